@@ -1,0 +1,110 @@
+// ref_harness.cpp -- drives the reference's own Hv calculators (compiled from
+// the reference sources by oracle/Makefile.ref into oracle/_ref/).  Test
+// infrastructure only: it calls HvInterfaceSetup / HvPrep / Hv exactly as the
+// reference driver does (lanczosEigenValuesVectors.cpp:93-103, :195) and
+// exposes the results to the oracle tests through a small C API.
+#include <cstring>
+#include <string>
+
+#include "interface.h"
+
+namespace {
+generalStor *g_general = nullptr;
+lanczosStor *g_lanczos = nullptr;
+void (*g_prep)(int, char **, generalStor *, lanczosStor *) = nullptr;
+void (*g_hv)(int, char **, generalStor *, lanczosStor *, double *, double *) = nullptr;
+char g_arg0[8] = "ref", g_arg1[8] = "unused";
+std::string g_hvdeck, g_potdeck;
+char *g_argv[4];
+}  // namespace
+
+extern "C" __attribute__((visibility("default"))) long long ref_setup(const char *calc, const char *hvdeck,
+                                                                      const char *potdeck)
+{
+    g_hvdeck = hvdeck;
+    g_potdeck = potdeck ? potdeck : "";
+    g_argv[0] = g_arg0;
+    g_argv[1] = g_arg1;
+    g_argv[2] = const_cast<char *>(g_hvdeck.c_str());
+    g_argv[3] = const_cast<char *>(g_potdeck.c_str());
+    g_lanczos = new lanczosStor();
+    HvInterfaceSetup(std::string(calc), &g_general, &g_prep, &g_hv);
+    (*g_prep)(potdeck ? 4 : 3, g_argv, g_general, g_lanczos);
+    return g_lanczos->total_basis_size;
+}
+
+extern "C" __attribute__((visibility("default"))) void ref_hv(const double *vec, double *uec)
+{
+    (*g_hv)(4, g_argv, g_general, g_lanczos, const_cast<double *>(vec), uec);
+}
+
+// V blocks of the NoQuad calculators, reference order [i][j][k][n][n']
+extern "C" __attribute__((visibility("default"))) int ref_vblocks(double *out, long long count)
+{
+    interfaceStor *s = reinterpret_cast<interfaceStor *>(g_general);
+    if (!s->potential || !s->potential->fullPotential) return -1;
+    std::memcpy(out, s->potential->fullPotential, sizeof(double) * (size_t)count);
+    return 0;
+}
+
+// kinetic matrix of axis 0/1/2 and the grid
+extern "C" __attribute__((visibility("default"))) void ref_kin(int axis, double *kin, double *grid)
+{
+    interfaceStor *s = reinterpret_cast<interfaceStor *>(g_general);
+    int n = axis == 0 ? s->grids->nx : axis == 1 ? s->grids->ny : s->grids->nz;
+    double *k = axis == 0 ? s->grids->xKinMat : axis == 1 ? s->grids->yKinMat : s->grids->zKinMat;
+    double *g = axis == 0 ? s->grids->x_Grid : axis == 1 ? s->grids->y_Grid : s->grids->z_Grid;
+    std::memcpy(kin, k, sizeof(double) * n * n);
+    std::memcpy(grid, g, sizeof(double) * n);
+}
+
+// quadrature nodes / weights and harmonic tables L_lm[n][a], S_m[m][b] (both phi halves)
+extern "C" __attribute__((visibility("default"))) void ref_quad(double *ct, double *wt, double *cp, double *wp,
+                                                                double *L, double *S1, double *S2)
+{
+    interfaceStor *s = reinterpret_cast<interfaceStor *>(g_general);
+    int na = s->quadrature->GLnum, nb = s->quadrature->GCnum;
+    int lmax = s->lmBasis->lmax, nn = s->lmBasis->length;
+    std::memcpy(ct, s->quadrature->GLabscissae, sizeof(double) * na);
+    std::memcpy(wt, s->quadrature->GLweights, sizeof(double) * na);
+    std::memcpy(cp, s->quadrature->GCabscissae, sizeof(double) * nb);
+    std::memcpy(wp, s->quadrature->GCweights, sizeof(double) * nb);
+    for (int n = 0; n < nn; n++)
+        for (int a = 0; a < na; a++) L[n * na + a] = s->tesseral->L_lm[n][a];
+    for (int m = 0; m < 2 * lmax + 1; m++)
+        for (int b = 0; b < nb; b++) {
+            S1[m * nb + b] = s->tesseral->S_m[m][b];
+            S2[m * nb + b] = s->tesseral2PI->S_m[m][b];
+        }
+}
+
+// water sites from a .xyz geometry (SPC/E model, getSPCEatoms)
+extern "C" __attribute__((visibility("default"))) int ref_spce_sites(const char *xyz, double *pos, char *type)
+{
+    char *atomType;
+    VECT *atomPos;
+    int nAtoms;
+    getSPCEatoms(&atomType, &atomPos, &nAtoms, std::string(xyz));
+    for (int i = 0; i < nAtoms; i++) {
+        for (int c = 0; c < 3; c++) pos[3 * i + c] = atomPos[i].COOR(c);
+        type[i] = atomType[i];
+    }
+    return nAtoms;
+}
+
+// point energies: Q_SPCE_Eng + LJ_SPCE_Eng_Fast at a position
+extern "C" __attribute__((visibility("default"))) double ref_spce_energy(const char *xyz, double x, double y,
+                                                                        double z, double q, int with_lj)
+{
+    char *atomType;
+    VECT *atomPos;
+    int nAtoms;
+    getSPCEatoms(&atomType, &atomPos, &nAtoms, std::string(xyz));
+    VECT pos(3);
+    pos.COOR(0) = x;
+    pos.COOR(1) = y;
+    pos.COOR(2) = z;
+    double e = Q_SPCE_Eng(pos, q, atomType, atomPos, nAtoms);
+    if (with_lj) e += LJ_SPCE_Eng_Fast(pos, atomType, atomPos, nAtoms);
+    return e;
+}

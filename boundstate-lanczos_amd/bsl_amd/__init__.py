@@ -1,0 +1,153 @@
+"""bsl_amd -- Python host interface of libbsl_hip.so, the MI355X
+implementation of the boundstate-lanczos Hv matvec and Lanczos recurrence.
+
+The Calculator mirrors the reference's HvPrep/Hv plugin (an opaque prepared
+operator with an accumulate-style Hv on reference-layout vectors) and adds
+the device-resident Lanczos loops.  lancbis/trivec are the Cullum-Willoughby
+post-processing of the reference driver."""
+import ctypes as C
+
+import numpy as np
+
+from . import cage, physics  # noqa: F401
+from ._lib import BASIS, CALC, POT, PREC, START, SYM, BslError, Params, check, lib  # noqa: F401
+
+
+class Calculator:
+    """A prepared Hv operator on the GPU (bsl_create).
+
+    Keyword arguments follow the Hv input deck of the reference
+    (HvPrep_Internal_NoQuad, linRotCartSphHvFullVPrepRoutines.cpp:248-321):
+    nx, ny, nz, x_max, y_max, z_max (nm, full box length), lmax, n_theta,
+    n_phi, mass, rot_const, ceiling, symmetry ("All"/"Even"/"Odd"), plus
+    basis ("full"/"reduced"), precision ("fp64"/"fp32"), potential
+    ("iho"/"cage") and its parameters."""
+
+    def __init__(self, *, nx, ny=None, nz=None, x_max, y_max=None, z_max=None, lmax, n_theta, n_phi,
+                 mass, rot_const, ceiling, symmetry="All", basis="full", calculator="noquad",
+                 precision="fp64", potential="iho", iho_factor=0.0, iho_ceil=0.0, sites=None,
+                 q_h2_h=physics.Q_H2_H, q_h2_cm=physics.Q_H2_CM, h2_half_bond=physics.H2_HALF_BOND,
+                 lj_a=physics.LJ_A_REF, lj_b=physics.LJ_B_REF):
+        p = Params()
+        p.nx, p.ny, p.nz = nx, ny or nx, nz or nx
+        p.x_max, p.y_max, p.z_max = x_max, y_max or x_max, z_max or x_max
+        p.lmax, p.n_theta, p.n_phi = lmax, n_theta, n_phi
+        p.mass, p.rot_const, p.ceiling = mass, rot_const, ceiling
+        p.symmetry = SYM[symmetry]
+        p.basis = BASIS[basis]
+        p.calculator = CALC[calculator]
+        p.precision = PREC[precision]
+        p.potential = POT[potential]
+        p.iho_factor, p.iho_ceil = iho_factor, iho_ceil
+        self._keep = []
+        if potential == "cage":
+            pos, q, lj = sites
+            pos = np.ascontiguousarray(pos, dtype=np.float64).ravel()
+            q = np.ascontiguousarray(q, dtype=np.float64)
+            lj = np.ascontiguousarray(lj, dtype=np.int32)
+            self._keep = [pos, q, lj]
+            p.n_sites = len(q)
+            p.site_pos, p.site_q, p.site_lj = pos.ctypes.data, q.ctypes.data, lj.ctypes.data
+        p.q_h2_h, p.q_h2_cm, p.h2_half_bond, p.lj_a, p.lj_b = q_h2_h, q_h2_cm, h2_half_bond, lj_a, lj_b
+        self.params = p
+        self.shape = (p.nx, p.ny, p.nz)
+        self.nn = (lmax + 1) ** 2
+        h = C.c_void_p()
+        check(lib().bsl_create(C.byref(p), C.byref(h)))
+        self.h = h
+        self.N = lib().bsl_basis_size(h)
+        self.N_krylov = lib().bsl_krylov_size(h)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().bsl_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # the reference's Hv contract: uec += H vec
+    def hv(self, vec, uec=None):
+        vec = np.ascontiguousarray(vec, dtype=np.float64)
+        if vec.size != self.N:
+            raise ValueError("vector length %d, basis size %d" % (vec.size, self.N))
+        out = np.zeros(self.N) if uec is None else np.ascontiguousarray(uec, dtype=np.float64).copy()
+        check(lib().bsl_hv(self.h, vec, out))
+        return out
+
+    def lanczos(self, niter, start="uniform", seed=0, v0=None):
+        a, b, b2 = np.zeros(niter), np.zeros(niter + 1), np.zeros(niter + 1)
+        ptr = None
+        if v0 is not None:
+            v0 = np.ascontiguousarray(v0, dtype=np.float64)
+            ptr = v0.ctypes.data
+        check(lib().bsl_lanczos(self.h, niter, ptr, START[start], seed, a, b, b2))
+        return a, b, b2
+
+    def ritz(self, niter, evtr, ngood, start="uniform", seed=0, v0=None):
+        out = np.zeros(self.N * max(ngood, 1))
+        ptr = None
+        if v0 is not None:
+            v0 = np.ascontiguousarray(v0, dtype=np.float64)
+            ptr = v0.ctypes.data
+        evtr = np.ascontiguousarray(evtr, dtype=np.float64)
+        check(lib().bsl_ritz(self.h, niter, ptr, START[start], seed, evtr, ngood, out))
+        return out[: self.N * ngood].reshape(ngood, self.N)
+
+    def rayleigh(self, y):
+        y = np.ascontiguousarray(np.atleast_2d(y), dtype=np.float64)
+        e, r = np.zeros(len(y)), np.zeros(len(y))
+        check(lib().bsl_rayleigh(self.h, len(y), y.ravel(), e, r))
+        return e, r
+
+    def vblocks(self):
+        nx, ny, nz = self.shape
+        out = np.zeros(nx * ny * nz * self.nn * self.nn)
+        check(lib().bsl_get_vblocks(self.h, out))
+        return out.reshape(nx, ny, nz, self.nn, self.nn)
+
+    def vgrid(self):
+        nx, ny, nz = self.shape
+        na, nb = self.params.n_theta, self.params.n_phi
+        v1, v2 = np.zeros(nx * ny * nz * na * nb), np.zeros(nx * ny * nz * na * nb)
+        check(lib().bsl_get_vgrid(self.h, v1, v2))
+        return v1.reshape(nx, ny, nz, na, nb), v2.reshape(nx, ny, nz, na, nb)
+
+    def time_steps(self, warmup, steps):
+        ms = np.zeros(4)
+        check(lib().bsl_time_steps(self.h, warmup, steps, ms))
+        return ms
+
+    def step_bytes(self):
+        out = np.zeros(5)
+        check(lib().bsl_step_bytes(self.h, out))
+        return out
+
+
+def lancbis(alpha, beta, beta2, emin, emax):
+    """Good eigenvalues and error estimates (lancbis.cpp:5-105)."""
+    niter = len(alpha)
+    ev, er = np.zeros(niter), np.zeros(niter)
+    ng = C.c_int32()
+    check(lib().bsl_lancbis(niter, np.ascontiguousarray(alpha, dtype=np.float64),
+                            np.ascontiguousarray(beta, dtype=np.float64),
+                            np.ascontiguousarray(beta2, dtype=np.float64), emin, emax, ev, er, C.byref(ng)))
+    return ev[: ng.value].copy(), er[: ng.value].copy()
+
+
+def trivec(alpha, beta, beta2, evals):
+    """Tm eigenvectors, column-major [niter][ngood] flattened (trivec.f)."""
+    niter, ng = len(alpha), len(evals)
+    evtr = np.zeros(niter * max(ng, 1))
+    mamin = np.zeros(max(ng, 1), dtype=np.int32)
+    check(lib().bsl_trivec(niter, np.ascontiguousarray(alpha, dtype=np.float64),
+                           np.ascontiguousarray(beta, dtype=np.float64),
+                           np.ascontiguousarray(beta2, dtype=np.float64),
+                           np.ascontiguousarray(evals, dtype=np.float64), ng, evtr, mamin))
+    return evtr[: niter * ng], mamin[:ng].copy()

@@ -1,0 +1,738 @@
+// bsl_capi.cpp -- the C ABI of libbsl_hip.so: context preparation (host
+// tables, device potential grid and V blocks), Hv on host or device vectors,
+// the device-resident Lanczos loops, and the host Cullum-Willoughby
+// post-processing.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "bsl_internal.h"
+
+namespace bsl {
+
+namespace {
+thread_local std::string g_err;
+}
+
+int fail(int code, const std::string &msg)
+{
+    g_err = msg;
+    return code;
+}
+
+int hip_fail(hipError_t e, const char *what, const char *file, int line)
+{
+    char buf[512];
+    std::snprintf(buf, sizeof buf, "%s failed: %s (%s:%d)", what, hipGetErrorString(e), file, line);
+    g_err = buf;
+    return e == hipErrorOutOfMemory ? BSL_E_NOMEM : BSL_E_HIP;
+}
+
+namespace {
+
+template <typename T>
+int dev_alloc(T **p, size_t count)
+{
+    *p = nullptr;
+    if (count == 0) count = 1;
+    BSL_HIP(hipMalloc((void **)p, sizeof(T) * count));
+    return BSL_OK;
+}
+
+template <typename T>
+int dev_upload(T **p, const std::vector<T> &v)
+{
+    int rc = dev_alloc(p, v.size());
+    if (rc) return rc;
+    BSL_HIP(hipMemcpy(*p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice));
+    return BSL_OK;
+}
+
+void dev_free(void *p)
+{
+    if (p) (void)hipFree(p);
+}
+
+int check_params(const bsl_params &p)
+{
+    if (p.nx < 2 || p.ny < 2 || p.nz < 2) return fail(BSL_E_ARG, "grid needs at least 2 points per axis");
+    if (p.nx > 64 || p.ny > 64 || p.nz > 64) return fail(BSL_E_ARG, "at most 64 grid points per axis");
+    if (!(p.x_max > 0) || !(p.y_max > 0) || !(p.z_max > 0)) return fail(BSL_E_ARG, "box lengths must be positive");
+    if (p.lmax < 0 || p.lmax > 6) return fail(BSL_E_ARG, "lmax must be in 0..6");
+    if (p.n_theta < 1 || p.n_phi < 1) return fail(BSL_E_ARG, "quadrature sizes must be positive");
+    if (!(p.mass > 0)) return fail(BSL_E_ARG, "mass must be positive");
+    if (p.symmetry < 0 || p.symmetry > 2) return fail(BSL_E_ARG, "symmetry flag not recognised");
+    if (p.basis < 0 || p.basis > 1) return fail(BSL_E_ARG, "basis flag not recognised");
+    if (p.calculator < 0 || p.calculator > 1) return fail(BSL_E_ARG, "calculator flag not recognised");
+    if (p.precision < 0 || p.precision > 2) return fail(BSL_E_ARG, "precision flag not recognised");
+    if (p.potential < 0 || p.potential > 1) return fail(BSL_E_ARG, "potential flag not recognised");
+    if (p.potential == BSL_POT_CAGE && (p.n_sites <= 0 || !p.site_pos || !p.site_q || !p.site_lj))
+        return fail(BSL_E_ARG, "cage potential needs sites");
+    if (p.potential == BSL_POT_CAGE && p.n_sites > 4096) return fail(BSL_E_ARG, "at most 4096 sites");
+    return BSL_OK;
+}
+
+// Build a DevOperator's potential part (packed blocks) from the grid
+int build_vpk(bsl_ctx *c, DevOperator &op, int vprec, const double *d_h)
+{
+    op.vprec = vprec;
+    const size_t es = vprec == BSL_PREC_FP64 ? sizeof(double) : sizeof(float);
+    BSL_HIP(hipMalloc(&op.vpk, es * (size_t)op.npair * (size_t)c->P));
+    int rc = launch_vblock_gemm(c->d_vgrid, d_h, op.vpk, vprec, c->P, c->Q, op.npair, c->stream);
+    if (rc) return rc;
+    return BSL_OK;
+}
+
+}  // namespace
+}  // namespace bsl
+
+using namespace bsl;
+
+extern "C" const char *bsl_last_error(void) { return g_err.c_str(); }
+
+extern "C" int bsl_destroy(bsl_ctx *c)
+{
+    if (!c) return BSL_OK;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    DevOperator *ops[2] = {&c->op, &c->op64};
+    for (int i = 0; i < 2; i++) {
+        DevOperator &o = *ops[i];
+        if (i == 1 && c->op64.vpk == c->op.vpk) break;  // shared with op
+        dev_free(o.vpk);
+    }
+    DevOperator &o = c->op;
+    dev_free(o.tx);
+    if (o.ty != o.tx) dev_free(o.ty);
+    if (o.tz != o.tx && o.tz != o.ty) dev_free(o.tz);
+    dev_free(o.rot);
+    dev_free(o.mask);
+    dev_free(o.plane_ref);
+    dev_free(o.partial);
+    dev_free(o.counter);
+    dev_free(c->d_gx);
+    dev_free(c->d_gy);
+    dev_free(c->d_gz);
+    dev_free(c->d_vgrid);
+    dev_free(c->d_X);
+    dev_free(c->d_R);
+    dev_free(c->d_T);
+    dev_free(c->d_scal);
+    dev_free(c->d_alpha);
+    dev_free(c->d_beta);
+    dev_free(c->d_beta2);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return BSL_OK;
+}
+
+extern "C" int bsl_create(const bsl_params *pp, bsl_ctx **out)
+{
+    if (!pp || !out) return fail(BSL_E_ARG, "null argument");
+    *out = nullptr;
+    int rc = check_params(*pp);
+    if (rc) return rc;
+    if (pp->calculator == BSL_CALC_QUAD) return fail(BSL_E_ARG, "the quadrature calculator is built separately");
+    std::unique_ptr<bsl_ctx> c(new bsl_ctx());
+    c->p = *pp;
+    if (pp->potential == BSL_POT_CAGE) {
+        c->site_pos.assign(pp->site_pos, pp->site_pos + 3 * pp->n_sites);
+        c->site_q.assign(pp->site_q, pp->site_q + pp->n_sites);
+        c->site_lj.assign(pp->site_lj, pp->site_lj + pp->n_sites);
+        c->p.site_pos = c->site_pos.data();
+        c->p.site_q = c->site_q.data();
+        c->p.site_lj = c->site_lj.data();
+    }
+    const bsl_params &p = c->p;
+    BSL_HIP(hipGetDevice(&c->device));
+    BSL_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+
+    // ---- host tables (HvPrep_Internal_NoQuad :356-455) ----
+    build_tables(p.lmax, p.n_theta, p.n_phi, c->tab);
+    const Tables &t = c->tab;
+    c->nn = t.nn;
+    c->P = (int64_t)p.nx * p.ny * p.nz;
+    c->n_full = c->P * c->nn;
+    std::vector<double> rowx, rowy, rowz, gx, gy, gz;
+    kinetic_row(p.x_max, p.nx, p.mass, rowx, gx);
+    kinetic_row(p.y_max, p.ny, p.mass, rowy, gy);
+    kinetic_row(p.z_max, p.nz, p.mass, rowz, gz);
+
+    // carried lm planes, grouped by parity block (even l first), each block
+    // in reference lm order
+    auto in_sym = [&](int l) { return p.symmetry == BSL_SYM_ALL || (l % 2 == 0) == (p.symmetry == BSL_SYM_EVEN); };
+    DevOperator &op = c->op;
+    op.nx = p.nx;
+    op.ny = p.ny;
+    op.nz = p.nz;
+    op.P = c->P;
+    int64_t pairs = 0;
+    for (int parity = 0; parity < 2; parity++) {
+        Block b;
+        b.plane0 = (int)c->carried.size();
+        for (int n = 0; n < t.nn; n++)
+            if (t.l[n] % 2 == parity && (p.basis == BSL_BASIS_FULL || in_sym(t.l[n]))) c->carried.push_back(n);
+        b.size = (int)c->carried.size() - b.plane0;
+        if (b.size == 0) continue;
+        b.pair_offset = pairs;
+        b.mask = (p.basis == BSL_BASIS_FULL && !in_sym(parity)) ? 0.0 : 1.0;
+        pairs += (int64_t)b.size * (b.size + 1) / 2;
+        op.blk[op.nblk++] = b;
+    }
+    op.npair = pairs;
+    const int nk = (int)c->carried.size();
+    op.nk = nk;
+    c->n_kry = c->P * nk;
+    for (int kidx = 0; kidx < nk; kidx++) {
+        const int n = c->carried[kidx];
+        const double l = (double)t.l[n];
+        c->rot.push_back(p.rot_const * l * (l + 1.0));  // rotKinEng :522-538
+        c->mask.push_back(p.basis == BSL_BASIS_FULL ? (in_sym(t.l[n]) ? 1.0 : 0.0) : 1.0);
+    }
+
+    // H[q][pair] = w_a w_b L_n(a) S_m(b) L_n'(a) S_m'(b), q = (half, a, b)
+    // (calc_Vlmlpmp_NoQuad linRotCartSphHvFullVPrepRoutines.cpp:116-131)
+    const int na = t.na, nb = t.nb, lmax = t.lmax;
+    c->Q = 2 * na * nb;
+    std::vector<double> h((size_t)c->Q * pairs);
+    for (int half = 0; half < 2; half++) {
+        const std::vector<double> &S = half == 0 ? t.S1 : t.S2;
+        for (int a = 0; a < na; a++)
+            for (int b = 0; b < nb; b++) {
+                const size_t q = (size_t)half * na * nb + (size_t)a * nb + b;
+                for (int g = 0; g < op.nblk; g++) {
+                    const Block &B = op.blk[g];
+                    int64_t pr = B.pair_offset;
+                    for (int r = 0; r < B.size; r++)
+                        for (int cc = r; cc < B.size; cc++) {
+                            const int n = c->carried[B.plane0 + r], np = c->carried[B.plane0 + cc];
+                            const int m = t.m[n], mp = t.m[np];
+                            h[q * pairs + pr] = t.wt[a] * t.wp[b] * t.L[(size_t)n * na + a] *
+                                                S[(size_t)(m + lmax) * nb + b] * t.L[(size_t)np * na + a] *
+                                                S[(size_t)(mp + lmax) * nb + b];
+                            pr++;
+                        }
+                }
+            }
+    }
+    // H atom offsets for each q (Alavi_ContGrid_H2_Routines.cpp:74-90)
+    std::vector<double> off((size_t)c->Q * 3);
+    for (int a = 0; a < na; a++)
+        for (int b = 0; b < nb; b++) {
+            const size_t q0 = (size_t)a * nb + b, q1 = (size_t)na * nb + q0;
+            off[3 * q0 + 0] = p.h2_half_bond * std::sin(t.th[a]) * t.cp[b];
+            off[3 * q0 + 1] = p.h2_half_bond * std::sin(t.th[a]) * std::sin(t.ph1[b]);
+            off[3 * q0 + 2] = p.h2_half_bond * t.ct[a];
+            off[3 * q1 + 0] = off[3 * q0 + 0];
+            off[3 * q1 + 1] = -1.0 * (off[3 * q0 + 1]);
+            off[3 * q1 + 2] = off[3 * q0 + 2];
+        }
+
+    // ---- device: tables ----
+    if ((rc = dev_upload(&op.tx, rowx))) return rc;
+    if ((rc = dev_upload(&op.ty, rowy))) return rc;
+    if ((rc = dev_upload(&op.tz, rowz))) return rc;
+    if ((rc = dev_upload(&op.rot, c->rot))) return rc;
+    if ((rc = dev_upload(&op.mask, c->mask))) return rc;
+    if ((rc = dev_upload(&op.plane_ref, c->carried))) return rc;
+    if ((rc = dev_upload(&c->d_gx, gx))) return rc;
+    if ((rc = dev_upload(&c->d_gy, gy))) return rc;
+    if ((rc = dev_upload(&c->d_gz, gz))) return rc;
+    op.npartial = (int)std::max<int64_t>(2 * ((c->P + 255) / 256), 4096);
+    if ((rc = dev_alloc(&op.partial, (size_t)op.npartial))) return rc;
+    if ((rc = dev_alloc(&op.counter, 1))) return rc;
+    BSL_HIP(hipMemset(op.counter, 0, sizeof(unsigned int)));
+
+    // ---- device: potential grid and V blocks ----
+    BSL_HIP(hipMalloc((void **)&c->d_vgrid, sizeof(double) * (size_t)c->P * c->Q));
+    if (p.potential == BSL_POT_CAGE) {
+        std::vector<double> site4((size_t)4 * p.n_sites);
+        for (int s = 0; s < p.n_sites; s++) {
+            for (int d = 0; d < 3; d++) site4[4 * s + d] = p.site_pos[3 * s + d];
+            site4[4 * s + 3] = p.site_q[s];
+        }
+        std::vector<int> lj(p.site_lj, p.site_lj + p.n_sites);
+        CageDev cage;
+        cage.nsites = p.n_sites;
+        double *d_off = nullptr;
+        if ((rc = dev_upload(&cage.site, site4))) return rc;
+        if ((rc = dev_upload(&cage.lj, lj))) return rc;
+        if ((rc = dev_upload(&d_off, off))) return rc;
+        rc = launch_vgrid_cage(c->d_vgrid, c->P, c->Q, c->d_gx, c->d_gy, c->d_gz, p.nx, p.ny, d_off, cage,
+                               p.q_h2_h, p.q_h2_cm, p.lj_a, p.lj_b, p.ceiling, c->stream);
+        BSL_HIP(hipStreamSynchronize(c->stream));
+        dev_free(cage.site);
+        dev_free(cage.lj);
+        dev_free(d_off);
+        if (rc) return rc;
+    } else {
+        rc = launch_vgrid_iho(c->d_vgrid, c->P, c->Q, c->d_gx, c->d_gy, c->d_gz, p.nx, p.ny, p.iho_factor,
+                              p.iho_ceil, p.ceiling, c->stream);
+        if (rc) return rc;
+    }
+    double *d_h = nullptr;
+    if ((rc = dev_upload(&d_h, h))) return rc;
+    c->op64 = op;
+    c->op64.vpk = nullptr;
+    if ((rc = build_vpk(c.get(), c->op64, BSL_PREC_FP64, d_h))) return rc;
+    if (p.precision == BSL_PREC_FP64) {
+        op.vpk = c->op64.vpk;
+        op.vprec = BSL_PREC_FP64;
+    } else {
+        if ((rc = build_vpk(c.get(), op, BSL_PREC_FP32, d_h))) return rc;
+    }
+    BSL_HIP(hipStreamSynchronize(c->stream));
+    dev_free(d_h);
+    dev_free(c->d_vgrid);
+    c->d_vgrid = nullptr;
+
+    // ---- device: Krylov vectors and scalars ----
+    if ((rc = dev_alloc(&c->d_X, (size_t)c->n_kry))) return rc;
+    if ((rc = dev_alloc(&c->d_R, (size_t)c->n_kry))) return rc;
+    if ((rc = dev_alloc(&c->d_scal, 1))) return rc;
+    BSL_HIP(hipMemset(c->d_scal, 0, sizeof(Scal)));
+    *out = c.release();
+    return BSL_OK;
+}
+
+extern "C" int64_t bsl_basis_size(const bsl_ctx *c) { return c ? c->n_full : -1; }
+extern "C" int64_t bsl_krylov_size(const bsl_ctx *c) { return c ? c->n_kry : -1; }
+
+namespace {
+// copy the carried planes of a reference-layout host vector into a device vector
+int upload_carried(bsl_ctx *c, const double *host, double *dev)
+{
+    for (size_t k = 0; k < c->carried.size(); k++)
+        BSL_HIP(hipMemcpyAsync(dev + k * c->P, host + (size_t)c->carried[k] * c->P, sizeof(double) * c->P,
+                               hipMemcpyHostToDevice, c->stream));
+    return BSL_OK;
+}
+
+int download_carried(bsl_ctx *c, const double *dev, double *host)
+{
+    for (size_t k = 0; k < c->carried.size(); k++)
+        BSL_HIP(hipMemcpyAsync(host + (size_t)c->carried[k] * c->P, dev + k * c->P, sizeof(double) * c->P,
+                               hipMemcpyDeviceToHost, c->stream));
+    return BSL_OK;
+}
+
+int ensure_tmp(bsl_ctx *c)
+{
+    if (!c->d_T) {
+        int rc = dev_alloc(&c->d_T, (size_t)2 * c->n_kry);
+        if (rc) return rc;
+    }
+    return BSL_OK;
+}
+
+int ensure_coeffs(bsl_ctx *c, int niter)
+{
+    if (c->niter_cap >= niter) return BSL_OK;
+    dev_free(c->d_alpha);
+    dev_free(c->d_beta);
+    dev_free(c->d_beta2);
+    c->d_alpha = c->d_beta = c->d_beta2 = nullptr;
+    int rc;
+    if ((rc = dev_alloc(&c->d_alpha, (size_t)niter + 1))) return rc;
+    if ((rc = dev_alloc(&c->d_beta, (size_t)niter + 1))) return rc;
+    if ((rc = dev_alloc(&c->d_beta2, (size_t)niter + 1))) return rc;
+    c->niter_cap = niter;
+    return BSL_OK;
+}
+
+// host Hv on the context operator `op`: uec += H vec
+int hv_host_with(bsl_ctx *c, const DevOperator &op, const double *vec, double *uec)
+{
+    int rc = ensure_tmp(c);
+    if (rc) return rc;
+    double *dX = c->d_T, *dR = c->d_T + c->n_kry;
+    Scal *sc = c->d_scal;
+    if ((rc = upload_carried(c, vec, dX))) return rc;
+    if ((rc = upload_carried(c, uec, dR))) return rc;
+    // keep the Lanczos scalars: stash them and restore afterwards
+    Scal saved;
+    BSL_HIP(hipMemcpyAsync(&saved, sc, sizeof(Scal), hipMemcpyDeviceToHost, c->stream));
+    if ((rc = launch_init_scal(sc, 1.0, 1.0, c->stream))) return rc;
+    if ((rc = launch_hv(op, dX, dR, sc, nullptr, c->stream, nullptr))) return rc;
+    if ((rc = download_carried(c, dR, uec))) return rc;
+    BSL_HIP(hipStreamSynchronize(c->stream));
+    BSL_HIP(hipMemcpyAsync(sc, &saved, sizeof(Scal), hipMemcpyHostToDevice, c->stream));
+    BSL_HIP(hipStreamSynchronize(c->stream));
+    return BSL_OK;
+}
+
+int start_lanczos(bsl_ctx *c, const double *v0, int start, uint64_t seed)
+{
+    int rc;
+    if (v0) {
+        if ((rc = upload_carried(c, v0, c->d_X))) return rc;
+        if ((rc = launch_init_scal(c->d_scal, 1.0, 0.0, c->stream))) return rc;
+    } else {
+        if (start != BSL_START_UNIFORM && start != BSL_START_RANDOM) return fail(BSL_E_ARG, "unknown start vector");
+        rc = launch_start_vector(c->d_X, c->n_kry, c->P, c->op.plane_ref, start, seed, c->d_scal, c->op.partial,
+                                 c->op.counter, c->op.npartial, c->stream);
+        if (rc) return rc;
+    }
+    return launch_zero(c->d_R, c->n_kry, c->stream);
+}
+
+// one Lanczos step j (1-based) on the current X/R; swaps the buffers
+int lanczos_step(bsl_ctx *c, const DevOperator &op, int j, hipEvent_t *ev)
+{
+    int rc = launch_hv(op, c->d_X, c->d_R, c->d_scal, c->d_alpha ? c->d_alpha + (j - 1) : nullptr, c->stream, ev);
+    if (rc) return rc;
+    rc = launch_update(op, c->d_X, c->d_R, c->d_scal, c->d_beta ? c->d_beta + j : nullptr,
+                       c->d_beta2 ? c->d_beta2 + j : nullptr, c->stream);
+    if (rc) return rc;
+    if (ev) BSL_HIP(hipEventRecord(ev[3], c->stream));
+    std::swap(c->d_X, c->d_R);
+    return BSL_OK;
+}
+}  // namespace
+
+extern "C" int bsl_hv(bsl_ctx *c, const double *vec, double *uec)
+{
+    if (!c || !vec || !uec) return fail(BSL_E_ARG, "null argument");
+    BSL_HIP(hipSetDevice(c->device));
+    return hv_host_with(c, c->op, vec, uec);
+}
+
+extern "C" int bsl_hv_device(bsl_ctx *c, const double *d_vec, double *d_uec)
+{
+    if (!c || !d_vec || !d_uec) return fail(BSL_E_ARG, "null argument");
+    BSL_HIP(hipSetDevice(c->device));
+    int rc = ensure_tmp(c);
+    if (rc) return rc;
+    double *dX = c->d_T, *dR = c->d_T + c->n_kry;
+    for (size_t k = 0; k < c->carried.size(); k++) {
+        BSL_HIP(hipMemcpyAsync(dX + k * c->P, d_vec + (size_t)c->carried[k] * c->P, sizeof(double) * c->P,
+                               hipMemcpyDeviceToDevice, c->stream));
+        BSL_HIP(hipMemcpyAsync(dR + k * c->P, d_uec + (size_t)c->carried[k] * c->P, sizeof(double) * c->P,
+                               hipMemcpyDeviceToDevice, c->stream));
+    }
+    Scal saved;
+    BSL_HIP(hipMemcpyAsync(&saved, c->d_scal, sizeof(Scal), hipMemcpyDeviceToHost, c->stream));
+    if ((rc = launch_init_scal(c->d_scal, 1.0, 1.0, c->stream))) return rc;
+    if ((rc = launch_hv(c->op, dX, dR, c->d_scal, nullptr, c->stream, nullptr))) return rc;
+    for (size_t k = 0; k < c->carried.size(); k++)
+        BSL_HIP(hipMemcpyAsync(d_uec + (size_t)c->carried[k] * c->P, dR + k * c->P, sizeof(double) * c->P,
+                               hipMemcpyDeviceToDevice, c->stream));
+    BSL_HIP(hipStreamSynchronize(c->stream));
+    BSL_HIP(hipMemcpyAsync(c->d_scal, &saved, sizeof(Scal), hipMemcpyHostToDevice, c->stream));
+    BSL_HIP(hipStreamSynchronize(c->stream));
+    return BSL_OK;
+}
+
+extern "C" int bsl_lanczos(bsl_ctx *c, int32_t niter, const double *v0, int32_t start, uint64_t seed,
+                           double *alpha, double *beta, double *beta2)
+{
+    if (!c || niter <= 0 || !alpha || !beta || !beta2) return fail(BSL_E_ARG, "bad arguments");
+    BSL_HIP(hipSetDevice(c->device));
+    int rc = ensure_coeffs(c, niter);
+    if (rc) return rc;
+    if ((rc = start_lanczos(c, v0, start, seed))) return rc;
+    for (int j = 1; j <= niter; j++)
+        if ((rc = lanczos_step(c, c->op, j, nullptr))) return rc;
+    BSL_HIP(hipMemcpyAsync(alpha, c->d_alpha, sizeof(double) * niter, hipMemcpyDeviceToHost, c->stream));
+    BSL_HIP(hipMemcpyAsync(beta + 1, c->d_beta + 1, sizeof(double) * niter, hipMemcpyDeviceToHost, c->stream));
+    BSL_HIP(hipMemcpyAsync(beta2 + 1, c->d_beta2 + 1, sizeof(double) * niter, hipMemcpyDeviceToHost, c->stream));
+    BSL_HIP(hipStreamSynchronize(c->stream));
+    beta[0] = 0.0;
+    beta2[0] = 0.0;
+    return BSL_OK;
+}
+
+extern "C" int bsl_lancbis(int32_t niter, const double *alpha, const double *beta, const double *beta2,
+                           double emin, double emax, double *eval, double *err, int32_t *ngood)
+{
+    if (!alpha || !beta || !beta2 || !eval || !err || !ngood) return fail(BSL_E_ARG, "null argument");
+    int ng = 0;
+    int rc = cw_lancbis(niter, alpha, beta, beta2, emin, emax, eval, err, &ng);
+    *ngood = ng;
+    return rc;
+}
+
+extern "C" int bsl_trivec(int32_t niter, const double *alpha, const double *beta, const double *beta2,
+                          const double *eval, int32_t ngood, double *evtr, int32_t *mamin)
+{
+    if (!alpha || !beta || !beta2 || (ngood > 0 && (!eval || !evtr || !mamin))) return fail(BSL_E_ARG, "null argument");
+    std::vector<int> mm(std::max(ngood, 1));
+    int rc = cw_trivec(niter, alpha, beta, beta2, eval, ngood, evtr, mm.data());
+    for (int i = 0; i < ngood; i++) mamin[i] = mm[i];
+    return rc;
+}
+
+extern "C" int bsl_ritz(bsl_ctx *c, int32_t niter, const double *v0, int32_t start, uint64_t seed,
+                        const double *evtr, int32_t ngood, double *ritz)
+{
+    if (!c || niter <= 0 || ngood < 0 || (ngood > 0 && (!evtr || !ritz))) return fail(BSL_E_ARG, "bad arguments");
+    if (ngood == 0) return BSL_OK;
+    BSL_HIP(hipSetDevice(c->device));
+    // coefficient of v_j for each vector, zero once the cumulative norm
+    // reaches 1 - 1e-16 (lanczosEigenValuesVectors.cpp:404-419)
+    std::vector<double> coef((size_t)niter * ngood);
+    for (int g = 0; g < ngood; g++) {
+        double cum = 0.0;
+        for (int j = 1; j <= niter; j++) {
+            const double e = evtr[(j - 1) + (size_t)niter * g];
+            cum += std::pow(e, 2.);
+            coef[(size_t)(j - 1) * ngood + g] = (cum < (1. - 1.e-16)) ? e : 0.0;
+        }
+    }
+    double *d_coef = nullptr, *d_ritz = nullptr;
+    int rc;
+    if ((rc = dev_upload(&d_coef, coef))) return rc;
+    if ((rc = dev_alloc(&d_ritz, (size_t)c->n_kry * ngood))) {
+        dev_free(d_coef);
+        return rc;
+    }
+    BSL_HIP(hipMemsetAsync(d_ritz, 0, sizeof(double) * (size_t)c->n_kry * ngood, c->stream));
+    rc = ensure_coeffs(c, niter);
+    if (!rc) rc = start_lanczos(c, v0, start, seed);
+    for (int j = 1; j <= niter && !rc; j++) {
+        rc = launch_ritz_accum(d_ritz, c->d_X, c->d_scal, d_coef + (size_t)(j - 1) * ngood, c->n_kry, ngood,
+                               c->stream);
+        if (!rc) rc = lanczos_step(c, c->op, j, nullptr);
+    }
+    if (!rc) {
+        std::memset(ritz, 0, sizeof(double) * (size_t)c->n_full * ngood);
+        for (int g = 0; g < ngood && !rc; g++) rc = download_carried(c, d_ritz + (size_t)g * c->n_kry, ritz + (size_t)g * c->n_full);
+        if (!rc) {
+            hipError_t e = hipStreamSynchronize(c->stream);
+            if (e != hipSuccess) rc = hip_fail(e, "hipStreamSynchronize", __FILE__, __LINE__);
+        }
+    }
+    dev_free(d_coef);
+    dev_free(d_ritz);
+    return rc;
+}
+
+extern "C" int bsl_rayleigh(bsl_ctx *c, int32_t count, const double *y, double *energy, double *resid)
+{
+    if (!c || count < 0 || (count > 0 && (!y || !energy || !resid))) return fail(BSL_E_ARG, "bad arguments");
+    BSL_HIP(hipSetDevice(c->device));
+    std::vector<double> hy((size_t)c->n_full);
+    for (int g = 0; g < count; g++) {
+        const double *v = y + (size_t)g * c->n_full;
+        std::fill(hy.begin(), hy.end(), 0.0);
+        int rc = hv_host_with(c, c->op64, v, hy.data());
+        if (rc) return rc;
+        double vv = 0, vh = 0;
+        for (int64_t i = 0; i < c->n_full; i++) {
+            vv += v[i] * v[i];
+            vh += v[i] * hy[i];
+        }
+        const double e = vh / vv;
+        double r2 = 0;
+        for (int64_t i = 0; i < c->n_full; i++) {
+            const double d = hy[i] - e * v[i];
+            r2 += d * d;
+        }
+        energy[g] = e;
+        resid[g] = std::sqrt(r2 / vv);
+    }
+    return BSL_OK;
+}
+
+extern "C" int bsl_water_sites(int32_t model, int32_t nmol, const double *cm_ea, double *pos, double *q,
+                               int32_t *lj)
+{
+    if (model != 0) return fail(BSL_E_ARG, "only the SPC/E water model (0) is provided");
+    if (nmol < 0 || (nmol > 0 && (!cm_ea || !pos || !q || !lj))) return fail(BSL_E_ARG, "bad arguments");
+    // body-frame SPC/E sites (nm) and charges (SPCE_AH2_EngRoutines.h:17-32)
+    static const double body[3][3] = {{0., 0., 0.006461408220},
+                                      {-8.164965810E-2, 0., -0.05127361870},
+                                      {8.164965810E-2, 0., -0.05127361870}};
+    static const double charge[3] = {-0.8476, 0.4238, 0.4238};
+    for (int i = 0; i < nmol; i++) {
+        const double *r = cm_ea + 6 * i;
+        const double cp = std::cos(r[1]), sp = std::sin(r[1]);
+        const double ct = std::cos(r[3]), st = std::sin(r[3]);
+        const double cx = std::cos(r[5]), sx = std::sin(r[5]);
+        const double rot[3][3] = {{(cp * ct * cx) - (sp * sx), (-1 * cp * ct * sx) - (sp * cx), cp * st},
+                                  {(sp * ct * cx) + (cp * sx), (-1 * sp * ct * sx) + (cp * cx), sp * st},
+                                  {-1 * st * cx, st * sx, ct}};
+        const double cm[3] = {r[0], r[2], r[4]};
+        for (int s = 0; s < 3; s++) {
+            for (int d = 0; d < 3; d++) {
+                double acc = 0.;
+                for (int e = 0; e < 3; e++) acc += rot[d][e] * body[s][e];
+                pos[(3 * i + s) * 3 + d] = acc + 0.1 * cm[d];
+            }
+            q[3 * i + s] = charge[s];
+            lj[3 * i + s] = s == 0 ? 1 : 0;
+        }
+    }
+    return BSL_OK;
+}
+
+extern "C" int bsl_get_vblocks(bsl_ctx *c, double *out)
+{
+    if (!c || !out) return fail(BSL_E_ARG, "null argument");
+    if (c->p.basis != BSL_BASIS_FULL) return fail(BSL_E_STATE, "V blocks are reported for FULL-basis contexts");
+    BSL_HIP(hipSetDevice(c->device));
+    const DevOperator &op = c->op64;
+    std::vector<double> vpk((size_t)op.npair * c->P);
+    BSL_HIP(hipMemcpy(vpk.data(), op.vpk, sizeof(double) * vpk.size(), hipMemcpyDeviceToHost));
+    const int nn = c->nn, nx = op.nx, ny = op.ny, nz = op.nz;
+    std::memset(out, 0, sizeof(double) * (size_t)c->P * nn * nn);
+    for (int g = 0; g < op.nblk; g++) {
+        const Block &B = op.blk[g];
+        int64_t pr = B.pair_offset;
+        for (int r = 0; r < B.size; r++)
+            for (int cc = r; cc < B.size; cc++, pr++) {
+                const int n = c->carried[B.plane0 + r], np = c->carried[B.plane0 + cc];
+                for (int k = 0; k < nz; k++)
+                    for (int j = 0; j < ny; j++)
+                        for (int i = 0; i < nx; i++) {
+                            const int64_t p = ((int64_t)k * ny + j) * nx + i;
+                            const int64_t ref = (((int64_t)i * ny + j) * nz + k);
+                            const double v = vpk[(size_t)pr * c->P + p];
+                            out[(ref * nn + n) * nn + np] = v;
+                            out[(ref * nn + np) * nn + n] = v;
+                        }
+            }
+    }
+    return BSL_OK;
+}
+
+extern "C" int bsl_get_vgrid(bsl_ctx *c, double *v1, double *v2)
+{
+    if (!c || !v1 || !v2) return fail(BSL_E_ARG, "null argument");
+    BSL_HIP(hipSetDevice(c->device));
+    // the grid is regenerated on demand (it is not kept after preparation)
+    const bsl_params &p = c->p;
+    double *d_vg = nullptr;
+    int rc = dev_alloc(&d_vg, (size_t)c->P * c->Q);
+    if (rc) return rc;
+    const Tables &t = c->tab;
+    const int na = t.na, nb = t.nb;
+    if (p.potential == BSL_POT_CAGE) {
+        std::vector<double> off((size_t)c->Q * 3), site4((size_t)4 * p.n_sites);
+        for (int a = 0; a < na; a++)
+            for (int b = 0; b < nb; b++) {
+                const size_t q0 = (size_t)a * nb + b, q1 = (size_t)na * nb + q0;
+                off[3 * q0 + 0] = p.h2_half_bond * std::sin(t.th[a]) * t.cp[b];
+                off[3 * q0 + 1] = p.h2_half_bond * std::sin(t.th[a]) * std::sin(t.ph1[b]);
+                off[3 * q0 + 2] = p.h2_half_bond * t.ct[a];
+                off[3 * q1 + 0] = off[3 * q0 + 0];
+                off[3 * q1 + 1] = -1.0 * (off[3 * q0 + 1]);
+                off[3 * q1 + 2] = off[3 * q0 + 2];
+            }
+        for (int s = 0; s < p.n_sites; s++) {
+            for (int d = 0; d < 3; d++) site4[4 * s + d] = p.site_pos[3 * s + d];
+            site4[4 * s + 3] = p.site_q[s];
+        }
+        std::vector<int> lj(p.site_lj, p.site_lj + p.n_sites);
+        CageDev cage;
+        cage.nsites = p.n_sites;
+        double *d_off = nullptr;
+        if (!rc) rc = dev_upload(&cage.site, site4);
+        if (!rc) rc = dev_upload(&cage.lj, lj);
+        if (!rc) rc = dev_upload(&d_off, off);
+        if (!rc)
+            rc = launch_vgrid_cage(d_vg, c->P, c->Q, c->d_gx, c->d_gy, c->d_gz, p.nx, p.ny, d_off, cage, p.q_h2_h,
+                                   p.q_h2_cm, p.lj_a, p.lj_b, p.ceiling, c->stream);
+        (void)hipStreamSynchronize(c->stream);
+        dev_free(cage.site);
+        dev_free(cage.lj);
+        dev_free(d_off);
+    } else {
+        rc = launch_vgrid_iho(d_vg, c->P, c->Q, c->d_gx, c->d_gy, c->d_gz, p.nx, p.ny, p.iho_factor, p.iho_ceil,
+                              p.ceiling, c->stream);
+    }
+    std::vector<double> vg;
+    if (!rc) {
+        vg.resize((size_t)c->P * c->Q);
+        hipError_t e = hipMemcpyAsync(vg.data(), d_vg, sizeof(double) * vg.size(), hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess) rc = hip_fail(e, "download potential grid", __FILE__, __LINE__);
+    }
+    dev_free(d_vg);
+    if (rc) return rc;
+    const int nx = p.nx, ny = p.ny, nz = p.nz;
+    for (int k = 0; k < nz; k++)
+        for (int j = 0; j < ny; j++)
+            for (int i = 0; i < nx; i++) {
+                const int64_t pp = ((int64_t)k * ny + j) * nx + i;
+                const int64_t ref = ((int64_t)i * ny + j) * nz + k;
+                for (int q = 0; q < na * nb; q++) {
+                    v1[ref * na * nb + q] = vg[(size_t)pp * c->Q + q];
+                    v2[ref * na * nb + q] = vg[(size_t)pp * c->Q + na * nb + q];
+                }
+            }
+    return BSL_OK;
+}
+
+extern "C" int bsl_step_bytes(const bsl_ctx *c, double *out)
+{
+    if (!c || !out) return fail(BSL_E_ARG, "null argument");
+    const DevOperator &op = c->op;
+    const double es = op.vprec == BSL_PREC_FP64 ? 8.0 : 4.0;
+    const double vec = 8.0 * (double)c->n_kry;
+    out[0] = es * (double)op.npair * (double)c->P + 3.0 * vec;  // potential kernel
+    out[1] = 6.0 * vec;                                          // kinetic kernels (xy and z)
+    out[2] = 3.0 * vec;                                          // update kernel
+    out[3] = es * (double)op.npair * (double)c->P + 6.0 * vec;   // fused-step minimum
+    double pair_fma = 0.0;
+    for (int g = 0; g < op.nblk; g++) pair_fma += (double)op.blk[g].size * op.blk[g].size;
+    out[4] = 2.0 * (double)c->P * pair_fma + 2.0 * (double)(op.nx + op.ny + op.nz) * (double)c->n_kry +
+             4.0 * (double)c->n_kry;
+    return BSL_OK;
+}
+
+extern "C" int bsl_time_steps(bsl_ctx *c, int32_t warmup, int32_t steps, double *ms)
+{
+    if (!c || steps <= 0 || warmup < 0 || !ms) return fail(BSL_E_ARG, "bad arguments");
+    BSL_HIP(hipSetDevice(c->device));
+    int rc = ensure_coeffs(c, warmup + steps + 1);
+    if (rc) return rc;
+    if ((rc = start_lanczos(c, nullptr, BSL_START_RANDOM, 0))) return rc;
+    int j = 1;
+    for (int w = 0; w < warmup; w++, j++)
+        if ((rc = lanczos_step(c, c->op, j, nullptr))) return rc;
+    hipEvent_t e0, e1;
+    BSL_HIP(hipEventCreate(&e0));
+    BSL_HIP(hipEventCreate(&e1));
+    BSL_HIP(hipStreamSynchronize(c->stream));
+    BSL_HIP(hipEventRecord(e0, c->stream));
+    const int j0 = j;
+    for (int s = 0; s < steps; s++, j++)
+        if ((rc = lanczos_step(c, c->op, std::min(j, warmup + steps), nullptr))) return rc;
+    BSL_HIP(hipEventRecord(e1, c->stream));
+    BSL_HIP(hipEventSynchronize(e1));
+    float total = 0;
+    BSL_HIP(hipEventElapsedTime(&total, e0, e1));
+    (void)j0;
+    // per-kernel durations: events around each kernel group, fewer steps
+    const int ksteps = std::min(steps, 64);
+    std::vector<hipEvent_t> ev((size_t)4 * ksteps);
+    for (auto &e : ev) BSL_HIP(hipEventCreate(&e));
+    double kin = 0, pot = 0, upd = 0;
+    for (int s = 0; s < ksteps; s++)
+        if ((rc = lanczos_step(c, c->op, std::min(j, warmup + steps), &ev[(size_t)4 * s]))) return rc;
+    BSL_HIP(hipStreamSynchronize(c->stream));
+    for (int s = 0; s < ksteps; s++) {
+        float a = 0, b = 0, d = 0;
+        BSL_HIP(hipEventElapsedTime(&a, ev[4 * s + 0], ev[4 * s + 1]));
+        BSL_HIP(hipEventElapsedTime(&b, ev[4 * s + 1], ev[4 * s + 2]));
+        BSL_HIP(hipEventElapsedTime(&d, ev[4 * s + 2], ev[4 * s + 3]));
+        kin += a;
+        pot += b;
+        upd += d;
+    }
+    for (auto &e : ev) (void)hipEventDestroy(e);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    ms[0] = total;
+    ms[1] = pot / ksteps;
+    ms[2] = kin / ksteps;
+    ms[3] = upd / ksteps;
+    return BSL_OK;
+}

@@ -1,0 +1,136 @@
+// bsl_internal.h -- shared declarations of libbsl_hip.so (host tables,
+// device buffers, kernel launchers).  Not part of the public ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "bsl_hip.h"
+
+namespace bsl {
+
+// recurrence scalars kept in device memory so that the Lanczos loop never
+// synchronises with the host: the actual Krylov vectors are sx*X and sr*R.
+struct Scal {
+    double sx, sr;
+    double alpha, beta, beta2;
+    double pad[3];
+};
+
+// error reporting (thread-local message + code)
+int fail(int code, const std::string &msg);
+int hip_fail(hipError_t e, const char *what, const char *file, int line);
+#define BSL_HIP(call)                                                              \
+    do {                                                                           \
+        hipError_t _e = (call);                                                    \
+        if (_e != hipSuccess) return ::bsl::hip_fail(_e, #call, __FILE__, __LINE__); \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// host tables (bsl_tables.cpp)
+// ---------------------------------------------------------------------------
+struct Tables {
+    int lmax = 0, nn = 0, na = 0, nb = 0;
+    std::vector<int> l, m;                 // per reference lm index n
+    std::vector<double> ct, wt, th;        // GL: cos(theta), weights, theta
+    std::vector<double> cp, wp, ph1, ph2;  // GC: cos(phi), weights, phi in (0,pi), 2pi - phi
+    std::vector<double> L;                 // [n][a] normalised associated Legendre
+    std::vector<double> S1, S2;            // [m+lmax][b] tesseral trig for both phi halves
+};
+void build_tables(int lmax, int na, int nb, Tables &t);
+// first row of the (symmetric Toeplitz) sinc-DVR kinetic matrix and the grid
+void kinetic_row(double x_max, int n, double mass, std::vector<double> &row, std::vector<double> &grid);
+
+// ---------------------------------------------------------------------------
+// Cullum-Willoughby post-processing (bsl_tridiag.cpp)
+// ---------------------------------------------------------------------------
+int cw_lancbis(int niter, const double *alpha, const double *beta, const double *beta2, double emin, double emax,
+               double *eval, double *err, int *ngood);
+int cw_trivec(int niter, const double *alpha, const double *beta, const double *beta2, const double *eval,
+              int ngood, double *evtr, int *mamin);
+
+// ---------------------------------------------------------------------------
+// device-side launchers (bsl_kernels.hip)
+// ---------------------------------------------------------------------------
+// a parity block of the potential: `size` consecutive carried planes from `plane0`
+struct Block {
+    int size = 0;
+    int plane0 = 0;
+    int64_t pair_offset = 0;      // first packed pair of this block
+    double mask = 1.0;            // symmetry mask of its planes (uniform within a block)
+};
+
+struct DevOperator {
+    int nx = 0, ny = 0, nz = 0;
+    int64_t P = 0;                // grid points
+    int nk = 0;                   // carried lm planes
+    int vprec = 0;                // BSL_PREC_* of the potential blocks
+    double *tx = nullptr, *ty = nullptr, *tz = nullptr;  // Toeplitz rows
+    double *rot = nullptr, *mask = nullptr;              // per carried plane
+    void *vpk = nullptr;          // packed V blocks [pair][P] (double or float)
+    int nblk = 0;
+    Block blk[2];
+    int *plane_ref = nullptr;     // reference lm index of each carried plane
+    int64_t npair = 0;
+    // reduction scratch
+    double *partial = nullptr;
+    unsigned int *counter = nullptr;
+    int npartial = 0;
+};
+
+// one Hv contribution pipeline: R = sr*R + mask*(H (sx*X)); on return the
+// device scalar alpha = (sx X).R is set when want_alpha.
+int launch_hv(const DevOperator &op, const double *X, double *R, Scal *sc, double *alpha_out, hipStream_t s,
+              hipEvent_t *ev /* 6 events or null */);
+// R -= alpha (sx X); beta2 = |R|^2; updates scalars for the swap
+int launch_update(const DevOperator &op, const double *X, double *R, Scal *sc, double *beta_out,
+                  double *beta2_out, hipStream_t s);
+int launch_init_scal(Scal *sc, double sx, double sr, hipStream_t s);
+int launch_start_vector(double *X, int64_t n, int64_t P, const int *plane_ref, int kind, uint64_t seed, Scal *sc,
+                        double *partial, unsigned int *counter, int npartial, hipStream_t s);
+int launch_zero(double *p, int64_t n, hipStream_t s);
+int launch_ritz_accum(double *ritz, const double *X, const Scal *sc, const double *coef, int64_t n, int ngood,
+                      hipStream_t s);
+
+// prep kernels
+struct CageDev {
+    int nsites = 0;
+    double *site = nullptr;  // [nsites][4]: x, y, z, q
+    int *lj = nullptr;
+};
+int launch_vgrid_cage(double *vg, int64_t P, int Q, const double *gx, const double *gy, const double *gz, int nx,
+                      int ny, const double *off /*[Q][3]*/, const CageDev &cage, double q_h, double q_cm,
+                      double lj_a, double lj_b, double ceiling, hipStream_t s);
+int launch_vgrid_iho(double *vg, int64_t P, int Q, const double *gx, const double *gy, const double *gz, int nx,
+                     int ny, double factor, double iho_ceil, double ceiling, hipStream_t s);
+// Vpk[pair][P] = sum_q Vg[P][q] * H[q][pair]
+int launch_vblock_gemm(const double *vg, const double *h, void *vpk, int vprec, int64_t P, int Q, int64_t npair,
+                       hipStream_t s);
+
+}  // namespace bsl
+
+struct bsl_ctx {
+    bsl_params p{};
+    std::vector<double> site_pos, site_q;
+    std::vector<int32_t> site_lj;
+    bsl::Tables tab;
+    int nn = 0;                        // full lm count
+    int64_t P = 0, n_full = 0, n_kry = 0;
+    std::vector<int> carried;          // carried plane -> reference lm index (grouped by parity block)
+    std::vector<double> rot, mask;     // per carried plane
+    bsl::DevOperator op;               // the operator in the context precision
+    bsl::DevOperator op64;             // FP64 operator (same as op when FP64)
+    void *vpk64 = nullptr;             // FP64 blocks kept for refinement when op is mixed
+    double *d_gx = nullptr, *d_gy = nullptr, *d_gz = nullptr;
+    double *d_vgrid = nullptr;         // [P][Q] potential grid (kept for the quadrature calculator)
+    int Q = 0;
+    double *d_X = nullptr, *d_R = nullptr, *d_T = nullptr;
+    bsl::Scal *d_scal = nullptr;
+    double *d_alpha = nullptr, *d_beta = nullptr, *d_beta2 = nullptr;
+    int niter_cap = 0;
+    hipStream_t stream = nullptr;
+    int device = 0;
+};

@@ -1,0 +1,694 @@
+// bsl_kernels.hip -- gfx950 kernels of the NoQuad Hv, the Lanczos recurrence
+// and the device-side preparation (potential grid, V-block contraction).
+//
+// Vector layout in HBM: [n][k][j][i] (i fastest), n over the carried lm
+// planes.  The per-point potential blocks are symmetric and, for a
+// homonuclear rotor on the parity-symmetric quadrature, block diagonal in
+// the parity of l: they are stored as packed upper triangles of each parity
+// block, structure-of-arrays [pair][P], so that one thread per grid point
+// streams them with fully coalesced loads.
+//
+// All reductions are deterministic: fixed per-workgroup partial sums and a
+// fixed-order final sum done by the last workgroup to finish.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "bsl_internal.h"
+
+namespace bsl {
+
+constexpr int kWG = 256;
+
+// ---------------------------------------------------------------------------
+// reductions
+// ---------------------------------------------------------------------------
+__device__ inline double wave_sum(double v)
+{
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    return v;
+}
+
+// sum over the workgroup; result valid in thread 0
+__device__ inline double block_sum(double v, double *lds)
+{
+    v = wave_sum(v);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) lds[wid] = v;
+    __syncthreads();
+    double s = 0.0;
+    if (threadIdx.x == 0)
+        for (int w = 0; w < (int)(blockDim.x >> 6); w++) s += lds[w];
+    return s;
+}
+
+// Thread 0 publishes `value` as this workgroup's partial; returns true in
+// every thread of the last workgroup to arrive, which then owns the total.
+__device__ inline bool publish_partial(double value, double *partial, unsigned int *counter, unsigned int nblocks,
+                                       unsigned int bid)
+{
+    __shared__ bool last;
+    if (threadIdx.x == 0) {
+        __hip_atomic_store(&partial[bid], value, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const unsigned int prev = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        last = (prev == nblocks - 1);
+    }
+    __syncthreads();
+    return last;
+}
+
+// fixed-order total of partial[0..n) by one workgroup; valid in thread 0
+__device__ inline double total_of_partials(const double *partial, unsigned int n, double *lds)
+{
+    double s = 0.0;
+    for (unsigned int i = threadIdx.x; i < n; i += blockDim.x)
+        s += __hip_atomic_load(&partial[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+    return block_sum(s, lds);
+}
+
+// ---------------------------------------------------------------------------
+// kinetic term, x and y axes: one workgroup per (k, n) plane
+// ---------------------------------------------------------------------------
+// The plane (sx * X) is staged in LDS, zero padded to multiples of 4; each
+// thread accumulates a 4x4 tile of outputs.  T is symmetric Toeplitz, so
+// only its first row is needed; it is stored reflected, trow[pad + d] =
+// t(|d|), and walked as a sliding window.
+// R = sr * R + mask[n] * ((Tx + Ty) v + rot[n] v)
+__global__ __launch_bounds__(kWG) void k_kin_xy(const double *__restrict__ X, double *__restrict__ R,
+                                                const double *__restrict__ tx, const double *__restrict__ ty,
+                                                const double *__restrict__ rot, const double *__restrict__ mask,
+                                                const Scal *__restrict__ sc, int nx, int ny, int nz)
+{
+    extern __shared__ double lds[];
+    const int nxp = (nx + 3) & ~3, nyp = (ny + 3) & ~3;
+    const int ld = nxp + 2;
+    double *pl = lds;                    // [nyp][ld]
+    double *rx = pl + (size_t)nyp * ld;  // [2*nxp+1], rx[nxp + d] = t_x(|d|)
+    double *ry = rx + 2 * nxp + 1;       // [2*nyp+1]
+    const int k = blockIdx.x, n = blockIdx.y;
+    const double sx = sc->sx, sr = sc->sr;
+    const size_t base = ((size_t)n * nz + k) * (size_t)ny * nx;
+    for (int t = threadIdx.x; t < nyp * ld; t += blockDim.x) {
+        const int j = t / ld, i = t - j * ld;
+        pl[t] = (j < ny && i < nx) ? sx * X[base + (size_t)j * nx + i] : 0.0;
+    }
+    for (int t = threadIdx.x; t < 2 * nxp + 1; t += blockDim.x) {
+        const int d = abs(t - nxp);
+        rx[t] = d < nx ? tx[d] : 0.0;
+    }
+    for (int t = threadIdx.x; t < 2 * nyp + 1; t += blockDim.x) {
+        const int d = abs(t - nyp);
+        ry[t] = d < ny ? ty[d] : 0.0;
+    }
+    __syncthreads();
+    const int tiles_i = nxp >> 2, ntiles = tiles_i * (nyp >> 2);
+    const double rn = rot[n], mn = mask[n];
+    for (int tile = threadIdx.x; tile < ntiles; tile += blockDim.x) {
+        const int j0 = (tile / tiles_i) << 2, i0 = (tile % tiles_i) << 2;
+        double acc[4][4];
+#pragma unroll
+        for (int r = 0; r < 4; r++)
+#pragma unroll
+            for (int c = 0; c < 4; c++) acc[r][c] = 0.0;
+        // x: acc[r][c] += t(|i0+c-ip|) * v[j0+r][ip]
+        {
+            double w0 = rx[nxp + i0 + 0], w1 = rx[nxp + i0 + 1], w2 = rx[nxp + i0 + 2], w3 = rx[nxp + i0 + 3];
+            for (int ip = 0; ip < nx; ip++) {
+                const double p0 = pl[(j0 + 0) * ld + ip], p1 = pl[(j0 + 1) * ld + ip];
+                const double p2 = pl[(j0 + 2) * ld + ip], p3 = pl[(j0 + 3) * ld + ip];
+                acc[0][0] += w0 * p0; acc[0][1] += w1 * p0; acc[0][2] += w2 * p0; acc[0][3] += w3 * p0;
+                acc[1][0] += w0 * p1; acc[1][1] += w1 * p1; acc[1][2] += w2 * p1; acc[1][3] += w3 * p1;
+                acc[2][0] += w0 * p2; acc[2][1] += w1 * p2; acc[2][2] += w2 * p2; acc[2][3] += w3 * p2;
+                acc[3][0] += w0 * p3; acc[3][1] += w1 * p3; acc[3][2] += w2 * p3; acc[3][3] += w3 * p3;
+                w3 = w2;
+                w2 = w1;
+                w1 = w0;
+                w0 = rx[nxp + i0 - ip - 1];
+            }
+        }
+        // y: acc[r][c] += t(|j0+r-jp|) * v[jp][i0+c]
+        {
+            double w0 = ry[nyp + j0 + 0], w1 = ry[nyp + j0 + 1], w2 = ry[nyp + j0 + 2], w3 = ry[nyp + j0 + 3];
+            for (int jp = 0; jp < ny; jp++) {
+                const double *row = pl + jp * ld + i0;
+                const double p0 = row[0], p1 = row[1], p2 = row[2], p3 = row[3];
+                acc[0][0] += w0 * p0; acc[0][1] += w0 * p1; acc[0][2] += w0 * p2; acc[0][3] += w0 * p3;
+                acc[1][0] += w1 * p0; acc[1][1] += w1 * p1; acc[1][2] += w1 * p2; acc[1][3] += w1 * p3;
+                acc[2][0] += w2 * p0; acc[2][1] += w2 * p1; acc[2][2] += w2 * p2; acc[2][3] += w2 * p3;
+                acc[3][0] += w3 * p0; acc[3][1] += w3 * p1; acc[3][2] += w3 * p2; acc[3][3] += w3 * p3;
+                w3 = w2;
+                w2 = w1;
+                w1 = w0;
+                w0 = ry[nyp + j0 - jp - 1];
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int j = j0 + r;
+            if (j >= ny) continue;
+#pragma unroll
+            for (int c = 0; c < 4; c++) {
+                const int i = i0 + c;
+                if (i >= nx) continue;
+                const size_t o = base + (size_t)j * nx + i;
+                const double h = acc[r][c] + rn * pl[j * ld + i];
+                R[o] = sr * R[o] + mn * h;
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// kinetic term, z axis: one workgroup per (64-column block, n)
+// ---------------------------------------------------------------------------
+// The 64 columns (j,i) of the block are staged in LDS over all k; thread
+// (lane, quarter) computes KT consecutive k outputs of its column.
+// R += mask[n] * Tz v
+template <int KT>
+__global__ __launch_bounds__(kWG) void k_kin_z(const double *__restrict__ X, double *__restrict__ R,
+                                               const double *__restrict__ tz, const double *__restrict__ mask,
+                                               const Scal *__restrict__ sc, int nxy, int nz)
+{
+    extern __shared__ double lds[];
+    const int nzp = 4 * KT;
+    double *col = lds;              // [nz][64]
+    double *rz = col + (size_t)nz * 64;  // [2*nzp+1]
+    const int lane = threadIdx.x & 63, quarter = threadIdx.x >> 6;
+    const int c0 = blockIdx.x * 64, n = blockIdx.y;
+    const double sx = sc->sx;
+    const size_t base = (size_t)n * nz * nxy;
+    const int c = c0 + lane;
+    for (int kk = quarter; kk < nz; kk += 4)
+        col[kk * 64 + lane] = c < nxy ? sx * X[base + (size_t)kk * nxy + c] : 0.0;
+    for (int t = threadIdx.x; t < 2 * nzp + 1; t += blockDim.x) {
+        const int d = abs(t - nzp);
+        rz[t] = d < nz ? tz[d] : 0.0;
+    }
+    __syncthreads();
+    const int k0 = quarter * KT;
+    double acc[KT], w[KT];
+#pragma unroll
+    for (int r = 0; r < KT; r++) {
+        acc[r] = 0.0;
+        w[r] = rz[nzp + k0 + r];
+    }
+    for (int kp = 0; kp < nz; kp++) {
+        const double p = col[kp * 64 + lane];
+#pragma unroll
+        for (int r = 0; r < KT; r++) acc[r] += w[r] * p;
+#pragma unroll
+        for (int r = KT - 1; r > 0; r--) w[r] = w[r - 1];
+        w[0] = rz[nzp + k0 - kp - 1];
+    }
+    if (c >= nxy) return;
+    const double mn = mask[n];
+#pragma unroll
+    for (int r = 0; r < KT; r++) {
+        const int k = k0 + r;
+        if (k < nz) {
+            const size_t o = base + (size_t)k * nxy + c;
+            R[o] += mn * acc[r];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// potential term: one thread per (grid point, parity block)
+// ---------------------------------------------------------------------------
+// R[n_r][P] += mask * sum_c V_P[r][c] (sx X[n_c][P]) over the packed upper
+// triangle; then the contribution of these planes to alpha = (sx X).R.
+template <int B, typename VT>
+__device__ inline void vblock_point(const double *__restrict__ X, double *__restrict__ R, const VT *__restrict__ V,
+                                    int plane0, double mask, double sx, int64_t P, int64_t p, double &dot)
+{
+    double x[B], acc[B];
+    const double *Xb = X + (size_t)plane0 * P + p;
+    double *Rb = R + (size_t)plane0 * P + p;
+#pragma unroll
+    for (int r = 0; r < B; r++) {
+        x[r] = sx * Xb[(size_t)r * P];
+        acc[r] = 0.0;
+    }
+    int q = 0;
+#pragma unroll
+    for (int r = 0; r < B; r++) {
+#pragma unroll
+        for (int c = r; c < B; c++) {
+            const double v = (double)V[(size_t)q * P + p];
+            acc[r] += v * x[c];
+            if (c != r) acc[c] += v * x[r];
+            q++;
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < B; r++) {
+        const double nr = Rb[(size_t)r * P] + mask * acc[r];
+        Rb[(size_t)r * P] = nr;
+        dot += x[r] * nr;
+    }
+}
+
+// The carried planes are ordered block by block: block 0 occupies planes
+// [0, B0), block 1 planes [B0, B0 + B1).  The symmetry mask is uniform
+// within a parity block.
+template <int B0, int B1, typename VT>
+__global__ __launch_bounds__(kWG) void k_vblock(const double *__restrict__ X, double *__restrict__ R,
+                                                const VT *__restrict__ V0, const VT *__restrict__ V1, double mask0,
+                                                double mask1, Scal *__restrict__ sc, int64_t P, double *partial,
+                                                unsigned int *counter, double *alpha_out)
+{
+    __shared__ double red[kWG / 64];
+    const double sx = sc->sx;
+    double dot = 0.0;
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < P) {
+        if (blockIdx.y == 0)
+            vblock_point<B0, VT>(X, R, V0, 0, mask0, sx, P, p, dot);
+        else if (B1 > 0)
+            vblock_point<(B1 > 0 ? B1 : 1), VT>(X, R, V1, B0, mask1, sx, P, p, dot);
+    }
+    const double s = block_sum(dot, red);
+    const unsigned int nblk = gridDim.x * gridDim.y, bid = blockIdx.y * gridDim.x + blockIdx.x;
+    if (publish_partial(s, partial, counter, nblk, bid)) {
+        const double a = total_of_partials(partial, nblk, red);
+        if (threadIdx.x == 0) {
+            sc->alpha = a;
+            if (alpha_out) *alpha_out = a;
+            *counter = 0u;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// recurrence update: R -= alpha (sx X), beta^2 = |R|^2, scalars for the swap
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kWG) void k_update(const double *__restrict__ X, double *__restrict__ R,
+                                                Scal *__restrict__ sc, int64_t n, double *partial,
+                                                unsigned int *counter, double *beta_out, double *beta2_out)
+{
+    __shared__ double red[kWG / 64];
+    const double a = sc->alpha, sx = sc->sx;
+    double s = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const double r = R[i] - a * (sx * X[i]);
+        R[i] = r;
+        s += r * r;
+    }
+    const double b = block_sum(s, red);
+    if (publish_partial(b, partial, counter, gridDim.x, blockIdx.x)) {
+        const double b2 = total_of_partials(partial, gridDim.x, red);
+        if (threadIdx.x == 0) {
+            const double beta = sqrt(b2);
+            sc->beta2 = b2;
+            sc->beta = beta;
+            if (beta_out) *beta_out = beta;
+            if (beta2_out) *beta2_out = b2;
+            // after the swap: X' = R / beta, R' = -beta * (sx X)
+            sc->sx = 1. / beta;
+            sc->sr = (-beta) * sx;
+            *counter = 0u;
+        }
+    }
+}
+
+__global__ void k_set_scal(Scal *sc, double sx, double sr)
+{
+    sc->sx = sx;
+    sc->sr = sr;
+    sc->alpha = 0.0;
+    sc->beta = 0.0;
+    sc->beta2 = 0.0;
+}
+
+__global__ void k_fill(double *p, int64_t n, double v)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+// counter-based U[-1,1) start vector (SplitMix64 finaliser of seed and index)
+__device__ inline double start_value(uint64_t seed, uint64_t idx)
+{
+    uint64_t z = seed * 0x9E3779B97F4A7C15ull + (idx + 1) * 0xBF58476D1CE4E5B9ull + 0x94D049BB133111EBull;
+    z ^= z >> 30;
+    z *= 0xBF58476D1CE4E5B9ull;
+    z ^= z >> 27;
+    z *= 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    const double u = (double)(z >> 11) * (1.0 / 9007199254740992.0);
+    return 2.0 * u - 1.0;
+}
+
+// value of device element i is that of reference element plane_ref[i / P] * P + i % P
+__global__ __launch_bounds__(kWG) void k_start_random(double *X, int64_t n, int64_t P,
+                                                      const int *__restrict__ plane_ref, uint64_t seed, Scal *sc,
+                                                      double *partial, unsigned int *counter)
+{
+    __shared__ double red[kWG / 64];
+    double s = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t plane = i / P;
+        const double v = start_value(seed, (uint64_t)(plane_ref[plane] * P + (i - plane * P)));
+        X[i] = v;
+        s += v * v;
+    }
+    const double b = block_sum(s, red);
+    if (publish_partial(b, partial, counter, gridDim.x, blockIdx.x)) {
+        const double tot = total_of_partials(partial, gridDim.x, red);
+        if (threadIdx.x == 0) {
+            sc->sx = 1.0 / sqrt(tot);
+            sc->sr = 0.0;
+            *counter = 0u;
+        }
+    }
+}
+
+// ritz[g][i] += coef[g] * (sx X[i]) for the vectors still accumulating
+__global__ void k_ritz_accum(double *ritz, const double *__restrict__ X, const Scal *__restrict__ sc,
+                             const double *__restrict__ coef, int64_t n, int ngood)
+{
+    const double sx = sc->sx;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const double v = sx * X[i];
+        for (int g = 0; g < ngood; g++) {
+            const double c = coef[g];
+            if (c != 0.0) ritz[(size_t)g * n + i] += c * v;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+static int grid_for(int64_t n, int cap = 4096)
+{
+    int64_t g = (n + kWG - 1) / kWG;
+    if (g > cap) g = cap;
+    if (g < 1) g = 1;
+    return (int)g;
+}
+
+template <typename VT>
+static int launch_vblock_t(const DevOperator &op, const double *X, double *R, Scal *sc, double *alpha_out,
+                           hipStream_t s)
+{
+    const VT *V = (const VT *)op.vpk;
+    const VT *V0 = V + op.blk[0].pair_offset * op.P;
+    const VT *V1 = op.nblk > 1 ? V + op.blk[1].pair_offset * op.P : V0;
+    const double mk0 = op.blk[0].mask, mk1 = op.nblk > 1 ? op.blk[1].mask : 0.0;
+    dim3 grid((unsigned)((op.P + kWG - 1) / kWG), (unsigned)op.nblk);
+    if ((int64_t)grid.x * grid.y > op.npartial) return fail(BSL_E_STATE, "partial buffer too small");
+    const int b0 = op.blk[0].size, b1 = op.nblk > 1 ? op.blk[1].size : 0;
+#define BSL_VB(A, Bb)                                                                                  \
+    if (b0 == A && b1 == Bb) {                                                                         \
+        hipLaunchKernelGGL((k_vblock<A, Bb, VT>), grid, dim3(kWG), 0, s, X, R, V0, V1, mk0, mk1, sc, \
+                           op.P, op.partial, op.counter, alpha_out);                                   \
+        return hipGetLastError() == hipSuccess ? BSL_OK : fail(BSL_E_HIP, "k_vblock launch");           \
+    }
+    // parity blocks (even l, odd l) for lmax 0..6, and single blocks
+    BSL_VB(1, 3) BSL_VB(6, 10) BSL_VB(6, 3) BSL_VB(15, 21) BSL_VB(15, 10) BSL_VB(28, 21)
+    BSL_VB(1, 0) BSL_VB(3, 0) BSL_VB(6, 0) BSL_VB(10, 0) BSL_VB(15, 0) BSL_VB(21, 0) BSL_VB(28, 0)
+#undef BSL_VB
+    return fail(BSL_E_ARG, "unsupported potential block sizes " + std::to_string(b0) + "/" + std::to_string(b1));
+}
+
+int launch_hv(const DevOperator &op, const double *X, double *R, Scal *sc, double *alpha_out, hipStream_t s,
+              hipEvent_t *ev)
+{
+    const int nxp = (op.nx + 3) & ~3, nyp = (op.ny + 3) & ~3;
+    const size_t lds_xy = sizeof(double) * ((size_t)nyp * (nxp + 2) + 2 * nxp + 1 + 2 * nyp + 1);
+    if (ev) BSL_HIP(hipEventRecord(ev[0], s));
+    hipLaunchKernelGGL(k_kin_xy, dim3(op.nz, op.nk), dim3(kWG), lds_xy, s, X, R, op.tx, op.ty, op.rot, op.mask,
+                       sc, op.nx, op.ny, op.nz);
+    BSL_HIP(hipGetLastError());
+    const int nxy = op.nx * op.ny;
+    const int kt = (op.nz + 3) / 4;
+    dim3 gz((nxy + 63) / 64, op.nk);
+    if (kt <= 4) {
+        const size_t lds = sizeof(double) * ((size_t)op.nz * 64 + 2 * 16 + 1);
+        hipLaunchKernelGGL(k_kin_z<4>, gz, dim3(kWG), lds, s, X, R, op.tz, op.mask, sc, nxy, op.nz);
+    } else if (kt <= 8) {
+        const size_t lds = sizeof(double) * ((size_t)op.nz * 64 + 2 * 32 + 1);
+        hipLaunchKernelGGL(k_kin_z<8>, gz, dim3(kWG), lds, s, X, R, op.tz, op.mask, sc, nxy, op.nz);
+    } else if (kt <= 16) {
+        const size_t lds = sizeof(double) * ((size_t)op.nz * 64 + 2 * 64 + 1);
+        hipLaunchKernelGGL(k_kin_z<16>, gz, dim3(kWG), lds, s, X, R, op.tz, op.mask, sc, nxy, op.nz);
+    } else {
+        return fail(BSL_E_ARG, "nz above 64 is not supported");
+    }
+    BSL_HIP(hipGetLastError());
+    if (ev) BSL_HIP(hipEventRecord(ev[1], s));
+    int rc = op.vprec == BSL_PREC_FP64 ? launch_vblock_t<double>(op, X, R, sc, alpha_out, s)
+                                       : launch_vblock_t<float>(op, X, R, sc, alpha_out, s);
+    if (rc) return rc;
+    if (ev) BSL_HIP(hipEventRecord(ev[2], s));
+    return BSL_OK;
+}
+
+int launch_update(const DevOperator &op, const double *X, double *R, Scal *sc, double *beta_out,
+                  double *beta2_out, hipStream_t s)
+{
+    const int64_t n = op.P * op.nk;
+    int g = grid_for(n, 2048);
+    if (g > op.npartial) g = op.npartial;
+    hipLaunchKernelGGL(k_update, dim3(g), dim3(kWG), 0, s, X, R, sc, n, op.partial, op.counter, beta_out,
+                       beta2_out);
+    BSL_HIP(hipGetLastError());
+    return BSL_OK;
+}
+
+int launch_init_scal(Scal *sc, double sx, double sr, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_set_scal, dim3(1), dim3(1), 0, s, sc, sx, sr);
+    BSL_HIP(hipGetLastError());
+    return BSL_OK;
+}
+
+int launch_zero(double *p, int64_t n, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_fill, dim3(grid_for(n)), dim3(kWG), 0, s, p, n, 0.0);
+    BSL_HIP(hipGetLastError());
+    return BSL_OK;
+}
+
+int launch_start_vector(double *X, int64_t n, int64_t P, const int *plane_ref, int kind, uint64_t seed, Scal *sc,
+                        double *partial, unsigned int *counter, int npartial, hipStream_t s)
+{
+    if (kind == BSL_START_UNIFORM) {
+        // X = 1, sx = 1/sqrt(N): sx*X is the reference's 1./sqrt((double)N) exactly
+        hipLaunchKernelGGL(k_fill, dim3(grid_for(n)), dim3(kWG), 0, s, X, n, 1.0);
+        BSL_HIP(hipGetLastError());
+        return launch_init_scal(sc, 1. / std::sqrt((double)n), 0.0, s);
+    }
+    int g = grid_for(n, 2048);
+    if (g > npartial) g = npartial;
+    hipLaunchKernelGGL(k_start_random, dim3(g), dim3(kWG), 0, s, X, n, P, plane_ref, seed, sc, partial, counter);
+    BSL_HIP(hipGetLastError());
+    return BSL_OK;
+}
+
+int launch_ritz_accum(double *ritz, const double *X, const Scal *sc, const double *coef, int64_t n, int ngood,
+                      hipStream_t s)
+{
+    hipLaunchKernelGGL(k_ritz_accum, dim3(grid_for(n)), dim3(kWG), 0, s, ritz, X, sc, coef, n, ngood);
+    BSL_HIP(hipGetLastError());
+    return BSL_OK;
+}
+
+// ---------------------------------------------------------------------------
+// preparation: potential grid
+// ---------------------------------------------------------------------------
+// Coulomb sum over the sites in site order, K q q_s / |s - pos|, and the LJ
+// term A/r^12 - B/r^6 on the flagged sites; FP contraction off so that the
+// grid matches the FP64 CPU evaluation of the same expression.
+__device__ inline double coulomb_sum(const double *site, int ns, double px, double py, double pz, double q)
+{
+#pragma clang fp contract(off)
+    double e = 0;
+    for (int s = 0; s < ns; s++) {
+        const double qs = site[4 * s + 3];
+        if (qs == 0.0) continue;
+        const double d0 = site[4 * s + 0] - px, d1 = site[4 * s + 1] - py, d2 = site[4 * s + 2] - pz;
+        double r2 = 0.;
+        r2 += d0 * d0;
+        r2 += d1 * d1;
+        r2 += d2 * d2;
+        e += 138.9354846111096 * q * qs / sqrt(r2);
+    }
+    return e;
+}
+
+__global__ __launch_bounds__(kWG) void k_cm_potential(double *cm, int64_t P, const double *__restrict__ gx,
+                                                      const double *__restrict__ gy, const double *__restrict__ gz,
+                                                      int nx, int ny, const double *__restrict__ site_g,
+                                                      const int *__restrict__ lj_g, int ns, double q_cm,
+                                                      double lj_a, double lj_b)
+{
+#pragma clang fp contract(off)
+    extern __shared__ double site[];
+    int *lj = (int *)(site + 4 * ns);
+    for (int t = threadIdx.x; t < 4 * ns; t += blockDim.x) site[t] = site_g[t];
+    for (int t = threadIdx.x; t < ns; t += blockDim.x) lj[t] = lj_g[t];
+    __syncthreads();
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= P) return;
+    const int i = (int)(p % nx), j = (int)((p / nx) % ny), k = (int)(p / ((int64_t)nx * ny));
+    const double x = gx[i], y = gy[j], z = gz[k];
+    double e = coulomb_sum(site, ns, x, y, z, q_cm);
+    double l = 0;
+    for (int s = 0; s < ns; s++) {
+        if (!lj[s]) continue;
+        const double d0 = site[4 * s + 0] - x, d1 = site[4 * s + 1] - y, d2 = site[4 * s + 2] - z;
+        double r2 = 0.;
+        r2 += d0 * d0;
+        r2 += d1 * d1;
+        r2 += d2 * d2;
+        const double r = sqrt(r2);
+        l += (lj_a / pow(r, 12.0)) - (lj_b / pow(r, 6.0));
+    }
+    cm[p] = e + l;
+}
+
+// vg[p][q], q = (half, a, b): H2 centre at grid point p, H atoms at +-off[q]
+__global__ __launch_bounds__(kWG) void k_vgrid_cage(double *vg, int64_t P, int Q, const double *__restrict__ gx,
+                                                    const double *__restrict__ gy, const double *__restrict__ gz,
+                                                    int nx, int ny, const double *__restrict__ off,
+                                                    const double *__restrict__ site_g, int ns,
+                                                    const double *__restrict__ cm, double q_h, double ceiling)
+{
+#pragma clang fp contract(off)
+    extern __shared__ double site[];
+    for (int t = threadIdx.x; t < 4 * ns; t += blockDim.x) site[t] = site_g[t];
+    __syncthreads();
+    const int64_t total = P * (int64_t)Q;
+    for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+         idx += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t p = idx / Q;
+        const int q = (int)(idx - p * Q);
+        const int i = (int)(p % nx), j = (int)((p / nx) % ny), k = (int)(p / ((int64_t)nx * ny));
+        const double x = gx[i], y = gy[j], z = gz[k];
+        const double ox = off[3 * q], oy = off[3 * q + 1], oz = off[3 * q + 2];
+        double v = coulomb_sum(site, ns, x + ox, y + oy, z + oz, q_h);
+        v += coulomb_sum(site, ns, x - ox, y - oy, z - oz, q_h);
+        v += cm[p];
+        if (v >= ceiling) v = ceiling;
+        vg[idx] = v;
+    }
+}
+
+__global__ __launch_bounds__(kWG) void k_vgrid_iho(double *vg, int64_t P, int Q, const double *__restrict__ gx,
+                                                   const double *__restrict__ gy, const double *__restrict__ gz,
+                                                   int nx, int ny, double factor, double iho_ceil, double ceiling)
+{
+#pragma clang fp contract(off)
+    const int64_t total = P * (int64_t)Q;
+    for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+         idx += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t p = idx / Q;
+        const int i = (int)(p % nx), j = (int)((p / nx) % ny), k = (int)(p / ((int64_t)nx * ny));
+        const double x = gx[i], y = gy[j], z = gz[k];
+        double d2 = 0.;
+        d2 += x * x;
+        d2 += y * y;
+        d2 += z * z;
+        double v = factor * d2;
+        if (fabs(v) > fabs(iho_ceil)) v = iho_ceil;
+        if (v >= ceiling) v = ceiling;
+        vg[idx] = v;
+    }
+}
+
+int launch_vgrid_cage(double *vg, int64_t P, int Q, const double *gx, const double *gy, const double *gz, int nx,
+                      int ny, const double *off, const CageDev &cage, double q_h, double q_cm, double lj_a,
+                      double lj_b, double ceiling, hipStream_t s)
+{
+    double *cm = nullptr;
+    BSL_HIP(hipMallocAsync((void **)&cm, sizeof(double) * (size_t)P, s));
+    const size_t lds_cm = sizeof(double) * 4 * cage.nsites + sizeof(int) * cage.nsites;
+    hipLaunchKernelGGL(k_cm_potential, dim3((unsigned)((P + kWG - 1) / kWG)), dim3(kWG), lds_cm, s, cm, P, gx, gy,
+                       gz, nx, ny, cage.site, cage.lj, cage.nsites, q_cm, lj_a, lj_b);
+    BSL_HIP(hipGetLastError());
+    const size_t lds = sizeof(double) * 4 * cage.nsites;
+    hipLaunchKernelGGL(k_vgrid_cage, dim3(grid_for(P * Q, 65536)), dim3(kWG), lds, s, vg, P, Q, gx, gy, gz, nx,
+                       ny, off, cage.site, cage.nsites, cm, q_h, ceiling);
+    BSL_HIP(hipGetLastError());
+    BSL_HIP(hipFreeAsync(cm, s));
+    return BSL_OK;
+}
+
+int launch_vgrid_iho(double *vg, int64_t P, int Q, const double *gx, const double *gy, const double *gz, int nx,
+                     int ny, double factor, double iho_ceil, double ceiling, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_vgrid_iho, dim3(grid_for(P * Q, 65536)), dim3(kWG), 0, s, vg, P, Q, gx, gy, gz, nx, ny,
+                       factor, iho_ceil, ceiling);
+    BSL_HIP(hipGetLastError());
+    return BSL_OK;
+}
+
+// ---------------------------------------------------------------------------
+// preparation: V blocks, Vpk[pair][P] = sum_q Vg[P][q] H[q][pair]  (FP64 GEMM)
+// ---------------------------------------------------------------------------
+// 64 points x 64 pairs per workgroup, K chunks of 16 quadrature nodes staged
+// in LDS, a 4x4 register tile per thread.
+template <typename VT>
+__global__ __launch_bounds__(kWG) void k_vblock_gemm(const double *__restrict__ vg, const double *__restrict__ h,
+                                                     VT *__restrict__ vpk, int64_t P, int Q, int64_t npair)
+{
+    __shared__ double As[16][64 + 1];  // [kk][point]
+    __shared__ double Bs[16][64 + 1];  // [kk][pair]
+    const int tm = threadIdx.x & 15, tn = threadIdx.x >> 4;
+    const int64_t p0 = (int64_t)blockIdx.x * 64;
+    const int64_t c0 = (int64_t)blockIdx.y * 64;
+    double acc[4][4] = {};
+    for (int q0 = 0; q0 < Q; q0 += 16) {
+        for (int t = threadIdx.x; t < 64 * 16; t += kWG) {
+            const int pp = t >> 4, kk = t & 15;  // A: 16 consecutive q per point
+            const int64_t p = p0 + pp;
+            const int q = q0 + kk;
+            As[kk][pp] = (p < P && q < Q) ? vg[(size_t)p * Q + q] : 0.0;
+            const int kb = t >> 6, cc = t & 63;  // B: 64 consecutive pairs per q
+            const int64_t c = c0 + cc;
+            const int qb = q0 + kb;
+            Bs[kb][cc] = (c < npair && qb < Q) ? h[(size_t)qb * npair + c] : 0.0;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < 16; kk++) {
+            double a[4], b[4];
+#pragma unroll
+            for (int r = 0; r < 4; r++) a[r] = As[kk][tm * 4 + r];
+#pragma unroll
+            for (int c = 0; c < 4; c++) b[c] = Bs[kk][tn * 4 + c];
+#pragma unroll
+            for (int r = 0; r < 4; r++)
+#pragma unroll
+                for (int c = 0; c < 4; c++) acc[r][c] += a[r] * b[c];
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+        const int64_t pair = c0 + tn * 4 + c;
+        if (pair >= npair) continue;
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int64_t p = p0 + tm * 4 + r;
+            if (p < P) vpk[(size_t)pair * P + p] = (VT)acc[r][c];
+        }
+    }
+}
+
+int launch_vblock_gemm(const double *vg, const double *h, void *vpk, int vprec, int64_t P, int Q, int64_t npair,
+                       hipStream_t s)
+{
+    dim3 grid((unsigned)((P + 63) / 64), (unsigned)((npair + 63) / 64));
+    if (vprec == BSL_PREC_FP64)
+        hipLaunchKernelGGL(k_vblock_gemm<double>, grid, dim3(kWG), 0, s, vg, h, (double *)vpk, P, Q, npair);
+    else
+        hipLaunchKernelGGL(k_vblock_gemm<float>, grid, dim3(kWG), 0, s, vg, h, (float *)vpk, P, Q, npair);
+    BSL_HIP(hipGetLastError());
+    return BSL_OK;
+}
+
+}  // namespace bsl

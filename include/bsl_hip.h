@@ -1,0 +1,165 @@
+/*
+ * bsl_hip.h -- C ABI of libbsl_hip.so, the MI355X (gfx950) implementation of
+ * the Hv matvec and Lanczos recurrence of jtcantin/boundstate-lanczos.
+ *
+ * Plain C types only.  Every entry point returns 0 on success and a non-zero
+ * BSL_E_* code on failure; bsl_last_error() then describes the failure (per
+ * thread).  A context is not thread-safe: calls on one context must be
+ * serialised, as the reference driver serialises its Hv calls.  Several
+ * contexts may coexist.
+ *
+ * Vector layout at this boundary is the reference's composite index
+ * [n][k][j][i] (i fastest; n = lm index in the reference order, m outer from
+ * -lmax..lmax, l inner from |m|..lmax), length nx*ny*nz*(lmax+1)^2, exactly
+ * what the reference's Hv function pointer receives.
+ */
+#ifndef BSL_HIP_H
+#define BSL_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    BSL_OK = 0,
+    BSL_E_ARG = 1,      /* invalid argument (bad size, flag or pointer) */
+    BSL_E_HIP = 2,      /* HIP runtime error */
+    BSL_E_NOMEM = 3,    /* device or host allocation failed */
+    BSL_E_STURM = 4,    /* BISEC: estimated number of Sturm sequences exceeds its limit */
+    BSL_E_NOTFOUND = 5, /* STURMI: eigenvalue not found */
+    BSL_E_NOCONV = 6,   /* TRIVEC: inverse iteration never converged */
+    BSL_E_STATE = 7     /* call not valid for this context */
+};
+
+enum { BSL_SYM_ALL = 0, BSL_SYM_EVEN = 1, BSL_SYM_ODD = 2 };
+/* FULL: all (lmax+1)^2 lm, output masked by the symmetry (reference
+ * semantics, calcSym linRotCartSphHvFullVPrepRoutines.cpp:530-573).
+ * REDUCED: only the lm of the symmetry class are carried (para / ortho
+ * blocks); the spurious zero level of the masked space does not appear. */
+enum { BSL_BASIS_FULL = 0, BSL_BASIS_REDUCED = 1 };
+/* NOQUAD: precomputed per-point <lm|V|l'm'> blocks streamed by a batched
+ * GEMV (linRotCartSph_NoQuad).  QUAD: Y_lm <-> (theta, phi) quadrature
+ * transforms around the resident potential grid, evaluated on MFMA
+ * (the on-the-fly linRotCartSph calculator, with its summation corrected). */
+enum { BSL_CALC_NOQUAD = 0, BSL_CALC_QUAD = 1 };
+/* Hv arithmetic; the recurrence, its dot products and norms are always FP64 */
+enum { BSL_PREC_FP64 = 0, BSL_PREC_FP32 = 1, BSL_PREC_BF16X3 = 2 };
+enum { BSL_POT_IHO = 0, BSL_POT_CAGE = 1 };
+enum { BSL_START_UNIFORM = 0, BSL_START_RANDOM = 1 };
+
+typedef struct bsl_params {
+    int32_t nx, ny, nz;          /* grid points per axis */
+    double x_max, y_max, z_max;  /* full box lengths (nm); points span [-x_max/2, x_max/2] */
+    int32_t lmax;
+    int32_t n_theta;             /* Gauss-Legendre points in cos(theta) */
+    int32_t n_phi;               /* Gauss-Chebyshev points; 2*n_phi phi nodes on [0, 2 pi) */
+    double mass;                 /* amu */
+    double rot_const;            /* B, kJ/mol */
+    double ceiling;              /* V >= ceiling is set to ceiling (kJ/mol) */
+    int32_t symmetry;            /* BSL_SYM_* */
+    int32_t basis;               /* BSL_BASIS_* */
+    int32_t calculator;          /* BSL_CALC_* */
+    int32_t precision;           /* BSL_PREC_* */
+    int32_t potential;           /* BSL_POT_* */
+    /* isotropic harmonic oscillator: V = iho_factor |r|^2, |V| > |iho_ceil| -> iho_ceil */
+    double iho_factor, iho_ceil;
+    /* cage: point charges (all sites) and the H2-centre LJ term (flagged sites) */
+    int32_t n_sites;
+    const double *site_pos;      /* [n_sites][3], nm */
+    const double *site_q;        /* [n_sites], e */
+    const int32_t *site_lj;      /* [n_sites], 1 = LJ centre (water O) */
+    double q_h2_h, q_h2_cm;      /* Alavi H2 charges on each H and at the centre, e */
+    double h2_half_bond;         /* nm */
+    double lj_a, lj_b;           /* A/r^12 - B/r^6, kJ nm^12/mol and kJ nm^6/mol */
+} bsl_params;
+
+typedef struct bsl_ctx bsl_ctx;
+
+/* Prepares a calculator (replaces HvPrep: HvPrep_Internal_NoQuad,
+ * linRotCartSphHvFullVPrepRoutines.cpp:229-528, and calc_Vlmlpmp_NoQuad[_ContGrid],
+ * linRotCartSphHvFullVPrepRoutines.cpp:7-227 / linRotCartSphHvContGridRoutines.cpp:7-152).
+ * Builds the basis tables on the host, then the potential grid and the
+ * per-point V blocks on the device, where they stay resident with the
+ * Krylov vectors. */
+int bsl_create(const bsl_params *p, bsl_ctx **out);
+int bsl_destroy(bsl_ctx *ctx);
+const char *bsl_last_error(void);
+
+/* total_basis_size in the reference convention (lanczosStor,
+ * interfaceContainers.h:6-10; set at linRotCartSphHvFullVPrepRoutines.cpp:500) */
+int64_t bsl_basis_size(const bsl_ctx *ctx);
+/* length of the device-resident Krylov vectors (differs from the above
+ * only for BSL_BASIS_REDUCED) */
+int64_t bsl_krylov_size(const bsl_ctx *ctx);
+
+/* uec += H vec on host vectors in the reference layout.  Replaces the Hv
+ * function pointer (Hv_linRotCartSph_NoQuad, linRotCartSphHvFullVPrepRoutines.cpp:807-822;
+ * called at lanczosEigenValuesVectors.cpp:195 and :426). */
+int bsl_hv(bsl_ctx *ctx, const double *vec, double *uec);
+/* the same on device pointers (reference layout, length bsl_basis_size) */
+int bsl_hv_device(bsl_ctx *ctx, const double *d_vec, double *d_uec);
+
+/* Lanczos loop 1 with the vectors resident in HBM
+ * (lanczosEigenValuesVectors.cpp:170-242).  Start vector: v0 (host, reference
+ * layout, normalised by the caller) if not NULL, else BSL_START_UNIFORM
+ * (1/sqrt(N), the reference's) or BSL_START_RANDOM (U[-1,1) from the
+ * counter-based generator with `seed`, normalised).  alpha[niter],
+ * beta[niter+1], beta2[niter+1]; index 0 of beta/beta2 is zero. */
+int bsl_lanczos(bsl_ctx *ctx, int32_t niter, const double *v0, int32_t start, uint64_t seed,
+                double *alpha, double *beta, double *beta2);
+
+/* Cullum-Willoughby post-processing (lancbis, lancbis.cpp:5-105: BISEC
+ * bisec.f:4-225, ISOEV isoev.f:4-55, INVERR inverr.f:4-167).  eval/err hold
+ * niter entries; *ngood receives the number of good eigenvalues. */
+int bsl_lancbis(int32_t niter, const double *alpha, const double *beta, const double *beta2, double emin,
+                double emax, double *eval, double *err, int32_t *ngood);
+
+/* Tm eigenvectors (lanczosvectors, lanczosEigenValuesVectors.cpp:509-561 ->
+ * TRIVEC trivec.f:4-95).  evtr column-major [niter][ngood]. */
+int bsl_trivec(int32_t niter, const double *alpha, const double *beta, const double *beta2, const double *eval,
+               int32_t ngood, double *evtr, int32_t *mamin);
+
+/* Lanczos loop 2 (lanczosEigenValuesVectors.cpp:371-469): recomputes the
+ * Krylov vectors from the same start and accumulates the Ritz vectors on
+ * the device; ritz is host memory [ngood][bsl_basis_size] (reference layout). */
+int bsl_ritz(bsl_ctx *ctx, int32_t niter, const double *v0, int32_t start, uint64_t seed, const double *evtr,
+             int32_t ngood, double *ritz);
+
+/* FP64 Rayleigh quotients <y|H|y>/<y|y> of host vectors (reference layout,
+ * [count][bsl_basis_size]) with an FP64 Hv, whatever the context precision;
+ * also returns the residual norms |Hy - e y|/|y|. */
+int bsl_rayleigh(bsl_ctx *ctx, int32_t count, const double *y, double *energy, double *resid);
+
+/* Water sites from centres of mass and ZY'Z'' Euler angles, as read from
+ * the reference's .xyz geometry files (getSPCEatoms,
+ * SPCE_AH2_EngRoutines.cpp:5-108; rotMat3D rotmat.cpp:6-32).  cm_ea holds
+ * per molecule CMx phi CMy theta CMz chi (Angstrom, radian) in file order.
+ * model 0 = SPC/E (O, H, H).  Writes 3*nmol sites: positions (nm),
+ * charges (e) and the LJ flag (1 on O). */
+int bsl_water_sites(int32_t model, int32_t nmol, const double *cm_ea, double *pos, double *q, int32_t *lj);
+
+/* Diagnostics: V blocks in the reference order [i][j][k][n][n'] (FULL basis
+ * contexts only; cross-parity couplings the device drops are returned as 0),
+ * and the potential grid [i][j][k][a][b] for both phi halves. */
+int bsl_get_vblocks(bsl_ctx *ctx, double *out);
+int bsl_get_vgrid(bsl_ctx *ctx, double *v1, double *v2);
+
+/* Timing of the device recurrence: runs `steps` Lanczos steps from the
+ * random start after `warmup` untimed ones.  ms[0] = total time of the timed
+ * steps (HIP events on the context stream, no host synchronisation inside);
+ * ms[1], ms[2], ms[3] = average per-step duration of the potential-block
+ * kernel, the two kinetic kernels and the recurrence update kernel, from
+ * events recorded around each of them in a further min(steps, 64) steps. */
+int bsl_time_steps(bsl_ctx *ctx, int32_t warmup, int32_t steps, double *ms);
+
+/* Bytes each kernel of one Lanczos step must move at minimum (algorithmic
+ * bytes): out[0] potential term, out[1] kinetic term, out[2] recurrence
+ * update, out[3] total; out[4] FLOPs of one Hv. */
+int bsl_step_bytes(const bsl_ctx *ctx, double *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,62 @@
+"""Shared problem definitions for the oracle and the GPU calculator."""
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLDEN = os.path.join(HERE, "golden")
+
+MASS = 2.01588
+B_ROT = 0.7096487130193394
+
+
+def cage11_cm_ea():
+    import bsl_amd.cage as cg
+    return cg.read_xyz(os.path.join(GOLDEN, "sII_smallCage11_CM_EA_ZYZ.xyz"))
+
+
+def cage11_sites():
+    """SPC/E sites of the reference's cage 11 geometry, built by the oracle."""
+    import _oracle as O
+    pos, q, typ = O.build_sites(cage11_cm_ea(), 0)
+    lj = np.array([1 if t == ord("O") else 0 for t in typ], dtype=np.int32)
+    return pos, q, typ, lj
+
+
+def synthetic_sites(seed=0):
+    import bsl_amd.cage as cg
+    pos, q, lj = cg.synthetic_cage(seed)
+    typ = bytes([ord("O") if f else ord("H") for f in lj])
+    return pos, q, typ, lj
+
+
+SYMS = {"All": 0, "Even": 1, "Odd": 2}
+
+
+def cage_case(nx, lmax, na, nb, sym="All", x_max=0.4, sites=None, lj=(1.364e-06, 1.533e-3), mass=MASS,
+              rot=B_ROT, ceiling=18.0):
+    """(oracle spec, calculator kwargs) for an Alavi H2 / SPC/E cage."""
+    pos, q, typ, ljf = sites if sites is not None else cage11_sites()
+    spec = dict(nx=nx, ny=nx, nz=nx, x_max=x_max, y_max=x_max, z_max=x_max, lmax=lmax, na=na, nb=nb, mass=mass,
+                rot_const=rot, ceiling=ceiling, sym=SYMS[sym], pot_kind=1, site_pos=pos, site_q=q, site_type=typ,
+                q_h2_h=0.4932, q_h2_cm=-0.9864, h2_half_bond=0.03707, lj_a=lj[0], lj_b=lj[1])
+    kw = dict(nx=nx, x_max=x_max, lmax=lmax, n_theta=na, n_phi=nb, mass=mass, rot_const=rot, ceiling=ceiling,
+              symmetry=sym, potential="cage", sites=(pos, q, ljf), q_h2_h=0.4932, q_h2_cm=-0.9864,
+              h2_half_bond=0.03707, lj_a=lj[0], lj_b=lj[1])
+    return spec, kw
+
+
+def iho_case(nx, lmax, na, nb, x_max, omega_cm1=100.0, mass=2.016, rot=0.709648713019339, sym="All",
+             ceiling=1.0e6):
+    from bsl_amd import physics
+    w = physics.omega_from_cm1(omega_cm1)
+    factor = 0.5 * mass * w * w
+    spec = dict(nx=nx, ny=nx, nz=nx, x_max=x_max, y_max=x_max, z_max=x_max, lmax=lmax, na=na, nb=nb, mass=mass,
+                rot_const=rot, ceiling=ceiling, sym=SYMS[sym], pot_kind=0, iho_factor=factor, iho_ceil=ceiling)
+    kw = dict(nx=nx, x_max=x_max, lmax=lmax, n_theta=na, n_phi=nb, mass=mass, rot_const=rot, ceiling=ceiling,
+              symmetry=sym, potential="iho", iho_factor=factor, iho_ceil=ceiling)
+    return spec, kw
+
+
+def ref_to_device_order(vblocks_ref):
+    return vblocks_ref

@@ -1,0 +1,243 @@
+"""GPU parity of the HIP path (libbsl_hip.so, through its C ABI) against the
+CPU oracle and the reference's own outputs.
+
+Tolerances: the potential grid and the V blocks are compared elementwise
+relative to their largest magnitude (1e-13 / 1e-12: the device sums the
+quadrature in a different order); Hv the same way (1e-12); alpha/beta on
+the prefix before loss of orthogonality at 1e-9 relative and converged
+eigenvalues at 1e-9 relative (the FP64 bar); FP32 levels within 0.01 cm^-1.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import _cases as cs
+import _oracle as O
+
+pytestmark = pytest.mark.gpu
+
+B = pytest.importorskip("bsl_amd")
+
+
+def _rel(a, b):
+    return np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300)
+
+
+def _ref_to_dev_blocks(vb_ref):
+    return vb_ref
+
+
+@pytest.fixture(scope="module")
+def cage_small():
+    spec, kw = cs.cage_case(6, 3, 8, 7, "All")
+    return O.Oracle(spec), B.Calculator(**kw)
+
+
+def test_vgrid_cage_matches_oracle(cage_small):
+    o, c = cage_small
+    g1, g2 = c.vgrid()
+    r1, r2 = o.vgrid()
+    assert _rel(g1, r1) < 1e-13 and _rel(g2, r2) < 1e-13
+
+
+def test_vblocks_cage_matches_oracle(cage_small):
+    o, c = cage_small
+    vg = c.vblocks()
+    vr = o.vblocks()
+    lsum = np.add.outer(o.qnum()[:, 0], o.qnum()[:, 0])
+    same = (lsum % 2 == 0)
+    # couplings of opposite l parity vanish for the homonuclear rotor; the
+    # device does not store them
+    assert np.max(np.abs(vr[..., ~same])) < 1e-12 * np.max(np.abs(vr))
+    assert _rel(vg[..., same], vr[..., same]) < 1e-12
+
+
+@pytest.mark.parametrize("sym", ["All", "Even", "Odd"])
+def test_hv_full_matches_oracle(sym):
+    spec, kw = cs.cage_case(5, 2, 6, 6, sym)
+    o, c = O.Oracle(spec), B.Calculator(**kw)
+    v = np.random.default_rng(7).uniform(-1, 1, o.N)
+    u0 = np.random.default_rng(8).uniform(-1, 1, o.N)
+    ref = u0.copy()
+    O.lib.bso_hv(o.h, v, ref)
+    got = c.hv(v, u0)
+    assert _rel(got, ref) < 1e-12
+
+
+@pytest.mark.parametrize("sym", ["Even", "Odd"])
+def test_hv_reduced_matches_oracle(sym):
+    spec, kw = cs.cage_case(5, 3, 8, 8, sym)
+    o, c = O.Oracle(spec), B.Calculator(basis="reduced", **kw)
+    v = np.random.default_rng(3).uniform(-1, 1, o.N)
+    ref = o.hv(v)
+    got = c.hv(v)
+    assert c.N_krylov < c.N
+    assert _rel(got, ref) < 1e-12
+
+
+def test_hv_iho_matches_oracle():
+    spec, kw = cs.iho_case(9, 2, 6, 8, x_max=0.4)
+    o, c = O.Oracle(spec), B.Calculator(**kw)
+    v = np.random.default_rng(5).uniform(-1, 1, o.N)
+    assert _rel(c.hv(v), o.hv(v)) < 1e-12
+
+
+def test_hv_matches_reference_fixture():
+    """Hv of the reference's own compiled calculator (committed vectors)."""
+    fx = np.load(os.path.join(cs.GOLDEN, "ref_hv_cage11.npz"))
+    for tag in ("a", "b"):
+        nx, lmax, na, nb, sym = [int(x) for x in fx["%s_params" % tag]]
+        _, kw = cs.cage_case(nx, lmax, na, nb, ["All", "Even", "Odd"][sym])
+        c = B.Calculator(**kw)
+        got = c.hv(fx["%s_v" % tag])
+        assert _rel(got, fx["%s_hv" % tag]) < 1e-12, tag
+
+
+def test_hv_device_entry_point():
+    """bsl_hv_device on device buffers agrees with bsl_hv on host buffers."""
+    import ctypes as C
+    spec, kw = cs.cage_case(5, 2, 6, 6, "Even")
+    c = B.Calculator(**kw)
+    v = np.random.default_rng(9).uniform(-1, 1, c.N)
+    u = np.random.default_rng(10).uniform(-1, 1, c.N)
+    ref = c.hv(v, u)
+    hip = C.CDLL("libamdhip64.so")
+    dv, du = C.c_void_p(), C.c_void_p()
+    nbytes = 8 * c.N
+    assert hip.hipMalloc(C.byref(dv), C.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(C.byref(du), C.c_size_t(nbytes)) == 0
+    try:
+        assert hip.hipMemcpy(dv, v.ctypes.data_as(C.c_void_p), C.c_size_t(nbytes), 1) == 0
+        assert hip.hipMemcpy(du, u.ctypes.data_as(C.c_void_p), C.c_size_t(nbytes), 1) == 0
+        B.check(B.lib().bsl_hv_device(c.h, dv, du))
+        out = np.zeros(c.N)
+        assert hip.hipMemcpy(out.ctypes.data_as(C.c_void_p), du, C.c_size_t(nbytes), 2) == 0
+    finally:
+        hip.hipFree(dv)
+        hip.hipFree(du)
+    assert np.array_equal(out, ref)
+
+
+@pytest.mark.parametrize("start", ["uniform", "random"])
+def test_lanczos_prefix_matches_oracle(start):
+    spec, kw = cs.cage_case(6, 2, 6, 6, "All")
+    o, c = O.Oracle(spec), B.Calculator(**kw)
+    niter = 40
+    v0 = O.start_vector(o.N, 11) if start == "random" else None
+    ar, br, b2r = o.lanczos(niter, v0)
+    ag, bg, b2g = c.lanczos(niter, start=start, seed=11)
+    assert np.max(np.abs(ag - ar) / np.abs(ar)) < 1e-9
+    assert np.max(np.abs(bg[1:] - br[1:]) / br[1:]) < 1e-9
+    assert np.max(np.abs(b2g[1:] - b2r[1:]) / b2r[1:]) < 1e-9
+
+
+def _converged(ev, err, tol=1e-8):
+    return ev[(np.abs(err) < tol) & (err != 0)]
+
+
+def test_eigenvalues_match_oracle():
+    spec, kw = cs.cage_case(6, 3, 8, 8, "All")
+    o, c = O.Oracle(spec), B.Calculator(**kw)
+    niter = 200
+    er_o = O.lancbis(*o.lanczos(niter), -10.0, 10.0)
+    er_g = B.lancbis(*c.lanczos(niter), -10.0, 10.0)
+    conv = _converged(*er_o)
+    assert len(conv) >= 3
+    for e in conv:
+        assert np.min(np.abs(er_g[0] - e)) / abs(e) < 1e-9
+
+
+def test_saved_run_eigenvalues():
+    """cage 11, 10^3 x (lmax 3), 10x10 quadrature, 300 steps, uniform start:
+    the levels the reference's saved run converged (boundstates.txt of
+    Compilation/SmallBasisTest2_2013-09-03_19_06_47)."""
+    runs = json.load(open(os.path.join(cs.GOLDEN, "runs.json")))
+    run = runs["SmallBasisTest2_2013-09-03_19_06_47"]
+    hv = run["hv_deck"]
+    nx, xmax, lmax, na, nb = int(hv[0]), float(hv[1]), int(hv[6]), int(hv[7]), int(hv[8])
+    _, kw = cs.cage_case(nx, lmax, na, nb, hv[21], x_max=xmax, mass=float(hv[15]), rot=float(hv[16]),
+                         ceiling=float(hv[19]))
+    c = B.Calculator(**kw)
+    ev, err = B.lancbis(*c.lanczos(int(run["lanczos_deck"][0])), float(run["lanczos_deck"][1]),
+                        float(run["lanczos_deck"][2]))
+    ref = np.array(run["boundstates"])[:, 0]
+    # the lowest levels are converged in the saved run (later ones depend on
+    # rounding through the loss of orthogonality)
+    for e in ref[:10]:
+        assert np.min(np.abs(ev - e)) / abs(e) < 1e-9, e
+
+
+def test_iho_analytic_levels():
+    """(n + 3/2) hbar w + B l(l+1) on the wide box (x_max 0.4 nm, 21 points)."""
+    from bsl_amd import physics
+    spec, kw = cs.iho_case(21, 3, 20, 30, x_max=0.4)
+    c = B.Calculator(**kw)
+    ev, err = B.lancbis(*c.lanczos(300, start="random", seed=0), 0.0, 8.0)
+    hw = physics.H_BAR * physics.omega_from_cm1(100.0)
+    Bc = kw["rot_const"]
+    levels = sorted({round((n + 1.5) * hw + Bc * l * (l + 1), 12) for n in range(8) for l in range(4)})
+    good = np.sort(_converged(ev, err, 1e-6))
+    low = [x for x in levels if x < 5.0]
+    for e in low:
+        assert np.min(np.abs(good - e)) / e < 1e-6, (e, good[:10])
+
+
+def test_ritz_vectors_match_oracle():
+    spec, kw = cs.cage_case(5, 2, 6, 6, "All")
+    o, c = O.Oracle(spec), B.Calculator(**kw)
+    niter = 60
+    a, b, b2 = o.lanczos(niter)
+    ev, err = O.lancbis(a, b, b2, -10.0, 10.0)
+    ev = ev[:3]
+    evtr, _ = O.trivec(a, b, b2, ev)
+    ref = o.ritz(niter, evtr, len(ev))
+    got = c.ritz(niter, evtr, len(ev))
+    for g in range(len(ev)):
+        assert _rel(got[g], ref[g]) < 1e-7
+
+
+def test_rayleigh_refinement():
+    spec, kw = cs.cage_case(5, 2, 6, 6, "All")
+    c = B.Calculator(**kw)
+    niter = 80
+    a, b, b2 = c.lanczos(niter)
+    ev, err = B.lancbis(a, b, b2, -10.0, 10.0)
+    evtr, _ = B.trivec(a, b, b2, ev[:2])
+    y = c.ritz(niter, evtr, 2)
+    e, r = c.rayleigh(y)
+    assert abs(e[0] - ev[0]) / abs(ev[0]) < 1e-9
+    assert r[0] < 1e-5
+
+
+def test_fp32_levels_within_target():
+    """FP32 potential blocks, FP64 recurrence: lowest levels within 0.01 cm^-1."""
+    from bsl_amd import physics
+    spec, kw = cs.cage_case(8, 3, 10, 10, "All")
+    c64 = B.Calculator(**kw)
+    c32 = B.Calculator(precision="fp32", **kw)
+    e64, r64 = B.lancbis(*c64.lanczos(300), -10.0, 10.0)
+    e32, r32 = B.lancbis(*c32.lanczos(300), -10.0, 10.0)
+    conv = _converged(e64, r64, 1e-9)[:5]
+    assert len(conv) >= 3
+    for e in conv:
+        d = np.min(np.abs(e32 - e)) / physics.KJMOL_PER_CM1
+        assert d < 0.01, (e, d)
+
+
+def test_reduced_basis_drops_spurious_zero():
+    """para (Even) block: the reduced basis reproduces the masked full-basis
+    levels without the spurious 0 of the masked subspace."""
+    spec, kw = cs.cage_case(6, 2, 6, 6, "Even")
+    full = B.Calculator(**kw)
+    red = B.Calculator(basis="reduced", **kw)
+    ef, rf = B.lancbis(*full.lanczos(200), -10.0, 10.0)
+    er, rr = B.lancbis(*red.lanczos(200), -10.0, 10.0)
+    assert np.min(np.abs(ef)) < 1e-8          # the masked subspace's zero
+    assert np.min(np.abs(er)) > 1e-3
+    conv = _converged(ef, rf)
+    conv = conv[np.abs(conv) > 1e-6]
+    assert len(conv) >= 2
+    for e in conv:
+        assert np.min(np.abs(er - e)) / abs(e) < 1e-9

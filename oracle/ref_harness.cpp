@@ -108,3 +108,19 @@ extern "C" __attribute__((visibility("default"))) double ref_spce_energy(const c
     if (with_lj) e += LJ_SPCE_Eng_Fast(pos, atomType, atomPos, nAtoms);
     return e;
 }
+
+// Coulomb energy of a charge q at (x,y,z) with the TIP4P sites of a .xyz
+// geometry (getTIP4Patoms + Q_TIP4P_Eng), as in the reference's commented test main
+extern "C" __attribute__((visibility("default"))) double ref_tip4p_coulomb(const char *xyz, double x, double y,
+                                                                          double z, double q)
+{
+    char *atomType;
+    VECT *atomPos;
+    int nAtoms;
+    getTIP4Patoms(&atomType, &atomPos, &nAtoms, std::string(xyz));
+    VECT pos(3);
+    pos.COOR(0) = x;
+    pos.COOR(1) = y;
+    pos.COOR(2) = z;
+    return Q_TIP4P_Eng(pos, q, atomType, atomPos, nAtoms);
+}

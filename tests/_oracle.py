@@ -147,7 +147,7 @@ class Oracle:
         return v1.reshape(shp), v2.reshape(shp)
 
     def kin(self, axis):
-        n = self.spec["nxyz"[axis] if False else ("nx", "ny", "nz")[axis]]
+        n = self.spec[("nx", "ny", "nz")[axis]]
         out = np.zeros(n * n)
         lib.bso_get_kin(self.h, axis, out)
         return out.reshape(n, n)

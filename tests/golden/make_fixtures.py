@@ -6,6 +6,9 @@ Fixtures (data only: inputs and expected outputs):
   sII_unitCelltest_CM_EA_ZYZ.xyz  unit cell of the -30.2331 kJ/mol known answer
   runs.json                       decks and boundstates.txt of the saved runs
                                   Compilation/SmallBasisTest2_2013-09-03_{18_36_00,19_06_47}
+  point_energies.json             point energies of the reference's compiled
+                                  site builders and pair sums (TIP4P Coulomb in
+                                  the sII unit cell, SPC/E Coulomb + LJ in cage 11)
   ref_hv_cage11.npz               V blocks and Hv of the reference's own compiled
                                   calculator (oracle/_ref) on two small cage11
                                   decks, for a seeded input vector
@@ -74,7 +77,21 @@ def main(ref):
         print("oracle/_ref not built; skipping ref_hv_cage11.npz")
         return
     R = O.ref_lib()
+    import ctypes as C
+    R.ref_tip4p_coulomb.restype = C.c_double
+    R.ref_tip4p_coulomb.argtypes = [C.c_char_p] + [C.c_double] * 4
     geo = os.path.join(HERE, "sII_smallCage11_CM_EA_ZYZ.xyz")
+    cell = os.path.join(HERE, "sII_unitCelltest_CM_EA_ZYZ.xyz")
+    pts = [(1.0824, 0.2164, 1.0819), (0.6491, 0.2165, 0.6492), (0.8651, 0.8655, 0.8654)]
+    energies = {"tip4p_coulomb_unit_cell": [[list(p), R.ref_tip4p_coulomb(cell.encode(), *p, 1.0)] for p in pts],
+                "spce_cage11": []}
+    rng = np.random.default_rng(5)
+    for _ in range(8):
+        p = [float(x) for x in rng.uniform(-0.2, 0.2, 3)]
+        energies["spce_cage11"].append([p, R.ref_spce_energy(geo.encode(), *p, -0.9864, 1),
+                                        R.ref_spce_energy(geo.encode(), *p, 0.4932, 0)])
+    with open(os.path.join(HERE, "point_energies.json"), "w") as f:
+        json.dump(energies, f, indent=1)
     out = {}
     for tag, (nx, lmax, na, nb, sym) in {"a": (5, 2, 5, 6, "All"), "b": (6, 3, 8, 7, "Even")}.items():
         deck = os.path.join("/tmp", "bsl_fixture_hv_%s.txt" % tag)

@@ -1,0 +1,184 @@
+"""Lanczos steps/s of the MI355X Hv + recurrence at a 54^3 x 36 basis.
+
+Workload (one GPU): Alavi H2 in a synthetic 5^12 cage of 20 rigid SPC/E
+waters (seed 0), sinc-DVR grid of 54 points per axis on a box of x_max
+(default 0.8 nm, i.e. [-0.4, 0.4] nm, spacing 0.0151 nm), tesseral
+harmonics l <= 5 (36 lm, full basis), 40 x 70 (theta, phi) quadrature,
+NoQuad potential blocks, FP64 Hv and FP64 recurrence.  A step is one Hv plus
+one update of the three-term recurrence; inputs (potential blocks, Krylov
+vectors) are resident in HBM when the timed region starts.
+
+Also reported: the FP32-potential ("mixed") step rate and its ratio to FP64,
+the error of the mixed run's lowest levels against the FP64 run (cm^-1), the
+roofline of the dominant (potential-block) kernel and the CPU oracle timed
+on this host on a bounded sample of the same operator.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(REPO, "boundstate-lanczos_amd"))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def make_calc(B, args, precision, basis="full", symmetry="All"):
+    from bsl_amd import cage, physics
+    pos, q, lj = cage.synthetic_cage(0)
+    lj_a, lj_b = cage.baseline_lj()
+    return B.Calculator(nx=args.n, x_max=args.x_max, lmax=5, n_theta=40, n_phi=70, mass=2.016,
+                        rot_const=physics.cm1_to_kjmol(59.322), ceiling=18.0, symmetry=symmetry, basis=basis,
+                        precision=precision, potential="cage", sites=(pos, q, lj), lj_a=lj_a, lj_b=lj_b)
+
+
+def traffic_from_profiles(tag):
+    """HBM bytes per launch of the potential kernel measured with rocprofv3
+    --pmc (FETCH_SIZE + WRITE_SIZE), committed under profiles/."""
+    path = os.path.join(REPO, "profiles", "traffic_%s.json" % tag)
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f).get("bytes_per_launch")
+
+
+def cpu_baseline(B, calc, args):
+    """The oracle (reference algorithm: serial kinetic term, OpenMP potential
+    term) on this host, a few steps of the same operator: the potential
+    blocks are taken from the GPU context so that the CPU does not redo the
+    preparation."""
+    sys.path.insert(0, os.path.join(REPO, "tests"))
+    import _oracle as O
+    from bsl_amd import cage, physics
+    pos, q, lj = cage.synthetic_cage(0)
+    typ = bytes([ord("O") if f else ord("H") for f in lj])
+    lj_a, lj_b = cage.baseline_lj()
+    spec = dict(nx=args.n, ny=args.n, nz=args.n, x_max=args.x_max, y_max=args.x_max, z_max=args.x_max, lmax=5,
+                na=40, nb=70, mass=2.016, rot_const=physics.cm1_to_kjmol(59.322), ceiling=18.0, sym=0,
+                pot_kind=1, site_pos=pos, site_q=q, site_type=typ, q_h2_h=physics.Q_H2_H,
+                q_h2_cm=physics.Q_H2_CM, h2_half_bond=physics.H2_HALF_BOND, lj_a=lj_a, lj_b=lj_b)
+    vb = calc.vblocks()
+    o = O.Oracle(spec, vblocks=vb)
+    del vb
+    steps = args.cpu_steps
+    o.lanczos(1)  # warm-up (page faults, thread pool)
+    t0 = time.perf_counter()
+    o.lanczos(steps)
+    dt = time.perf_counter() - t0
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": steps / dt, "unit": "steps/s", "cores": cores, "kind": "port",
+            "sample": "%d Lanczos steps of the same 54^3 x 36 operator (oracle Hv: serial kinetic term, "
+                      "OpenMP potential term, as the reference), %.1f s" % (steps, dt)}
+
+
+def eig_error(B, c64, c32, niter):
+    """Lowest 20 good levels of FP64 and mixed runs (same random start)."""
+    from bsl_amd import physics
+    e64, r64 = B.lancbis(*c64.lanczos(niter, start="random", seed=0), -10.0, 10.0)
+    e32, r32 = B.lancbis(*c32.lanczos(niter, start="random", seed=0), -10.0, 10.0)
+    low = e64[:20]
+    d = [np.min(np.abs(e32 - e)) / physics.KJMOL_PER_CM1 for e in low]
+    return float(np.max(d)), len(low), [float(x) for x in low[:5]]
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--n", type=int, default=54)
+    ap.add_argument("--x-max", type=float, default=0.8)
+    ap.add_argument("--eig-steps", type=int, default=3000)
+    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-eig", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: every box in this setup has one GPU, and one Lanczos "
+              "recurrence per GPU is the unit of work (independent recurrences shard with no "
+              "communication). Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+    import bsl_amd as B
+
+    t0 = time.perf_counter()
+    c64 = make_calc(B, args, "fp64")
+    prep64 = time.perf_counter() - t0
+    ms = c64.time_steps(args.warmup, args.steps)
+    sb = c64.step_bytes()
+    steps_s = args.steps / (ms[0] / 1e3)
+    pot_ms = ms[1]
+    achieved = sb[0] / (pot_ms * 1e-3) / 1e9
+    print("fp64: prep %.2f s, %.1f steps/s, per step %.3f ms (potential %.3f, kinetic %.3f, update %.3f)"
+          % (prep64, steps_s, ms[0] / args.steps, ms[1], ms[2], ms[3]), file=sys.stderr, flush=True)
+
+    t0 = time.perf_counter()
+    c32 = make_calc(B, args, "fp32")
+    prep32 = time.perf_counter() - t0
+    ms32 = c32.time_steps(args.warmup, args.steps)
+    steps_s32 = args.steps / (ms32[0] / 1e3)
+    sb32 = c32.step_bytes()
+    print("fp32: prep %.2f s, %.1f steps/s, per step %.3f ms (potential %.3f, kinetic %.3f, update %.3f)"
+          % (prep32, steps_s32, ms32[0] / args.steps, ms32[1], ms32[2], ms32[3]), file=sys.stderr, flush=True)
+
+    eig = None
+    if not args.no_eig:
+        err, nlev, low = eig_error(B, c64, c32, args.eig_steps)
+        eig = {"max_err_cm1": err, "levels": nlev, "steps": args.eig_steps, "lowest_kjmol": low}
+        print("mixed vs fp64 lowest %d levels: max err %.2e cm^-1" % (nlev, err), file=sys.stderr, flush=True)
+    del c32
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(B, c64, args)
+        print("cpu baseline: %.4f steps/s on %d threads" % (cpu["value"], cpu["cores"]), file=sys.stderr,
+              flush=True)
+
+    line = {
+        "metric": "Lanczos steps/s at 54³×36 basis, mixed vs FP64 GPU; eig err cm⁻¹ vs FP64",
+        "value": steps_s,
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms[0] / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": "Alavi H2 in a synthetic 5^12 SPC/E cage (seed 0); %d^3 sinc-DVR grid, x_max %.3g nm; "
+                        "lmax 5 (36 lm, full basis); 40x70 quadrature; NoQuad potential blocks; FP64 Hv and "
+                        "recurrence" % (args.n, args.x_max),
+            "basis_size": int(c64.N),
+            "mixed_fp32_steps_per_s": steps_s32,
+            "mixed_vs_fp64": steps_s32 / steps_s,
+            "eig_err_vs_fp64": eig,
+            "prep_s": {"fp64": prep64, "fp32": prep32},
+            "per_step_ms": {"potential": ms[1], "kinetic": ms[2], "update": ms[3]},
+            "algorithmic_bytes_per_step": sb[3],
+            "step_bytes_roofline_frac": sb[3] / (ms[0] / args.steps * 1e-3) / 1e9 / HBM_PEAK_GBS,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "k_vblock (potential blocks, FP64)",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": traffic_from_profiles("vblock_fp64_54"),
+            "algorithmic_bytes_per_launch": sb[0],
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

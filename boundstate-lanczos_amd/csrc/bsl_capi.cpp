@@ -89,6 +89,59 @@ int build_vpk(bsl_ctx *c, DevOperator &op, int vprec, const double *d_h)
     return BSL_OK;
 }
 
+// Picks the occupancy variant of the potential-block kernel by timing each
+// on the context's own operator (the variants compute identical results).
+int time_potential(bsl_ctx *c, DevOperator &op, float *ms)
+{
+    hipEvent_t e0, e1;
+    BSL_HIP(hipEventCreate(&e0));
+    BSL_HIP(hipEventCreate(&e1));
+    int rc = launch_init_scal(c->d_scal, 1.0, 1.0, c->stream);
+    for (int i = 0; i < 2 && !rc; i++) rc = launch_potential(op, c->d_X, c->d_R, c->d_scal, c->stream);
+    if (!rc) {
+        BSL_HIP(hipEventRecord(e0, c->stream));
+        for (int i = 0; i < 5 && !rc; i++) rc = launch_potential(op, c->d_X, c->d_R, c->d_scal, c->stream);
+        BSL_HIP(hipEventRecord(e1, c->stream));
+        BSL_HIP(hipEventSynchronize(e1));
+        BSL_HIP(hipEventElapsedTime(ms, e0, e1));
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return rc;
+}
+
+int autotune(bsl_ctx *c)
+{
+    if (c->op.calc == BSL_CALC_QUAD) return BSL_OK;
+    int rc = launch_zero(c->d_X, c->n_kry, c->stream);
+    if (!rc) rc = launch_zero(c->d_R, c->n_kry, c->stream);
+    DevOperator *ops[2] = {&c->op, &c->op64};
+    for (int o = 0; o < 2 && !rc; o++) {
+        if (o == 1 && c->op64.vpk == c->op.vpk) {
+            c->op64.vb_variant = c->op.vb_variant;
+            break;
+        }
+        float best = 1e30f;
+        int pick = 1;
+        for (int v = 0; v < 3 && !rc; v++) {
+            ops[o]->vb_variant = v;
+            float ms = 0;
+            rc = time_potential(c, *ops[o], &ms);
+            if (!rc && ms < best) {
+                best = ms;
+                pick = v;
+            }
+        }
+        ops[o]->vb_variant = pick;
+    }
+    if (!rc) rc = launch_init_scal(c->d_scal, 0.0, 0.0, c->stream);
+    if (!rc) {
+        hipError_t e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess) rc = hip_fail(e, "autotune", __FILE__, __LINE__);
+    }
+    return rc;
+}
+
 }  // namespace
 }  // namespace bsl
 
@@ -114,8 +167,12 @@ extern "C" int bsl_destroy(bsl_ctx *c)
     dev_free(o.rot);
     dev_free(o.mask);
     dev_free(o.plane_ref);
+    dev_free(o.wgrid);
+    dev_free(o.qL);
+    dev_free(o.qS);
+    dev_free(o.mplane);
     dev_free(o.partial);
-    dev_free(o.counter);
+    dev_free(o.partial_b);
     dev_free(c->d_gx);
     dev_free(c->d_gy);
     dev_free(c->d_gz);
@@ -138,7 +195,6 @@ extern "C" int bsl_create(const bsl_params *pp, bsl_ctx **out)
     *out = nullptr;
     int rc = check_params(*pp);
     if (rc) return rc;
-    if (pp->calculator == BSL_CALC_QUAD) return fail(BSL_E_ARG, "the quadrature calculator is built separately");
     std::unique_ptr<bsl_ctx> c(new bsl_ctx());
     c->p = *pp;
     if (pp->potential == BSL_POT_CAGE) {
@@ -209,15 +265,14 @@ extern "C" int bsl_create(const bsl_params *pp, bsl_ctx **out)
                 for (int g = 0; g < op.nblk; g++) {
                     const Block &B = op.blk[g];
                     int64_t pr = B.pair_offset;
-                    for (int r = 0; r < B.size; r++)
-                        for (int cc = r; cc < B.size; cc++) {
-                            const int n = c->carried[B.plane0 + r], np = c->carried[B.plane0 + cc];
-                            const int m = t.m[n], mp = t.m[np];
-                            h[q * pairs + pr] = t.wt[a] * t.wp[b] * t.L[(size_t)n * na + a] *
-                                                S[(size_t)(m + lmax) * nb + b] * t.L[(size_t)np * na + a] *
-                                                S[(size_t)(mp + lmax) * nb + b];
-                            pr++;
-                        }
+                    for (const auto &rc : block_pair_order(B.size)) {
+                        const int n = c->carried[B.plane0 + rc.first], np = c->carried[B.plane0 + rc.second];
+                        const int m = t.m[n], mp = t.m[np];
+                        h[q * pairs + pr] = t.wt[a] * t.wp[b] * t.L[(size_t)n * na + a] *
+                                            S[(size_t)(m + lmax) * nb + b] * t.L[(size_t)np * na + a] *
+                                            S[(size_t)(mp + lmax) * nb + b];
+                        pr++;
+                    }
                 }
             }
     }
@@ -244,10 +299,10 @@ extern "C" int bsl_create(const bsl_params *pp, bsl_ctx **out)
     if ((rc = dev_upload(&c->d_gx, gx))) return rc;
     if ((rc = dev_upload(&c->d_gy, gy))) return rc;
     if ((rc = dev_upload(&c->d_gz, gz))) return rc;
-    op.npartial = (int)std::max<int64_t>(2 * ((c->P + 255) / 256), 4096);
+    op.nalpha = ((c->P + 255) / 256) * op.nblk;
+    op.npartial = (int)std::max<int64_t>(op.nalpha, 4096);
     if ((rc = dev_alloc(&op.partial, (size_t)op.npartial))) return rc;
-    if ((rc = dev_alloc(&op.counter, 1))) return rc;
-    BSL_HIP(hipMemset(op.counter, 0, sizeof(unsigned int)));
+    if ((rc = dev_alloc(&op.partial_b, (size_t)op.npartial))) return rc;
 
     // ---- device: potential grid and V blocks ----
     BSL_HIP(hipMalloc((void **)&c->d_vgrid, sizeof(double) * (size_t)c->P * c->Q));
@@ -281,7 +336,48 @@ extern "C" int bsl_create(const bsl_params *pp, bsl_ctx **out)
     c->op64 = op;
     c->op64.vpk = nullptr;
     if ((rc = build_vpk(c.get(), c->op64, BSL_PREC_FP64, d_h))) return rc;
-    if (p.precision == BSL_PREC_FP64) {
+    if (p.calculator == BSL_CALC_QUAD) {
+        // weighted grid W[P][a][half*nb + b] and the transform tables
+        op.calc = BSL_CALC_QUAD;
+        op.wprec = p.precision == BSL_PREC_FP64 ? BSL_PREC_FP64 : BSL_PREC_FP32;
+        op.na = na;
+        op.nphi = 2 * nb;
+        op.nm = 2 * lmax + 1;
+        const size_t es = op.wprec == BSL_PREC_FP64 ? sizeof(double) : sizeof(float);
+        BSL_HIP(hipMalloc(&op.wgrid, es * (size_t)c->P * c->Q));
+        double *d_wt = nullptr, *d_wp = nullptr;
+        if ((rc = dev_upload(&d_wt, t.wt))) return rc;
+        if ((rc = dev_upload(&d_wp, t.wp))) return rc;
+        rc = launch_weight_grid(c->d_vgrid, op.wgrid, op.wprec, d_wt, d_wp, c->P, na, nb, c->stream);
+        BSL_HIP(hipStreamSynchronize(c->stream));
+        dev_free(d_wt);
+        dev_free(d_wp);
+        if (rc) return rc;
+        std::vector<double> qL((size_t)nk * na), qS((size_t)op.nm * op.nphi);
+        std::vector<int> mpl(nk);
+        for (int r = 0; r < nk; r++) {
+            const int n = c->carried[r];
+            for (int a = 0; a < na; a++) qL[(size_t)r * na + a] = t.L[(size_t)n * na + a];
+            mpl[r] = t.m[n] + lmax;
+        }
+        for (int mi = 0; mi < op.nm; mi++)
+            for (int b = 0; b < nb; b++) {
+                qS[(size_t)mi * op.nphi + b] = t.S1[(size_t)mi * nb + b];
+                qS[(size_t)mi * op.nphi + nb + b] = t.S2[(size_t)mi * nb + b];
+            }
+        if ((rc = dev_upload(&op.qL, qL))) return rc;
+        if ((rc = dev_upload(&op.qS, qS))) return rc;
+        if ((rc = dev_upload(&op.mplane, mpl))) return rc;
+        op.nalpha = (c->P + 7) / 8;
+        op.npartial = std::max<int>(op.npartial, (int)op.nalpha + 1);
+        dev_free(op.partial);
+        dev_free(op.partial_b);
+        if ((rc = dev_alloc(&op.partial, (size_t)op.npartial))) return rc;
+        if ((rc = dev_alloc(&op.partial_b, (size_t)op.npartial))) return rc;
+        c->op64.partial = op.partial;
+        c->op64.partial_b = op.partial_b;
+        c->op64.npartial = op.npartial;
+    } else if (p.precision == BSL_PREC_FP64) {
         op.vpk = c->op64.vpk;
         op.vprec = BSL_PREC_FP64;
     } else {
@@ -297,6 +393,7 @@ extern "C" int bsl_create(const bsl_params *pp, bsl_ctx **out)
     if ((rc = dev_alloc(&c->d_R, (size_t)c->n_kry))) return rc;
     if ((rc = dev_alloc(&c->d_scal, 1))) return rc;
     BSL_HIP(hipMemset(c->d_scal, 0, sizeof(Scal)));
+    if ((rc = autotune(c.get()))) return rc;
     *out = c.release();
     return BSL_OK;
 }
@@ -359,7 +456,7 @@ int hv_host_with(bsl_ctx *c, const DevOperator &op, const double *vec, double *u
     Scal saved;
     BSL_HIP(hipMemcpyAsync(&saved, sc, sizeof(Scal), hipMemcpyDeviceToHost, c->stream));
     if ((rc = launch_init_scal(sc, 1.0, 1.0, c->stream))) return rc;
-    if ((rc = launch_hv(op, dX, dR, sc, nullptr, c->stream, nullptr))) return rc;
+    if ((rc = launch_hv(op, dX, dR, sc, c->stream, nullptr))) return rc;
     if ((rc = download_carried(c, dR, uec))) return rc;
     BSL_HIP(hipStreamSynchronize(c->stream));
     BSL_HIP(hipMemcpyAsync(sc, &saved, sizeof(Scal), hipMemcpyHostToDevice, c->stream));
@@ -376,7 +473,7 @@ int start_lanczos(bsl_ctx *c, const double *v0, int start, uint64_t seed)
     } else {
         if (start != BSL_START_UNIFORM && start != BSL_START_RANDOM) return fail(BSL_E_ARG, "unknown start vector");
         rc = launch_start_vector(c->d_X, c->n_kry, c->P, c->op.plane_ref, start, seed, c->d_scal, c->op.partial,
-                                 c->op.counter, c->op.npartial, c->stream);
+                                 c->op.npartial, c->stream);
         if (rc) return rc;
     }
     return launch_zero(c->d_R, c->n_kry, c->stream);
@@ -385,9 +482,10 @@ int start_lanczos(bsl_ctx *c, const double *v0, int start, uint64_t seed)
 // one Lanczos step j (1-based) on the current X/R; swaps the buffers
 int lanczos_step(bsl_ctx *c, const DevOperator &op, int j, hipEvent_t *ev)
 {
-    int rc = launch_hv(op, c->d_X, c->d_R, c->d_scal, c->d_alpha ? c->d_alpha + (j - 1) : nullptr, c->stream, ev);
+    int rc = launch_hv(op, c->d_X, c->d_R, c->d_scal, c->stream, ev);
     if (rc) return rc;
-    rc = launch_update(op, c->d_X, c->d_R, c->d_scal, c->d_beta ? c->d_beta + j : nullptr,
+    rc = launch_update(op, c->d_X, c->d_R, c->d_scal, c->d_alpha ? c->d_alpha + (j - 1) : nullptr,
+                       c->d_beta ? c->d_beta + j : nullptr,
                        c->d_beta2 ? c->d_beta2 + j : nullptr, c->stream);
     if (rc) return rc;
     if (ev) BSL_HIP(hipEventRecord(ev[3], c->stream));
@@ -419,7 +517,7 @@ extern "C" int bsl_hv_device(bsl_ctx *c, const double *d_vec, double *d_uec)
     Scal saved;
     BSL_HIP(hipMemcpyAsync(&saved, c->d_scal, sizeof(Scal), hipMemcpyDeviceToHost, c->stream));
     if ((rc = launch_init_scal(c->d_scal, 1.0, 1.0, c->stream))) return rc;
-    if ((rc = launch_hv(c->op, dX, dR, c->d_scal, nullptr, c->stream, nullptr))) return rc;
+    if ((rc = launch_hv(c->op, dX, dR, c->d_scal, c->stream, nullptr))) return rc;
     for (size_t k = 0; k < c->carried.size(); k++)
         BSL_HIP(hipMemcpyAsync(d_uec + (size_t)c->carried[k] * c->P, dR + k * c->P, sizeof(double) * c->P,
                                hipMemcpyDeviceToDevice, c->stream));
@@ -543,13 +641,20 @@ extern "C" int bsl_rayleigh(bsl_ctx *c, int32_t count, const double *y, double *
 extern "C" int bsl_water_sites(int32_t model, int32_t nmol, const double *cm_ea, double *pos, double *q,
                                int32_t *lj)
 {
-    if (model != 0) return fail(BSL_E_ARG, "only the SPC/E water model (0) is provided");
+    if (model != 0 && model != 1) return fail(BSL_E_ARG, "water model must be 0 (SPC/E) or 1 (TIP4P)");
     if (nmol < 0 || (nmol > 0 && (!cm_ea || !pos || !q || !lj))) return fail(BSL_E_ARG, "bad arguments");
-    // body-frame SPC/E sites (nm) and charges (SPCE_AH2_EngRoutines.h:17-32)
-    static const double body[3][3] = {{0., 0., 0.006461408220},
+    // body-frame sites (nm) and charges: SPC/E (SPCE_AH2_EngRoutines.h:17-32)
+    // and TIP4P with its massless M site (TIP4P_AH2_EngRoutines.h:16-35)
+    static const double spce[3][3] = {{0., 0., 0.006461408220},
                                       {-8.164965810E-2, 0., -0.05127361870},
                                       {8.164965810E-2, 0., -0.05127361870}};
-    static const double charge[3] = {-0.8476, 0.4238, 0.4238};
+    static const double spce_q[3] = {-0.8476, 0.4238, 0.4238};
+    static const double tip4p[4][3] = {{0., 0., 0.00655538},
+                                       {-0.075695, 0., -0.0520326},
+                                       {0.075695, 0., -0.0520326},
+                                       {0., 0., -0.00844462}};
+    static const double tip4p_q[4] = {0.0, 0.520, 0.520, -1.040};
+    const int ns = model == 0 ? 3 : 4;
     for (int i = 0; i < nmol; i++) {
         const double *r = cm_ea + 6 * i;
         const double cp = std::cos(r[1]), sp = std::sin(r[1]);
@@ -559,14 +664,15 @@ extern "C" int bsl_water_sites(int32_t model, int32_t nmol, const double *cm_ea,
                                   {(sp * ct * cx) + (cp * sx), (-1 * sp * ct * sx) + (cp * cx), sp * st},
                                   {-1 * st * cx, st * sx, ct}};
         const double cm[3] = {r[0], r[2], r[4]};
-        for (int s = 0; s < 3; s++) {
+        for (int s = 0; s < ns; s++) {
+            const double *body = model == 0 ? spce[s] : tip4p[s];
             for (int d = 0; d < 3; d++) {
                 double acc = 0.;
-                for (int e = 0; e < 3; e++) acc += rot[d][e] * body[s][e];
-                pos[(3 * i + s) * 3 + d] = acc + 0.1 * cm[d];
+                for (int e = 0; e < 3; e++) acc += rot[d][e] * body[e];
+                pos[((size_t)ns * i + s) * 3 + d] = acc + 0.1 * cm[d];
             }
-            q[3 * i + s] = charge[s];
-            lj[3 * i + s] = s == 0 ? 1 : 0;
+            q[(size_t)ns * i + s] = model == 0 ? spce_q[s] : tip4p_q[s];
+            lj[(size_t)ns * i + s] = s == 0 ? 1 : 0;
         }
     }
     return BSL_OK;
@@ -585,19 +691,19 @@ extern "C" int bsl_get_vblocks(bsl_ctx *c, double *out)
     for (int g = 0; g < op.nblk; g++) {
         const Block &B = op.blk[g];
         int64_t pr = B.pair_offset;
-        for (int r = 0; r < B.size; r++)
-            for (int cc = r; cc < B.size; cc++, pr++) {
-                const int n = c->carried[B.plane0 + r], np = c->carried[B.plane0 + cc];
-                for (int k = 0; k < nz; k++)
-                    for (int j = 0; j < ny; j++)
-                        for (int i = 0; i < nx; i++) {
-                            const int64_t p = ((int64_t)k * ny + j) * nx + i;
-                            const int64_t ref = (((int64_t)i * ny + j) * nz + k);
-                            const double v = vpk[(size_t)pr * c->P + p];
-                            out[(ref * nn + n) * nn + np] = v;
-                            out[(ref * nn + np) * nn + n] = v;
-                        }
-            }
+        for (const auto &rc : block_pair_order(B.size)) {
+            const int n = c->carried[B.plane0 + rc.first], np = c->carried[B.plane0 + rc.second];
+            for (int k = 0; k < nz; k++)
+                for (int j = 0; j < ny; j++)
+                    for (int i = 0; i < nx; i++) {
+                        const int64_t p = ((int64_t)k * ny + j) * nx + i;
+                        const int64_t ref = (((int64_t)i * ny + j) * nz + k);
+                        const double v = vpk[(size_t)pr * c->P + p];
+                        out[(ref * nn + n) * nn + np] = v;
+                        out[(ref * nn + np) * nn + n] = v;
+                    }
+            pr++;
+        }
     }
     return BSL_OK;
 }
@@ -674,16 +780,25 @@ extern "C" int bsl_step_bytes(const bsl_ctx *c, double *out)
 {
     if (!c || !out) return fail(BSL_E_ARG, "null argument");
     const DevOperator &op = c->op;
-    const double es = op.vprec == BSL_PREC_FP64 ? 8.0 : 4.0;
     const double vec = 8.0 * (double)c->n_kry;
-    out[0] = es * (double)op.npair * (double)c->P + 3.0 * vec;  // potential kernel
-    out[1] = 6.0 * vec;                                          // kinetic kernels (xy and z)
-    out[2] = 3.0 * vec;                                          // update kernel
-    out[3] = es * (double)op.npair * (double)c->P + 6.0 * vec;   // fused-step minimum
-    double pair_fma = 0.0;
-    for (int g = 0; g < op.nblk; g++) pair_fma += (double)op.blk[g].size * op.blk[g].size;
-    out[4] = 2.0 * (double)c->P * pair_fma + 2.0 * (double)(op.nx + op.ny + op.nz) * (double)c->n_kry +
-             4.0 * (double)c->n_kry;
+    double pot_bytes, pot_flops;
+    if (op.calc == BSL_CALC_QUAD) {
+        const double es = op.wprec == BSL_PREC_FP64 ? 8.0 : 4.0;
+        pot_bytes = es * (double)c->P * (double)c->Q;
+        // two Legendre and two phi transforms plus the pointwise product
+        pot_flops = (double)c->P * (4.0 * op.na * op.nk + 4.0 * op.na * op.nphi * op.nm + op.na * op.nphi);
+    } else {
+        const double es = op.vprec == BSL_PREC_FP64 ? 8.0 : 4.0;
+        pot_bytes = es * (double)op.npair * (double)c->P;
+        double pair_fma = 0.0;
+        for (int g = 0; g < op.nblk; g++) pair_fma += (double)op.blk[g].size * op.blk[g].size;
+        pot_flops = 2.0 * (double)c->P * pair_fma;
+    }
+    out[0] = pot_bytes + 3.0 * vec;  // potential kernel: V + read X, read and write R
+    out[1] = 6.0 * vec;              // kinetic kernels (xy and z): read X, read and write R each
+    out[2] = 3.0 * vec;              // update kernel
+    out[3] = pot_bytes + 6.0 * vec;  // fused-step minimum: V + the six vector passes of the recurrence
+    out[4] = pot_flops + 2.0 * (double)(op.nx + op.ny + op.nz) * (double)c->n_kry + 4.0 * (double)c->n_kry;
     return BSL_OK;
 }
 

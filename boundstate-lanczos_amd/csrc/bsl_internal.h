@@ -6,6 +6,7 @@
 
 #include <cstdint>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "bsl_hip.h"
@@ -55,6 +56,20 @@ int cw_trivec(int niter, const double *alpha, const double *beta, const double *
 // ---------------------------------------------------------------------------
 // device-side launchers (bsl_kernels.hip)
 // ---------------------------------------------------------------------------
+// Storage order of the packed upper triangle of a B x B potential block:
+// row r then row B-1-r, for r = 0 .. (B-1)/2 (rows as columns c >= row).
+inline std::vector<std::pair<int, int>> block_pair_order(int B)
+{
+    std::vector<std::pair<int, int>> out;
+    for (int r = 0; r <= (B - 1) / 2; r++) {
+        const int r2 = B - 1 - r;
+        for (int c = r; c < B; c++) out.emplace_back(r, c);
+        if (r2 != r)
+            for (int c = r2; c < B; c++) out.emplace_back(r2, c);
+    }
+    return out;
+}
+
 // a parity block of the potential: `size` consecutive carried planes from `plane0`
 struct Block {
     int size = 0;
@@ -75,22 +90,34 @@ struct DevOperator {
     Block blk[2];
     int *plane_ref = nullptr;     // reference lm index of each carried plane
     int64_t npair = 0;
-    // reduction scratch
-    double *partial = nullptr;
-    unsigned int *counter = nullptr;
+    int vb_variant = 1;           // occupancy variant of the potential kernel (autotuned)
+    // quadrature calculator (calc == BSL_CALC_QUAD)
+    int calc = 0;
+    void *wgrid = nullptr;        // [P][na][nphi] weighted potential grid
+    int wprec = 0;
+    double *qL = nullptr;         // [nk][na] normalised Legendre of the carried planes
+    double *qS = nullptr;         // [nm][nphi] tesseral trig factors, nphi = 2*nb
+    int *mplane = nullptr;        // m + lmax of each carried plane
+    int na = 0, nphi = 0, nm = 0;
+    // reduction scratch: `partial` holds the nalpha per-workgroup partials of
+    // alpha written by the potential kernel, partial_b those of |r|^2
+    double *partial = nullptr, *partial_b = nullptr;
     int npartial = 0;
+    int64_t nalpha = 0;
 };
 
-// one Hv contribution pipeline: R = sr*R + mask*(H (sx*X)); on return the
-// device scalar alpha = (sx X).R is set when want_alpha.
-int launch_hv(const DevOperator &op, const double *X, double *R, Scal *sc, double *alpha_out, hipStream_t s,
-              hipEvent_t *ev /* 6 events or null */);
-// R -= alpha (sx X); beta2 = |R|^2; updates scalars for the swap
-int launch_update(const DevOperator &op, const double *X, double *R, Scal *sc, double *beta_out,
+// one Hv: R = sr*R + mask*(H (sx*X)); the potential kernel leaves the
+// nalpha partial sums of alpha = (sx X).R in op.partial
+int launch_hv(const DevOperator &op, const double *X, double *R, Scal *sc, hipStream_t s,
+              hipEvent_t *ev /* 3 events or null */);
+// the potential kernel alone (used by the autotuner)
+int launch_potential(const DevOperator &op, const double *X, double *R, Scal *sc, hipStream_t s);
+// alpha from the partials, R -= alpha (sx X), beta^2 = |R|^2 and the scalars of the swap
+int launch_update(const DevOperator &op, const double *X, double *R, Scal *sc, double *alpha_out, double *beta_out,
                   double *beta2_out, hipStream_t s);
 int launch_init_scal(Scal *sc, double sx, double sr, hipStream_t s);
 int launch_start_vector(double *X, int64_t n, int64_t P, const int *plane_ref, int kind, uint64_t seed, Scal *sc,
-                        double *partial, unsigned int *counter, int npartial, hipStream_t s);
+                        double *partial, int npartial, hipStream_t s);
 int launch_zero(double *p, int64_t n, hipStream_t s);
 int launch_ritz_accum(double *ritz, const double *X, const Scal *sc, const double *coef, int64_t n, int ngood,
                       hipStream_t s);
@@ -109,6 +136,10 @@ int launch_vgrid_iho(double *vg, int64_t P, int Q, const double *gx, const doubl
 // Vpk[pair][P] = sum_q Vg[P][q] * H[q][pair]
 int launch_vblock_gemm(const double *vg, const double *h, void *vpk, int vprec, int64_t P, int Q, int64_t npair,
                        hipStream_t s);
+// quadrature calculator (bsl_quad.hip)
+int launch_weight_grid(const double *vg, void *W, int wprec, const double *wt, const double *wp, int64_t P,
+                       int na, int nb, hipStream_t s);
+int launch_quad_potential(const DevOperator &op, const double *X, double *R, Scal *sc, hipStream_t s);
 
 }  // namespace bsl
 

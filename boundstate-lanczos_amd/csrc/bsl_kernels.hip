@@ -42,29 +42,19 @@ __device__ inline double block_sum(double v, double *lds)
     return s;
 }
 
-// Thread 0 publishes `value` as this workgroup's partial; returns true in
-// every thread of the last workgroup to arrive, which then owns the total.
-__device__ inline bool publish_partial(double value, double *partial, unsigned int *counter, unsigned int nblocks,
-                                       unsigned int bid)
+// Fixed-order total of partial[0..n) computed by one workgroup; valid in
+// every thread.  Partials are written with plain stores by the kernel before
+// (kernel boundaries make them visible), so no device-scope fence or atomic
+// is needed in the producers.
+__device__ inline double total_of_partials(const double *__restrict__ partial, unsigned int n, double *lds)
 {
-    __shared__ bool last;
-    if (threadIdx.x == 0) {
-        __hip_atomic_store(&partial[bid], value, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        const unsigned int prev = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        last = (prev == nblocks - 1);
-    }
-    __syncthreads();
-    return last;
-}
-
-// fixed-order total of partial[0..n) by one workgroup; valid in thread 0
-__device__ inline double total_of_partials(const double *partial, unsigned int n, double *lds)
-{
+    __shared__ double tot;
     double s = 0.0;
-    for (unsigned int i = threadIdx.x; i < n; i += blockDim.x)
-        s += __hip_atomic_load(&partial[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    for (unsigned int i = threadIdx.x; i < n; i += blockDim.x) s += partial[i];
+    s = block_sum(s, lds);
+    if (threadIdx.x == 0) tot = s;
     __syncthreads();
-    return block_sum(s, lds);
+    return tot;
 }
 
 // ---------------------------------------------------------------------------
@@ -89,9 +79,29 @@ __global__ __launch_bounds__(kWG) void k_kin_xy(const double *__restrict__ X, do
     const int k = blockIdx.x, n = blockIdx.y;
     const double sx = sc->sx, sr = sc->sr;
     const size_t base = ((size_t)n * nz + k) * (size_t)ny * nx;
-    for (int t = threadIdx.x; t < nyp * ld; t += blockDim.x) {
-        const int j = t / ld, i = t - j * ld;
-        pl[t] = (j < ny && i < nx) ? sx * X[base + (size_t)j * nx + i] : 0.0;
+    // stage the plane: all of this thread's loads are issued before any is
+    // consumed (at most 16 per thread: nx * ny <= 4096)
+    {
+        const double *Xp = X + base;
+        const int np = nx * ny;
+        double tmp[16];
+#pragma unroll
+        for (int e = 0; e < 16; e++) {
+            const int t = threadIdx.x + e * kWG;
+            tmp[e] = t < np ? Xp[t] : 0.0;
+        }
+        for (int t = threadIdx.x; t < nyp * ld; t += blockDim.x) {
+            const int j = t / ld, i = t - j * ld;
+            if (j >= ny || i >= nx) pl[t] = 0.0;
+        }
+#pragma unroll
+        for (int e = 0; e < 16; e++) {
+            const int t = threadIdx.x + e * kWG;
+            if (t < np) {
+                const int j = t / nx, i = t - j * nx;
+                pl[j * ld + i] = sx * tmp[e];
+            }
+        }
     }
     for (int t = threadIdx.x; t < 2 * nxp + 1; t += blockDim.x) {
         const int d = abs(t - nxp);
@@ -179,8 +189,20 @@ __global__ __launch_bounds__(kWG) void k_kin_z(const double *__restrict__ X, dou
     const double sx = sc->sx;
     const size_t base = (size_t)n * nz * nxy;
     const int c = c0 + lane;
-    for (int kk = quarter; kk < nz; kk += 4)
-        col[kk * 64 + lane] = c < nxy ? sx * X[base + (size_t)kk * nxy + c] : 0.0;
+    {
+        double tmp[16];
+        const double *Xc = X + base + c;
+#pragma unroll
+        for (int e = 0; e < 16; e++) {
+            const int kk = quarter + 4 * e;
+            tmp[e] = (kk < nz && c < nxy) ? Xc[(size_t)kk * nxy] : 0.0;
+        }
+#pragma unroll
+        for (int e = 0; e < 16; e++) {
+            const int kk = quarter + 4 * e;
+            if (kk < nz) col[kk * 64 + lane] = sx * tmp[e];
+        }
+    }
     for (int t = threadIdx.x; t < 2 * nzp + 1; t += blockDim.x) {
         const int d = abs(t - nzp);
         rz[t] = d < nz ? tz[d] : 0.0;
@@ -218,33 +240,66 @@ __global__ __launch_bounds__(kWG) void k_kin_z(const double *__restrict__ X, dou
 // ---------------------------------------------------------------------------
 // R[n_r][P] += mask * sum_c V_P[r][c] (sx X[n_c][P]) over the packed upper
 // triangle; then the contribution of these planes to alpha = (sx X).R.
+// One thread per grid point.  The packed upper triangle of a B x B block is
+// stored row pair by row pair: row r (columns r..B-1) followed by row
+// B-1-r (columns B-1-r..B-1), for r = 0 .. (B-1)/2, so that each row pair
+// is B+1 consecutive streams.  The B+1 loads of a row pair are issued back
+// to back through a running pointer before they are consumed, which keeps
+// B+1 loads per wave in flight.
 template <int B, typename VT>
 __device__ inline void vblock_point(const double *__restrict__ X, double *__restrict__ R, const VT *__restrict__ V,
-                                    int plane0, double mask, double sx, int64_t P, int64_t p, double &dot)
+                                    int plane0, double mask, double sx, int64_t P, int p, double &dot)
 {
     double x[B], acc[B];
-    const double *Xb = X + (size_t)plane0 * P + p;
-    double *Rb = R + (size_t)plane0 * P + p;
+    const double *xp = X + (size_t)plane0 * P + p;
 #pragma unroll
     for (int r = 0; r < B; r++) {
-        x[r] = sx * Xb[(size_t)r * P];
+        x[r] = sx * *xp;
+        xp += P;
         acc[r] = 0.0;
     }
-    int q = 0;
+    const VT *vp = V + p;
 #pragma unroll
-    for (int r = 0; r < B; r++) {
+    for (int r = 0; r <= (B - 1) / 2; r++) {
+        const int r2 = B - 1 - r;
+        VT vv[B + 1];
+        int k = 0;
 #pragma unroll
         for (int c = r; c < B; c++) {
-            const double v = (double)V[(size_t)q * P + p];
+            vv[k++] = *vp;
+            vp += P;
+        }
+        if (r2 != r) {
+#pragma unroll
+            for (int c = r2; c < B; c++) {
+                vv[k++] = *vp;
+                vp += P;
+            }
+        }
+        k = 0;
+#pragma unroll
+        for (int c = r; c < B; c++) {
+            const double v = (double)vv[k++];
             acc[r] += v * x[c];
             if (c != r) acc[c] += v * x[r];
-            q++;
+        }
+        if (r2 != r) {
+#pragma unroll
+            for (int c = r2; c < B; c++) {
+                const double v = (double)vv[k++];
+                acc[r2] += v * x[c];
+                if (c != r2) acc[c] += v * x[r2];
+            }
         }
     }
+    double *rp = R + (size_t)plane0 * P + p;
+    double rr[B];
+#pragma unroll
+    for (int r = 0; r < B; r++) rr[r] = rp[(size_t)r * P];
 #pragma unroll
     for (int r = 0; r < B; r++) {
-        const double nr = Rb[(size_t)r * P] + mask * acc[r];
-        Rb[(size_t)r * P] = nr;
+        const double nr = rr[r] + mask * acc[r];
+        rp[(size_t)r * P] = nr;
         dot += x[r] * nr;
     }
 }
@@ -252,16 +307,16 @@ __device__ inline void vblock_point(const double *__restrict__ X, double *__rest
 // The carried planes are ordered block by block: block 0 occupies planes
 // [0, B0), block 1 planes [B0, B0 + B1).  The symmetry mask is uniform
 // within a parity block.
-template <int B0, int B1, typename VT>
-__global__ __launch_bounds__(kWG) void k_vblock(const double *__restrict__ X, double *__restrict__ R,
+template <int B0, int B1, typename VT, int W>
+__global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(W, 8))) void k_vblock(const double *__restrict__ X, double *__restrict__ R,
                                                 const VT *__restrict__ V0, const VT *__restrict__ V1, double mask0,
-                                                double mask1, Scal *__restrict__ sc, int64_t P, double *partial,
-                                                unsigned int *counter, double *alpha_out)
+                                                double mask1, const Scal *__restrict__ sc, int64_t P,
+                                                double *__restrict__ partial)
 {
     __shared__ double red[kWG / 64];
     const double sx = sc->sx;
     double dot = 0.0;
-    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int p = (int)(blockIdx.x * blockDim.x + threadIdx.x);
     if (p < P) {
         if (blockIdx.y == 0)
             vblock_point<B0, VT>(X, R, V0, 0, mask0, sx, P, p, dot);
@@ -269,46 +324,77 @@ __global__ __launch_bounds__(kWG) void k_vblock(const double *__restrict__ X, do
             vblock_point<(B1 > 0 ? B1 : 1), VT>(X, R, V1, B0, mask1, sx, P, p, dot);
     }
     const double s = block_sum(dot, red);
-    const unsigned int nblk = gridDim.x * gridDim.y, bid = blockIdx.y * gridDim.x + blockIdx.x;
-    if (publish_partial(s, partial, counter, nblk, bid)) {
-        const double a = total_of_partials(partial, nblk, red);
-        if (threadIdx.x == 0) {
-            sc->alpha = a;
-            if (alpha_out) *alpha_out = a;
-            *counter = 0u;
-        }
-    }
+    if (threadIdx.x == 0) partial[blockIdx.y * gridDim.x + blockIdx.x] = s;
 }
 
 // ---------------------------------------------------------------------------
-// recurrence update: R -= alpha (sx X), beta^2 = |R|^2, scalars for the swap
+// recurrence update
 // ---------------------------------------------------------------------------
+// alpha = (sx X).R is the fixed-order sum of the potential kernel's partials,
+// recomputed identically by every workgroup; then R -= alpha (sx X) and the
+// partial sums of |R|^2 go to partial_b for k_finalize.
 __global__ __launch_bounds__(kWG) void k_update(const double *__restrict__ X, double *__restrict__ R,
-                                                Scal *__restrict__ sc, int64_t n, double *partial,
-                                                unsigned int *counter, double *beta_out, double *beta2_out)
+                                                Scal *__restrict__ sc, int64_t n,
+                                                const double *__restrict__ partial_a, unsigned int na,
+                                                double *__restrict__ partial_b, double *alpha_out)
 {
     __shared__ double red[kWG / 64];
-    const double a = sc->alpha, sx = sc->sx;
+    const double a = total_of_partials(partial_a, na, red);
+    const double sx = sc->sx;
     double s = 0.0;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i + 3 * stride < n; i += 4 * stride) {
+        double xv[4], rv[4];
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+            xv[e] = X[i + e * stride];
+            rv[e] = R[i + e * stride];
+        }
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+            const double r = rv[e] - a * (sx * xv[e]);
+            R[i + e * stride] = r;
+            s += r * r;
+        }
+    }
+    for (; i < n; i += stride) {
         const double r = R[i] - a * (sx * X[i]);
         R[i] = r;
         s += r * r;
     }
     const double b = block_sum(s, red);
-    if (publish_partial(b, partial, counter, gridDim.x, blockIdx.x)) {
-        const double b2 = total_of_partials(partial, gridDim.x, red);
-        if (threadIdx.x == 0) {
-            const double beta = sqrt(b2);
-            sc->beta2 = b2;
-            sc->beta = beta;
-            if (beta_out) *beta_out = beta;
-            if (beta2_out) *beta2_out = b2;
-            // after the swap: X' = R / beta, R' = -beta * (sx X)
-            sc->sx = 1. / beta;
-            sc->sr = (-beta) * sx;
-            *counter = 0u;
+    if (threadIdx.x == 0) {
+        partial_b[blockIdx.x] = b;
+        if (blockIdx.x == 0) {
+            sc->alpha = a;
+            if (alpha_out) *alpha_out = a;
         }
+    }
+}
+
+// mode 0: beta^2 = sum of partials; set beta and the scalars of the swap
+//         (X' = R / beta, R' = -beta (sx X)).
+// mode 1: start vector norm^2; sx = 1/norm.
+__global__ __launch_bounds__(kWG) void k_finalize(const double *__restrict__ partial, unsigned int n,
+                                                  Scal *__restrict__ sc, int mode, double *beta_out,
+                                                  double *beta2_out)
+{
+    __shared__ double red[kWG / 64];
+    const double t = total_of_partials(partial, n, red);
+    if (threadIdx.x != 0) return;
+    if (mode == 0) {
+        const double beta = sqrt(t);
+        const double sx = sc->sx;
+        sc->beta2 = t;
+        sc->beta = beta;
+        if (beta_out) *beta_out = beta;
+        if (beta2_out) *beta2_out = t;
+        sc->sx = 1. / beta;
+        sc->sr = (-beta) * sx;
+    } else {
+        sc->sx = 1.0 / sqrt(t);
+        sc->sr = 0.0;
     }
 }
 
@@ -342,8 +428,8 @@ __device__ inline double start_value(uint64_t seed, uint64_t idx)
 
 // value of device element i is that of reference element plane_ref[i / P] * P + i % P
 __global__ __launch_bounds__(kWG) void k_start_random(double *X, int64_t n, int64_t P,
-                                                      const int *__restrict__ plane_ref, uint64_t seed, Scal *sc,
-                                                      double *partial, unsigned int *counter)
+                                                      const int *__restrict__ plane_ref, uint64_t seed,
+                                                      double *partial)
 {
     __shared__ double red[kWG / 64];
     double s = 0.0;
@@ -354,14 +440,7 @@ __global__ __launch_bounds__(kWG) void k_start_random(double *X, int64_t n, int6
         s += v * v;
     }
     const double b = block_sum(s, red);
-    if (publish_partial(b, partial, counter, gridDim.x, blockIdx.x)) {
-        const double tot = total_of_partials(partial, gridDim.x, red);
-        if (threadIdx.x == 0) {
-            sc->sx = 1.0 / sqrt(tot);
-            sc->sr = 0.0;
-            *counter = 0u;
-        }
-    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = b;
 }
 
 // ritz[g][i] += coef[g] * (sx X[i]) for the vectors still accumulating
@@ -390,31 +469,41 @@ static int grid_for(int64_t n, int cap = 4096)
 }
 
 template <typename VT>
-static int launch_vblock_t(const DevOperator &op, const double *X, double *R, Scal *sc, double *alpha_out,
-                           hipStream_t s)
+static int launch_vblock_t(const DevOperator &op, const double *X, double *R, Scal *sc, hipStream_t s)
 {
     const VT *V = (const VT *)op.vpk;
     const VT *V0 = V + op.blk[0].pair_offset * op.P;
     const VT *V1 = op.nblk > 1 ? V + op.blk[1].pair_offset * op.P : V0;
     const double mk0 = op.blk[0].mask, mk1 = op.nblk > 1 ? op.blk[1].mask : 0.0;
     dim3 grid((unsigned)((op.P + kWG - 1) / kWG), (unsigned)op.nblk);
-    if ((int64_t)grid.x * grid.y > op.npartial) return fail(BSL_E_STATE, "partial buffer too small");
+    if ((int64_t)grid.x * grid.y > op.npartial || (int64_t)grid.x * grid.y != op.nalpha)
+        return fail(BSL_E_STATE, "partial buffer does not match the potential kernel grid");
     const int b0 = op.blk[0].size, b1 = op.nblk > 1 ? op.blk[1].size : 0;
 #define BSL_VB(A, Bb)                                                                                  \
     if (b0 == A && b1 == Bb) {                                                                         \
-        hipLaunchKernelGGL((k_vblock<A, Bb, VT>), grid, dim3(kWG), 0, s, X, R, V0, V1, mk0, mk1, sc, \
-                           op.P, op.partial, op.counter, alpha_out);                                   \
+        switch (op.vb_variant) {                                                                       \
+        case 0:                                                                                        \
+            hipLaunchKernelGGL((k_vblock<A, Bb, VT, 1>), grid, dim3(kWG), 0, s, X, R, V0, V1, mk0, mk1, sc, \
+                               op.P, op.partial);                                                      \
+            break;                                                                                     \
+        case 1:                                                                                        \
+            hipLaunchKernelGGL((k_vblock<A, Bb, VT, 2>), grid, dim3(kWG), 0, s, X, R, V0, V1, mk0, mk1, sc, \
+                               op.P, op.partial);                                                      \
+            break;                                                                                     \
+        default:                                                                                       \
+            hipLaunchKernelGGL((k_vblock<A, Bb, VT, 3>), grid, dim3(kWG), 0, s, X, R, V0, V1, mk0, mk1, sc, \
+                               op.P, op.partial);                                                      \
+            break;                                                                                     \
+        }                                                                                              \
         return hipGetLastError() == hipSuccess ? BSL_OK : fail(BSL_E_HIP, "k_vblock launch");           \
     }
-    // parity blocks (even l, odd l) for lmax 0..6, and single blocks
     BSL_VB(1, 3) BSL_VB(6, 10) BSL_VB(6, 3) BSL_VB(15, 21) BSL_VB(15, 10) BSL_VB(28, 21)
     BSL_VB(1, 0) BSL_VB(3, 0) BSL_VB(6, 0) BSL_VB(10, 0) BSL_VB(15, 0) BSL_VB(21, 0) BSL_VB(28, 0)
 #undef BSL_VB
     return fail(BSL_E_ARG, "unsupported potential block sizes " + std::to_string(b0) + "/" + std::to_string(b1));
 }
 
-int launch_hv(const DevOperator &op, const double *X, double *R, Scal *sc, double *alpha_out, hipStream_t s,
-              hipEvent_t *ev)
+int launch_hv(const DevOperator &op, const double *X, double *R, Scal *sc, hipStream_t s, hipEvent_t *ev)
 {
     const int nxp = (op.nx + 3) & ~3, nyp = (op.ny + 3) & ~3;
     const size_t lds_xy = sizeof(double) * ((size_t)nyp * (nxp + 2) + 2 * nxp + 1 + 2 * nyp + 1);
@@ -439,21 +528,31 @@ int launch_hv(const DevOperator &op, const double *X, double *R, Scal *sc, doubl
     }
     BSL_HIP(hipGetLastError());
     if (ev) BSL_HIP(hipEventRecord(ev[1], s));
-    int rc = op.vprec == BSL_PREC_FP64 ? launch_vblock_t<double>(op, X, R, sc, alpha_out, s)
-                                       : launch_vblock_t<float>(op, X, R, sc, alpha_out, s);
+    int rc = op.calc == BSL_CALC_QUAD        ? launch_quad_potential(op, X, R, sc, s)
+             : op.vprec == BSL_PREC_FP64 ? launch_vblock_t<double>(op, X, R, sc, s)
+                                         : launch_vblock_t<float>(op, X, R, sc, s);
     if (rc) return rc;
     if (ev) BSL_HIP(hipEventRecord(ev[2], s));
     return BSL_OK;
 }
 
-int launch_update(const DevOperator &op, const double *X, double *R, Scal *sc, double *beta_out,
+int launch_potential(const DevOperator &op, const double *X, double *R, Scal *sc, hipStream_t s)
+{
+    return op.calc == BSL_CALC_QUAD        ? launch_quad_potential(op, X, R, sc, s)
+           : op.vprec == BSL_PREC_FP64 ? launch_vblock_t<double>(op, X, R, sc, s)
+                                       : launch_vblock_t<float>(op, X, R, sc, s);
+}
+
+int launch_update(const DevOperator &op, const double *X, double *R, Scal *sc, double *alpha_out, double *beta_out,
                   double *beta2_out, hipStream_t s)
 {
     const int64_t n = op.P * op.nk;
     int g = grid_for(n, 2048);
     if (g > op.npartial) g = op.npartial;
-    hipLaunchKernelGGL(k_update, dim3(g), dim3(kWG), 0, s, X, R, sc, n, op.partial, op.counter, beta_out,
-                       beta2_out);
+    hipLaunchKernelGGL(k_update, dim3(g), dim3(kWG), 0, s, X, R, sc, n, op.partial, (unsigned)op.nalpha,
+                       op.partial_b, alpha_out);
+    BSL_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kWG), 0, s, op.partial_b, (unsigned)g, sc, 0, beta_out, beta2_out);
     BSL_HIP(hipGetLastError());
     return BSL_OK;
 }
@@ -473,7 +572,7 @@ int launch_zero(double *p, int64_t n, hipStream_t s)
 }
 
 int launch_start_vector(double *X, int64_t n, int64_t P, const int *plane_ref, int kind, uint64_t seed, Scal *sc,
-                        double *partial, unsigned int *counter, int npartial, hipStream_t s)
+                        double *partial, int npartial, hipStream_t s)
 {
     if (kind == BSL_START_UNIFORM) {
         // X = 1, sx = 1/sqrt(N): sx*X is the reference's 1./sqrt((double)N) exactly
@@ -483,7 +582,9 @@ int launch_start_vector(double *X, int64_t n, int64_t P, const int *plane_ref, i
     }
     int g = grid_for(n, 2048);
     if (g > npartial) g = npartial;
-    hipLaunchKernelGGL(k_start_random, dim3(g), dim3(kWG), 0, s, X, n, P, plane_ref, seed, sc, partial, counter);
+    hipLaunchKernelGGL(k_start_random, dim3(g), dim3(kWG), 0, s, X, n, P, plane_ref, seed, partial);
+    BSL_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kWG), 0, s, partial, (unsigned)g, sc, 1, nullptr, nullptr);
     BSL_HIP(hipGetLastError());
     return BSL_OK;
 }

@@ -136,8 +136,9 @@ int bsl_rayleigh(bsl_ctx *ctx, int32_t count, const double *y, double *energy, d
  * the reference's .xyz geometry files (getSPCEatoms,
  * SPCE_AH2_EngRoutines.cpp:5-108; rotMat3D rotmat.cpp:6-32).  cm_ea holds
  * per molecule CMx phi CMy theta CMz chi (Angstrom, radian) in file order.
- * model 0 = SPC/E (O, H, H).  Writes 3*nmol sites: positions (nm),
- * charges (e) and the LJ flag (1 on O). */
+ * model 0 = SPC/E (O, H, H; 3 sites per molecule), 1 = TIP4P (O, H, H, M;
+ * getTIP4Patoms TIP4P_AH2_EngRoutines.cpp:5-117, 4 sites, O uncharged).
+ * Writes the sites' positions (nm), charges (e) and LJ flags (1 on O). */
 int bsl_water_sites(int32_t model, int32_t nmol, const double *cm_ea, double *pos, double *q, int32_t *lj);
 
 /* Diagnostics: V blocks in the reference order [i][j][k][n][n'] (FULL basis

@@ -419,7 +419,17 @@ static void build_vblocks(bso_state *s)
 /* HvPrep_Internal_NoQuad linRotCartSphHvFullVPrepRoutines.cpp:229-528, with
  * QuadraturePrep linRotCartSphHvRoutines.cpp:1480-1532, TesseralPrep
  * :1534-1735 and calcSym linRotCartSphHvFullVPrepRoutines.cpp:530-573 */
-bso_state *bso_prepare(const bso_problem *p)
+static bso_state *prepare(const bso_problem *p, const double *vblk_ext);
+
+bso_state *bso_prepare(const bso_problem *p) { return prepare(p, NULL); }
+
+/* Same operator with the V blocks supplied by the caller (reference order
+ * [i][j][k][n][n']); the potential grid and the contraction are skipped.
+ * Used to time the CPU Hv on operators whose preparation would take hours
+ * on the CPU. */
+bso_state *bso_prepare_external(const bso_problem *p, const double *vblk) { return prepare(p, vblk); }
+
+static bso_state *prepare(const bso_problem *p, const double *vblk_ext)
 {
     if (p->nx <= 0 || p->ny <= 0 || p->nz <= 0 || p->lmax < 0 || p->na <= 0 || p->nb <= 0) return NULL;
     if (p->sym < 0 || p->sym > 2 || p->pot_kind < 0 || p->pot_kind > 1) return NULL;
@@ -494,6 +504,13 @@ bso_state *bso_prepare(const bso_problem *p)
         }
     }
     free(tmp);
+    if (vblk_ext) {
+        s->v1 = dalloc(1);
+        s->v2 = dalloc(1);
+        s->vblk = (double *)malloc((size_t)s->P * s->nn * s->nn * sizeof(double));
+        memcpy(s->vblk, vblk_ext, (size_t)s->P * s->nn * s->nn * sizeof(double));
+        return s;
+    }
     s->v1 = dalloc((size_t)s->P * na * nb);
     s->v2 = dalloc((size_t)s->P * na * nb);
     build_vgrid(s);

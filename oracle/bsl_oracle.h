@@ -87,6 +87,7 @@ typedef struct bso_problem {
 typedef struct bso_state bso_state;
 
 bso_state *bso_prepare(const bso_problem *p);
+bso_state *bso_prepare_external(const bso_problem *p, const double *vblk);
 void       bso_free(bso_state *s);
 long long  bso_basis_size(const bso_state *s);
 int        bso_nn(const bso_state *s);

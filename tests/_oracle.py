@@ -35,6 +35,8 @@ def _load():
     lib = C.CDLL(ORACLE_SO)
     lib.bso_prepare.restype = C.c_void_p
     lib.bso_prepare.argtypes = [C.POINTER(Problem)]
+    lib.bso_prepare_external.restype = C.c_void_p
+    lib.bso_prepare_external.argtypes = [C.POINTER(Problem), _dp]
     lib.bso_free.argtypes = [C.c_void_p]
     lib.bso_basis_size.restype = C.c_longlong
     lib.bso_basis_size.argtypes = [C.c_void_p]
@@ -93,7 +95,7 @@ def build_sites(cm_ea, model=0):
 class Oracle:
     """A prepared oracle operator (bso_prepare)."""
 
-    def __init__(self, spec):
+    def __init__(self, spec, vblocks=None):
         self.spec = dict(spec)
         p = Problem()
         for f in ("nx", "ny", "nz", "lmax", "na", "nb", "sym", "pot_kind"):
@@ -115,7 +117,10 @@ class Oracle:
             for f in ("q_h2_h", "q_h2_cm", "h2_half_bond", "lj_a", "lj_b"):
                 setattr(p, f, float(spec[f]))
         self._p = p
-        self.h = lib.bso_prepare(C.byref(p))
+        if vblocks is None:
+            self.h = lib.bso_prepare(C.byref(p))
+        else:
+            self.h = lib.bso_prepare_external(C.byref(p), np.ascontiguousarray(vblocks, dtype=np.float64).ravel())
         if not self.h:
             raise ValueError("bso_prepare rejected the problem")
         self.N = lib.bso_basis_size(self.h)

@@ -1,0 +1,113 @@
+"""The standalone driver (lib/bsl_lanczos): the reference's command line,
+decks and output files over the HIP calculators.  Error paths run on the CPU
+(they stop before any device work); full runs are GPU tests."""
+import json
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+import _cases as cs
+
+DRIVER = os.path.join(os.path.dirname(cs.HERE), "boundstate-lanczos_amd", "lib", "bsl_lanczos")
+
+
+def write_deck(path, values):
+    with open(path, "w") as f:
+        f.write("#\n#\n")
+        for i, v in enumerate(values):
+            f.write("name%d= %s\n" % (i, v))
+
+
+def lanczos_deck(tmp, niter=300, calc="linRotCartSph_HIP_NoQuad_ContGrid_Alavi_SPCE", word="partial", nvec=2):
+    p = os.path.join(tmp, "lanczos.txt")
+    write_deck(p, [niter, -10, 10, word, nvec, calc, tmp + "/"])
+    return p
+
+
+def hv_deck(tmp, nx=10, lmax=3, na=10, nb=10, sym="All", xmax=0.4, pxmax=0.48):
+    shutil.copyfile(os.path.join(cs.GOLDEN, "sII_smallCage11_CM_EA_ZYZ.xyz"), os.path.join(tmp, "cage.xyz"))
+    p = os.path.join(tmp, "hv.txt")
+    write_deck(p, [nx, xmax, nx, xmax, nx, xmax, lmax, na, nb, 793, pxmax, 793, pxmax, 793, pxmax, 2.01588,
+                   0.7096487130193394, "cage.xyz", "DriverTest", 18, "FALSE", sym])
+    return p
+
+
+def run(args, cwd):
+    return subprocess.run([DRIVER] + args, cwd=cwd, capture_output=True, text=True, timeout=600)
+
+
+pytestmark = pytest.mark.skipif(not os.path.exists(DRIVER), reason="driver not built")
+
+
+def test_unknown_calculator_rejected(tmp_path):
+    t = str(tmp_path)
+    r = run([lanczos_deck(t, calc="linRotCartSph_NoQuad_Alavi_SPCE"), hv_deck(t)], t)
+    assert r.returncode == 1 and "not recognized" in r.stderr
+
+
+def test_bad_eigenvector_word_rejected(tmp_path):
+    t = str(tmp_path)
+    r = run([lanczos_deck(t, word="some"), hv_deck(t)], t)
+    assert r.returncode == 1 and "NumberOfEigenvectors" in r.stderr
+
+
+def test_bad_symmetry_flag_rejected(tmp_path):
+    t = str(tmp_path)
+    r = run([lanczos_deck(t), hv_deck(t, sym="Both")], t)
+    assert r.returncode == 1 and "Symmeterizer" in r.stderr
+
+
+def test_box_larger_than_potential_universe_rejected(tmp_path):
+    t = str(tmp_path)
+    r = run([lanczos_deck(t), hv_deck(t, xmax=0.6, pxmax=0.48)], t)
+    assert r.returncode == 1 and "potential universe" in r.stderr
+
+
+def test_missing_deck_rejected(tmp_path):
+    t = str(tmp_path)
+    r = run([os.path.join(t, "nope.txt"), hv_deck(t)], t)
+    assert r.returncode == 1 and "could not be opened" in r.stderr
+
+
+def _read_boundstates(d):
+    rows = []
+    with open(os.path.join(d, "boundstates.txt")) as f:
+        lines = f.read().split("\n")
+    assert lines[0] == ""
+    assert lines[1].startswith("The following are the eigenvalues and errors for the simulation DriverTest")
+    for line in lines[4:]:
+        t = line.split()
+        if len(t) == 3:
+            rows.append([float(t[1]), float(t[2])])
+    return np.array(rows)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("path", ["device", "host"])
+def test_driver_reproduces_saved_run(tmp_path, path):
+    """The reference's saved run Compilation/SmallBasisTest2_2013-09-03_19_06_47
+    through the drop-in: same decks, same output files, same converged levels."""
+    t = str(tmp_path)
+    env = dict(os.environ)
+    if path == "host":
+        env["BSL_HV_PATH"] = "host"
+    r = subprocess.run([DRIVER, lanczos_deck(t), hv_deck(t)], cwd=t, capture_output=True, text=True, timeout=600,
+                       env=env)
+    assert r.returncode == 0, r.stderr[-2000:]
+    out = [d for d in os.listdir(t) if d.startswith("DriverTest_")]
+    assert len(out) == 1
+    d = os.path.join(t, out[0])
+    for f in ("boundstates.txt", "states_zpe.txt", "lv", "eigVecFile.txt", "lanczos.txt", "hv.txt"):
+        assert os.path.exists(os.path.join(d, f)), f
+    got = _read_boundstates(d)
+    run_fx = json.load(open(os.path.join(cs.GOLDEN, "runs.json")))["SmallBasisTest2_2013-09-03_19_06_47"]
+    ref = np.array(run_fx["boundstates"])
+    for e in ref[:10, 0]:
+        assert np.min(np.abs(got[:, 0] - e)) / abs(e) < 1e-9
+    # Ritz vectors: two columns, unit norm for the converged ground state
+    vec = np.loadtxt(os.path.join(d, "eigVecFile.txt"), skiprows=4)
+    assert vec.shape == (16000, 3)
+    assert abs(np.linalg.norm(vec[:, 1]) - 1.0) < 1e-6

@@ -38,14 +38,16 @@ def make_calc(B, args, precision, basis="full", symmetry="All"):
                         precision=precision, potential="cage", sites=(pos, q, lj), lj_a=lj_a, lj_b=lj_b)
 
 
-def traffic_from_profiles(tag):
-    """HBM bytes per launch of the potential kernel measured with rocprofv3
-    --pmc (FETCH_SIZE + WRITE_SIZE), committed under profiles/."""
+def traffic_from_profiles(tag, kernel):
+    """HBM bytes per launch of `kernel` measured with two rocprofv3 --pmc
+    runs (FETCH_SIZE, then WRITE_SIZE, each scaled by what the counter reads
+    for a plain copy kernel of known size), committed under profiles/."""
     path = os.path.join(REPO, "profiles", "traffic_%s.json" % tag)
     if not os.path.exists(path):
         return None
     with open(path) as f:
-        return json.load(f).get("bytes_per_launch")
+        k = json.load(f)["kernels"].get("void bsl::" + kernel)
+    return None if k is None else k["bytes_per_launch"]
 
 
 def cpu_baseline(B, calc, args):
@@ -98,12 +100,25 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-eig", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the FP32-potential context")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain copy kernel over 1 GiB of doubles three times (the reference point "
+                         "for the HBM byte counters of rocprofv3 --pmc) and exit")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: every box in this setup has one GPU, and one Lanczos "
               "recurrence per GPU is the unit of work (independent recurrences shard with no "
               "communication). Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
+    if args.copy_calibration:
+        import torch
+        a = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            torch.mul(a, 1.0, out=b)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_kernel_bytes_read": a.numel() * 8, "copy_kernel_bytes_written": b.numel() * 8}))
+        return
     import bsl_amd as B
 
     t0 = time.perf_counter()
@@ -114,20 +129,23 @@ def main():
     steps_s = args.steps / (ms[0] / 1e3)
     pot_ms = ms[1]
     achieved = sb[0] / (pot_ms * 1e-3) / 1e9
+    kname = c64.potential_kernel()
     print("fp64: prep %.2f s, %.1f steps/s, per step %.3f ms (potential %.3f, kinetic %.3f, update %.3f)"
           % (prep64, steps_s, ms[0] / args.steps, ms[1], ms[2], ms[3]), file=sys.stderr, flush=True)
 
-    t0 = time.perf_counter()
-    c32 = make_calc(B, args, "fp32")
-    prep32 = time.perf_counter() - t0
-    ms32 = c32.time_steps(args.warmup, args.steps)
-    steps_s32 = args.steps / (ms32[0] / 1e3)
-    sb32 = c32.step_bytes()
-    print("fp32: prep %.2f s, %.1f steps/s, per step %.3f ms (potential %.3f, kinetic %.3f, update %.3f)"
-          % (prep32, steps_s32, ms32[0] / args.steps, ms32[1], ms32[2], ms32[3]), file=sys.stderr, flush=True)
+    c32, steps_s32, prep32 = None, None, None
+    if not args.no_mixed:
+        t0 = time.perf_counter()
+        c32 = make_calc(B, args, "fp32")
+        prep32 = time.perf_counter() - t0
+        ms32 = c32.time_steps(args.warmup, args.steps)
+        steps_s32 = args.steps / (ms32[0] / 1e3)
+        print("fp32: prep %.2f s, %.1f steps/s, per step %.3f ms (potential %.3f, kinetic %.3f, update %.3f)"
+              % (prep32, steps_s32, ms32[0] / args.steps, ms32[1], ms32[2], ms32[3]), file=sys.stderr,
+              flush=True)
 
     eig = None
-    if not args.no_eig:
+    if not args.no_eig and c32 is not None:
         err, nlev, low = eig_error(B, c64, c32, args.eig_steps)
         eig = {"max_err_cm1": err, "levels": nlev, "steps": args.eig_steps, "lowest_kjmol": low}
         print("mixed vs fp64 lowest %d levels: max err %.2e cm^-1" % (nlev, err), file=sys.stderr, flush=True)
@@ -158,7 +176,7 @@ def main():
                         "recurrence" % (args.n, args.x_max),
             "basis_size": int(c64.N),
             "mixed_fp32_steps_per_s": steps_s32,
-            "mixed_vs_fp64": steps_s32 / steps_s,
+            "mixed_vs_fp64": steps_s32 / steps_s if steps_s32 else None,
             "eig_err_vs_fp64": eig,
             "prep_s": {"fp64": prep64, "fp32": prep32},
             "per_step_ms": {"potential": ms[1], "kinetic": ms[2], "update": ms[3]},
@@ -167,12 +185,12 @@ def main():
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_vblock (potential blocks, FP64)",
+            "kernel": kname,
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": traffic_from_profiles("vblock_fp64_54"),
+            "traffic": traffic_from_profiles("cage54_lmax5_fp64", kname),
             "algorithmic_bytes_per_launch": sb[0],
         },
         "cpu_baseline": cpu,

@@ -129,6 +129,12 @@ class Calculator:
         check(lib().bsl_step_bytes(self.h, out))
         return out
 
+    def potential_kernel(self):
+        """Name of the potential kernel (and autotuned variant) in use."""
+        buf = ctypes.create_string_buffer(128)
+        check(lib().bsl_potential_kernel(self.h, buf, 128))
+        return buf.value.decode()
+
 
 def lancbis(alpha, beta, beta2, emin, emax):
     """Good eigenvalues and error estimates (lancbis.cpp:5-105)."""

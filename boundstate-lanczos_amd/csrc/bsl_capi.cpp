@@ -802,6 +802,20 @@ extern "C" int bsl_step_bytes(const bsl_ctx *c, double *out)
     return BSL_OK;
 }
 
+extern "C" int bsl_potential_kernel(const bsl_ctx *c, char *name, int32_t len)
+{
+    if (!c || !name || len <= 0) return fail(BSL_E_ARG, "bsl_potential_kernel: null argument");
+    const DevOperator &op = c->op;
+    char buf[128];
+    if (op.calc == BSL_CALC_QUAD)
+        std::snprintf(buf, sizeof buf, "k_quad_fp64<%s>", op.wprec == BSL_PREC_FP64 ? "double" : "float");
+    else
+        std::snprintf(buf, sizeof buf, "k_vblock<%d, %d, %s, %d>", op.blk[0].size, op.nblk > 1 ? op.blk[1].size : 0,
+                      op.vprec == BSL_PREC_FP64 ? "double" : "float", op.vb_variant + 1);
+    std::snprintf(name, (size_t)len, "%s", buf);
+    return BSL_OK;
+}
+
 extern "C" int bsl_time_steps(bsl_ctx *c, int32_t warmup, int32_t steps, double *ms)
 {
     if (!c || steps <= 0 || warmup < 0 || !ms) return fail(BSL_E_ARG, "bad arguments");

@@ -160,6 +160,12 @@ int bsl_time_steps(bsl_ctx *ctx, int32_t warmup, int32_t steps, double *ms);
  * update, out[3] total; out[4] FLOPs of one Hv. */
 int bsl_step_bytes(const bsl_ctx *ctx, double *out);
 
+// Name (with template arguments, as a profiler prints it) of the potential
+// kernel the context launches, e.g. "k_vblock<15, 21, double, 1>"; the last
+// argument is the occupancy variant picked by the autotuner at bsl_create.
+// No reference counterpart (measurement support).
+int bsl_potential_kernel(const bsl_ctx *ctx, char *name, int32_t len);
+
 #ifdef __cplusplus
 }
 #endif

@@ -29,13 +29,39 @@ sys.path.insert(0, os.path.join(REPO, "boundstate-lanczos_amd"))
 HBM_PEAK_GBS = 8000.0
 
 
-def make_calc(B, args, precision, basis="full", symmetry="All"):
+def make_calc(B, args, precision, basis="full", symmetry="All", calculator="noquad"):
     from bsl_amd import cage, physics
     pos, q, lj = cage.synthetic_cage(0)
     lj_a, lj_b = cage.baseline_lj()
     return B.Calculator(nx=args.n, x_max=args.x_max, lmax=5, n_theta=40, n_phi=70, mass=2.016,
                         rot_const=physics.cm1_to_kjmol(59.322), ceiling=18.0, symmetry=symmetry, basis=basis,
-                        precision=precision, potential="cage", sites=(pos, q, lj), lj_a=lj_a, lj_b=lj_b)
+                        precision=precision, potential="cage", sites=(pos, q, lj), lj_a=lj_a, lj_b=lj_b,
+                        calculator=calculator)
+
+
+def quad_leg(B, args):
+    """On-the-fly quadrature calculator at the same sizes: FP64, FP32 and
+    BF16x3 MFMA transforms over the resident weighted V grid."""
+    out = {}
+    for prec in ("fp64", "fp32", "bf16x3"):
+        t0 = time.perf_counter()
+        c = make_calc(B, args, prec, calculator="quad")
+        prep = time.perf_counter() - t0
+        ms = c.time_steps(args.warmup, args.quad_steps)
+        sb = c.step_bytes()
+        sps = args.quad_steps / (ms[0] / 1e3)
+        ach = sb[0] / (ms[1] * 1e-3) / 1e9
+        out[prec] = {"steps_per_s": sps, "prep_s": prep, "potential_ms": ms[1], "kinetic_ms": ms[2],
+                     "update_ms": ms[3], "kernel": c.potential_kernel(), "algorithmic_bytes_per_launch": sb[0],
+                     "achieved_GBs": ach, "hbm_frac": ach / HBM_PEAK_GBS,
+                     "flops_per_launch": sb[4] - 2.0 * 3 * args.n * c.N - 4.0 * c.N,
+                     "tflops": (sb[4] - 2.0 * 3 * args.n * c.N - 4.0 * c.N) / (ms[1] * 1e-3) / 1e12}
+        print("quad %s: prep %.2f s, %.1f steps/s, potential %.3f ms (%.0f GB/s)"
+              % (prec, prep, sps, ms[1], ach), file=sys.stderr, flush=True)
+        del c
+    out["fp32_vs_fp64"] = out["fp32"]["steps_per_s"] / out["fp64"]["steps_per_s"]
+    out["bf16x3_vs_fp64"] = out["bf16x3"]["steps_per_s"] / out["fp64"]["steps_per_s"]
+    return out
 
 
 def traffic_from_profiles(tag, kernel):
@@ -101,6 +127,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-eig", action="store_true")
     ap.add_argument("--no-mixed", action="store_true", help="skip the FP32-potential context")
+    ap.add_argument("--quad-steps", type=int, default=200,
+                    help="steps timed per quadrature-calculator precision (0 skips the quadrature leg)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain copy kernel over 1 GiB of doubles three times (the reference point "
                          "for the HBM byte counters of rocprofv3 --pmc) and exit")
@@ -151,6 +179,8 @@ def main():
         print("mixed vs fp64 lowest %d levels: max err %.2e cm^-1" % (nlev, err), file=sys.stderr, flush=True)
     del c32
 
+    quad = quad_leg(B, args) if args.quad_steps > 0 else None
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(B, c64, args)
@@ -180,6 +210,7 @@ def main():
             "eig_err_vs_fp64": eig,
             "prep_s": {"fp64": prep64, "fp32": prep32},
             "per_step_ms": {"potential": ms[1], "kinetic": ms[2], "update": ms[3]},
+            "quadrature": quad,
             "algorithmic_bytes_per_step": sb[3],
             "step_bytes_roofline_frac": sb[3] / (ms[0] / args.steps * 1e-3) / 1e9 / HBM_PEAK_GBS,
         },

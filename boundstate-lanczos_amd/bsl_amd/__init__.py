@@ -131,7 +131,7 @@ class Calculator:
 
     def potential_kernel(self):
         """Name of the potential kernel (and autotuned variant) in use."""
-        buf = ctypes.create_string_buffer(128)
+        buf = C.create_string_buffer(128)
         check(lib().bsl_potential_kernel(self.h, buf, 128))
         return buf.value.decode()
 

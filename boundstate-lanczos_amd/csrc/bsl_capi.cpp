@@ -71,6 +71,8 @@ int check_params(const bsl_params &p)
     if (p.basis < 0 || p.basis > 1) return fail(BSL_E_ARG, "basis flag not recognised");
     if (p.calculator < 0 || p.calculator > 1) return fail(BSL_E_ARG, "calculator flag not recognised");
     if (p.precision < 0 || p.precision > 2) return fail(BSL_E_ARG, "precision flag not recognised");
+    if (p.precision == BSL_PREC_BF16X3 && p.calculator != BSL_CALC_QUAD)
+        return fail(BSL_E_ARG, "BF16x3 split products apply to the quadrature calculator's MFMA transforms only");
     if (p.potential < 0 || p.potential > 1) return fail(BSL_E_ARG, "potential flag not recognised");
     if (p.potential == BSL_POT_CAGE && (p.n_sites <= 0 || !p.site_pos || !p.site_q || !p.site_lj))
         return fail(BSL_E_ARG, "cage potential needs sites");
@@ -171,6 +173,7 @@ extern "C" int bsl_destroy(bsl_ctx *c)
     dev_free(o.qL);
     dev_free(o.qS);
     dev_free(o.mplane);
+    dev_free(o.q_rl);
     dev_free(o.partial);
     dev_free(o.partial_b);
     dev_free(c->d_gx);
@@ -340,15 +343,18 @@ extern "C" int bsl_create(const bsl_params *pp, bsl_ctx **out)
         // weighted grid W[P][a][half*nb + b] and the transform tables
         op.calc = BSL_CALC_QUAD;
         op.wprec = p.precision == BSL_PREC_FP64 ? BSL_PREC_FP64 : BSL_PREC_FP32;
+        op.qmode = p.precision;
         op.na = na;
         op.nphi = 2 * nb;
         op.nm = 2 * lmax + 1;
+        op.q_npts = quad_points_per_group(na, nk);
         const size_t es = op.wprec == BSL_PREC_FP64 ? sizeof(double) : sizeof(float);
-        BSL_HIP(hipMalloc(&op.wgrid, es * (size_t)c->P * c->Q));
+        const int64_t ntiles = quad_tiles(c->P, na, op.q_npts);
+        BSL_HIP(hipMalloc(&op.wgrid, es * (size_t)ntiles * quad_phi_tiles(op.nphi) * 256));
         double *d_wt = nullptr, *d_wp = nullptr;
         if ((rc = dev_upload(&d_wt, t.wt))) return rc;
         if ((rc = dev_upload(&d_wp, t.wp))) return rc;
-        rc = launch_weight_grid(c->d_vgrid, op.wgrid, op.wprec, d_wt, d_wp, c->P, na, nb, c->stream);
+        rc = launch_weight_grid(c->d_vgrid, op.wgrid, op.wprec, d_wt, d_wp, c->P, na, nb, op.q_npts, c->stream);
         BSL_HIP(hipStreamSynchronize(c->stream));
         dev_free(d_wt);
         dev_free(d_wp);
@@ -368,7 +374,14 @@ extern "C" int bsl_create(const bsl_params *pp, bsl_ctx **out)
         if ((rc = dev_upload(&op.qL, qL))) return rc;
         if ((rc = dev_upload(&op.qS, qS))) return rc;
         if ((rc = dev_upload(&op.mplane, mpl))) return rc;
-        op.nalpha = (c->P + 7) / 8;
+        std::vector<int> qrl(16 * 9, 0);
+        for (int r = 0; r < nk; r++) {
+            int &cnt = qrl[(size_t)mpl[r] * 9];
+            if (cnt >= 8) return fail(BSL_E_ARG, "quadrature kernel: more than 8 carried planes share one m");
+            qrl[(size_t)mpl[r] * 9 + 1 + cnt++] = r;
+        }
+        if ((rc = dev_upload(&op.q_rl, qrl))) return rc;
+        op.nalpha = (c->P + op.q_npts - 1) / op.q_npts;
         op.npartial = std::max<int>(op.npartial, (int)op.nalpha + 1);
         dev_free(op.partial);
         dev_free(op.partial_b);
@@ -808,7 +821,7 @@ extern "C" int bsl_potential_kernel(const bsl_ctx *c, char *name, int32_t len)
     const DevOperator &op = c->op;
     char buf[128];
     if (op.calc == BSL_CALC_QUAD)
-        std::snprintf(buf, sizeof buf, "k_quad_fp64<%s>", op.wprec == BSL_PREC_FP64 ? "double" : "float");
+        std::snprintf(buf, sizeof buf, "k_quad_mfma<%d>", op.qmode);
     else
         std::snprintf(buf, sizeof buf, "k_vblock<%d, %d, %s, %d>", op.blk[0].size, op.nblk > 1 ? op.blk[1].size : 0,
                       op.vprec == BSL_PREC_FP64 ? "double" : "float", op.vb_variant + 1);

@@ -93,12 +93,15 @@ struct DevOperator {
     int vb_variant = 1;           // occupancy variant of the potential kernel (autotuned)
     // quadrature calculator (calc == BSL_CALC_QUAD)
     int calc = 0;
-    void *wgrid = nullptr;        // [P][na][nphi] weighted potential grid
+    void *wgrid = nullptr;        // weighted potential grid in MFMA accumulator layout
     int wprec = 0;
     double *qL = nullptr;         // [nk][na] normalised Legendre of the carried planes
     double *qS = nullptr;         // [nm][nphi] tesseral trig factors, nphi = 2*nb
     int *mplane = nullptr;        // m + lmax of each carried plane
     int na = 0, nphi = 0, nm = 0;
+    int qmode = 0;                // 0 FP64 MFMA, 1 FP32 MFMA, 2 BF16x3 MFMA
+    int q_npts = 0;               // grid points per workgroup of the quadrature kernel
+    int *q_rl = nullptr;          // [16][9]: per m (+lmax) the count, then the carried planes
     // reduction scratch: `partial` holds the nalpha per-workgroup partials of
     // alpha written by the potential kernel, partial_b those of |r|^2
     double *partial = nullptr, *partial_b = nullptr;
@@ -137,8 +140,13 @@ int launch_vgrid_iho(double *vg, int64_t P, int Q, const double *gx, const doubl
 int launch_vblock_gemm(const double *vg, const double *h, void *vpk, int vprec, int64_t P, int Q, int64_t npair,
                        hipStream_t s);
 // quadrature calculator (bsl_quad.hip)
+// weighted quadrature grid in the MFMA accumulator layout of the quadrature
+// kernel (npts grid points per workgroup)
 int launch_weight_grid(const double *vg, void *W, int wprec, const double *wt, const double *wp, int64_t P,
-                       int na, int nb, hipStream_t s);
+                       int na, int nb, int npts, hipStream_t s);
+int quad_points_per_group(int na, int nk);
+int64_t quad_tiles(int64_t P, int na, int npts);
+int quad_phi_tiles(int nphi);
 int launch_quad_potential(const DevOperator &op, const double *X, double *R, Scal *sc, hipStream_t s);
 
 }  // namespace bsl

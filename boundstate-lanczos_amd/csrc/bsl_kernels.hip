@@ -49,8 +49,19 @@ __device__ inline double block_sum(double v, double *lds)
 __device__ inline double total_of_partials(const double *__restrict__ partial, unsigned int n, double *lds)
 {
     __shared__ double tot;
-    double s = 0.0;
-    for (unsigned int i = threadIdx.x; i < n; i += blockDim.x) s += partial[i];
+    // four independent chains per thread (the loads of a long list overlap),
+    // combined in a fixed order
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    const unsigned int st = blockDim.x;
+    unsigned int i = threadIdx.x;
+    for (; i + 3 * st < n; i += 4 * st) {
+        s0 += partial[i];
+        s1 += partial[i + st];
+        s2 += partial[i + 2 * st];
+        s3 += partial[i + 3 * st];
+    }
+    for (; i < n; i += st) s0 += partial[i];
+    double s = (s0 + s1) + (s2 + s3);
     s = block_sum(s, lds);
     if (threadIdx.x == 0) tot = s;
     __syncthreads();
@@ -331,15 +342,17 @@ __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(W, 8))) voi
 // recurrence update
 // ---------------------------------------------------------------------------
 // alpha = (sx X).R is the fixed-order sum of the potential kernel's partials,
-// recomputed identically by every workgroup; then R -= alpha (sx X) and the
-// partial sums of |R|^2 go to partial_b for k_finalize.
+// recomputed identically by every workgroup (na > 0), or taken from
+// sc->alpha when k_finalize has summed a long partial list before (na == 0);
+// then R -= alpha (sx X) and the partial sums of |R|^2 go to partial_b for
+// k_finalize.
 __global__ __launch_bounds__(kWG) void k_update(const double *__restrict__ X, double *__restrict__ R,
                                                 Scal *__restrict__ sc, int64_t n,
                                                 const double *__restrict__ partial_a, unsigned int na,
                                                 double *__restrict__ partial_b, double *alpha_out)
 {
     __shared__ double red[kWG / 64];
-    const double a = total_of_partials(partial_a, na, red);
+    const double a = na ? total_of_partials(partial_a, na, red) : sc->alpha;
     const double sx = sc->sx;
     double s = 0.0;
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
@@ -376,6 +389,7 @@ __global__ __launch_bounds__(kWG) void k_update(const double *__restrict__ X, do
 // mode 0: beta^2 = sum of partials; set beta and the scalars of the swap
 //         (X' = R / beta, R' = -beta (sx X)).
 // mode 1: start vector norm^2; sx = 1/norm.
+// mode 2: alpha = sum of partials.
 __global__ __launch_bounds__(kWG) void k_finalize(const double *__restrict__ partial, unsigned int n,
                                                   Scal *__restrict__ sc, int mode, double *beta_out,
                                                   double *beta2_out)
@@ -392,9 +406,11 @@ __global__ __launch_bounds__(kWG) void k_finalize(const double *__restrict__ par
         if (beta2_out) *beta2_out = t;
         sc->sx = 1. / beta;
         sc->sr = (-beta) * sx;
-    } else {
+    } else if (mode == 1) {
         sc->sx = 1.0 / sqrt(t);
         sc->sr = 0.0;
+    } else {
+        sc->alpha = t;
     }
 }
 
@@ -549,8 +565,15 @@ int launch_update(const DevOperator &op, const double *X, double *R, Scal *sc, d
     const int64_t n = op.P * op.nk;
     int g = grid_for(n, 2048);
     if (g > op.npartial) g = op.npartial;
-    hipLaunchKernelGGL(k_update, dim3(g), dim3(kWG), 0, s, X, R, sc, n, op.partial, (unsigned)op.nalpha,
-                       op.partial_b, alpha_out);
+    // a long list of alpha partials (the quadrature kernel's) is summed once
+    // instead of by every workgroup of k_update
+    unsigned na = (unsigned)op.nalpha;
+    if (na > 4096) {
+        hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kWG), 0, s, op.partial, na, sc, 2, nullptr, nullptr);
+        BSL_HIP(hipGetLastError());
+        na = 0;
+    }
+    hipLaunchKernelGGL(k_update, dim3(g), dim3(kWG), 0, s, X, R, sc, n, op.partial, na, op.partial_b, alpha_out);
     BSL_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kWG), 0, s, op.partial_b, (unsigned)g, sc, 0, beta_out, beta2_out);
     BSL_HIP(hipGetLastError());

@@ -11,27 +11,72 @@
 // are corrected here).  The weighted potential grid W = w_a w_phi V stays
 // resident in HBM and is streamed once per Hv.
 //
-// FP64: one workgroup walks a run of grid points, with the transform tables
-// in LDS.  FP32: the two phi transforms are batched GEMMs over the points of
-// a tile on MFMA (v_mfma_f32_32x32x2f32), with the pointwise multiply fused
-// between them.
+// The two phi transforms are GEMMs batched over columns (P, a): with
+// columns as N and phi as M (forward) or K (backward), on MFMA 16x16x4
+// (FP64 or FP32, FP32 accumulation for FP32) or 16x16x16 BF16 (three split
+// products hi*hi + hi*lo + lo*hi of FP32 operands, FP32 accumulation).  The
+// forward accumulator tile (16 phi x 16 columns) is multiplied by W in
+// registers and fed straight back as the B operand of the backward MFMA, so
+// the (a, phi) intermediate never leaves the registers.  W is stored in that
+// accumulator layout: [column tile][phi tile][lane][4], one contiguous
+// 16-byte (FP32) or 32-byte (FP64) load per lane.  Accumulator rows of lane
+// j (item i): 4*(j/16) + i for the FP32 and BF16 instructions, j/16 + 4*i for
+// the FP64 one.
 #include <hip/hip_runtime.h>
+
+#include <type_traits>
 
 #include "bsl_internal.h"
 
 namespace bsl {
 
-constexpr int kQWG = 256;
+constexpr int kQWG = 256;  // 4 waves
 
-__device__ inline double q_wave_sum(double v)
+typedef float f4v __attribute__((ext_vector_type(4)));
+typedef double d4v __attribute__((ext_vector_type(4)));
+typedef short s4v __attribute__((ext_vector_type(4)));
+
+template <typename T>
+struct AccT;
+template <>
+struct AccT<float> {
+    typedef f4v type;
+};
+template <>
+struct AccT<double> {
+    typedef d4v type;
+};
+
+__device__ inline f4v mfma16(float a, float b, f4v c)
 {
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-    return v;
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+__device__ inline d4v mfma16(double a, double b, d4v c)
+{
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+// bfloat16 split of a float: x ~ hi + lo, both round-to-nearest-even
+__device__ inline short bf16_rne(float x)
+{
+    unsigned u = __float_as_uint(x);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (short)(u >> 16);
+}
+__device__ inline float bf16_to_f(short h) { return __uint_as_float(((unsigned)(unsigned short)h) << 16); }
+__device__ inline void bf16_split(float x, short &hi, short &lo)
+{
+    hi = bf16_rne(x);
+    lo = bf16_rne(x - bf16_to_f(hi));
+}
+__device__ inline f4v mfma_bf16(s4v a, s4v b, f4v c)
+{
+    return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
 }
 
 __device__ inline double q_block_sum(double v, double *lds)
 {
-    v = q_wave_sum(v);
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     if (lane == 0) lds[wid] = v;
     __syncthreads();
@@ -42,120 +87,435 @@ __device__ inline double q_block_sum(double v, double *lds)
     return s;
 }
 
-// W: [P][na][nphi] (weights folded in), L: [nk][na], S: [nm][nphi],
-// mplane[r]: m + lmax of carried plane r.
-template <typename WT>
-__global__ __launch_bounds__(kQWG) void k_quad_fp64(const double *__restrict__ X, double *__restrict__ R,
-                                                    const WT *__restrict__ W, const double *__restrict__ Lg,
-                                                    const double *__restrict__ Sg, const int *__restrict__ mplane,
-                                                    const double *__restrict__ mask, const Scal *__restrict__ sc,
-                                                    int64_t P, int nk, int na, int nphi, int nm, int ppw,
-                                                    double *__restrict__ partial)
+// accumulator row of item i in lane group kq = lane/16
+template <bool F64>
+__host__ __device__ inline int acc_row(int kq, int i)
 {
-    extern __shared__ double lds[];
-    double *Ls = lds;                      // nk * na
-    double *Ss = Ls + nk * na;             // nm * nphi
-    double *c = Ss + nm * nphi;            // nk
-    double *f = c + nk;                    // na * nm  (forward, then backward)
-    double *h = f + na * nm;               // na * nphi
-    __shared__ double red[kQWG / 64];
-    __shared__ int ms[64];
-    for (int t = threadIdx.x; t < nk * na; t += blockDim.x) Ls[t] = Lg[t];
-    for (int t = threadIdx.x; t < nm * nphi; t += blockDim.x) Ss[t] = Sg[t];
-    for (int t = threadIdx.x; t < nk; t += blockDim.x) ms[t] = mplane[t];
-    __syncthreads();
+    return F64 ? kq + 4 * i : 4 * kq + i;
+}
+
+// Shared-memory plan of one workgroup: kQW waves, each owning ppw grid
+// points (ppw*na a multiple of 16: whole column tiles per wave).
+struct QuadLds {
+    int sp, fm, bp;               // row strides (odd, against bank conflicts)
+    size_t off_s, off_b, off_x, off_u, off_l, bytes;
+};
+
+constexpr int kQW = kQWG / 64;
+// sched_barrier mask: VALU, SALU and LDS instructions may cross; global
+// loads and MFMAs keep their order around it
+constexpr int kKeepLoads = 0x2 | 0x4 | 0x100 | 0x200;
+
+__host__ __device__ inline QuadLds quad_lds(int nk, int na, int npts, int kf, int mode, int nphp)
+{
+    QuadLds q;
+    const int es = mode == 0 ? 8 : 4;
+    q.sp = 17;
+    q.fm = kf + 1;
+    q.bp = nphp + 4;
+    size_t o = sizeof(double) * (size_t)npts * nk;  // c [npts][nk]
+    q.off_s = o;
+    if (mode == 2) {
+        // bf16 splits: forward A [phi][16] hi, lo; backward A [16][phi] hi, lo
+        o += 2 * sizeof(short) * (size_t)nphp * 16;
+        q.off_b = o;
+        o += 2 * sizeof(short) * (size_t)16 * q.bp;
+    } else {
+        o += (size_t)es * nphp * q.sp;             // S [phi][m] (m padded to 16)
+        q.off_b = o;
+    }
+    o = (o + 15) & ~(size_t)15;
+    q.off_x = o;
+    o += (size_t)es * kQW * 16 * q.fm;             // per-wave F' of one column tile [16][m]
+    o = (o + 15) & ~(size_t)15;
+    q.off_u = o;
+    o += sizeof(double) * (size_t)npts * nk;       // u [npts][nk]
+    o = (o + 15) & ~(size_t)15;
+    q.off_l = o;
+    o += (size_t)es * nk * na;                     // L [nk][na]
+    q.bytes = o;
+    return q;
+}
+
+template <int MODE>
+struct QuadTypes;
+template <>
+struct QuadTypes<0> {
+    typedef double T;
+    typedef d4v acc;
+};
+template <>
+struct QuadTypes<1> {
+    typedef float T;
+    typedef f4v acc;
+};
+template <>
+struct QuadTypes<2> {
+    typedef float T;
+    typedef f4v acc;
+};
+
+template <int MODE>
+__device__ inline typename QuadTypes<MODE>::acc load_w(const typename QuadTypes<MODE>::T *p)
+{
+    if constexpr (MODE == 0) {
+        const double2 w0 = *(const double2 *)p, w1 = *(const double2 *)(p + 2);
+        d4v w = {w0.x, w0.y, w1.x, w1.y};
+        return w;
+    } else {
+        return *(const f4v *)p;
+    }
+}
+
+// W: [tile][NT][64][4] (tile = global 16-column tile, NT phi tiles, the
+// ones beyond nphi zero; see k_weight_grid); Lg: [nk][na]; Sg: [nm][nphi];
+// mplane[r]: m + lmax of carried plane r; qrl[m*9]: number of carried planes
+// of m, qrl[m*9 + 1 + u]: those planes.  MODE 0: FP64 MFMA; 1: FP32 MFMA;
+// 2: BF16x3 MFMA (W in FP32).  KS: forward K steps of 4 m (m padded to
+// 4*KS with zeros).  W of the next column tile is prefetched into NT register
+// tiles while the current one is transformed.
+template <int MODE, int NT, int KS>
+__global__ __launch_bounds__(kQWG) void k_quad_mfma(const double *__restrict__ X, double *__restrict__ R,
+                                                    const void *__restrict__ Wv, const double *__restrict__ Lg,
+                                                    const double *__restrict__ Sg, const int *__restrict__ mplane,
+                                                    const int *__restrict__ qrl, const double *__restrict__ mask,
+                                                    const Scal *__restrict__ sc, int64_t P, int nk, int na,
+                                                    int nphi, int nm, int ppw, double *__restrict__ partial)
+{
+    typedef typename QuadTypes<MODE>::T T;
+    typedef typename QuadTypes<MODE>::acc acc_t;
+    constexpr bool F64 = MODE == 0;
+    constexpr int KF = 4 * KS;  // m rows carried through the transforms
+    extern __shared__ __align__(16) unsigned char smem[];
+    __shared__ double red[kQW];
+    const int npts = kQW * ppw;
+    const QuadLds q = quad_lds(nk, na, npts, KF, MODE, 16 * NT);
+    double *cs = (double *)smem;
+    double *us = (double *)(smem + q.off_u);
+    T *Ls = (T *)(smem + q.off_l);
+    const int64_t p0 = (int64_t)blockIdx.x * npts;
     const double sx = sc->sx;
+
+    // ---- tables and inputs ----
+    for (int t = threadIdx.x; t < nk * na; t += blockDim.x) Ls[t] = (T)Lg[t];
+    if constexpr (MODE == 2) {
+        short *fh = (short *)(smem + q.off_s), *fl = fh + 16 * NT * 16;
+        short *bh = (short *)(smem + q.off_b), *bl = bh + 16 * q.bp;
+        for (int t = threadIdx.x; t < NT * 16 * 16; t += blockDim.x) {
+            const int ph = t >> 4, mi = t & 15;
+            const float v = (ph < nphi && mi < nm) ? (float)Sg[(size_t)mi * nphi + ph] : 0.f;
+            short h, l;
+            bf16_split(v, h, l);
+            fh[ph * 16 + mi] = h;
+            fl[ph * 16 + mi] = l;
+            bh[mi * q.bp + ph] = h;
+            bl[mi * q.bp + ph] = l;
+        }
+    } else {
+        T *Ss = (T *)(smem + q.off_s);
+        for (int t = threadIdx.x; t < NT * 16 * 16; t += blockDim.x) {
+            const int ph = t >> 4, mi = t & 15;
+            Ss[ph * q.sp + mi] = (ph < nphi && mi < nm) ? (T)Sg[(size_t)mi * nphi + ph] : (T)0;
+        }
+    }
+    for (int t = threadIdx.x; t < nk * npts; t += blockDim.x) {
+        const int r = t / npts, p = t - r * npts;
+        cs[p * nk + r] = p0 + p < P ? sx * X[(size_t)r * P + p0 + p] : 0.0;
+    }
+    __syncthreads();
+
+    // ---- per wave: its ppw points, tw column tiles ----
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int l16 = lane & 15, kq = lane >> 4;
+    const int tw = ppw * na / 16;
+    const int pw0 = wid * ppw;  // first point of this wave within the workgroup
+    const int64_t tile0 = (int64_t)blockIdx.x * kQW * tw + (int64_t)wid * tw;
+    const T *W = (const T *)Wv + (size_t)lane * 4;
+    T *xs = (T *)(smem + q.off_x) + wid * 16 * q.fm;
+    const int mr = lane < nk ? mplane[lane] : 0;
+    int pcur = -1;
+    double uacc = 0.0;
+
+    // One column tile.  Each phi tile's W register tile is refilled with the
+    // next column tile's W right after its use; the scheduling barrier keeps
+    // the load there (the compiler would otherwise sink it to the end of the
+    // tile), so its latency overlaps the rest of this tile's transforms.
+    auto tile = [&](int n, acc_t(&wc)[NT]) {
+        const int colw = 16 * n + l16;  // column within the wave
+        const int pc = pw0 + colw / na, ac = colw % na;
+        // next tile's W (the last tile reloads itself: no read past the grid)
+        const T *Wn = W + (size_t)((tile0 + (n + 1 < tw ? n + 1 : n)) * NT) * 256;
+        // Legendre forward of this lane's column for every m (uniform loops
+        // over the planes of each m), then the B operand rows of this lane
+        double f[KF];
+#pragma unroll
+        for (int m = 0; m < KF; m++) {
+            double acc = 0.0;
+            if (m < nm) {
+                const int cnt = qrl[m * 9];
+                for (int u = 0; u < cnt; u++) {
+                    const int r = qrl[m * 9 + 1 + u];
+                    acc += (double)Ls[r * na + ac] * cs[pc * nk + r];
+                }
+            }
+            f[m] = acc;
+        }
+        acc_t acc2 = {0, 0, 0, 0};
+        if constexpr (MODE == 2) {
+            const short *fh = (const short *)(smem + q.off_s), *fl = fh + 16 * NT * 16;
+            const short *bh = (const short *)(smem + q.off_b), *bl = bh + 16 * q.bp;
+            s4v bfh, bfl;
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                // row 4*kq + i of the 16-row K block (rows >= KF are zero)
+                double v = 0.0;
+#pragma unroll
+                for (int g = 0; g < 4; g++)
+                    if (4 * g + i < KF) v = kq == g ? f[4 * g + i] : v;
+                short h, l;
+                bf16_split((float)v, h, l);
+                bfh[i] = h;
+                bfl[i] = l;
+            }
+#pragma unroll
+            for (int t = 0; t < NT; t++) {
+                const s4v ah = *(const s4v *)(fh + (16 * t + l16) * 16 + 4 * kq);
+                const s4v al = *(const s4v *)(fl + (16 * t + l16) * 16 + 4 * kq);
+                f4v g = {0.f, 0.f, 0.f, 0.f};
+                g = mfma_bf16(ah, bfh, g);
+                g = mfma_bf16(ah, bfl, g);
+                g = mfma_bf16(al, bfh, g);
+                const f4v h4 = g * wc[t];
+                wc[t] = load_w<MODE>(Wn + (size_t)t * 256);
+                __builtin_amdgcn_sched_barrier(kKeepLoads);
+                s4v hh, hl;
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    short h, l;
+                    bf16_split(h4[i], h, l);
+                    hh[i] = h;
+                    hl[i] = l;
+                }
+                const s4v a2h = *(const s4v *)(bh + l16 * q.bp + 16 * t + 4 * kq);
+                const s4v a2l = *(const s4v *)(bl + l16 * q.bp + 16 * t + 4 * kq);
+                acc2 = mfma_bf16(a2h, hh, acc2);
+                acc2 = mfma_bf16(a2h, hl, acc2);
+                acc2 = mfma_bf16(a2l, hh, acc2);
+            }
+        } else {
+            const T *Ss = (const T *)(smem + q.off_s);
+            T bF[KS];
+#pragma unroll
+            for (int s = 0; s < KS; s++) {
+                double v = f[4 * s];
+#pragma unroll
+                for (int g = 1; g < 4; g++) v = kq == g ? f[4 * s + g] : v;
+                bF[s] = (T)v;
+            }
+#pragma unroll
+            for (int t = 0; t < NT; t++) {
+                acc_t g = {0, 0, 0, 0};
+#pragma unroll
+                for (int s = 0; s < KS; s++) g = mfma16(Ss[(16 * t + l16) * q.sp + 4 * s + kq], bF[s], g);
+                const acc_t h4 = g * wc[t];
+                wc[t] = load_w<MODE>(Wn + (size_t)t * 256);
+                __builtin_amdgcn_sched_barrier(kKeepLoads);
+#pragma unroll
+                for (int i = 0; i < 4; i++)
+                    acc2 = mfma16(Ss[(16 * t + acc_row<F64>(kq, i)) * q.sp + l16], h4[i], acc2);
+            }
+        }
+        // F' of the tile -> per-wave LDS, then the Legendre backward of its
+        // 16 columns by lane r (one carried plane per lane)
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const int m = acc_row<F64>(kq, i);
+            if (m < KF) xs[l16 * q.fm + m] = acc2[i];
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        if (lane < nk) {
+            // independent products first, then a segmented sum over the
+            // points the 16 columns belong to
+            const int a0 = (16 * n) % na;
+            double v[16];
+            int aa = a0;
+#pragma unroll
+            for (int c = 0; c < 16; c++) {
+                v[c] = (double)Ls[lane * na + aa] * (double)xs[c * q.fm + mr];
+                if (++aa == na) aa = 0;
+            }
+            int pp = pw0 + (16 * n) / na;
+            aa = a0;
+#pragma unroll
+            for (int c = 0; c < 16; c++) {
+                if (pp != pcur) {
+                    if (pcur >= 0) us[pcur * nk + lane] = uacc;
+                    pcur = pp;
+                    uacc = 0.0;
+                }
+                uacc += v[c];
+                if (++aa == na) {
+                    aa = 0;
+                    pp++;
+                }
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    };
+    acc_t wb[NT];
+#pragma unroll
+    for (int t = 0; t < NT; t++) wb[t] = load_w<MODE>(W + (size_t)(tile0 * NT + t) * 256);
+    for (int n = 0; n < tw; n++) tile(n, wb);
+    if (lane < nk && pcur >= 0) us[pcur * nk + lane] = uacc;
+    __syncthreads();
+
+    // ---- R update and the alpha partial ----
     double dot = 0.0;
-    const int64_t p0 = (int64_t)blockIdx.x * ppw;
-    for (int64_t p = p0; p < p0 + ppw && p < P; p++) {
-        for (int r = threadIdx.x; r < nk; r += blockDim.x) c[r] = sx * X[(size_t)r * P + p];
-        __syncthreads();
-        for (int t = threadIdx.x; t < na * nm; t += blockDim.x) {
-            const int a = t / nm, mi = t - a * nm;
-            double s = 0.0;
-            for (int r = 0; r < nk; r++)
-                if (ms[r] == mi) s += Ls[r * na + a] * c[r];
-            f[t] = s;
-        }
-        __syncthreads();
-        const WT *Wp = W + (size_t)p * na * nphi;
-        for (int t = threadIdx.x; t < na * nphi; t += blockDim.x) {
-            const int a = t / nphi, ph = t - a * nphi;
-            double g = 0.0;
-            for (int mi = 0; mi < nm; mi++) g += f[a * nm + mi] * Ss[mi * nphi + ph];
-            h[t] = (double)Wp[t] * g;
-        }
-        __syncthreads();
-        for (int t = threadIdx.x; t < na * nm; t += blockDim.x) {
-            const int a = t / nm, mi = t - a * nm;
-            double s = 0.0;
-            for (int ph = 0; ph < nphi; ph++) s += h[a * nphi + ph] * Ss[mi * nphi + ph];
-            f[t] = s;
-        }
-        __syncthreads();
-        for (int r = threadIdx.x; r < nk; r += blockDim.x) {
-            const int mi = ms[r];
-            double u = 0.0;
-            for (int a = 0; a < na; a++) u += Ls[r * na + a] * f[a * nm + mi];
-            const size_t o = (size_t)r * P + p;
-            const double nr = R[o] + mask[r] * u;
-            R[o] = nr;
-            dot += c[r] * nr;
-        }
-        __syncthreads();
+    for (int t = threadIdx.x; t < nk * npts; t += blockDim.x) {
+        const int r = t / npts, p = t - r * npts;
+        if (p0 + p >= P) continue;
+        const size_t o = (size_t)r * P + p0 + p;
+        const double nr = R[o] + mask[r] * us[p * nk + r];
+        R[o] = nr;
+        dot += cs[p * nk + r] * nr;
     }
     const double s = q_block_sum(dot, red);
     if (threadIdx.x == 0) partial[blockIdx.x] = s;
 }
 
-// W[p][a][half*nb + b] = w_a w_b vg[p][half*na*nb + a*nb + b]
+// W[tile][t][lane][i] = w_a w_b vg[p][half*na*nb + a*nb + b] for column
+// col = 16*tile + lane%16 = p*na + a and phi = 16*t + acc_row(lane/16, i) =
+// half*nb + b; zero outside the grid (padding columns and phi).  FP64 grids
+// feed the FP64 instruction, FP32 grids the FP32 and BF16 ones.
 template <typename WT>
 __global__ void k_weight_grid(const double *__restrict__ vg, WT *__restrict__ W, const double *__restrict__ wt,
-                              const double *__restrict__ wp, int64_t P, int na, int nb)
+                              const double *__restrict__ wp, int64_t P, int na, int nb, int64_t ntiles, int ntp)
 {
-    const int64_t Q = (int64_t)2 * na * nb;
-    const int64_t total = P * Q;
+    const int nphi = 2 * nb;
+    const int64_t total = ntiles * ntp * 256;
     for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
          idx += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t p = idx / Q;
-        const int q = (int)(idx - p * Q);
-        const int a = q / (2 * nb), rem = q - a * 2 * nb, half = rem / nb, b = rem - half * nb;
-        W[idx] = (WT)(wt[a] * wp[b] * vg[(size_t)p * Q + (size_t)half * na * nb + (size_t)a * nb + b]);
+        const int i = (int)(idx & 3), lane = (int)((idx >> 2) & 63);
+        const int64_t rest = idx >> 8;
+        const int t = (int)(rest % ntp);
+        const int64_t tile = rest / ntp;
+        const int64_t col = 16 * tile + (lane & 15);
+        const int ph = 16 * t + acc_row<sizeof(WT) == 8>(lane >> 4, i);
+        const int64_t p = col / na;
+        const int a = (int)(col - p * na);
+        double v = 0.0;
+        if (p < P && ph < nphi) {
+            const int half = ph / nb, b = ph - half * nb;
+            v = wt[a] * wp[b] * vg[(size_t)p * 2 * na * nb + (size_t)half * na * nb + (size_t)a * nb + b];
+        }
+        W[idx] = (WT)v;
     }
 }
 
-int launch_weight_grid(const double *vg, void *W, int wprec, const double *wt, const double *wp, int64_t P,
-                       int na, int nb, hipStream_t s)
+// phi tiles of the kernel instantiations (zero padded beyond nphi)
+int quad_phi_tiles(int nphi)
 {
-    const int64_t total = P * 2 * na * nb;
+    const int nt = (nphi + 15) / 16;
+    return nt <= 1 ? 1 : nt <= 2 ? 2 : nt <= 4 ? 4 : nt <= 9 ? 9 : 16;
+}
+
+int quad_points_per_group(int na, int nk)
+{
+    // each wave owns ppw points with ppw*na a multiple of 16 (whole column
+    // tiles), at least 4 tiles per wave so that the W prefetch has a next tile
+    int g = 16;
+    while (na % g) g >>= 1;
+    const int unit = 16 / g, tiles_per_unit = na / g;
+    int k = (4 + tiles_per_unit - 1) / tiles_per_unit;
+    while (k > 1 && quad_lds(nk, na, kQW * unit * k, 16, 0, 256).bytes > 65536) k--;
+    return kQW * unit * k;
+}
+
+int64_t quad_tiles(int64_t P, int na, int npts)
+{
+    const int64_t groups = (P + npts - 1) / npts;
+    return groups * (int64_t)npts * na / 16;
+}
+
+int launch_weight_grid(const double *vg, void *W, int wprec, const double *wt, const double *wp, int64_t P,
+                       int na, int nb, int npts, hipStream_t s)
+{
+    const int64_t ntiles = quad_tiles(P, na, npts);
+    const int ntp = quad_phi_tiles(2 * nb);
+    const int64_t total = ntiles * ntp * 256;
     int64_t g = (total + 255) / 256;
     if (g > 65536) g = 65536;
     if (wprec == BSL_PREC_FP64)
         hipLaunchKernelGGL(k_weight_grid<double>, dim3((unsigned)g), dim3(256), 0, s, vg, (double *)W, wt, wp, P,
-                           na, nb);
+                           na, nb, ntiles, ntp);
     else
         hipLaunchKernelGGL(k_weight_grid<float>, dim3((unsigned)g), dim3(256), 0, s, vg, (float *)W, wt, wp, P,
-                           na, nb);
+                           na, nb, ntiles, ntp);
     BSL_HIP(hipGetLastError());
     return BSL_OK;
 }
 
+#define BSL_QUAD_LAUNCH(M, NTV, KSV)                                                                       \
+    hipLaunchKernelGGL((k_quad_mfma<M, NTV, KSV>), dim3(grid), dim3(kQWG), lds, s, X, R, op.wgrid, op.qL, op.qS, \
+                       op.mplane, op.q_rl, op.mask, sc, op.P, op.nk, op.na, op.nphi, op.nm, ppw, op.partial)
+
+template <int M, int NTV>
+static void launch_quad_ks(const DevOperator &op, const double *X, double *R, Scal *sc, unsigned grid, size_t lds,
+                           int ppw, int ks, hipStream_t s)
+{
+    if constexpr (M == 2) {
+        BSL_QUAD_LAUNCH(M, NTV, 4);
+    } else {
+        if (ks >= 4)
+            BSL_QUAD_LAUNCH(M, NTV, 4);
+        else if (ks <= 2)
+            BSL_QUAD_LAUNCH(M, NTV, 2);
+        else
+            BSL_QUAD_LAUNCH(M, NTV, 3);
+    }
+}
+#undef BSL_QUAD_LAUNCH
+
+template <int M>
+static void launch_quad_mode(const DevOperator &op, const double *X, double *R, Scal *sc, unsigned grid, size_t lds,
+                             int ppw, int ks, hipStream_t s)
+{
+    switch (quad_phi_tiles(op.nphi)) {
+    case 1: launch_quad_ks<M, 1>(op, X, R, sc, grid, lds, ppw, ks, s); break;
+    case 2: launch_quad_ks<M, 2>(op, X, R, sc, grid, lds, ppw, ks, s); break;
+    case 4: launch_quad_ks<M, 4>(op, X, R, sc, grid, lds, ppw, ks, s); break;
+    case 9: launch_quad_ks<M, 9>(op, X, R, sc, grid, lds, ppw, ks, s); break;
+    default: launch_quad_ks<M, 16>(op, X, R, sc, grid, lds, ppw, ks, s); break;
+    }
+}
+
+int quad_kf(int nm, int mode)
+{
+    // m rows carried: 4*KS with KS >= 2 (16 for the BF16 instruction's K)
+    const int ks = (nm + 3) / 4;
+    return mode == 2 ? 16 : 4 * (ks < 2 ? 2 : ks);
+}
+
 int launch_quad_potential(const DevOperator &op, const double *X, double *R, Scal *sc, hipStream_t s)
 {
-    const int ppw = 8;
-    const unsigned grid = (unsigned)((op.P + ppw - 1) / ppw);
+    const int npts = op.q_npts, ppw = npts / kQW;
+    const unsigned grid = (unsigned)((op.P + npts - 1) / npts);
     if ((int64_t)grid > op.npartial || (int64_t)grid != op.nalpha)
         return fail(BSL_E_STATE, "partial buffer does not match the quadrature kernel grid");
-    const size_t lds = sizeof(double) * ((size_t)op.nk * op.na + (size_t)op.nm * op.nphi + op.nk +
-                                         (size_t)op.na * op.nm + (size_t)op.na * op.nphi);
-    if (op.wprec == BSL_PREC_FP64)
-        hipLaunchKernelGGL(k_quad_fp64<double>, dim3(grid), dim3(kQWG), lds, s, X, R, (const double *)op.wgrid,
-                           op.qL, op.qS, op.mplane, op.mask, sc, op.P, op.nk, op.na, op.nphi, op.nm, ppw,
-                           op.partial);
+    const int kf = quad_kf(op.nm, op.qmode), ntp = quad_phi_tiles(op.nphi);
+    const size_t lds = quad_lds(op.nk, op.na, npts, kf, op.qmode, 16 * ntp).bytes;
+    if (op.nm > 16 || op.nk > 64 || op.nphi > 256 || npts % kQW || (ppw * op.na) % 16 || lds > 65536)
+        return fail(BSL_E_ARG, "quadrature kernel: unsupported sizes");
+    if (op.qmode == 0)
+        launch_quad_mode<0>(op, X, R, sc, grid, lds, ppw, kf / 4, s);
+    else if (op.qmode == 1)
+        launch_quad_mode<1>(op, X, R, sc, grid, lds, ppw, kf / 4, s);
     else
-        hipLaunchKernelGGL(k_quad_fp64<float>, dim3(grid), dim3(kQWG), lds, s, X, R, (const float *)op.wgrid,
-                           op.qL, op.qS, op.mplane, op.mask, sc, op.P, op.nk, op.na, op.nphi, op.nm, ppw,
-                           op.partial);
+        launch_quad_mode<2>(op, X, R, sc, grid, lds, ppw, kf / 4, s);
     BSL_HIP(hipGetLastError());
     return BSL_OK;
 }

@@ -267,6 +267,48 @@ def test_quad_fp32_hv_close_to_oracle():
     assert _rel(c.hv(v), o.hv(v)) < 1e-6
 
 
+QUAD_TOL = {"fp64": 1e-12, "fp32": 1e-6, "bf16x3": 2e-5}
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32", "bf16x3"])
+def test_quad_hv_production_shape(precision):
+    """lmax 5 (11 m, K padded to 12) and the 40 x 70 quadrature of the 54^3
+    workload (140 phi nodes in 9 MFMA tiles of 16, the last one partial) on
+    a small grid whose last workgroup holds fewer points."""
+    spec, kw = cs.cage_case(3, 5, 40, 70, "All")
+    o = O.Oracle(spec)
+    c = B.Calculator(calculator="quad", precision=precision, **kw)
+    v = np.random.default_rng(23).uniform(-1, 1, o.N)
+    assert _rel(c.hv(v), o.hv(v)) < QUAD_TOL[precision]
+
+
+@pytest.mark.parametrize("na,nb,sym,basis", [(5, 7, "Even", "reduced"), (3, 4, "Odd", "full"),
+                                             (12, 16, "All", "full")])
+def test_quad_ragged_tiles(na, nb, sym, basis):
+    """theta counts that are not multiples of 16 (16 points per workgroup for
+    odd na), phi counts below and at one MFMA tile."""
+    spec, kw = cs.cage_case(4, 2, na, nb, sym)
+    o = O.Oracle(spec)
+    c = B.Calculator(calculator="quad", basis=basis, **kw)
+    v = np.random.default_rng(24).uniform(-1, 1, o.N)
+    assert _rel(c.hv(v), o.hv(v)) < 1e-12
+
+
+def test_quad_bf16x3_levels_within_target():
+    """BF16x3 split products on MFMA: converged levels within 0.01 cm^-1 of
+    the FP64 NoQuad operator's."""
+    from bsl_amd import physics
+    spec, kw = cs.cage_case(6, 3, 8, 8, "All")
+    cq = B.Calculator(calculator="quad", precision="bf16x3", **kw)
+    cn = B.Calculator(**kw)
+    eq, rq = B.lancbis(*cq.lanczos(200), -10.0, 10.0)
+    en, rn = B.lancbis(*cn.lanczos(200), -10.0, 10.0)
+    conv = _converged(en, rn)
+    assert len(conv) >= 3
+    for e in conv:
+        assert np.min(np.abs(eq - e)) / physics.KJMOL_PER_CM1 < 0.01
+
+
 def test_quad_eigenvalues_match_noquad():
     spec, kw = cs.cage_case(6, 3, 8, 8, "All")
     cq = B.Calculator(calculator="quad", **kw)

@@ -106,6 +106,38 @@ class Calculator:
         check(lib().bsl_rayleigh(self.h, len(y), y.ravel(), e, r))
         return e, r
 
+    def levels(self, niter, count, emin=-1.0e4, emax=1.0e4, start="random", seed=0, tol=1e-8):
+        """The `count` lowest converged good levels: Loop 1, Cullum-Willoughby,
+        Tm eigenvectors, then Loop 2 rebuilds their Ritz vectors by
+        recomputing the Krylov vectors (lanczosEigenValuesVectors.cpp:
+        170-242, 251-295, 329-469), and an FP64 Rayleigh quotient refines
+        each level.  Returns (energies, unit Ritz vectors (count, N),
+        residual norms)."""
+        a, b, b2 = self.lanczos(niter, start=start, seed=seed)
+        ev, er = lancbis(a, b, b2, emin, emax)
+        order = np.argsort(ev)[:count]
+        if len(order) < count or not np.all(np.abs(er[order]) < tol):
+            raise BslError(0, "the %d lowest levels are not all converged after %d steps" % (count, niter))
+        sel = ev[order]
+        evtr, _ = trivec(a, b, b2, sel)
+        psi = self.ritz(niter, evtr, count, start=start, seed=seed)
+        psi /= np.linalg.norm(psi, axis=1, keepdims=True)
+        e, r = self.rayleigh(psi)
+        return e, psi, r
+
+    def hf_gradient(self, psi, g):
+        """Hellmann-Feynman gradient sum_k g[k] <psi_k|dV/dtheta|psi_k> of the
+        cage potential (bsl_hf_gradient).  Returns a dict: site_q (n_sites),
+        site_pos (n_sites, 3), q_h2_h, q_h2_cm, lj_a, lj_b."""
+        psi = np.ascontiguousarray(np.atleast_2d(psi), dtype=np.float64)
+        g = np.ascontiguousarray(np.atleast_1d(g), dtype=np.float64)
+        if psi.shape[1] != self.N or len(g) != psi.shape[0]:
+            raise ValueError("psi must be (len(g), basis size)")
+        ns = self.params.n_sites
+        out = np.zeros(4 * ns + 4)
+        check(lib().bsl_hf_gradient(self.h, psi.shape[0], psi.ravel(), g, out))
+        return split_gradient(out, ns)
+
     def vblocks(self):
         nx, ny, nz = self.shape
         out = np.zeros(nx * ny * nz * self.nn * self.nn)
@@ -134,6 +166,28 @@ class Calculator:
         buf = C.create_string_buffer(128)
         check(lib().bsl_potential_kernel(self.h, buf, 128))
         return buf.value.decode()
+
+
+def split_gradient(grad, ns):
+    """Named parts of a bsl_hf_gradient vector."""
+    sg = grad[: 4 * ns].reshape(ns, 4)
+    return {"site_q": sg[:, 0].copy(), "site_pos": sg[:, 1:4].copy(), "q_h2_h": float(grad[4 * ns]),
+            "q_h2_cm": float(grad[4 * ns + 1]), "lj_a": float(grad[4 * ns + 2]), "lj_b": float(grad[4 * ns + 3])}
+
+
+def multiplets(energies, tol=1e-6):
+    """Groups of indices of (near-)degenerate levels: consecutive sorted
+    levels closer than tol (kJ/mol) share a group."""
+    order = np.argsort(energies)
+    groups, cur = [], [int(order[0])]
+    for i, j in zip(order[:-1], order[1:]):
+        if energies[j] - energies[i] < tol:
+            cur.append(int(j))
+        else:
+            groups.append(cur)
+            cur = [int(j)]
+    groups.append(cur)
+    return groups
 
 
 def lancbis(alpha, beta, beta2, emin, emax):

@@ -22,7 +22,7 @@ EXPORTS = (
     "bsl_create", "bsl_destroy", "bsl_last_error", "bsl_basis_size", "bsl_krylov_size", "bsl_hv",
     "bsl_hv_device", "bsl_lanczos", "bsl_lancbis", "bsl_trivec", "bsl_ritz", "bsl_rayleigh",
     "bsl_water_sites", "bsl_get_vblocks", "bsl_get_vgrid", "bsl_time_steps", "bsl_step_bytes",
-    "bsl_potential_kernel",
+    "bsl_potential_kernel", "bsl_hf_gradient",
 )
 
 
@@ -73,6 +73,7 @@ def load():
     lib.bsl_time_steps.argtypes = [C.c_void_p, C.c_int32, C.c_int32, _dp]
     lib.bsl_step_bytes.argtypes = [C.c_void_p, _dp]
     lib.bsl_potential_kernel.argtypes = [C.c_void_p, C.c_char_p, C.c_int32]
+    lib.bsl_hf_gradient.argtypes = [C.c_void_p, C.c_int32, _dp, _dp, _dp]
     return lib
 
 

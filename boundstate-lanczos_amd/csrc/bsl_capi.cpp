@@ -91,6 +91,61 @@ int build_vpk(bsl_ctx *c, DevOperator &op, int vprec, const double *d_h)
     return BSL_OK;
 }
 
+// H offsets of the quadrature nodes (q = half*na*nb + a*nb + b; the second
+// half mirrors y), site [x, y, z, q] rows and LJ flags of a cage context
+void cage_arrays(const bsl_ctx *c, std::vector<double> &off, std::vector<double> &site4, std::vector<int> &lj)
+{
+    const bsl_params &p = c->p;
+    const Tables &t = c->tab;
+    const int na = t.na, nb = t.nb;
+    off.assign((size_t)c->Q * 3, 0.0);
+    for (int a = 0; a < na; a++)
+        for (int b = 0; b < nb; b++) {
+            const size_t q0 = (size_t)a * nb + b, q1 = (size_t)na * nb + q0;
+            off[3 * q0 + 0] = p.h2_half_bond * std::sin(t.th[a]) * t.cp[b];
+            off[3 * q0 + 1] = p.h2_half_bond * std::sin(t.th[a]) * std::sin(t.ph1[b]);
+            off[3 * q0 + 2] = p.h2_half_bond * t.ct[a];
+            off[3 * q1 + 0] = off[3 * q0 + 0];
+            off[3 * q1 + 1] = -1.0 * (off[3 * q0 + 1]);
+            off[3 * q1 + 2] = off[3 * q0 + 2];
+        }
+    site4.assign((size_t)4 * p.n_sites, 0.0);
+    for (int s = 0; s < p.n_sites; s++) {
+        for (int d = 0; d < 3; d++) site4[4 * s + d] = c->site_pos[3 * s + d];
+        site4[4 * s + 3] = c->site_q[s];
+    }
+    lj.assign(c->site_lj.begin(), c->site_lj.end());
+}
+
+// the clamped potential grid [P][Q] (device point order) into d_vg
+int make_vgrid(bsl_ctx *c, double *d_vg)
+{
+    const bsl_params &p = c->p;
+    int rc = BSL_OK;
+    if (p.potential == BSL_POT_CAGE) {
+        std::vector<double> off, site4;
+        std::vector<int> lj;
+        cage_arrays(c, off, site4, lj);
+        CageDev cage;
+        cage.nsites = p.n_sites;
+        double *d_off = nullptr;
+        if (!rc) rc = dev_upload(&cage.site, site4);
+        if (!rc) rc = dev_upload(&cage.lj, lj);
+        if (!rc) rc = dev_upload(&d_off, off);
+        if (!rc)
+            rc = launch_vgrid_cage(d_vg, c->P, c->Q, c->d_gx, c->d_gy, c->d_gz, p.nx, p.ny, d_off, cage, p.q_h2_h,
+                                   p.q_h2_cm, p.lj_a, p.lj_b, p.ceiling, c->stream);
+        (void)hipStreamSynchronize(c->stream);
+        dev_free(cage.site);
+        dev_free(cage.lj);
+        dev_free(d_off);
+    } else {
+        rc = launch_vgrid_iho(d_vg, c->P, c->Q, c->d_gx, c->d_gy, c->d_gz, p.nx, p.ny, p.iho_factor, p.iho_ceil,
+                              p.ceiling, c->stream);
+    }
+    return rc;
+}
+
 // Picks the occupancy variant of the potential-block kernel by timing each
 // on the context's own operator (the variants compute identical results).
 int time_potential(bsl_ctx *c, DevOperator &op, float *ms)
@@ -732,40 +787,7 @@ extern "C" int bsl_get_vgrid(bsl_ctx *c, double *v1, double *v2)
     if (rc) return rc;
     const Tables &t = c->tab;
     const int na = t.na, nb = t.nb;
-    if (p.potential == BSL_POT_CAGE) {
-        std::vector<double> off((size_t)c->Q * 3), site4((size_t)4 * p.n_sites);
-        for (int a = 0; a < na; a++)
-            for (int b = 0; b < nb; b++) {
-                const size_t q0 = (size_t)a * nb + b, q1 = (size_t)na * nb + q0;
-                off[3 * q0 + 0] = p.h2_half_bond * std::sin(t.th[a]) * t.cp[b];
-                off[3 * q0 + 1] = p.h2_half_bond * std::sin(t.th[a]) * std::sin(t.ph1[b]);
-                off[3 * q0 + 2] = p.h2_half_bond * t.ct[a];
-                off[3 * q1 + 0] = off[3 * q0 + 0];
-                off[3 * q1 + 1] = -1.0 * (off[3 * q0 + 1]);
-                off[3 * q1 + 2] = off[3 * q0 + 2];
-            }
-        for (int s = 0; s < p.n_sites; s++) {
-            for (int d = 0; d < 3; d++) site4[4 * s + d] = p.site_pos[3 * s + d];
-            site4[4 * s + 3] = p.site_q[s];
-        }
-        std::vector<int> lj(p.site_lj, p.site_lj + p.n_sites);
-        CageDev cage;
-        cage.nsites = p.n_sites;
-        double *d_off = nullptr;
-        if (!rc) rc = dev_upload(&cage.site, site4);
-        if (!rc) rc = dev_upload(&cage.lj, lj);
-        if (!rc) rc = dev_upload(&d_off, off);
-        if (!rc)
-            rc = launch_vgrid_cage(d_vg, c->P, c->Q, c->d_gx, c->d_gy, c->d_gz, p.nx, p.ny, d_off, cage, p.q_h2_h,
-                                   p.q_h2_cm, p.lj_a, p.lj_b, p.ceiling, c->stream);
-        (void)hipStreamSynchronize(c->stream);
-        dev_free(cage.site);
-        dev_free(cage.lj);
-        dev_free(d_off);
-    } else {
-        rc = launch_vgrid_iho(d_vg, c->P, c->Q, c->d_gx, c->d_gy, c->d_gz, p.nx, p.ny, p.iho_factor, p.iho_ceil,
-                              p.ceiling, c->stream);
-    }
+    rc = make_vgrid(c, d_vg);
     std::vector<double> vg;
     if (!rc) {
         vg.resize((size_t)c->P * c->Q);
@@ -787,6 +809,70 @@ extern "C" int bsl_get_vgrid(bsl_ctx *c, double *v1, double *v2)
                 }
             }
     return BSL_OK;
+}
+
+extern "C" int bsl_hf_gradient(bsl_ctx *c, int32_t nvec, const double *psi, const double *g, double *grad)
+{
+    if (!c || nvec <= 0 || !psi || !g || !grad) return fail(BSL_E_ARG, "bsl_hf_gradient: bad arguments");
+    if (c->p.potential != BSL_POT_CAGE) return fail(BSL_E_STATE, "bsl_hf_gradient: cage potential only");
+    BSL_HIP(hipSetDevice(c->device));
+    const bsl_params &p = c->p;
+    const Tables &t = c->tab;
+    const int na = t.na, nb = t.nb, nk = (int)c->carried.size(), ns = p.n_sites, Q = c->Q;
+    const int npb = 4, nent = 4 * ns + 4;
+    const int nwg = hf_grad_groups(c->P, npb);
+    // quadrature-node values of the carried harmonics and the node weights
+    std::vector<double> Y((size_t)nk * Q), wq(Q);
+    for (int a = 0; a < na; a++)
+        for (int b = 0; b < nb; b++) {
+            const size_t q0 = (size_t)a * nb + b, q1 = (size_t)na * nb + q0;
+            wq[q0] = wq[q1] = t.wt[a] * t.wp[b];
+            for (int r = 0; r < nk; r++) {
+                const int n = c->carried[r], mi = t.m[n] + p.lmax;
+                Y[(size_t)r * Q + q0] = t.L[(size_t)n * na + a] * t.S1[(size_t)mi * nb + b];
+                Y[(size_t)r * Q + q1] = t.L[(size_t)n * na + a] * t.S2[(size_t)mi * nb + b];
+            }
+        }
+    std::vector<double> off, site4, gv(g, g + nvec);
+    std::vector<int> lj;
+    cage_arrays(c, off, site4, lj);
+    double *d_psi = nullptr, *d_vg = nullptr, *d_Y = nullptr, *d_wq = nullptr, *d_g = nullptr, *d_off = nullptr,
+           *d_site = nullptr, *d_part = nullptr, *d_out = nullptr;
+    int *d_lj = nullptr;
+    int rc = dev_alloc(&d_psi, (size_t)nvec * c->n_kry);
+    for (int k = 0; k < nvec && !rc; k++)
+        rc = upload_carried(c, psi + (size_t)k * c->n_full, d_psi + (size_t)k * c->n_kry);
+    if (!rc) rc = dev_alloc(&d_vg, (size_t)c->P * Q);
+    if (!rc) rc = make_vgrid(c, d_vg);
+    if (!rc) rc = dev_upload(&d_Y, Y);
+    if (!rc) rc = dev_upload(&d_wq, wq);
+    if (!rc) rc = dev_upload(&d_g, gv);
+    if (!rc) rc = launch_hf_density(d_psi, nvec, d_g, d_Y, d_wq, d_vg, c->P, nk, Q, p.ceiling, c->stream);
+    if (!rc) rc = dev_upload(&d_off, off);
+    if (!rc) rc = dev_upload(&d_site, site4);
+    if (!rc) rc = dev_upload(&d_lj, lj);
+    if (!rc) rc = dev_alloc(&d_part, (size_t)nwg * nent);
+    if (!rc) rc = dev_alloc(&d_out, (size_t)nent);
+    if (!rc)
+        rc = launch_hf_grad(d_vg, c->P, Q, c->d_gx, c->d_gy, c->d_gz, p.nx, p.ny, d_off, d_site, d_lj, ns, p.q_h2_h,
+                            p.q_h2_cm, p.lj_a, p.lj_b, npb, d_part, d_out, c->stream);
+    if (!rc) {
+        hipError_t e = hipMemcpyAsync(grad, d_out, sizeof(double) * nent, hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess) rc = hip_fail(e, "download gradient", __FILE__, __LINE__);
+    }
+    (void)hipStreamSynchronize(c->stream);
+    dev_free(d_psi);
+    dev_free(d_vg);
+    dev_free(d_Y);
+    dev_free(d_wq);
+    dev_free(d_g);
+    dev_free(d_off);
+    dev_free(d_site);
+    dev_free(d_lj);
+    dev_free(d_part);
+    dev_free(d_out);
+    return rc;
 }
 
 extern "C" int bsl_step_bytes(const bsl_ctx *c, double *out)

@@ -144,6 +144,13 @@ int launch_vblock_gemm(const double *vg, const double *h, void *vpk, int vprec, 
 // kernel (npts grid points per workgroup)
 int launch_weight_grid(const double *vg, void *W, int wprec, const double *wt, const double *wp, int64_t P,
                        int na, int nb, int npts, hipStream_t s);
+// Hellmann-Feynman adjoint (bsl_adjoint.hip)
+int launch_hf_density(const double *psi, int nvec, const double *g, const double *Y, const double *wq, double *vg,
+                      int64_t P, int nk, int Q, double ceiling, hipStream_t s);
+int hf_grad_groups(int64_t P, int npb);
+int launch_hf_grad(const double *om, int64_t P, int Q, const double *gx, const double *gy, const double *gz, int nx,
+                   int ny, const double *off, const double *site4, const int *lj, int ns, double q_h, double q_cm,
+                   double lj_a, double lj_b, int npb, double *partial, double *out, hipStream_t s);
 int quad_points_per_group(int na, int nk);
 int64_t quad_tiles(int64_t P, int na, int npts);
 int quad_phi_tiles(int nphi);

@@ -160,6 +160,17 @@ int bsl_time_steps(bsl_ctx *ctx, int32_t warmup, int32_t steps, double *ms);
  * update, out[3] total; out[4] FLOPs of one Hv. */
 int bsl_step_bytes(const bsl_ctx *ctx, double *out);
 
+// Hellmann-Feynman gradients: grad = sum_k g[k] <psi_k| dV/dtheta |psi_k>
+// for the cage potential (Alavi H2 in point-charge/LJ sites,
+// Alavi_ContGrid_H2_Routines.cpp:6-147; clamped nodes contribute nothing).
+// psi: nvec eigenvectors of length bsl_basis_size, reference layout,
+// unit norm; g: nvec weights (e.g. the gradient of a loss with respect to
+// each eigenvalue).  grad (4*n_sites + 4 doubles): d/d site_q[s] at [4*s],
+// d/d site_pos[s][d] at [4*s + 1 + d]; then d/d q_h2_h, q_h2_cm, lj_a, lj_b
+// at [4*n_sites + 0..3].  No reference counterpart (the reference has no
+// derivatives).
+int bsl_hf_gradient(bsl_ctx *ctx, int32_t nvec, const double *psi, const double *g, double *grad);
+
 // Name (with template arguments, as a profiler prints it) of the potential
 // kernel the context launches, e.g. "k_vblock<15, 21, double, 1>"; the last
 // argument is the occupancy variant picked by the autotuner at bsl_create.

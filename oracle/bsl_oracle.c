@@ -1240,3 +1240,90 @@ void bso_ritz(const bso_state *s, int niter, const double *v0, const double *evt
     free(uec);
     free(cum);
 }
+
+/* Hellmann-Feynman gradient: sum_k g_k <psi_k| dV/dtheta |psi_k>, with
+ * <psi|dV|psi> = sum_{P,q} w_q (sum_n Y_n(q) psi(P,n))^2 dV(P,q)
+ * (the contraction of build_vblocks) and V(P,q) the pair sums of
+ * build_vgrid: V = sum_s K q_cm q_s/|c - s| + LJ(|c - s|) on 'O' sites
+ *              + sum_{h=+-} sum_s K q_h q_s/|c + h off(q) - s|,
+ * clamped at the ceiling (no dependence there). */
+void bso_hf_gradient(const bso_state *s, int nvec, const double *psi, const double *g, double *grad)
+{
+    const bso_problem *p = &s->p;
+    int nx = s->nx, ny = s->ny, nz = s->nz, nn = s->nn, na = s->na, nb = s->nb, lmax = s->lmax;
+    int ns = p->nsites, nent = 4 * ns + 4;
+    long long P = s->P;
+    for (int e = 0; e < nent; e++) grad[e] = 0.0;
+    int nthr = 1;
+#ifdef _OPENMP
+    nthr = omp_get_max_threads();
+#endif
+    double *acc = dalloc((size_t)nthr * nent);
+#pragma omp parallel
+    {
+        int tid = 0;
+#ifdef _OPENMP
+        tid = omp_get_thread_num();
+#endif
+        double *my = acc + (size_t)tid * nent;
+#pragma omp for schedule(guided)
+        for (long long ip = 0; ip < P; ip++) {
+            int k = (int)(ip % nz), j = (int)((ip / nz) % ny), i = (int)(ip / ((long long)nz * ny));
+            double c[3] = {s->gx[i], s->gy[j], s->gz[k]};
+            long long pv = ((long long)k * ny + j) * nx + i; /* vector layout [n][k][j][i] */
+            double wcm = 0.0;
+            for (int a = 0; a < na; a++)
+                for (int b = 0; b < nb; b++)
+                    for (int half = 0; half < 2; half++) {
+                        size_t ind2 = ((size_t)ip * na + a) * nb + b;
+                        double v = half == 0 ? s->v1[ind2] : s->v2[ind2];
+                        if (v >= p->ceiling) continue;
+                        const double *S = half == 0 ? s->S1 : s->S2;
+                        double w = 0.0;
+                        for (int kv = 0; kv < nvec; kv++) {
+                            double y = 0.0;
+                            for (int n = 0; n < nn; n++)
+                                y += s->L[n * na + a] * S[(s->qnum[2 * n + 1] + lmax) * nb + b] *
+                                     psi[(size_t)kv * s->N + (size_t)n * P + pv];
+                            w += g[kv] * y * y;
+                        }
+                        w *= s->wt[a] * s->wp[b];
+                        wcm += w;
+                        double o[3] = {p->h2_half_bond * sin(s->th[a]) * s->cp[b],
+                                       p->h2_half_bond * sin(s->th[a]) * sin(s->ph1[b]),
+                                       p->h2_half_bond * s->ct[a]};
+                        if (half == 1) o[1] = -o[1];
+                        for (int hh = 0; hh < 2; hh++) {
+                            double sg = hh == 0 ? 1.0 : -1.0;
+                            for (int st = 0; st < ns; st++) {
+                                const double *sp = p->site_pos + 3 * st;
+                                double e[3] = {c[0] + sg * o[0] - sp[0], c[1] + sg * o[1] - sp[1],
+                                               c[2] + sg * o[2] - sp[2]};
+                                double r = sqrt(e[0] * e[0] + e[1] * e[1] + e[2] * e[2]);
+                                my[4 * st] += w * BSO_K_CONST * p->q_h2_h / r;
+                                my[4 * ns] += w * BSO_K_CONST * p->site_q[st] / r;
+                                double f = w * BSO_K_CONST * p->q_h2_h * p->site_q[st] / (r * r * r);
+                                for (int d = 0; d < 3; d++) my[4 * st + 1 + d] += f * e[d];
+                            }
+                        }
+                    }
+            for (int st = 0; st < ns; st++) {
+                const double *sp = p->site_pos + 3 * st;
+                double e[3] = {c[0] - sp[0], c[1] - sp[1], c[2] - sp[2]};
+                double r = sqrt(e[0] * e[0] + e[1] * e[1] + e[2] * e[2]);
+                my[4 * st] += wcm * BSO_K_CONST * p->q_h2_cm / r;
+                my[4 * ns + 1] += wcm * BSO_K_CONST * p->site_q[st] / r;
+                double f = wcm * BSO_K_CONST * p->q_h2_cm * p->site_q[st] / (r * r * r);
+                if (p->site_type[st] == 'O') {
+                    my[4 * ns + 2] += wcm / pow(r, 12.0);
+                    my[4 * ns + 3] -= wcm / pow(r, 6.0);
+                    f += wcm * (12.0 * p->lj_a / pow(r, 14.0) - 6.0 * p->lj_b / pow(r, 8.0));
+                }
+                for (int d = 0; d < 3; d++) my[4 * st + 1 + d] += f * e[d];
+            }
+        }
+    }
+    for (int t = 0; t < nthr; t++)
+        for (int e = 0; e < nent; e++) grad[e] += acc[(size_t)t * nent + e];
+    free(acc);
+}

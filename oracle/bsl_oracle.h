@@ -15,12 +15,17 @@
  *   kinetic + rotor + mask          closed form, HO spectrum, oracle/_ref Hv
  *   V blocks (NoQuad, ContGrid)     constant / radial / cos(theta) potentials,
  *                                   symmetry, oracle/_ref V blocks + Hv
- *   cage potential (SPC/E, Alavi)   -30.2331 kJ/mol known answer (TIP4P sites),
- *                                   closed-form pair sums, oracle/_ref
+ *   cage potential (SPC/E, Alavi)   point energies of the compiled reference
+ *                                   sources (oracle/_ref; the -30.2331 kJ/mol
+ *                                   of a reference comment is stale: those
+ *                                   sources give -58.37227147344424),
+ *                                   closed-form pair sums
  *   Lanczos loop 1                  numpy Lanczos on the dense operator,
  *                                   reference run-dir eigenvalues
  *   BISEC / ISOEV                   brute force (eigvals of T and T-hat),
  *                                   reference run-dir eigenvalues
+ *   Hellmann-Feynman gradient       central finite differences of eigenvalues
+ *                                   of this oracle (tests/test_adjoint.py)
  *   INVERR / TRIVEC / INVERM /
  *   STURMI / GENRAN                 residual checks; run-dir error column and
  *                                   lv / eigVecFile fixtures.  The GNU RAND
@@ -131,6 +136,14 @@ int  bso_trivec(int niter, const double *alpha, const double *beta,
 void bso_ritz(const bso_state *s, int niter, const double *v0,
               const double *evtr, int ngood, double *ritz);
 
+/* Hellmann-Feynman gradient of sum_k g[k] E_k for the cage potential:
+ * grad[4*s] = d/d site_q[s], grad[4*s+1+d] = d/d site_pos[s][d],
+ * grad[4*nsites + 0..3] = d/d q_h2_h, q_h2_cm, lj_a, lj_b.  psi: nvec unit
+ * eigenvectors in reference layout.  Derivatives of the pair sums of
+ * Alavi_ContGrid_H2_Routines.cpp:6-147 (no reference counterpart: the
+ * reference has no derivatives); clamped nodes (V >= ceiling) contribute
+ * nothing.  Needs the V grids (bso_prepare, not bso_prepare_external). */
+void bso_hf_gradient(const bso_state *s, int nvec, const double *psi, const double *g, double *grad);
 /* GNU Fortran style SRAND/RAND stream (Park-Miller minimal standard) */
 void bso_genran(int seed, double *r, int n);
 

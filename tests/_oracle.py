@@ -62,6 +62,7 @@ def _load():
     lib.bso_trivec.argtypes = [C.c_int, _dp, _dp, _dp, _dp, C.c_int, _dp, _ip]
     lib.bso_ritz.argtypes = [C.c_void_p, C.c_int, C.c_void_p, _dp, C.c_int, _dp]
     lib.bso_genran.argtypes = [C.c_int, _dp, C.c_int]
+    lib.bso_hf_gradient.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _dp]
     lib.bso_gen_indices_lm.restype = C.c_int
     lib.bso_gen_indices_lm.argtypes = [C.c_int, _ip]
     lib.bso_gauleg.argtypes = [C.c_double, C.c_double, _dp, _dp, C.c_int]
@@ -208,6 +209,15 @@ class Oracle:
         ptr = None if v0 is None else np.ascontiguousarray(v0, dtype=np.float64).ctypes.data
         lib.bso_ritz(self.h, niter, ptr, evtr, ngood, out)
         return out.reshape(ngood, self.N)
+
+
+    def hf_gradient(self, psi, g):
+        """sum_k g[k] dE_k/dtheta (bso_hf_gradient); psi: (nvec, N)."""
+        psi = np.ascontiguousarray(np.atleast_2d(psi), dtype=np.float64)
+        g = np.ascontiguousarray(np.atleast_1d(g), dtype=np.float64)
+        out = np.zeros(4 * len(self.spec["site_q"]) + 4)
+        lib.bso_hf_gradient(self.h, psi.shape[0], psi.ravel(), g, out)
+        return out
 
 
 def start_vector(n, seed):

@@ -1,0 +1,175 @@
+"""Hellmann-Feynman gradients of eigenvalues with respect to the cage
+potential's parameters (site charges and positions, Alavi charges, LJ A/B).
+
+CPU: the oracle's gradient (bso_hf_gradient) against central finite
+differences of dense-diagonalisation eigenvalues of the oracle operator.
+GPU: the library's adjoint kernels against the oracle on the same vectors,
+against central finite differences of the GPU's own converged levels, and
+through the torch.autograd wrapper (chain rule to water-model charges and
+LJ sigma/epsilon)."""
+import copy
+
+import numpy as np
+import pytest
+
+import _cases as cs
+import _oracle as O
+
+
+def _dense(o):
+    n = o.N
+    h = np.zeros((n, n))
+    e = np.zeros(n)
+    for j in range(n):
+        e[j] = 1.0
+        h[:, j] = o.hv(e)
+        e[j] = 0.0
+    return 0.5 * (h + h.T)
+
+
+def _perturb(spec, name, idx, delta):
+    s = copy.deepcopy(spec)
+    if name in ("site_q", "site_pos"):
+        arr = np.array(s[name], dtype=np.float64, copy=True)
+        arr.reshape(-1)[idx] += delta
+        s[name] = arr
+    else:
+        s[name] = s[name] + delta
+    return s
+
+
+def _value(spec, name, idx):
+    return np.asarray(spec[name], dtype=np.float64).reshape(-1)[idx] if name in ("site_q", "site_pos") \
+        else spec[name]
+
+
+def _grad_entry(g, ns, name, idx):
+    if name == "site_q":
+        return g[4 * idx]
+    if name == "site_pos":
+        s, d = divmod(idx, 3)
+        return g[4 * s + 1 + d]
+    return g[4 * ns + ["q_h2_h", "q_h2_cm", "lj_a", "lj_b"].index(name)]
+
+
+PARAMS = [("site_q", 0), ("site_q", 1), ("site_pos", 0), ("site_pos", 5), ("site_pos", 14), ("q_h2_h", 0),
+          ("q_h2_cm", 0), ("lj_a", 0), ("lj_b", 0)]
+
+
+def test_oracle_gradient_matches_finite_differences():
+    """The oracle's Hellmann-Feynman gradient of the two lowest levels
+    against central differences (h = 1e-4 relative) of dense eigenvalues."""
+    spec, _ = cs.cage_case(4, 2, 6, 6, "All")
+    o = O.Oracle(spec)
+    w, v = np.linalg.eigh(_dense(o))
+    assert w[1] - w[0] > 1e-3 and w[2] - w[1] > 1e-3
+    ns = len(spec["site_q"])
+    g0 = o.hf_gradient(v[:, 0], [1.0])
+    g1 = o.hf_gradient(v[:, 1], [1.0])
+    g01 = o.hf_gradient(np.stack([v[:, 0], v[:, 1]]), [1.0, -2.0])
+    assert np.allclose(g01, g0 - 2.0 * g1, rtol=1e-12, atol=1e-12 * np.max(np.abs(g0)))
+    for name, idx in PARAMS:
+        x = _value(spec, name, idx)
+        h = 1e-4 * abs(x)
+        wp = np.linalg.eigvalsh(_dense(O.Oracle(_perturb(spec, name, idx, h))))
+        wm = np.linalg.eigvalsh(_dense(O.Oracle(_perturb(spec, name, idx, -h))))
+        for lev, g in ((0, g0), (1, g1)):
+            fd = (wp[lev] - wm[lev]) / (2 * h)
+            an = _grad_entry(g, ns, name, idx)
+            assert abs(an - fd) <= 1e-5 * max(abs(fd), 1e-3 * abs(x) ** -1), (name, idx, lev, an, fd)
+
+
+@pytest.mark.gpu
+def test_gpu_gradient_matches_oracle():
+    """Same quadratic form on the same (random, unit) vectors and weights."""
+    import bsl_amd as B
+    spec, kw = cs.cage_case(5, 3, 8, 9, "All")
+    o = O.Oracle(spec)
+    c = B.Calculator(**kw)
+    rng = np.random.default_rng(31)
+    psi = rng.uniform(-1, 1, (2, o.N))
+    psi /= np.linalg.norm(psi, axis=1, keepdims=True)
+    g = np.array([0.7, -0.3])
+    ref = o.hf_gradient(psi, g)
+    ns = len(spec["site_q"])
+    got = c.hf_gradient(psi, g)
+    flat = np.zeros(4 * ns + 4)
+    flat[0:4 * ns:4] = got["site_q"]
+    flat[:4 * ns].reshape(ns, 4)[:, 1:4] = got["site_pos"]
+    flat[4 * ns:] = [got["q_h2_h"], got["q_h2_cm"], got["lj_a"], got["lj_b"]]
+    scale = np.max(np.abs(ref[:4 * ns]))
+    assert np.max(np.abs(flat[:4 * ns] - ref[:4 * ns])) / scale < 1e-10
+    for e in range(4 * ns, 4 * ns + 4):
+        assert abs(flat[e] - ref[e]) <= 1e-10 * abs(ref[e])
+
+
+NITER = 800
+
+
+@pytest.mark.gpu
+def test_gpu_gradient_matches_finite_differences():
+    """Gradients of the lowest levels from converged Ritz vectors against
+    central differences (h = 1e-4 relative) of the GPU's refined levels."""
+    import bsl_amd as B
+    spec, kw = cs.cage_case(6, 3, 8, 8, "All")
+    ns = len(spec["site_q"])
+    c = B.Calculator(**kw)
+    e, psi, res = c.levels(NITER, 2)
+    assert np.all(res < 1e-6)
+    grads = [c.hf_gradient(psi[k], [1.0]) for k in range(2)]
+    c.close()
+    kwargs_of = {"site_q": lambda s: dict(kw, sites=(s["site_pos"], s["site_q"], kw["sites"][2])),
+                 "site_pos": lambda s: dict(kw, sites=(s["site_pos"], s["site_q"], kw["sites"][2]))}
+    for name, idx in PARAMS:
+        x = _value(spec, name, idx)
+        h = 1e-4 * abs(x)
+        fds = []
+        for sgn in (1, -1):
+            sp = _perturb(spec, name, idx, sgn * h)
+            kk = kwargs_of[name](sp) if name in kwargs_of else dict(kw, **{name: sp[name]})
+            with B.Calculator(**kk) as cp:
+                fds.append(cp.levels(NITER, 2)[0])
+        for lev in range(2):
+            fd = (fds[0][lev] - fds[1][lev]) / (2 * h)
+            gr = grads[lev]
+            an = {"site_q": lambda: gr["site_q"][idx], "site_pos": lambda: gr["site_pos"].reshape(-1)[idx]}.get(
+                name, lambda: gr[name])()
+            assert abs(an - fd) <= 1e-4 * max(abs(fd), 1e-3 / abs(x)), (name, idx, lev, an, fd)
+
+
+@pytest.mark.gpu
+def test_autograd_wrapper_chain_rule():
+    """torch.autograd through the water-model parameters: dL/d(q_O, q_H,
+    sigma_O, eps_O) of L = E0 + 2 E1 against central differences."""
+    import torch
+    import bsl_amd as B
+    from bsl_amd import adjoint
+    spec, kw = cs.cage_case(6, 3, 8, 8, "All")
+    pos, q, typ, ljf = cs.cage11_sites()
+    kinds = ["O" if t == ord("O") else "H" for t in typ]
+    calc_kw = {k: v for k, v in kw.items() if k not in ("potential", "sites", "q_h2_h", "q_h2_cm", "lj_a",
+                                                        "lj_b")}
+    config = {"calc": calc_kw, "lj_sites": ljf, "niter": NITER, "count": 2}
+    q_h2 = torch.tensor([0.4932, -0.9864], dtype=torch.float64)
+
+    def loss(charges, lj):
+        site_pos, site_q, lj_ab = adjoint.water_model_sites(charges, lj, torch.as_tensor(pos), kinds)
+        e = adjoint.cage_levels(site_pos, site_q, q_h2, lj_ab, config)
+        return e[0] + 2.0 * e[1]
+
+    charges = torch.tensor([-0.8476, 0.4238], dtype=torch.float64, requires_grad=True)
+    lj = torch.tensor([0.3166, 0.650, 0.296, 0.2852], dtype=torch.float64, requires_grad=True)
+    L = loss(charges, lj)
+    L.backward()
+    for which, t, i in (("charges", charges, 0), ("charges", charges, 1), ("lj", lj, 0), ("lj", lj, 1)):
+        x0 = t.detach().clone()
+        h = 1e-4 * abs(float(x0[i]))
+        vals = []
+        for sgn in (1, -1):
+            x = x0.clone()
+            x[i] += sgn * h
+            with torch.no_grad():
+                vals.append(float(loss(x, lj.detach()) if which == "charges" else loss(charges.detach(), x)))
+        fd = (vals[0] - vals[1]) / (2 * h)
+        an = float(t.grad[i])
+        assert abs(an - fd) <= 1e-4 * max(abs(fd), 1e-3), (which, i, an, fd)

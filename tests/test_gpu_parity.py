@@ -103,7 +103,9 @@ def test_hv_device_entry_point():
     v = np.random.default_rng(9).uniform(-1, 1, c.N)
     u = np.random.default_rng(10).uniform(-1, 1, c.N)
     ref = c.hv(v, u)
-    hip = C.CDLL("libamdhip64.so")
+    # the HIP runtime the library is linked against, by its soname (a torch
+    # import may have loaded a second runtime under the bare file name)
+    hip = C.CDLL("libamdhip64.so.7")
     dv, du = C.c_void_p(), C.c_void_p()
     nbytes = 8 * c.N
     assert hip.hipMalloc(C.byref(dv), C.c_size_t(nbytes)) == 0

@@ -23,44 +23,35 @@ class Calculator:
     basis ("full"/"reduced"), precision ("fp64"/"fp32"), potential
     ("iho"/"cage") and its parameters."""
 
-    def __init__(self, *, nx, ny=None, nz=None, x_max, y_max=None, z_max=None, lmax, n_theta, n_phi,
-                 mass, rot_const, ceiling, symmetry="All", basis="full", calculator="noquad",
-                 precision="fp64", potential="iho", iho_factor=0.0, iho_ceil=0.0, sites=None,
-                 q_h2_h=physics.Q_H2_H, q_h2_cm=physics.Q_H2_CM, h2_half_bond=physics.H2_HALF_BOND,
-                 lj_a=physics.LJ_A_REF, lj_b=physics.LJ_B_REF):
-        p = Params()
-        p.nx, p.ny, p.nz = nx, ny or nx, nz or nx
-        p.x_max, p.y_max, p.z_max = x_max, y_max or x_max, z_max or x_max
-        p.lmax, p.n_theta, p.n_phi = lmax, n_theta, n_phi
-        p.mass, p.rot_const, p.ceiling = mass, rot_const, ceiling
-        p.symmetry = SYM[symmetry]
-        p.basis = BASIS[basis]
-        p.calculator = CALC[calculator]
-        p.precision = PREC[precision]
-        p.potential = POT[potential]
-        p.iho_factor, p.iho_ceil = iho_factor, iho_ceil
-        self._keep = []
-        if potential == "cage":
-            pos, q, lj = sites
-            pos = np.ascontiguousarray(pos, dtype=np.float64).ravel()
-            q = np.ascontiguousarray(q, dtype=np.float64)
-            lj = np.ascontiguousarray(lj, dtype=np.int32)
-            self._keep = [pos, q, lj]
-            p.n_sites = len(q)
-            p.site_pos, p.site_q, p.site_lj = pos.ctypes.data, q.ctypes.data, lj.ctypes.data
-        p.q_h2_h, p.q_h2_cm, p.h2_half_bond, p.lj_a, p.lj_b = q_h2_h, q_h2_cm, h2_half_bond, lj_a, lj_b
+    def __init__(self, **kw):
+        p, keep = make_params(**kw)
+        self._keep = keep
         self.params = p
-        self.shape = (p.nx, p.ny, p.nz)
-        self.nn = (lmax + 1) ** 2
         h = C.c_void_p()
         check(lib().bsl_create(C.byref(p), C.byref(h)))
+        self._owned = True
+        self._attach(h)
+
+    def _attach(self, h):
+        p = self.params
+        self.shape = (p.nx, p.ny, p.nz)
+        self.nn = (p.lmax + 1) ** 2
         self.h = h
         self.N = lib().bsl_basis_size(h)
         self.N_krylov = lib().bsl_krylov_size(h)
 
+    @classmethod
+    def _member(cls, h, params, owner):
+        """Non-owning view of a batch member's context."""
+        self = cls.__new__(cls)
+        self.params, self._keep, self._owned, self._owner = params, [], False, owner
+        self._attach(h)
+        return self
+
     def close(self):
         if getattr(self, "h", None):
-            lib().bsl_destroy(self.h)
+            if getattr(self, "_owned", False):
+                lib().bsl_destroy(self.h)
             self.h = None
 
     def __del__(self):
@@ -166,6 +157,84 @@ class Calculator:
         buf = C.create_string_buffer(128)
         check(lib().bsl_potential_kernel(self.h, buf, 128))
         return buf.value.decode()
+
+
+def make_params(*, nx, ny=None, nz=None, x_max, y_max=None, z_max=None, lmax, n_theta, n_phi, mass, rot_const,
+                ceiling, symmetry="All", basis="full", calculator="noquad", precision="fp64", potential="iho",
+                iho_factor=0.0, iho_ceil=0.0, sites=None, q_h2_h=physics.Q_H2_H, q_h2_cm=physics.Q_H2_CM,
+                h2_half_bond=physics.H2_HALF_BOND, lj_a=physics.LJ_A_REF, lj_b=physics.LJ_B_REF):
+    """bsl_params from keyword arguments; returns (params, arrays to keep alive)."""
+    p = Params()
+    p.nx, p.ny, p.nz = nx, ny or nx, nz or nx
+    p.x_max, p.y_max, p.z_max = x_max, y_max or x_max, z_max or x_max
+    p.lmax, p.n_theta, p.n_phi = lmax, n_theta, n_phi
+    p.mass, p.rot_const, p.ceiling = mass, rot_const, ceiling
+    p.symmetry = SYM[symmetry]
+    p.basis = BASIS[basis]
+    p.calculator = CALC[calculator]
+    p.precision = PREC[precision]
+    p.potential = POT[potential]
+    p.iho_factor, p.iho_ceil = iho_factor, iho_ceil
+    keep = []
+    if potential == "cage":
+        pos, q, lj = sites
+        pos = np.ascontiguousarray(pos, dtype=np.float64).ravel()
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        lj = np.ascontiguousarray(lj, dtype=np.int32)
+        keep = [pos, q, lj]
+        p.n_sites = len(q)
+        p.site_pos, p.site_q, p.site_lj = pos.ctypes.data, q.ctypes.data, lj.ctypes.data
+    p.q_h2_h, p.q_h2_cm, p.h2_half_bond, p.lj_a, p.lj_b = q_h2_h, q_h2_cm, h2_half_bond, lj_a, lj_b
+    return p, keep
+
+
+class Batch:
+    """Independent recurrences advanced in lockstep on one GPU (bsl_batch_*):
+    `members` is a list of Calculator keyword dicts (shapes, symmetry blocks
+    and precisions may differ).  Member i starts from seed + i."""
+
+    def __init__(self, members):
+        built = [make_params(**m) for m in members]
+        self._keep = [k for _, k in built]
+        arr = (Params * len(built))(*[p for p, _ in built])
+        self._params = [p for p, _ in built]
+        h = C.c_void_p()
+        check(lib().bsl_batch_create(len(built), arr, C.byref(h)))
+        self.h = h
+        self.n = len(built)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().bsl_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def member(self, i):
+        h = C.c_void_p()
+        check(lib().bsl_batch_member(self.h, i, C.byref(h)))
+        return Calculator._member(h, self._params[i], self)
+
+    def lanczos(self, niter, start="random", seed=0, graph=True):
+        """Per member (alpha, beta, beta2), as Calculator.lanczos."""
+        a = np.zeros((self.n, niter))
+        b = np.zeros((self.n, niter + 1))
+        b2 = np.zeros((self.n, niter + 1))
+        check(lib().bsl_batch_lanczos(self.h, niter, START[start], seed, 1 if graph else 0, a, b, b2))
+        return [(a[i], b[i], b2[i]) for i in range(self.n)]
+
+    def time_steps(self, warmup, steps, graph=True):
+        """Wall time (ms) of `steps` lockstep iterations of all members."""
+        ms = np.zeros(1)
+        check(lib().bsl_batch_time_steps(self.h, warmup, steps, 1 if graph else 0, ms))
+        return float(ms[0])
 
 
 def split_gradient(grad, ns):

@@ -22,7 +22,8 @@ EXPORTS = (
     "bsl_create", "bsl_destroy", "bsl_last_error", "bsl_basis_size", "bsl_krylov_size", "bsl_hv",
     "bsl_hv_device", "bsl_lanczos", "bsl_lancbis", "bsl_trivec", "bsl_ritz", "bsl_rayleigh",
     "bsl_water_sites", "bsl_get_vblocks", "bsl_get_vgrid", "bsl_time_steps", "bsl_step_bytes",
-    "bsl_potential_kernel", "bsl_hf_gradient",
+    "bsl_potential_kernel", "bsl_hf_gradient", "bsl_batch_create", "bsl_batch_destroy", "bsl_batch_size",
+    "bsl_batch_member", "bsl_batch_lanczos", "bsl_batch_time_steps",
 )
 
 
@@ -74,6 +75,12 @@ def load():
     lib.bsl_step_bytes.argtypes = [C.c_void_p, _dp]
     lib.bsl_potential_kernel.argtypes = [C.c_void_p, C.c_char_p, C.c_int32]
     lib.bsl_hf_gradient.argtypes = [C.c_void_p, C.c_int32, _dp, _dp, _dp]
+    lib.bsl_batch_create.argtypes = [C.c_int32, C.c_void_p, C.c_void_p]
+    lib.bsl_batch_destroy.argtypes = [C.c_void_p]
+    lib.bsl_batch_size.argtypes = [C.c_void_p]
+    lib.bsl_batch_member.argtypes = [C.c_void_p, C.c_int32, C.c_void_p]
+    lib.bsl_batch_lanczos.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_uint64, C.c_int32, _dp, _dp, _dp]
+    lib.bsl_batch_time_steps.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, _dp]
     return lib
 
 

@@ -552,15 +552,20 @@ int lanczos_step(bsl_ctx *c, const DevOperator &op, int j, hipEvent_t *ev)
 {
     int rc = launch_hv(op, c->d_X, c->d_R, c->d_scal, c->stream, ev);
     if (rc) return rc;
-    rc = launch_update(op, c->d_X, c->d_R, c->d_scal, c->d_alpha ? c->d_alpha + (j - 1) : nullptr,
-                       c->d_beta ? c->d_beta + j : nullptr,
-                       c->d_beta2 ? c->d_beta2 + j : nullptr, c->stream);
+    (void)j;  // the device step counter indexes the coefficient outputs
+    rc = launch_update(op, c->d_X, c->d_R, c->d_scal, c->d_alpha, c->d_beta, c->d_beta2, c->stream);
     if (rc) return rc;
     if (ev) BSL_HIP(hipEventRecord(ev[3], c->stream));
     std::swap(c->d_X, c->d_R);
     return BSL_OK;
 }
 }  // namespace
+
+namespace bsl {
+int ctx_reserve_steps(bsl_ctx *c, int niter) { return ensure_coeffs(c, niter); }
+int ctx_start(bsl_ctx *c, const double *v0, int start, uint64_t seed) { return start_lanczos(c, v0, start, seed); }
+int ctx_step(bsl_ctx *c) { return lanczos_step(c, c->op, 0, nullptr); }
+}  // namespace bsl
 
 extern "C" int bsl_hv(bsl_ctx *c, const double *vec, double *uec)
 {
@@ -919,7 +924,9 @@ extern "C" int bsl_time_steps(bsl_ctx *c, int32_t warmup, int32_t steps, double 
 {
     if (!c || steps <= 0 || warmup < 0 || !ms) return fail(BSL_E_ARG, "bad arguments");
     BSL_HIP(hipSetDevice(c->device));
-    int rc = ensure_coeffs(c, warmup + steps + 1);
+    // every step writes its coefficients at the device step counter
+    const int ksteps = std::min(steps, 64);
+    int rc = ensure_coeffs(c, warmup + steps + ksteps + 1);
     if (rc) return rc;
     if ((rc = start_lanczos(c, nullptr, BSL_START_RANDOM, 0))) return rc;
     int j = 1;
@@ -932,19 +939,18 @@ extern "C" int bsl_time_steps(bsl_ctx *c, int32_t warmup, int32_t steps, double 
     BSL_HIP(hipEventRecord(e0, c->stream));
     const int j0 = j;
     for (int s = 0; s < steps; s++, j++)
-        if ((rc = lanczos_step(c, c->op, std::min(j, warmup + steps), nullptr))) return rc;
+        if ((rc = lanczos_step(c, c->op, j, nullptr))) return rc;
     BSL_HIP(hipEventRecord(e1, c->stream));
     BSL_HIP(hipEventSynchronize(e1));
     float total = 0;
     BSL_HIP(hipEventElapsedTime(&total, e0, e1));
     (void)j0;
     // per-kernel durations: events around each kernel group, fewer steps
-    const int ksteps = std::min(steps, 64);
     std::vector<hipEvent_t> ev((size_t)4 * ksteps);
     for (auto &e : ev) BSL_HIP(hipEventCreate(&e));
     double kin = 0, pot = 0, upd = 0;
     for (int s = 0; s < ksteps; s++)
-        if ((rc = lanczos_step(c, c->op, std::min(j, warmup + steps), &ev[(size_t)4 * s]))) return rc;
+        if ((rc = lanczos_step(c, c->op, j + s, &ev[(size_t)4 * s]))) return rc;
     BSL_HIP(hipStreamSynchronize(c->stream));
     for (int s = 0; s < ksteps; s++) {
         float a = 0, b = 0, d = 0;

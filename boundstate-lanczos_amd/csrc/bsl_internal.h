@@ -18,7 +18,8 @@ namespace bsl {
 struct Scal {
     double sx, sr;
     double alpha, beta, beta2;
-    double pad[3];
+    long long step;  // completed Lanczos steps: alpha/beta outputs are indexed by it
+    double pad[2];
 };
 
 // error reporting (thread-local message + code)
@@ -115,7 +116,9 @@ int launch_hv(const DevOperator &op, const double *X, double *R, Scal *sc, hipSt
               hipEvent_t *ev /* 3 events or null */);
 // the potential kernel alone (used by the autotuner)
 int launch_potential(const DevOperator &op, const double *X, double *R, Scal *sc, hipStream_t s);
-// alpha from the partials, R -= alpha (sx X), beta^2 = |R|^2 and the scalars of the swap
+// alpha from the partials, R -= alpha (sx X), beta^2 = |R|^2 and the scalars
+// of the swap; alpha_out[step], beta_out[step + 1], beta2_out[step + 1] with
+// step the device counter of completed steps, which advances by one
 int launch_update(const DevOperator &op, const double *X, double *R, Scal *sc, double *alpha_out, double *beta_out,
                   double *beta2_out, hipStream_t s);
 int launch_init_scal(Scal *sc, double sx, double sr, hipStream_t s);
@@ -144,6 +147,13 @@ int launch_vblock_gemm(const double *vg, const double *h, void *vpk, int vprec, 
 // kernel (npts grid points per workgroup)
 int launch_weight_grid(const double *vg, void *W, int wprec, const double *wt, const double *wp, int64_t P,
                        int na, int nb, int npts, hipStream_t s);
+// context run helpers (bsl_capi.cpp) used by the batch driver: coefficient
+// capacity for niter steps, start vector, one step on the context stream
+// (swaps the context's X/R; enqueues kernels only, so it can be captured)
+int ctx_reserve_steps(bsl_ctx *c, int niter);
+int ctx_start(bsl_ctx *c, const double *v0, int start, uint64_t seed);
+int ctx_step(bsl_ctx *c);
+
 // Hellmann-Feynman adjoint (bsl_adjoint.hip)
 int launch_hf_density(const double *psi, int nvec, const double *g, const double *Y, const double *wq, double *vg,
                       int64_t P, int nk, int Q, double ceiling, hipStream_t s);

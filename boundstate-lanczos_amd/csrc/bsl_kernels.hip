@@ -381,7 +381,7 @@ __global__ __launch_bounds__(kWG) void k_update(const double *__restrict__ X, do
         partial_b[blockIdx.x] = b;
         if (blockIdx.x == 0) {
             sc->alpha = a;
-            if (alpha_out) *alpha_out = a;
+            if (alpha_out) alpha_out[sc->step] = a;
         }
     }
 }
@@ -402,13 +402,16 @@ __global__ __launch_bounds__(kWG) void k_finalize(const double *__restrict__ par
         const double sx = sc->sx;
         sc->beta2 = t;
         sc->beta = beta;
-        if (beta_out) *beta_out = beta;
-        if (beta2_out) *beta2_out = t;
+        const long long j = sc->step;
+        if (beta_out) beta_out[j + 1] = beta;
+        if (beta2_out) beta2_out[j + 1] = t;
+        sc->step = j + 1;
         sc->sx = 1. / beta;
         sc->sr = (-beta) * sx;
     } else if (mode == 1) {
         sc->sx = 1.0 / sqrt(t);
         sc->sr = 0.0;
+        sc->step = 0;
     } else {
         sc->alpha = t;
     }
@@ -421,6 +424,7 @@ __global__ void k_set_scal(Scal *sc, double sx, double sr)
     sc->alpha = 0.0;
     sc->beta = 0.0;
     sc->beta2 = 0.0;
+    sc->step = 0;
 }
 
 __global__ void k_fill(double *p, int64_t n, double v)

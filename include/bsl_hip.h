@@ -171,6 +171,24 @@ int bsl_step_bytes(const bsl_ctx *ctx, double *out);
 // derivatives).
 int bsl_hf_gradient(bsl_ctx *ctx, int32_t nvec, const double *psi, const double *g, double *grad);
 
+// ---- batched recurrences (no reference counterpart: the reference runs one
+// recurrence per process, lanczosEigenValuesVectors.cpp:158-242) ----
+// A batch owns n contexts (cages, para/ortho blocks, grid sizes may differ)
+// whose Loop-1 recurrences advance in lockstep; with use_graph != 0 each pair
+// of lockstep iterations is one replayed hipGraph whose member chains run as
+// independent branches.  Member i starts from seed + i.
+typedef struct bsl_batch bsl_batch;
+int bsl_batch_create(int32_t n, const bsl_params *params /* [n] */, bsl_batch **out);
+int bsl_batch_destroy(bsl_batch *b);
+int bsl_batch_size(const bsl_batch *b);
+// the context of member i (owned by the batch: Ritz vectors, Rayleigh, Hv)
+int bsl_batch_member(bsl_batch *b, int32_t i, bsl_ctx **out);
+// alpha [n][niter], beta and beta2 [n][niter + 1] (index 0 zero), as bsl_lanczos
+int bsl_batch_lanczos(bsl_batch *b, int32_t niter, int32_t start, uint64_t seed, int32_t use_graph,
+                      double *alpha, double *beta, double *beta2);
+// ms[0]: wall time (ms, HIP events) of `steps` lockstep iterations after `warmup`
+int bsl_batch_time_steps(bsl_batch *b, int32_t warmup, int32_t steps, int32_t use_graph, double *ms);
+
 // Name (with template arguments, as a profiler prints it) of the potential
 // kernel the context launches, e.g. "k_vblock<15, 21, double, 1>"; the last
 // argument is the occupancy variant picked by the autotuner at bsl_create.

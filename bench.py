@@ -64,6 +64,71 @@ def quad_leg(B, args):
     return out
 
 
+def batch_members(args, calculator, precision, n=32):
+    """16 synthetic cages (seeds 0-15) x {para (even l), ortho (odd l)} on
+    grids cycled through 41^3, 47^3, 54^3 (same box, lmax 5, 40 x 70)."""
+    from bsl_amd import cage, physics
+    out = []
+    grids = (41, 47, 54)
+    for i in range(n):
+        seed, block = i // 2, ("Even", "Odd")[i % 2]
+        pos, q, lj = cage.synthetic_cage(seed)
+        lj_a, lj_b = cage.baseline_lj()
+        out.append(dict(nx=grids[i % 3], x_max=args.x_max, lmax=5, n_theta=40, n_phi=70, mass=2.016,
+                        rot_const=physics.cm1_to_kjmol(59.322), ceiling=18.0, symmetry=block, basis="reduced",
+                        calculator=calculator, precision=precision, potential="cage", sites=(pos, q, lj),
+                        lj_a=lj_a, lj_b=lj_b))
+    return out
+
+
+def batch_leg(B, args):
+    """32 independent recurrences in lockstep (hipGraph-replayed pairs of
+    iterations), FP32 potential, FP64 recurrence: aggregate member steps/s."""
+    out = {}
+    for calculator in ("quad", "noquad"):
+        t0 = time.perf_counter()
+        bt = B.Batch(batch_members(args, calculator, "fp32"))
+        prep = time.perf_counter() - t0
+        ms = bt.time_steps(4, args.batch_steps, graph=True)
+        bt.close()
+        out[calculator] = {"members": 32, "member_steps_per_s": 32 * args.batch_steps / (ms / 1e3),
+                           "ms_per_lockstep_step": ms / args.batch_steps, "prep_s": prep,
+                           "steps_timed": args.batch_steps}
+        print("batch %s fp32: prep %.1f s, %.3f ms per lockstep step, %.0f member-steps/s"
+              % (calculator, prep, ms / args.batch_steps, out[calculator]["member_steps_per_s"]),
+              file=sys.stderr, flush=True)
+    return out
+
+
+COPY_PROBE = r"""
+import json, torch
+a = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+b = torch.empty_like(a)
+for _ in range(3):
+    torch.mul(a, 1.0, out=b)
+e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+torch.cuda.synchronize()
+e0.record()
+for _ in range(20):
+    torch.mul(a, 1.0, out=b)
+e1.record()
+torch.cuda.synchronize()
+print(json.dumps({"GBs": 20 * 2 * a.numel() * 8 / (e0.elapsed_time(e1) * 1e-3) / 1e9}))
+"""
+
+
+def copy_bandwidth():
+    """Achievable HBM bandwidth of a plain copy kernel (1 GiB read + 1 GiB
+    written, torch elementwise), in a child process so that torch's HIP
+    runtime never shares a process with the library's."""
+    import subprocess
+    try:
+        r = subprocess.run([sys.executable, "-c", COPY_PROBE], capture_output=True, text=True, timeout=300)
+        return json.loads(r.stdout.strip().splitlines()[-1])["GBs"]
+    except Exception:
+        return None
+
+
 def traffic_from_profiles(tag, kernel):
     """HBM bytes per launch of `kernel` measured with two rocprofv3 --pmc
     runs (FETCH_SIZE, then WRITE_SIZE, each scaled by what the counter reads
@@ -123,10 +188,12 @@ def main():
     ap.add_argument("--n", type=int, default=54)
     ap.add_argument("--x-max", type=float, default=0.8)
     ap.add_argument("--eig-steps", type=int, default=3000)
-    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--cpu-steps", type=int, default=30)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-eig", action="store_true")
     ap.add_argument("--no-mixed", action="store_true", help="skip the FP32-potential context")
+    ap.add_argument("--batch-steps", type=int, default=20,
+                    help="lockstep iterations timed for the 32-member batch (0 skips the batch leg)")
     ap.add_argument("--quad-steps", type=int, default=200,
                     help="steps timed per quadrature-calculator precision (0 skips the quadrature leg)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -147,6 +214,7 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"copy_kernel_bytes_read": a.numel() * 8, "copy_kernel_bytes_written": b.numel() * 8}))
         return
+    copy_gbs = copy_bandwidth() if not args.no_cpu else None
     import bsl_amd as B
 
     t0 = time.perf_counter()
@@ -180,6 +248,7 @@ def main():
     del c32
 
     quad = quad_leg(B, args) if args.quad_steps > 0 else None
+    batch = batch_leg(B, args) if args.batch_steps > 0 else None
 
     cpu = None
     if not args.no_cpu:
@@ -211,6 +280,7 @@ def main():
             "prep_s": {"fp64": prep64, "fp32": prep32},
             "per_step_ms": {"potential": ms[1], "kinetic": ms[2], "update": ms[3]},
             "quadrature": quad,
+            "batch_32": batch,
             "algorithmic_bytes_per_step": sb[3],
             "step_bytes_roofline_frac": sb[3] / (ms[0] / args.steps * 1e-3) / 1e9 / HBM_PEAK_GBS,
         },
@@ -223,6 +293,8 @@ def main():
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic_from_profiles("cage54_lmax5_fp64", kname),
             "algorithmic_bytes_per_launch": sb[0],
+            "achievable_copy_GBs": copy_gbs,
+            "frac_of_copy": achieved / copy_gbs if copy_gbs else None,
         },
         "cpu_baseline": cpu,
     }

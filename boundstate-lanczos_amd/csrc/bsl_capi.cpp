@@ -393,6 +393,9 @@ extern "C" int bsl_create(const bsl_params *pp, bsl_ctx **out)
     if ((rc = dev_upload(&d_h, h))) return rc;
     c->op64 = op;
     c->op64.vpk = nullptr;
+    c->op64.kprec = BSL_PREC_FP64;
+    // mixed precisions contract the kinetic term in FP32 as well
+    op.kprec = p.precision == BSL_PREC_FP64 ? BSL_PREC_FP64 : BSL_PREC_FP32;
     if ((rc = build_vpk(c.get(), c->op64, BSL_PREC_FP64, d_h))) return rc;
     if (p.calculator == BSL_CALC_QUAD) {
         // weighted grid W[P][a][half*nb + b] and the transform tables

@@ -69,181 +69,219 @@ __device__ inline double total_of_partials(const double *__restrict__ partial, u
 }
 
 // ---------------------------------------------------------------------------
-// kinetic term, x and y axes: one workgroup per (k, n) plane
+// kinetic terms on MFMA (v_mfma_f64_16x16x4f64): the contractions are small
+// GEMMs with the slab (or column block) staged in LDS, zero padded to 64,
+// and the Toeplitz operand read from a reflected first row, t[64 + d] =
+// t(|d|).  FP64 accumulator layout: lane l holds rows (l/16) + 4*q, q = 0..3,
+// of column l%16; A operand lane l: row l%16, k = l/16; B operand lane l:
+// k = l/16, column l%16.
 // ---------------------------------------------------------------------------
-// The plane (sx * X) is staged in LDS, zero padded to multiples of 4; each
-// thread accumulates a 4x4 tile of outputs.  T is symmetric Toeplitz, so
-// only its first row is needed; it is stored reflected, trow[pad + d] =
-// t(|d|), and walked as a sliding window.
-// R = sr * R + mask[n] * ((Tx + Ty) v + rot[n] v)
-__global__ __launch_bounds__(kWG) void k_kin_xy(const double *__restrict__ X, double *__restrict__ R,
-                                                const double *__restrict__ tx, const double *__restrict__ ty,
-                                                const double *__restrict__ rot, const double *__restrict__ mask,
-                                                const Scal *__restrict__ sc, int nx, int ny, int nz)
+typedef double kd4 __attribute__((ext_vector_type(4)));
+typedef float kf4 __attribute__((ext_vector_type(4)));
+
+__device__ inline kd4 kmfma(double a, double b, kd4 c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+__device__ inline kf4 kmfma(float a, float b, kf4 c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+
+template <typename T>
+struct KinAcc {
+    typedef kd4 type;
+};
+template <>
+struct KinAcc<float> {
+    typedef kf4 type;
+};
+
+// accumulator row of item q in lane group kq: FP64 instruction kq + 4q,
+// FP32 instruction 4kq + q
+template <typename T>
+__device__ inline int kin_row(int kq, int q)
 {
-    extern __shared__ double lds[];
-    const int nxp = (nx + 3) & ~3, nyp = (ny + 3) & ~3;
-    const int ld = nxp + 2;
-    double *pl = lds;                    // [nyp][ld]
-    double *rx = pl + (size_t)nyp * ld;  // [2*nxp+1], rx[nxp + d] = t_x(|d|)
-    double *ry = rx + 2 * nxp + 1;       // [2*nyp+1]
+    return sizeof(T) == 8 ? kq + 4 * q : 4 * kq + q;
+}
+
+constexpr int kSlabLd = 65;
+
+// x and y axes of one (k, n) slab: D[j][i] = sum_ip v[j][ip] t_x(|ip - i|)
+// + sum_jp t_y(|j - jp|) v[jp][i], both into the same accumulator tiles;
+// each wave owns a 2 x 2 block of 16 x 16 output tiles.  T: FP64, or FP32
+// (products and accumulation in FP32, the recurrence vectors stay FP64).
+// R = sr * R + mask[n] * (D + rot[n] v)
+template <typename T>
+__global__ __launch_bounds__(kWG) void k_kin_xy_mfma(const double *__restrict__ X, double *__restrict__ R,
+                                                     const double *__restrict__ tx, const double *__restrict__ ty,
+                                                     const double *__restrict__ rot, const double *__restrict__ mask,
+                                                     const Scal *__restrict__ sc, int nx, int ny, int nz)
+{
+    typedef typename KinAcc<T>::type acc_t;
+    extern __shared__ __align__(16) unsigned char kin_lds[];
+    T *pl = (T *)kin_lds;              // [64][kSlabLd]
+    T *rx = pl + 64 * kSlabLd;         // [129]
+    T *ry = rx + 129;                  // [129]
     const int k = blockIdx.x, n = blockIdx.y;
     const double sx = sc->sx, sr = sc->sr;
     const size_t base = ((size_t)n * nz + k) * (size_t)ny * nx;
-    // stage the plane: all of this thread's loads are issued before any is
-    // consumed (at most 16 per thread: nx * ny <= 4096)
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int l16 = lane & 15, kq = lane >> 4;
+    const int tj = (ny + 15) >> 4, ti = (nx + 15) >> 4;
+    const int bj = (tj + 1) >> 1, bi = (ti + 1) >> 1;  // at most 2 x 2 blocks: one per wave
+    const int j0 = 32 * (wid / bi), i0 = 32 * (wid % bi);
+    const bool active = wid < bj * bi;
+    // this lane's R outputs are loaded together with the slab, so that both
+    // latencies overlap
+    double rp[2][2][4];
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++)
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const int j = j0 + 16 * a + kin_row<T>(kq, q), i = i0 + 16 * b + l16;
+                rp[a][b][q] = (active && j < ny && i < nx) ? R[base + (size_t)j * nx + i] : 0.0;
+            }
     {
         const double *Xp = X + base;
-        const int np = nx * ny;
         double tmp[16];
 #pragma unroll
         for (int e = 0; e < 16; e++) {
-            const int t = threadIdx.x + e * kWG;
-            tmp[e] = t < np ? Xp[t] : 0.0;
-        }
-        for (int t = threadIdx.x; t < nyp * ld; t += blockDim.x) {
-            const int j = t / ld, i = t - j * ld;
-            if (j >= ny || i >= nx) pl[t] = 0.0;
+            const int t = threadIdx.x + e * kWG, j = t >> 6, i = t & 63;
+            tmp[e] = (j < ny && i < nx) ? Xp[j * nx + i] : 0.0;
         }
 #pragma unroll
         for (int e = 0; e < 16; e++) {
-            const int t = threadIdx.x + e * kWG;
-            if (t < np) {
-                const int j = t / nx, i = t - j * nx;
-                pl[j * ld + i] = sx * tmp[e];
-            }
+            const int t = threadIdx.x + e * kWG, j = t >> 6, i = t & 63;
+            pl[j * kSlabLd + i] = (T)(sx * tmp[e]);
         }
     }
-    for (int t = threadIdx.x; t < 2 * nxp + 1; t += blockDim.x) {
-        const int d = abs(t - nxp);
-        rx[t] = d < nx ? tx[d] : 0.0;
-    }
-    for (int t = threadIdx.x; t < 2 * nyp + 1; t += blockDim.x) {
-        const int d = abs(t - nyp);
-        ry[t] = d < ny ? ty[d] : 0.0;
+    for (int t = threadIdx.x; t < 129; t += blockDim.x) {
+        const int d = abs(t - 64);
+        rx[t] = d < nx ? (T)tx[d] : (T)0;
+        ry[t] = d < ny ? (T)ty[d] : (T)0;
     }
     __syncthreads();
-    const int tiles_i = nxp >> 2, ntiles = tiles_i * (nyp >> 2);
+    const int kx = (nx + 3) >> 2, ky = (ny + 3) >> 2;
     const double rn = rot[n], mn = mask[n];
-    for (int tile = threadIdx.x; tile < ntiles; tile += blockDim.x) {
-        const int j0 = (tile / tiles_i) << 2, i0 = (tile % tiles_i) << 2;
-        double acc[4][4];
+    if (active) {
+        acc_t acc[2][2];
 #pragma unroll
-        for (int r = 0; r < 4; r++)
+        for (int a = 0; a < 2; a++)
 #pragma unroll
-            for (int c = 0; c < 4; c++) acc[r][c] = 0.0;
-        // x: acc[r][c] += t(|i0+c-ip|) * v[j0+r][ip]
-        {
-            double w0 = rx[nxp + i0 + 0], w1 = rx[nxp + i0 + 1], w2 = rx[nxp + i0 + 2], w3 = rx[nxp + i0 + 3];
-            for (int ip = 0; ip < nx; ip++) {
-                const double p0 = pl[(j0 + 0) * ld + ip], p1 = pl[(j0 + 1) * ld + ip];
-                const double p2 = pl[(j0 + 2) * ld + ip], p3 = pl[(j0 + 3) * ld + ip];
-                acc[0][0] += w0 * p0; acc[0][1] += w1 * p0; acc[0][2] += w2 * p0; acc[0][3] += w3 * p0;
-                acc[1][0] += w0 * p1; acc[1][1] += w1 * p1; acc[1][2] += w2 * p1; acc[1][3] += w3 * p1;
-                acc[2][0] += w0 * p2; acc[2][1] += w1 * p2; acc[2][2] += w2 * p2; acc[2][3] += w3 * p2;
-                acc[3][0] += w0 * p3; acc[3][1] += w1 * p3; acc[3][2] += w2 * p3; acc[3][3] += w3 * p3;
-                w3 = w2;
-                w2 = w1;
-                w1 = w0;
-                w0 = rx[nxp + i0 - ip - 1];
-            }
+            for (int b = 0; b < 2; b++) acc[a][b] = acc_t{0, 0, 0, 0};
+        for (int s = 0; s < kx; s++) {
+            const int ip = 4 * s + kq;
+            const T a0 = pl[(j0 + l16) * kSlabLd + ip], a1 = pl[(j0 + 16 + l16) * kSlabLd + ip];
+            const T b0 = rx[64 + ip - (i0 + l16)], b1 = rx[64 + ip - (i0 + 16 + l16)];
+            acc[0][0] = kmfma(a0, b0, acc[0][0]);
+            acc[0][1] = kmfma(a0, b1, acc[0][1]);
+            acc[1][0] = kmfma(a1, b0, acc[1][0]);
+            acc[1][1] = kmfma(a1, b1, acc[1][1]);
         }
-        // y: acc[r][c] += t(|j0+r-jp|) * v[jp][i0+c]
-        {
-            double w0 = ry[nyp + j0 + 0], w1 = ry[nyp + j0 + 1], w2 = ry[nyp + j0 + 2], w3 = ry[nyp + j0 + 3];
-            for (int jp = 0; jp < ny; jp++) {
-                const double *row = pl + jp * ld + i0;
-                const double p0 = row[0], p1 = row[1], p2 = row[2], p3 = row[3];
-                acc[0][0] += w0 * p0; acc[0][1] += w0 * p1; acc[0][2] += w0 * p2; acc[0][3] += w0 * p3;
-                acc[1][0] += w1 * p0; acc[1][1] += w1 * p1; acc[1][2] += w1 * p2; acc[1][3] += w1 * p3;
-                acc[2][0] += w2 * p0; acc[2][1] += w2 * p1; acc[2][2] += w2 * p2; acc[2][3] += w2 * p3;
-                acc[3][0] += w3 * p0; acc[3][1] += w3 * p1; acc[3][2] += w3 * p2; acc[3][3] += w3 * p3;
-                w3 = w2;
-                w2 = w1;
-                w1 = w0;
-                w0 = ry[nyp + j0 - jp - 1];
-            }
+        for (int s = 0; s < ky; s++) {
+            const int jp = 4 * s + kq;
+            const T a0 = ry[64 + (j0 + l16) - jp], a1 = ry[64 + (j0 + 16 + l16) - jp];
+            const T b0 = pl[jp * kSlabLd + i0 + l16], b1 = pl[jp * kSlabLd + i0 + 16 + l16];
+            acc[0][0] = kmfma(a0, b0, acc[0][0]);
+            acc[0][1] = kmfma(a0, b1, acc[0][1]);
+            acc[1][0] = kmfma(a1, b0, acc[1][0]);
+            acc[1][1] = kmfma(a1, b1, acc[1][1]);
         }
 #pragma unroll
-        for (int r = 0; r < 4; r++) {
-            const int j = j0 + r;
-            if (j >= ny) continue;
+        for (int a = 0; a < 2; a++)
 #pragma unroll
-            for (int c = 0; c < 4; c++) {
-                const int i = i0 + c;
-                if (i >= nx) continue;
-                const size_t o = base + (size_t)j * nx + i;
-                const double h = acc[r][c] + rn * pl[j * ld + i];
-                R[o] = sr * R[o] + mn * h;
+            for (int b = 0; b < 2; b++) {
+                const int i = i0 + 16 * b + l16;
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    const int j = j0 + 16 * a + kin_row<T>(kq, q);
+                    if (j < ny && i < nx)
+                        R[base + (size_t)j * nx + i] =
+                            sr * rp[a][b][q] + mn * ((double)acc[a][b][q] + rn * (double)pl[j * kSlabLd + i]);
+                }
             }
-        }
     }
 }
 
-// ---------------------------------------------------------------------------
-// kinetic term, z axis: one workgroup per (64-column block, n)
-// ---------------------------------------------------------------------------
-// The 64 columns (j,i) of the block are staged in LDS over all k; thread
-// (lane, quarter) computes KT consecutive k outputs of its column.
-// R += mask[n] * Tz v
-template <int KT>
-__global__ __launch_bounds__(kWG) void k_kin_z(const double *__restrict__ X, double *__restrict__ R,
-                                               const double *__restrict__ tz, const double *__restrict__ mask,
-                                               const Scal *__restrict__ sc, int nxy, int nz)
+// z axis of 64 columns (j, i) of plane n: D[k][c] = sum_kp t_z(|k - kp|) v[kp][c];
+// wave w owns the 16 columns 16w.. and every 16-row tile of k.
+// R += mask[n] * D
+template <typename T>
+__global__ __launch_bounds__(kWG) void k_kin_z_mfma(const double *__restrict__ X, double *__restrict__ R,
+                                                    const double *__restrict__ tz, const double *__restrict__ mask,
+                                                    const Scal *__restrict__ sc, int nxy, int nz)
 {
-    extern __shared__ double lds[];
-    const int nzp = 4 * KT;
-    double *col = lds;              // [nz][64]
-    double *rz = col + (size_t)nz * 64;  // [2*nzp+1]
-    const int lane = threadIdx.x & 63, quarter = threadIdx.x >> 6;
+    typedef typename KinAcc<T>::type acc_t;
+    extern __shared__ __align__(16) unsigned char kin_lds[];
+    T *col = (T *)kin_lds;             // [64][kSlabLd]
+    T *rz = col + 64 * kSlabLd;        // [129]
     const int c0 = blockIdx.x * 64, n = blockIdx.y;
     const double sx = sc->sx;
     const size_t base = (size_t)n * nz * nxy;
-    const int c = c0 + lane;
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int l16 = lane & 15, kq = lane >> 4;
+    const int tk = (nz + 15) >> 4, kz = (nz + 3) >> 2;
+    const int cw = 16 * wid + l16;
+    const int c = c0 + cw;
+    // this lane's R outputs, loaded together with the column block
+    double rp[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int kk = 16 * a + kin_row<T>(kq, q);
+            rp[a][q] = (a < tk && kk < nz && c < nxy) ? R[base + (size_t)kk * nxy + c] : 0.0;
+        }
     {
+        const double *Xc = X + base + c0;
         double tmp[16];
-        const double *Xc = X + base + c;
 #pragma unroll
         for (int e = 0; e < 16; e++) {
-            const int kk = quarter + 4 * e;
-            tmp[e] = (kk < nz && c < nxy) ? Xc[(size_t)kk * nxy] : 0.0;
+            const int t = threadIdx.x + e * kWG, kk = t >> 6, cc = t & 63;
+            tmp[e] = (kk < nz && c0 + cc < nxy) ? Xc[(size_t)kk * nxy + cc] : 0.0;
         }
 #pragma unroll
         for (int e = 0; e < 16; e++) {
-            const int kk = quarter + 4 * e;
-            if (kk < nz) col[kk * 64 + lane] = sx * tmp[e];
+            const int t = threadIdx.x + e * kWG, kk = t >> 6, cc = t & 63;
+            col[kk * kSlabLd + cc] = (T)(sx * tmp[e]);
         }
     }
-    for (int t = threadIdx.x; t < 2 * nzp + 1; t += blockDim.x) {
-        const int d = abs(t - nzp);
-        rz[t] = d < nz ? tz[d] : 0.0;
+    for (int t = threadIdx.x; t < 129; t += blockDim.x) {
+        const int d = abs(t - 64);
+        rz[t] = d < nz ? (T)tz[d] : (T)0;
     }
     __syncthreads();
-    const int k0 = quarter * KT;
-    double acc[KT], w[KT];
+    acc_t acc[4];
 #pragma unroll
-    for (int r = 0; r < KT; r++) {
-        acc[r] = 0.0;
-        w[r] = rz[nzp + k0 + r];
+    for (int a = 0; a < 4; a++) acc[a] = acc_t{0, 0, 0, 0};
+    for (int s = 0; s < kz; s++) {
+        const int kp = 4 * s + kq;
+        const T b = col[kp * kSlabLd + cw];
+#pragma unroll
+        for (int a = 0; a < 4; a++)
+            if (a < tk) acc[a] = kmfma(rz[64 + 16 * a + l16 - kp], b, acc[a]);
     }
-    for (int kp = 0; kp < nz; kp++) {
-        const double p = col[kp * 64 + lane];
-#pragma unroll
-        for (int r = 0; r < KT; r++) acc[r] += w[r] * p;
-#pragma unroll
-        for (int r = KT - 1; r > 0; r--) w[r] = w[r - 1];
-        w[0] = rz[nzp + k0 - kp - 1];
-    }
-    if (c >= nxy) return;
     const double mn = mask[n];
+    if (c >= nxy) return;
 #pragma unroll
-    for (int r = 0; r < KT; r++) {
-        const int k = k0 + r;
-        if (k < nz) {
-            const size_t o = base + (size_t)k * nxy + c;
-            R[o] += mn * acc[r];
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int kk = 16 * a + kin_row<T>(kq, q);
+            if (a < tk && kk < nz) R[base + (size_t)kk * nxy + c] = rp[a][q] + mn * (double)acc[a][q];
         }
-    }
+}
+
+template <typename T>
+static int launch_kinetic_t(const DevOperator &op, const double *X, double *R, Scal *sc, hipStream_t s)
+{
+    const size_t lds_xy = sizeof(T) * ((size_t)64 * kSlabLd + 2 * 129);
+    hipLaunchKernelGGL(k_kin_xy_mfma<T>, dim3(op.nz, op.nk), dim3(kWG), lds_xy, s, X, R, op.tx, op.ty, op.rot,
+                       op.mask, sc, op.nx, op.ny, op.nz);
+    BSL_HIP(hipGetLastError());
+    const int nxy = op.nx * op.ny;
+    const size_t lds_z = sizeof(T) * ((size_t)64 * kSlabLd + 129);
+    hipLaunchKernelGGL(k_kin_z_mfma<T>, dim3((nxy + 63) / 64, op.nk), dim3(kWG), lds_z, s, X, R, op.tz, op.mask,
+                       sc, nxy, op.nz);
+    BSL_HIP(hipGetLastError());
+    return BSL_OK;
 }
 
 // ---------------------------------------------------------------------------
@@ -525,28 +563,11 @@ static int launch_vblock_t(const DevOperator &op, const double *X, double *R, Sc
 
 int launch_hv(const DevOperator &op, const double *X, double *R, Scal *sc, hipStream_t s, hipEvent_t *ev)
 {
-    const int nxp = (op.nx + 3) & ~3, nyp = (op.ny + 3) & ~3;
-    const size_t lds_xy = sizeof(double) * ((size_t)nyp * (nxp + 2) + 2 * nxp + 1 + 2 * nyp + 1);
     if (ev) BSL_HIP(hipEventRecord(ev[0], s));
-    hipLaunchKernelGGL(k_kin_xy, dim3(op.nz, op.nk), dim3(kWG), lds_xy, s, X, R, op.tx, op.ty, op.rot, op.mask,
-                       sc, op.nx, op.ny, op.nz);
-    BSL_HIP(hipGetLastError());
-    const int nxy = op.nx * op.ny;
-    const int kt = (op.nz + 3) / 4;
-    dim3 gz((nxy + 63) / 64, op.nk);
-    if (kt <= 4) {
-        const size_t lds = sizeof(double) * ((size_t)op.nz * 64 + 2 * 16 + 1);
-        hipLaunchKernelGGL(k_kin_z<4>, gz, dim3(kWG), lds, s, X, R, op.tz, op.mask, sc, nxy, op.nz);
-    } else if (kt <= 8) {
-        const size_t lds = sizeof(double) * ((size_t)op.nz * 64 + 2 * 32 + 1);
-        hipLaunchKernelGGL(k_kin_z<8>, gz, dim3(kWG), lds, s, X, R, op.tz, op.mask, sc, nxy, op.nz);
-    } else if (kt <= 16) {
-        const size_t lds = sizeof(double) * ((size_t)op.nz * 64 + 2 * 64 + 1);
-        hipLaunchKernelGGL(k_kin_z<16>, gz, dim3(kWG), lds, s, X, R, op.tz, op.mask, sc, nxy, op.nz);
-    } else {
-        return fail(BSL_E_ARG, "nz above 64 is not supported");
-    }
-    BSL_HIP(hipGetLastError());
+    if (op.nx > 64 || op.ny > 64 || op.nz > 64) return fail(BSL_E_ARG, "grid axes above 64 points are not supported");
+    int krc = op.kprec == BSL_PREC_FP64 ? launch_kinetic_t<double>(op, X, R, sc, s)
+                                        : launch_kinetic_t<float>(op, X, R, sc, s);
+    if (krc) return krc;
     if (ev) BSL_HIP(hipEventRecord(ev[1], s));
     int rc = op.calc == BSL_CALC_QUAD        ? launch_quad_potential(op, X, R, sc, s)
              : op.vprec == BSL_PREC_FP64 ? launch_vblock_t<double>(op, X, R, sc, s)

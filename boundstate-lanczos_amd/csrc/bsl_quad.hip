@@ -237,14 +237,14 @@ __global__ __launch_bounds__(kQWG) void k_quad_mfma(const double *__restrict__ X
     double uacc = 0.0;
 
     // One column tile.  Each phi tile's W register tile is refilled with the
-    // next column tile's W right after its use; the scheduling barrier keeps
+    // next column tile's W right after its use (prefetch: std::true_type); the scheduling barrier keeps
     // the load there (the compiler would otherwise sink it to the end of the
     // tile), so its latency overlaps the rest of this tile's transforms.
-    auto tile = [&](int n, acc_t(&wc)[NT]) {
+    auto tile = [&](int n, acc_t(&wc)[NT], auto prefetch) {
         const int colw = 16 * n + l16;  // column within the wave
         const int pc = pw0 + colw / na, ac = colw % na;
         // next tile's W (the last tile reloads itself: no read past the grid)
-        const T *Wn = W + (size_t)((tile0 + (n + 1 < tw ? n + 1 : n)) * NT) * 256;
+        const T *Wn = W + (size_t)((tile0 + n + 1) * NT) * 256;  // read only when prefetching
         // Legendre forward of this lane's column for every m (uniform loops
         // over the planes of each m), then the B operand rows of this lane
         double f[KF];
@@ -286,8 +286,10 @@ __global__ __launch_bounds__(kQWG) void k_quad_mfma(const double *__restrict__ X
                 g = mfma_bf16(ah, bfl, g);
                 g = mfma_bf16(al, bfh, g);
                 const f4v h4 = g * wc[t];
-                wc[t] = load_w<MODE>(Wn + (size_t)t * 256);
-                __builtin_amdgcn_sched_barrier(kKeepLoads);
+                if constexpr (decltype(prefetch)::value) {
+                    wc[t] = load_w<MODE>(Wn + (size_t)t * 256);
+                    __builtin_amdgcn_sched_barrier(kKeepLoads);
+                }
                 s4v hh, hl;
 #pragma unroll
                 for (int i = 0; i < 4; i++) {
@@ -318,8 +320,10 @@ __global__ __launch_bounds__(kQWG) void k_quad_mfma(const double *__restrict__ X
 #pragma unroll
                 for (int s = 0; s < KS; s++) g = mfma16(Ss[(16 * t + l16) * q.sp + 4 * s + kq], bF[s], g);
                 const acc_t h4 = g * wc[t];
-                wc[t] = load_w<MODE>(Wn + (size_t)t * 256);
-                __builtin_amdgcn_sched_barrier(kKeepLoads);
+                if constexpr (decltype(prefetch)::value) {
+                    wc[t] = load_w<MODE>(Wn + (size_t)t * 256);
+                    __builtin_amdgcn_sched_barrier(kKeepLoads);
+                }
 #pragma unroll
                 for (int i = 0; i < 4; i++)
                     acc2 = mfma16(Ss[(16 * t + acc_row<F64>(kq, i)) * q.sp + l16], h4[i], acc2);
@@ -369,7 +373,9 @@ __global__ __launch_bounds__(kQWG) void k_quad_mfma(const double *__restrict__ X
     acc_t wb[NT];
 #pragma unroll
     for (int t = 0; t < NT; t++) wb[t] = load_w<MODE>(W + (size_t)(tile0 * NT + t) * 256);
-    for (int n = 0; n < tw; n++) tile(n, wb);
+    // the last tile has no successor to prefetch
+    for (int n = 0; n + 1 < tw; n++) tile(n, wb, std::true_type());
+    tile(tw - 1, wb, std::false_type());
     if (lane < nk && pcur >= 0) us[pcur * nk + lane] = uacc;
     __syncthreads();
 

@@ -259,12 +259,14 @@ def multiplets(energies, tol=1e-6):
     return groups
 
 
-def lancbis(alpha, beta, beta2, emin, emax):
-    """Good eigenvalues and error estimates (lancbis.cpp:5-105)."""
+def lancbis(alpha, beta, beta2, emin, emax, device=False):
+    """Good eigenvalues and error estimates (lancbis.cpp:5-105); device=True
+    runs the Sturm bisections and inverse iterations on the GPU."""
     niter = len(alpha)
     ev, er = np.zeros(niter), np.zeros(niter)
     ng = C.c_int32()
-    check(lib().bsl_lancbis(niter, np.ascontiguousarray(alpha, dtype=np.float64),
+    fn = lib().bsl_lancbis_device if device else lib().bsl_lancbis
+    check(fn(niter, np.ascontiguousarray(alpha, dtype=np.float64),
                             np.ascontiguousarray(beta, dtype=np.float64),
                             np.ascontiguousarray(beta2, dtype=np.float64), emin, emax, ev, er, C.byref(ng)))
     return ev[: ng.value].copy(), er[: ng.value].copy()

@@ -632,6 +632,20 @@ extern "C" int bsl_lancbis(int32_t niter, const double *alpha, const double *bet
     return rc;
 }
 
+extern "C" int bsl_lancbis_device(int32_t niter, const double *alpha, const double *beta, const double *beta2,
+                                  double emin, double emax, double *eval, double *err, int32_t *ngood)
+{
+    (void)beta2;
+    if (!alpha || !beta || !eval || !err || !ngood) return fail(BSL_E_ARG, "null argument");
+    hipStream_t s = nullptr;
+    BSL_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    int ng = 0;
+    int rc = cw_lancbis_device(niter, alpha, beta, emin, emax, s, eval, err, &ng);
+    (void)hipStreamDestroy(s);
+    *ngood = ng;
+    return rc;
+}
+
 extern "C" int bsl_trivec(int32_t niter, const double *alpha, const double *beta, const double *beta2,
                           const double *eval, int32_t ngood, double *evtr, int32_t *mamin)
 {

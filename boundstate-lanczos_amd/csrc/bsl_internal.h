@@ -49,6 +49,14 @@ void kinetic_row(double x_max, int n, double mass, std::vector<double> &row, std
 // ---------------------------------------------------------------------------
 // Cullum-Willoughby post-processing (bsl_tridiag.cpp)
 // ---------------------------------------------------------------------------
+// device Sturm bisection and error estimates (bsl_tridiag_dev.hip)
+int launch_tri_bisect(const double *al, const double *be, int m, double lo, double hi, int above_hi, int mt,
+                      double ep0, double ep1, double *out, hipStream_t s);
+int launch_tri_inverr(const double *al, const double *be, int m, const double *xs, int nx, const double *g,
+                      double eps3, double eps4, double beta_m1, int maxit, double *sv, double *sm, unsigned char *sf,
+                      double *err, hipStream_t s);
+int cw_lancbis_device(int niter, const double *alpha, const double *beta, double emin, double emax,
+                      hipStream_t stream, double *eval, double *err, int *ngood);
 int cw_lancbis(int niter, const double *alpha, const double *beta, const double *beta2, double emin, double emax,
                double *eval, double *err, int *ngood);
 int cw_trivec(int niter, const double *alpha, const double *beta, const double *beta2, const double *eval,

@@ -285,6 +285,155 @@ int cw_lancbis(int niter, const double *alpha, const double *beta, const double 
     return BSL_OK;
 }
 
+// The same post-processing with the Sturm bisections and the inverse
+// iterations on the GPU (bsl_tridiag_dev.hip); the cluster bookkeeping of
+// BISEC and ISOEV stay here.  Eigenvalues agree with cw_lancbis to the
+// bisection tolerance (each index is bisected from the full interval rather
+// than from the bounds the previous indices refined).
+int cw_lancbis_device(int niter, const double *alpha, const double *beta, double emin, double emax,
+                      hipStream_t stream, double *eval, double *err, int *ngood)
+{
+    if (niter <= 0) return fail(BSL_E_ARG, "lancbis: niter must be positive");
+    const int m = niter;
+    Tri t{m, alpha, std::vector<double>(m + 2, 0.0), std::vector<double>(m + 2, 0.0)};
+    for (int j = 1; j <= m; j++) t.be[j] = beta[j];
+    const double beta_m1 = beta[m];
+    double tkmax = 0.0;
+    for (int i = 1; i <= m; i++) tkmax = std::fmax(std::fmax(std::fabs(t.a(i)), t.b(i)), tkmax);
+    const double ttol = kEps2 * tkmax, scale = (double)(m + 1000);
+    const double ep0 = scale * ttol, ep1 = std::sqrt(scale) * ttol;
+    t.be[m] = 0.0;
+    for (int i = 0; i <= m; i++) t.b2[i] = t.be[i] * t.be[i];
+    const int above_hi = sturm_forward(t, emax);
+    auto cnt = [&](double x) { return sturm_forward(t, x) - above_hi; };
+    const int mt = cnt(emin);
+    if (30 * mt >= 100 * m) return fail(BSL_E_STURM, "BISEC: estimated number of Sturm sequences exceeds its limit");
+    *ngood = 0;
+    if (mt == 0) return BSL_OK;
+    // device copies: alpha, beta with b(1) = b(m+1) = 0
+    std::vector<double> bev(t.be.begin(), t.be.begin() + m + 1);
+    double *d_al = nullptr, *d_be = nullptr, *d_out = nullptr;
+    auto cleanup = [&]() {
+        (void)hipStreamSynchronize(stream);
+        if (d_al) (void)hipFree(d_al);
+        if (d_be) (void)hipFree(d_be);
+        if (d_out) (void)hipFree(d_out);
+    };
+    hipError_t e = hipMalloc(&d_al, sizeof(double) * m);
+    if (e == hipSuccess) e = hipMalloc(&d_be, sizeof(double) * (m + 1));
+    if (e == hipSuccess) e = hipMalloc(&d_out, sizeof(double) * 8 * (size_t)mt);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_al, alpha, sizeof(double) * m, hipMemcpyHostToDevice, stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_be, bev.data(), sizeof(double) * (m + 1), hipMemcpyHostToDevice, stream);
+    if (e != hipSuccess) {
+        cleanup();
+        return hip_fail(e, "lancbis device buffers", __FILE__, __LINE__);
+    }
+    int rc = launch_tri_bisect(d_al, d_be, m, emin, emax, above_hi, mt, ep0, ep1, d_out, stream);
+    std::vector<double> res(8 * (size_t)mt);
+    if (!rc) {
+        e = hipMemcpyAsync(res.data(), d_out, sizeof(double) * res.size(), hipMemcpyDeviceToHost, stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(stream);
+        if (e != hipSuccess) rc = hip_fail(e, "lancbis bisection", __FILE__, __LINE__);
+    }
+    if (rc) {
+        cleanup();
+        return rc;
+    }
+    // BISEC's walk from the top: one value per simple eigenvalue (spurious
+    // when T-hat has one in the same window), one per cluster
+    std::vector<double> vs;
+    std::vector<int> mp;
+    int k = mt;
+    while (k > 0) {
+        const double *r = res.data() + 8 * (size_t)(k - 1);
+        double x1 = r[0];
+        const double xu = r[2];
+        const int nu = (int)r[3], kl = (int)r[4], jl = (int)r[5];
+        int ku = (int)r[6];
+        const int ju = (int)r[7];
+        vs.push_back(x1);
+        int mult, knew;
+        if (kl - ku - 1 == 0) {
+            mult = (ju < jl) ? 0 : 1;
+            knew = k - 1;
+        } else {
+            x1 = xu + ep0;
+            if (ku != nu) {
+                int change;
+                do {
+                    x1 = x1 + ep0;
+                    const int nev = cnt(x1);
+                    change = ku - nev;
+                    ku = nev;
+                } while (change != 0);
+            }
+            mult = kl - ku;
+            knew = ku;
+        }
+        mp.push_back(mult);
+        k = knew;
+    }
+    isolate(vs, mp);
+    // INVERR on the device for the good, isolated values
+    std::vector<double> es(vs.size(), 0.0), xs;
+    std::vector<int> idx;
+    for (size_t j = 0; j < vs.size(); j++)
+        if (mp[j] == 1) {
+            xs.push_back(vs[j]);
+            idx.push_back((int)j);
+        }
+    if (!xs.empty()) {
+        const int nx = (int)xs.size(), maxit = 10;
+        double tsum = std::fabs(t.a(1));
+        for (int i = 2; i <= m; i++) tsum = tsum + std::fabs(t.a(i)) + t.b(i);
+        const double eps3 = kEps2 * tsum, eps4 = (double)m * eps3;
+        std::vector<double> g(m);
+        park_miller(1, g.data(), m);
+        double gsum = 0.0;
+        for (int i = 0; i < m; i++) gsum = gsum + std::fabs(g[i]);
+        gsum = eps4 / gsum;
+        for (int i = 0; i < m; i++) g[i] = gsum * g[i];
+        double *d_g = nullptr, *d_xs = nullptr, *d_sv = nullptr, *d_sm = nullptr, *d_err = nullptr;
+        unsigned char *d_sf = nullptr;
+        e = hipMalloc(&d_g, sizeof(double) * m);
+        if (e == hipSuccess) e = hipMalloc(&d_xs, sizeof(double) * nx);
+        if (e == hipSuccess) e = hipMalloc(&d_sv, sizeof(double) * (size_t)m * nx);
+        if (e == hipSuccess) e = hipMalloc(&d_sm, sizeof(double) * (size_t)m * nx);
+        if (e == hipSuccess) e = hipMalloc(&d_sf, (size_t)(m + 1) * nx);
+        if (e == hipSuccess) e = hipMalloc(&d_err, sizeof(double) * nx);
+        if (e == hipSuccess) e = hipMemcpyAsync(d_g, g.data(), sizeof(double) * m, hipMemcpyHostToDevice, stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(d_xs, xs.data(), sizeof(double) * nx, hipMemcpyHostToDevice, stream);
+        if (e != hipSuccess) rc = hip_fail(e, "lancbis error estimates", __FILE__, __LINE__);
+        if (!rc)
+            rc = launch_tri_inverr(d_al, d_be, m, d_xs, nx, d_g, eps3, eps4, beta_m1, maxit, d_sv, d_sm, d_sf, d_err,
+                                   stream);
+        std::vector<double> ev(nx);
+        if (!rc) {
+            e = hipMemcpyAsync(ev.data(), d_err, sizeof(double) * nx, hipMemcpyDeviceToHost, stream);
+            if (e == hipSuccess) e = hipStreamSynchronize(stream);
+            if (e != hipSuccess) rc = hip_fail(e, "lancbis error estimates", __FILE__, __LINE__);
+        }
+        (void)hipStreamSynchronize(stream);
+        for (double *p : {d_g, d_xs, d_sv, d_sm, d_err})
+            if (p) (void)hipFree(p);
+        if (d_sf) (void)hipFree(d_sf);
+        if (!rc)
+            for (int q = 0; q < nx; q++) es[idx[q]] = ev[q];
+    }
+    cleanup();
+    if (rc) return rc;
+    // vs is ascending (the walk starts from the lowest index), as BISEC's
+    int kk = 0;
+    for (size_t i = 0; i < vs.size(); i++)
+        if (mp[i] != 0) {
+            eval[kk] = vs[i];
+            err[kk] = es[i];
+            kk++;
+        }
+    *ngood = kk;
+    return BSL_OK;
+}
+
 namespace {
 // STURMI (sturmi.f:4-116): smallest orders at which x first appears (mk1)
 // and appears twice (mk2, or m).  Returns false when x is not an eigenvalue.

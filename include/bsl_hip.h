@@ -116,6 +116,14 @@ int bsl_lanczos(bsl_ctx *ctx, int32_t niter, const double *v0, int32_t start, ui
 int bsl_lancbis(int32_t niter, const double *alpha, const double *beta, const double *beta2, double emin,
                 double emax, double *eval, double *err, int32_t *ngood);
 
+/* The same as bsl_lancbis with the Sturm bisections and the inverse-iteration
+ * error estimates on the GPU (current device), one eigenvalue per thread; the
+ * cluster bookkeeping and ISOEV stay on the host.  Eigenvalues agree with
+ * bsl_lancbis to BISEC's bisection tolerance.  No reference counterpart (the
+ * reference's BISEC/INVERR are serial Fortran, bisec.f:4-225, inverr.f:4-167). */
+int bsl_lancbis_device(int32_t niter, const double *alpha, const double *beta, const double *beta2, double emin,
+                       double emax, double *eval, double *err, int32_t *ngood);
+
 /* Tm eigenvectors (lanczosvectors, lanczosEigenValuesVectors.cpp:509-561 ->
  * TRIVEC trivec.f:4-95).  evtr column-major [niter][ngood]. */
 int bsl_trivec(int32_t niter, const double *alpha, const double *beta, const double *beta2, const double *eval,

@@ -171,13 +171,22 @@ def cpu_baseline(B, calc, args):
 
 
 def eig_error(B, c64, c32, niter):
-    """Lowest 20 good levels of FP64 and mixed runs (same random start)."""
+    """Lowest 20 good levels of FP64 and mixed runs (same random start), and
+    the Cullum-Willoughby eigensolve timed on the host and on the GPU."""
     from bsl_amd import physics
-    e64, r64 = B.lancbis(*c64.lanczos(niter, start="random", seed=0), -10.0, 10.0)
+    coef = c64.lanczos(niter, start="random", seed=0)
+    t0 = time.perf_counter()
+    e64, r64 = B.lancbis(*coef, -10.0, 10.0)
+    t_host = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    e64d, _ = B.lancbis(*coef, -10.0, 10.0, device=True)
+    t_dev = time.perf_counter() - t0
     e32, r32 = B.lancbis(*c32.lanczos(niter, start="random", seed=0), -10.0, 10.0)
     low = e64[:20]
     d = [np.min(np.abs(e32 - e)) / physics.KJMOL_PER_CM1 for e in low]
-    return float(np.max(d)), len(low), [float(x) for x in low[:5]]
+    tri = {"host_s": t_host, "device_s": t_dev, "good_levels": int(len(e64)),
+           "max_abs_diff_kjmol": float(np.max(np.abs(e64 - e64d))) if len(e64) == len(e64d) else None}
+    return float(np.max(d)), len(low), [float(x) for x in low[:5]], tri
 
 
 def main():
@@ -242,8 +251,9 @@ def main():
 
     eig = None
     if not args.no_eig and c32 is not None:
-        err, nlev, low = eig_error(B, c64, c32, args.eig_steps)
-        eig = {"max_err_cm1": err, "levels": nlev, "steps": args.eig_steps, "lowest_kjmol": low}
+        err, nlev, low, tri = eig_error(B, c64, c32, args.eig_steps)
+        eig = {"max_err_cm1": err, "levels": nlev, "steps": args.eig_steps, "lowest_kjmol": low,
+               "cullum_willoughby": tri}
         print("mixed vs fp64 lowest %d levels: max err %.2e cm^-1" % (nlev, err), file=sys.stderr, flush=True)
     del c32
 

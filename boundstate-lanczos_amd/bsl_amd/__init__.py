@@ -162,8 +162,13 @@ class Calculator:
 def make_params(*, nx, ny=None, nz=None, x_max, y_max=None, z_max=None, lmax, n_theta, n_phi, mass, rot_const,
                 ceiling, symmetry="All", basis="full", calculator="noquad", precision="fp64", potential="iho",
                 iho_factor=0.0, iho_ceil=0.0, sites=None, q_h2_h=physics.Q_H2_H, q_h2_cm=physics.Q_H2_CM,
-                h2_half_bond=physics.H2_HALF_BOND, lj_a=physics.LJ_A_REF, lj_b=physics.LJ_B_REF):
-    """bsl_params from keyword arguments; returns (params, arrays to keep alive)."""
+                h2_half_bond=physics.H2_HALF_BOND, lj_a=physics.LJ_A_REF, lj_b=physics.LJ_B_REF, universe=None,
+                coulomb=None):
+    """bsl_params from keyword arguments; returns (params, arrays to keep alive).
+    potential: "iho", "cage" (pair sums at the exact H positions), "cage_grid"
+    (universe=(pnx, pny, pnz, px_max, py_max, pz_max): tabulated potentials
+    looked up at the nearest universe point) or "coulomb"
+    (coulomb=(q_centre, q_mol, v_ceil))."""
     p = Params()
     p.nx, p.ny, p.nz = nx, ny or nx, nz or nx
     p.x_max, p.y_max, p.z_max = x_max, y_max or x_max, z_max or x_max
@@ -176,7 +181,12 @@ def make_params(*, nx, ny=None, nz=None, x_max, y_max=None, z_max=None, lmax, n_
     p.potential = POT[potential]
     p.iho_factor, p.iho_ceil = iho_factor, iho_ceil
     keep = []
-    if potential == "cage":
+    if potential == "cage_grid":
+        p.pnx, p.pny, p.pnz = universe[0], universe[1], universe[2]
+        p.px_max, p.py_max, p.pz_max = universe[3], universe[4], universe[5]
+    if potential == "coulomb":
+        p.coul_q_centre, p.coul_q_mol, p.coul_ceil = coulomb
+    if potential in ("cage", "cage_grid"):
         pos, q, lj = sites
         pos = np.ascontiguousarray(pos, dtype=np.float64).ravel()
         q = np.ascontiguousarray(q, dtype=np.float64)

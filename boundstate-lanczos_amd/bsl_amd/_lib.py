@@ -15,7 +15,7 @@ SYM = {"All": 0, "Even": 1, "Odd": 2}
 BASIS = {"full": 0, "reduced": 1}
 CALC = {"noquad": 0, "quad": 1}
 PREC = {"fp64": 0, "fp32": 1, "bf16x3": 2}
-POT = {"iho": 0, "cage": 1}
+POT = {"iho": 0, "cage": 1, "cage_grid": 2, "coulomb": 3}
 START = {"uniform": 0, "random": 1}
 
 EXPORTS = (
@@ -40,6 +40,9 @@ class Params(C.Structure):
         ("site_pos", C.c_void_p), ("site_q", C.c_void_p), ("site_lj", C.c_void_p),
         ("q_h2_h", C.c_double), ("q_h2_cm", C.c_double), ("h2_half_bond", C.c_double),
         ("lj_a", C.c_double), ("lj_b", C.c_double),
+        ("pnx", C.c_int32), ("pny", C.c_int32), ("pnz", C.c_int32),
+        ("px_max", C.c_double), ("py_max", C.c_double), ("pz_max", C.c_double),
+        ("coul_q_centre", C.c_double), ("coul_q_mol", C.c_double), ("coul_ceil", C.c_double),
     ]
 
 

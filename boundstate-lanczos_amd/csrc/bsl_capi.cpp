@@ -73,10 +73,18 @@ int check_params(const bsl_params &p)
     if (p.precision < 0 || p.precision > 2) return fail(BSL_E_ARG, "precision flag not recognised");
     if (p.precision == BSL_PREC_BF16X3 && p.calculator != BSL_CALC_QUAD)
         return fail(BSL_E_ARG, "BF16x3 split products apply to the quadrature calculator's MFMA transforms only");
-    if (p.potential < 0 || p.potential > 1) return fail(BSL_E_ARG, "potential flag not recognised");
-    if (p.potential == BSL_POT_CAGE && (p.n_sites <= 0 || !p.site_pos || !p.site_q || !p.site_lj))
+    if (p.potential < 0 || p.potential > 3) return fail(BSL_E_ARG, "potential flag not recognised");
+    const bool sites = p.potential == BSL_POT_CAGE || p.potential == BSL_POT_CAGE_GRID;
+    if (sites && (p.n_sites <= 0 || !p.site_pos || !p.site_q || !p.site_lj))
         return fail(BSL_E_ARG, "cage potential needs sites");
-    if (p.potential == BSL_POT_CAGE && p.n_sites > 4096) return fail(BSL_E_ARG, "at most 4096 sites");
+    if (sites && p.n_sites > 4096) return fail(BSL_E_ARG, "at most 4096 sites");
+    if (p.potential == BSL_POT_CAGE_GRID) {
+        if (p.pnx < 2 || p.pny < 2 || p.pnz < 2) return fail(BSL_E_ARG, "potential universe needs 2 points per axis");
+        if ((double)p.pnx * p.pny * p.pnz > 2.0e9) return fail(BSL_E_ARG, "potential universe too large");
+        // HvPrep_Internal_NoQuad linRotCartSphHvFullVPrepRoutines.cpp:330-348
+        if (p.x_max > p.px_max || p.y_max > p.py_max || p.z_max > p.pz_max)
+            return fail(BSL_E_ARG, "the system's dimensions must not exceed the potential universe's");
+    }
     return BSL_OK;
 }
 
@@ -139,6 +147,55 @@ int make_vgrid(bsl_ctx *c, double *d_vg)
         dev_free(cage.site);
         dev_free(cage.lj);
         dev_free(d_off);
+    } else if (p.potential == BSL_POT_CAGE_GRID) {
+        std::vector<double> off, site4;
+        std::vector<int> lj;
+        cage_arrays(c, off, site4, lj);
+        // H-H axis directions of the nodes, as the reference evaluates them
+        // (theta = acos(x_a), phi = acos(x_b) and 2 pi - acos(x_b))
+        const Tables &t = c->tab;
+        const int na = t.na, nb = t.nb;
+        std::vector<double> dir((size_t)c->Q * 3);
+        for (int half = 0; half < 2; half++)
+            for (int a = 0; a < na; a++)
+                for (int b = 0; b < nb; b++) {
+                    const size_t q = (size_t)half * na * nb + (size_t)a * nb + b;
+                    const double th = std::acos(t.ct[a]);
+                    const double ph = half == 0 ? std::acos(t.cp[b]) : 2 * M_PI - std::acos(t.cp[b]);
+                    dir[3 * q + 0] = std::sin(th) * std::cos(ph);
+                    dir[3 * q + 1] = std::sin(th) * std::sin(ph);
+                    dir[3 * q + 2] = std::cos(th);
+                }
+        Universe u;
+        const int un[3] = {p.pnx, p.pny, p.pnz};
+        const double um[3] = {p.px_max, p.py_max, p.pz_max};
+        for (int d = 0; d < 3; d++) {
+            u.n[d] = un[d];
+            u.mx[d] = um[d];
+            u.d[d] = um[d] / (double(un[d] - 1));
+        }
+        const size_t nu = (size_t)p.pnx * p.pny * p.pnz;
+        CageDev cage;
+        cage.nsites = p.n_sites;
+        double *d_dir = nullptr, *d_cm = nullptr, *d_hp = nullptr;
+        if (!rc) rc = dev_upload(&cage.site, site4);
+        if (!rc) rc = dev_upload(&cage.lj, lj);
+        if (!rc) rc = dev_upload(&d_dir, dir);
+        if (!rc) rc = dev_alloc(&d_cm, nu);
+        if (!rc) rc = dev_alloc(&d_hp, nu);
+        if (!rc) rc = launch_universe_cage(d_cm, d_hp, u, cage, p.q_h2_cm, p.q_h2_h, p.lj_a, p.lj_b, c->stream);
+        if (!rc)
+            rc = launch_vgrid_mapped(d_vg, c->P, c->Q, c->d_gx, c->d_gy, c->d_gz, p.nx, p.ny, d_dir, d_cm, d_hp, u,
+                                     p.h2_half_bond, p.ceiling, c->stream);
+        (void)hipStreamSynchronize(c->stream);
+        dev_free(cage.site);
+        dev_free(cage.lj);
+        dev_free(d_dir);
+        dev_free(d_cm);
+        dev_free(d_hp);
+    } else if (p.potential == BSL_POT_COULOMB) {
+        rc = launch_vgrid_coulomb(d_vg, c->P, c->Q, c->d_gx, c->d_gy, c->d_gz, p.nx, p.ny, p.coul_q_centre,
+                                  p.coul_q_mol, p.coul_ceil, p.ceiling, c->stream);
     } else {
         rc = launch_vgrid_iho(d_vg, c->P, c->Q, c->d_gx, c->d_gy, c->d_gz, p.nx, p.ny, p.iho_factor, p.iho_ceil,
                               p.ceiling, c->stream);
@@ -255,7 +312,7 @@ extern "C" int bsl_create(const bsl_params *pp, bsl_ctx **out)
     if (rc) return rc;
     std::unique_ptr<bsl_ctx> c(new bsl_ctx());
     c->p = *pp;
-    if (pp->potential == BSL_POT_CAGE) {
+    if (pp->potential == BSL_POT_CAGE || pp->potential == BSL_POT_CAGE_GRID) {
         c->site_pos.assign(pp->site_pos, pp->site_pos + 3 * pp->n_sites);
         c->site_q.assign(pp->site_q, pp->site_q + pp->n_sites);
         c->site_lj.assign(pp->site_lj, pp->site_lj + pp->n_sites);
@@ -334,18 +391,6 @@ extern "C" int bsl_create(const bsl_params *pp, bsl_ctx **out)
                 }
             }
     }
-    // H atom offsets for each q (Alavi_ContGrid_H2_Routines.cpp:74-90)
-    std::vector<double> off((size_t)c->Q * 3);
-    for (int a = 0; a < na; a++)
-        for (int b = 0; b < nb; b++) {
-            const size_t q0 = (size_t)a * nb + b, q1 = (size_t)na * nb + q0;
-            off[3 * q0 + 0] = p.h2_half_bond * std::sin(t.th[a]) * t.cp[b];
-            off[3 * q0 + 1] = p.h2_half_bond * std::sin(t.th[a]) * std::sin(t.ph1[b]);
-            off[3 * q0 + 2] = p.h2_half_bond * t.ct[a];
-            off[3 * q1 + 0] = off[3 * q0 + 0];
-            off[3 * q1 + 1] = -1.0 * (off[3 * q0 + 1]);
-            off[3 * q1 + 2] = off[3 * q0 + 2];
-        }
 
     // ---- device: tables ----
     if ((rc = dev_upload(&op.tx, rowx))) return rc;
@@ -364,31 +409,7 @@ extern "C" int bsl_create(const bsl_params *pp, bsl_ctx **out)
 
     // ---- device: potential grid and V blocks ----
     BSL_HIP(hipMalloc((void **)&c->d_vgrid, sizeof(double) * (size_t)c->P * c->Q));
-    if (p.potential == BSL_POT_CAGE) {
-        std::vector<double> site4((size_t)4 * p.n_sites);
-        for (int s = 0; s < p.n_sites; s++) {
-            for (int d = 0; d < 3; d++) site4[4 * s + d] = p.site_pos[3 * s + d];
-            site4[4 * s + 3] = p.site_q[s];
-        }
-        std::vector<int> lj(p.site_lj, p.site_lj + p.n_sites);
-        CageDev cage;
-        cage.nsites = p.n_sites;
-        double *d_off = nullptr;
-        if ((rc = dev_upload(&cage.site, site4))) return rc;
-        if ((rc = dev_upload(&cage.lj, lj))) return rc;
-        if ((rc = dev_upload(&d_off, off))) return rc;
-        rc = launch_vgrid_cage(c->d_vgrid, c->P, c->Q, c->d_gx, c->d_gy, c->d_gz, p.nx, p.ny, d_off, cage,
-                               p.q_h2_h, p.q_h2_cm, p.lj_a, p.lj_b, p.ceiling, c->stream);
-        BSL_HIP(hipStreamSynchronize(c->stream));
-        dev_free(cage.site);
-        dev_free(cage.lj);
-        dev_free(d_off);
-        if (rc) return rc;
-    } else {
-        rc = launch_vgrid_iho(c->d_vgrid, c->P, c->Q, c->d_gx, c->d_gy, c->d_gz, p.nx, p.ny, p.iho_factor,
-                              p.iho_ceil, p.ceiling, c->stream);
-        if (rc) return rc;
-    }
+    if ((rc = make_vgrid(c.get(), c->d_vgrid))) return rc;
     double *d_h = nullptr;
     if ((rc = dev_upload(&d_h, h))) return rc;
     c->op64 = op;

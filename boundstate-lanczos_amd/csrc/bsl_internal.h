@@ -148,6 +148,18 @@ int launch_vgrid_cage(double *vg, int64_t P, int Q, const double *gx, const doub
                       double lj_a, double lj_b, double ceiling, hipStream_t s);
 int launch_vgrid_iho(double *vg, int64_t P, int Q, const double *gx, const double *gy, const double *gz, int nx,
                      int ny, double factor, double iho_ceil, double ceiling, hipStream_t s);
+// potential universe of the grid-mapped cage potential
+struct Universe {
+    int n[3];
+    double d[3], mx[3];  // spacing max/(n-1) and full length per axis
+};
+int launch_universe_cage(double *cm, double *hp, const Universe &u, const CageDev &cage, double q_cm, double q_h,
+                         double lj_a, double lj_b, hipStream_t s);
+int launch_vgrid_mapped(double *vg, int64_t P, int Q, const double *gx, const double *gy, const double *gz, int nx,
+                        int ny, const double *dir, const double *cm, const double *hp, const Universe &u, double dist,
+                        double ceiling, hipStream_t s);
+int launch_vgrid_coulomb(double *vg, int64_t P, int Q, const double *gx, const double *gy, const double *gz, int nx,
+                         int ny, double q1, double q2, double v_ceil, double ceiling, hipStream_t s);
 // Vpk[pair][P] = sum_q Vg[P][q] * H[q][pair]
 int launch_vblock_gemm(const double *vg, const double *h, void *vpk, int vprec, int64_t P, int Q, int64_t npair,
                        hipStream_t s);

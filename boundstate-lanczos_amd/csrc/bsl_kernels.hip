@@ -775,6 +775,145 @@ int launch_vgrid_iho(double *vg, int64_t P, int Q, const double *gx, const doubl
     return BSL_OK;
 }
 
+// Potential universe (Alavi_H2_Routines.cpp:189-317, gridFcns.cpp:6-58):
+// point u = i0 + n0 (i1 + n1 i2) at c_d = i_d dx_d - max_d / 2, dx_d =
+// max_d / (n_d - 1); cm[u] = centre Coulomb + LJ, hp[u] = H-atom Coulomb.
+__global__ __launch_bounds__(kWG) void k_universe_cage(double *cm, double *hp, Universe u,
+                                                       const double *__restrict__ site_g,
+                                                       const int *__restrict__ lj_g, int ns, double q_cm, double q_h,
+                                                       double lj_a, double lj_b)
+{
+#pragma clang fp contract(off)
+    extern __shared__ double site[];
+    int *lj = (int *)(site + 4 * ns);
+    for (int t = threadIdx.x; t < 4 * ns; t += blockDim.x) site[t] = site_g[t];
+    for (int t = threadIdx.x; t < ns; t += blockDim.x) lj[t] = lj_g[t];
+    __syncthreads();
+    const int64_t total = (int64_t)u.n[0] * u.n[1] * u.n[2];
+    for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+         idx += (int64_t)gridDim.x * blockDim.x) {
+        const int i0 = (int)(idx % u.n[0]), i1 = (int)((idx / u.n[0]) % u.n[1]),
+                  i2 = (int)(idx / ((int64_t)u.n[0] * u.n[1]));
+        const double x = ((double)i0 * u.d[0]) - (u.mx[0] / 2.0), y = ((double)i1 * u.d[1]) - (u.mx[1] / 2.0),
+                     z = ((double)i2 * u.d[2]) - (u.mx[2] / 2.0);
+        double e = coulomb_sum(site, ns, x, y, z, q_cm);
+        double l = 0;
+        for (int s = 0; s < ns; s++) {
+            if (!lj[s]) continue;
+            const double d0 = site[4 * s + 0] - x, d1 = site[4 * s + 1] - y, d2 = site[4 * s + 2] - z;
+            double r2 = 0.;
+            r2 += d0 * d0;
+            r2 += d1 * d1;
+            r2 += d2 * d2;
+            const double r = sqrt(r2);
+            l += (lj_a / pow(r, 12.0)) - (lj_b / pow(r, 6.0));
+        }
+        cm[idx] = e + l;
+        hp[idx] = coulomb_sum(site, ns, x, y, z, q_h);
+    }
+}
+
+// nearest universe index along axis d of coordinate c (AlaviMapH2GridToHGrid
+// Alavi_H2_Routines.cpp:5-54: round half away from zero, kept inside)
+__device__ inline int universe_index(const Universe &u, int d, double c)
+{
+#pragma clang fp contract(off)
+    const double cmin = (-0.5) * u.mx[d];
+    int i = (int)round((c - cmin) / u.d[d]);
+    if (i <= 0) i = 0;
+    else if (i >= u.n[d]) i = u.n[d] - 1;
+    return i;
+}
+
+// vg[p][q] = cm[nearest(c)] + hp[nearest(c + dist e_q)] + hp[nearest(c - dist e_q)],
+// V >= ceiling -> ceiling (Alavi_H2_Eng_Point Alavi_H2_Routines.cpp:133-187;
+// calc_Vlmlpmp_NoQuad linRotCartSphHvFullVPrepRoutines.cpp:134-184)
+__global__ __launch_bounds__(kWG) void k_vgrid_mapped(double *vg, int64_t P, int Q, const double *__restrict__ gx,
+                                                      const double *__restrict__ gy, const double *__restrict__ gz,
+                                                      int nx, int ny, const double *__restrict__ dir,
+                                                      const double *__restrict__ cm, const double *__restrict__ hp,
+                                                      Universe u, double dist, double ceiling)
+{
+#pragma clang fp contract(off)
+    const int64_t total = P * (int64_t)Q;
+    for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+         idx += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t p = idx / Q;
+        const int q = (int)(idx - p * Q);
+        const int i = (int)(p % nx), j = (int)((p / nx) % ny), k = (int)(p / ((int64_t)nx * ny));
+        const double c[3] = {gx[i], gy[j], gz[k]};
+        int ic[3], i1[3], i2[3];
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            ic[d] = universe_index(u, d, c[d]);
+            i1[d] = universe_index(u, d, c[d] + dir[3 * q + d] * dist);
+            i2[d] = universe_index(u, d, c[d] + dir[3 * q + d] * (-dist));
+        }
+        const int64_t n01 = (int64_t)u.n[0] * u.n[1];
+        double v = 0.0;
+        v += cm[ic[0] + (int64_t)u.n[0] * ic[1] + n01 * ic[2]];
+        v += hp[i1[0] + (int64_t)u.n[0] * i1[1] + n01 * i1[2]];
+        v += hp[i2[0] + (int64_t)u.n[0] * i2[1] + n01 * i2[2]];
+        if (v >= ceiling) v = ceiling;
+        vg[idx] = v;
+    }
+}
+
+// V = K q_centre q_mol / |c| (a charge at the origin), |V| > |v_ceil| -> v_ceil,
+// then V >= ceiling -> ceiling (CoulombPotential Coulomb_Routines.cpp:6-31)
+__global__ __launch_bounds__(kWG) void k_vgrid_coulomb(double *vg, int64_t P, int Q, const double *__restrict__ gx,
+                                                       const double *__restrict__ gy, const double *__restrict__ gz,
+                                                       int nx, int ny, double q1, double q2, double v_ceil,
+                                                       double ceiling)
+{
+#pragma clang fp contract(off)
+    const int64_t total = P * (int64_t)Q;
+    for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+         idx += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t p = idx / Q;
+        const int i = (int)(p % nx), j = (int)((p / nx) % ny), k = (int)(p / ((int64_t)nx * ny));
+        const double x = gx[i], y = gy[j], z = gz[k];
+        double r2 = 0.;
+        r2 += x * x;
+        r2 += y * y;
+        r2 += z * z;
+        double v = 138.9354846111096 * q1 * q2 / sqrt(r2);
+        if (fabs(v) > fabs(v_ceil)) v = v_ceil;
+        if (v >= ceiling) v = ceiling;
+        vg[idx] = v;
+    }
+}
+
+int launch_universe_cage(double *cm, double *hp, const Universe &u, const CageDev &cage, double q_cm, double q_h,
+                         double lj_a, double lj_b, hipStream_t s)
+{
+    const int64_t total = (int64_t)u.n[0] * u.n[1] * u.n[2];
+    const size_t lds = sizeof(double) * 4 * cage.nsites + sizeof(int) * cage.nsites;
+    hipLaunchKernelGGL(k_universe_cage, dim3(grid_for(total, 65536)), dim3(kWG), lds, s, cm, hp, u, cage.site,
+                       cage.lj, cage.nsites, q_cm, q_h, lj_a, lj_b);
+    BSL_HIP(hipGetLastError());
+    return BSL_OK;
+}
+
+int launch_vgrid_mapped(double *vg, int64_t P, int Q, const double *gx, const double *gy, const double *gz, int nx,
+                        int ny, const double *dir, const double *cm, const double *hp, const Universe &u, double dist,
+                        double ceiling, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_vgrid_mapped, dim3(grid_for(P * Q, 65536)), dim3(kWG), 0, s, vg, P, Q, gx, gy, gz, nx, ny,
+                       dir, cm, hp, u, dist, ceiling);
+    BSL_HIP(hipGetLastError());
+    return BSL_OK;
+}
+
+int launch_vgrid_coulomb(double *vg, int64_t P, int Q, const double *gx, const double *gy, const double *gz, int nx,
+                         int ny, double q1, double q2, double v_ceil, double ceiling, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_vgrid_coulomb, dim3(grid_for(P * Q, 65536)), dim3(kWG), 0, s, vg, P, Q, gx, gy, gz, nx, ny,
+                       q1, q2, v_ceil, ceiling);
+    BSL_HIP(hipGetLastError());
+    return BSL_OK;
+}
+
 // ---------------------------------------------------------------------------
 // preparation: V blocks, Vpk[pair][P] = sum_q Vg[P][q] H[q][pair]  (FP64 GEMM)
 // ---------------------------------------------------------------------------

@@ -31,15 +31,50 @@ void check(int rc, const char *what)
 
 const char *kPrefix = "linRotCartSph_HIP_";
 
+// potential part of a calculator string: ContGrid_Alavi_* (pair sums at the
+// exact H positions), Alavi_* (grid-mapped universe), Coulomb, IsoHarmOsc
+bool parse_potential(const std::string &rest, bslPluginStor *st)
+{
+    if (rest == "ContGrid_Alavi_SPCE" || rest == "ContGrid_Alavi_TIP4P" || rest == "Alavi_SPCE" ||
+        rest == "Alavi_TIP4P") {
+        st->potential = rest.compare(0, 9, "ContGrid_") == 0 ? BSL_POT_CAGE : BSL_POT_CAGE_GRID;
+        st->water = rest.size() > 5 && rest.compare(rest.size() - 5, 5, "TIP4P") == 0 ? 1 : 0;
+        return true;
+    }
+    if (rest == "Coulomb") {
+        st->potential = BSL_POT_COULOMB;
+        return true;
+    }
+    if (rest == "IsoHarmOsc") {
+        st->potential = BSL_POT_IHO;
+        return true;
+    }
+    return false;
+}
+
 bool parse_calculator(const std::string &s, bslPluginStor *st)
 {
+    // the reference's NoQuad strings (interface.cpp:90-211) are accepted as
+    // aliases of the FP64 NoQuad GPU calculators; its on-the-fly strings are
+    // not (the reference's on-the-fly Hv computes another operator)
+    const std::string ref = "linRotCartSph_NoQuad_";
+    if (s.compare(0, ref.size(), ref) == 0) {
+        if (!parse_potential(s.substr(ref.size()), st)) return false;
+        st->calculator = s;
+        st->calc = BSL_CALC_NOQUAD;
+        st->precision = BSL_PREC_FP64;
+        return true;
+    }
     if (s.compare(0, std::string(kPrefix).size(), kPrefix) != 0) return false;
     std::string rest = s.substr(std::string(kPrefix).size());
     int precision = BSL_PREC_FP64;
-    const std::string fp32 = "_FP32";
+    const std::string fp32 = "_FP32", bf = "_BF16X3";
     if (rest.size() > fp32.size() && rest.compare(rest.size() - fp32.size(), fp32.size(), fp32) == 0) {
         precision = BSL_PREC_FP32;
         rest = rest.substr(0, rest.size() - fp32.size());
+    } else if (rest.size() > bf.size() && rest.compare(rest.size() - bf.size(), bf.size(), bf) == 0) {
+        precision = BSL_PREC_BF16X3;
+        rest = rest.substr(0, rest.size() - bf.size());
     }
     int calc;
     if (rest.compare(0, 7, "NoQuad_") == 0) {
@@ -51,21 +86,13 @@ bool parse_calculator(const std::string &s, bslPluginStor *st)
     } else {
         return false;
     }
-    int potential, water = 0;
-    if (rest == "ContGrid_Alavi_SPCE")
-        potential = BSL_POT_CAGE;
-    else if (rest == "ContGrid_Alavi_TIP4P") {
-        potential = BSL_POT_CAGE;
-        water = 1;
-    } else if (rest == "IsoHarmOsc")
-        potential = BSL_POT_IHO;
-    else
-        return false;
+    if (precision == BSL_PREC_BF16X3 && calc != BSL_CALC_QUAD) return false;
+    bslPluginStor probe = *st;
+    if (!parse_potential(rest, &probe)) return false;
+    *st = probe;
     st->calculator = s;
     st->calc = calc;
     st->precision = precision;
-    st->potential = potential;
-    st->water = water;
     return true;
 }
 
@@ -186,9 +213,17 @@ void Hv_Prep_linRotCartSph_HIP(int argc, char **argv, generalStor *general_data,
     p.potential = st->potential;
     std::vector<double> pos, q;
     std::vector<int32_t> lj;
-    if (st->potential == BSL_POT_CAGE) {
+    if (st->potential == BSL_POT_CAGE || st->potential == BSL_POT_CAGE_GRID) {
         // Alavi H2 (Alavi_Parameters.h:5-14) on SPC/E (SPCE_AH2_EngRoutines.h:17-38)
-        // or TIP4P (TIP4P_AH2_EngRoutines.h:16-41) waters
+        // or TIP4P (TIP4P_AH2_EngRoutines.h:16-41) waters; the grid-mapped
+        // potentials are tabulated on the deck's potential universe
+        // (preCalcPotential_Alavi Alavi_H2_Routines.cpp:289-317)
+        p.pnx = d.pnx;
+        p.pny = d.pny;
+        p.pnz = d.pnz;
+        p.px_max = d.px_max;
+        p.py_max = d.py_max;
+        p.pz_max = d.pz_max;
         const std::vector<double> rows = read_geometry(d.geometryFilename);
         const int nmol = (int)(rows.size() / 6);
         const int ns = st->water == 0 ? 3 : 4;
@@ -205,6 +240,16 @@ void Hv_Prep_linRotCartSph_HIP(int argc, char **argv, generalStor *general_data,
         p.h2_half_bond = 0.03707;
         p.lj_a = st->water == 0 ? 1.364e-06 : 1.362e-06;
         p.lj_b = st->water == 0 ? 1.533e-3 : 1.532e-3;
+    } else if (st->potential == BSL_POT_COULOMB) {
+        // preCalcPotential_Coulomb Coulomb_Routines.cpp:36-86: argv[3] holds
+        // the centre charge, the molecule charge and the |V| cap
+        if (argc < 4) die("Coulomb Potential input file (argv[3]) is missing.");
+        std::ifstream in(argv[3]);
+        if (!in.is_open()) die(std::string("Coulomb Potential input file '") + argv[3] + "' could not be opened.");
+        std::string line, junk;
+        std::getline(in, line);
+        std::getline(in, line);
+        in >> junk >> p.coul_q_centre >> junk >> p.coul_q_mol >> junk >> p.coul_ceil;
     } else {
         // preCalcPotential_IsoHarmOsc IsoHarmOsc_Routines.cpp:41-78
         if (argc < 4) die("Isotropic Harmonic Oscillator Potential input file (argv[3]) is missing.");

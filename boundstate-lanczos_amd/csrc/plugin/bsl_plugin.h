@@ -7,14 +7,22 @@
 // below and returns a storage struct plus HvPrep/Hv adapters with the exact
 // reference signatures; the adapters call libbsl_hip.so through its C ABI.
 //
-//   linRotCartSph_HIP_{NoQuad,Quad}_ContGrid_Alavi_SPCE[_FP32]
-//   linRotCartSph_HIP_{NoQuad,Quad}_ContGrid_Alavi_TIP4P[_FP32]
-//   linRotCartSph_HIP_{NoQuad,Quad}_IsoHarmOsc[_FP32]
+//   linRotCartSph_HIP_{NoQuad,Quad}_{ContGrid_Alavi_SPCE, ContGrid_Alavi_TIP4P,
+//                                    Alavi_SPCE, Alavi_TIP4P, Coulomb, IsoHarmOsc}[_FP32]
+//   linRotCartSph_HIP_Quad_<potential>_BF16X3
+//
+// ContGrid_Alavi_*: pair sums at the exact H positions; Alavi_*: the
+// grid-mapped potential universe of the reference's linRotCartSph_NoQuad_Alavi_*.
+// The reference's own NoQuad strings (linRotCartSph_NoQuad_{ContGrid_Alavi_SPCE,
+// Alavi_SPCE, Alavi_TIP4P, Coulomb, IsoHarmOsc}) are accepted too, as the FP64
+// NoQuad GPU calculators, so that a driver linked against this library reads
+// the reference's decks unchanged.
 //
 // The decks are the reference's: argv[2] is the Hv deck (22 positional
 // values, HvPrep_Internal_NoQuad linRotCartSphHvFullVPrepRoutines.cpp:248-321)
 // and argv[3] the isotropic harmonic oscillator deck
-// (preCalcPotential_IsoHarmOsc IsoHarmOsc_Routines.cpp:41-58).
+// (preCalcPotential_IsoHarmOsc IsoHarmOsc_Routines.cpp:41-58) or the Coulomb
+// deck (preCalcPotential_Coulomb Coulomb_Routines.cpp:48-71).
 //
 // The types generalStor and lanczosStor must be the reference's
 // (interfaceContainers.h:6-14); a translation unit that includes this header

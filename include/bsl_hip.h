@@ -46,7 +46,15 @@ enum { BSL_BASIS_FULL = 0, BSL_BASIS_REDUCED = 1 };
 enum { BSL_CALC_NOQUAD = 0, BSL_CALC_QUAD = 1 };
 /* Hv arithmetic; the recurrence, its dot products and norms are always FP64 */
 enum { BSL_PREC_FP64 = 0, BSL_PREC_FP32 = 1, BSL_PREC_BF16X3 = 2 };
-enum { BSL_POT_IHO = 0, BSL_POT_CAGE = 1 };
+/* IHO: isotropic harmonic oscillator (IsoHarmOsc_Routines.cpp:6-79).
+ * CAGE: Alavi H2 pair sums at the exact H positions of every quadrature
+ *   node (Alavi_ContGrid_H2_Routines.cpp:6-291, the ContGrid calculators).
+ * CAGE_GRID: the same sites, with the centre and H-atom potentials tabulated
+ *   on a separate "potential universe" grid and looked up at the nearest
+ *   universe point (Alavi_H2_Routines.cpp:5-317, the grid-mapped calculators).
+ * COULOMB: a point charge at the origin acting on the molecule's centre,
+ *   |V| > |coul_ceil| -> coul_ceil (Coulomb_Routines.cpp:6-86). */
+enum { BSL_POT_IHO = 0, BSL_POT_CAGE = 1, BSL_POT_CAGE_GRID = 2, BSL_POT_COULOMB = 3 };
 enum { BSL_START_UNIFORM = 0, BSL_START_RANDOM = 1 };
 
 typedef struct bsl_params {
@@ -73,6 +81,12 @@ typedef struct bsl_params {
     double q_h2_h, q_h2_cm;      /* Alavi H2 charges on each H and at the centre, e */
     double h2_half_bond;         /* nm */
     double lj_a, lj_b;           /* A/r^12 - B/r^6, kJ nm^12/mol and kJ nm^6/mol */
+    /* CAGE_GRID: potential universe, points per axis and full lengths (nm);
+     * must contain the system box (HvPrep_Internal_NoQuad :330-348) */
+    int32_t pnx, pny, pnz;
+    double px_max, py_max, pz_max;
+    /* COULOMB: charges (e) at the origin and on the molecule, and |V| cap */
+    double coul_q_centre, coul_q_mol, coul_ceil;
 } bsl_params;
 
 typedef struct bsl_ctx bsl_ctx;

@@ -293,6 +293,70 @@ static double iho_point(const bso_problem *p, const double *cm)
     return v;
 }
 
+/* Grid-mapped Alavi cage (Alavi_H2_Routines.cpp:5-317, gridFcns.cpp:6-101):
+ * centre and H-atom potentials tabulated on the universe grid (point
+ * u = i0 + n0 (i1 + n1 i2) at i_d dx_d - max_d/2, dx_d = max_d/(n_d - 1)),
+ * looked up at the nearest universe point of the centre and of the two H
+ * atoms (round half away from zero, kept inside the universe). */
+static int universe_index(double c, double maxd, double dd, int n)
+{
+    double cmin = (-0.5) * maxd;
+    int i = (int)round((c - cmin) / dd);
+    if (i <= 0) i = 0;
+    else if (i >= n) i = n - 1;
+    return i;
+}
+
+static void build_vgrid_universe(bso_state *s)
+{
+    const bso_problem *p = &s->p;
+    int nx = s->nx, ny = s->ny, nz = s->nz, na = s->na, nb = s->nb;
+    int un[3] = {p->pnx, p->pny, p->pnz};
+    double um[3] = {p->px_max, p->py_max, p->pz_max}, ud[3];
+    for (int d = 0; d < 3; d++) ud[d] = um[d] / ((double)(un[d] - 1));
+    long long U = (long long)un[0] * un[1] * un[2];
+    double *cmu = dalloc((size_t)U), *hu = dalloc((size_t)U);
+#pragma omp parallel for schedule(guided)
+    for (long long u = 0; u < U; u++) {
+        int i0 = (int)(u % un[0]), i1 = (int)((u / un[0]) % un[1]), i2 = (int)(u / ((long long)un[0] * un[1]));
+        double g[3] = {((double)i0 * ud[0]) - (um[0] / 2.0), ((double)i1 * ud[1]) - (um[1] / 2.0),
+                       ((double)i2 * ud[2]) - (um[2] / 2.0)};
+        cmu[u] = bso_coulomb_sum(g, p->q_h2_cm, p->nsites, p->site_pos, p->site_q);
+        cmu[u] += bso_lj_sum(g, p->nsites, p->site_pos, p->site_type, p->lj_a, p->lj_b);
+        hu[u] = bso_coulomb_sum(g, p->q_h2_h, p->nsites, p->site_pos, p->site_q);
+    }
+#pragma omp parallel for schedule(guided) collapse(3)
+    for (int i = 0; i < nx; i++)
+        for (int j = 0; j < ny; j++)
+            for (int k = 0; k < nz; k++) {
+                double c[3] = {s->gx[i], s->gy[j], s->gz[k]};
+                size_t ind = (((size_t)i * ny + j) * nz + k) * na;
+                for (int a = 0; a < na; a++)
+                    for (int b = 0; b < nb; b++)
+                        for (int half = 0; half < 2; half++) {
+                            double th = s->th[a], ph = half == 0 ? s->ph1[b] : s->ph2[b];
+                            double e[3] = {sin(th) * cos(ph), sin(th) * sin(ph), cos(th)};
+                            int ic[3], i1[3], i2[3];
+                            for (int d = 0; d < 3; d++) {
+                                ic[d] = universe_index(c[d], um[d], ud[d], un[d]);
+                                i1[d] = universe_index(c[d] + e[d] * p->h2_half_bond, um[d], ud[d], un[d]);
+                                i2[d] = universe_index(c[d] + e[d] * (-p->h2_half_bond), um[d], ud[d], un[d]);
+                            }
+                            double v = 0.0;
+                            v += cmu[ic[0] + (long long)un[0] * (ic[1] + (long long)un[1] * ic[2])];
+                            v += hu[i1[0] + (long long)un[0] * (i1[1] + (long long)un[1] * i1[2])];
+                            v += hu[i2[0] + (long long)un[0] * (i2[1] + (long long)un[1] * i2[2])];
+                            if (v >= p->ceiling) v = p->ceiling;
+                            if (half == 0)
+                                s->v1[(ind + a) * nb + b] = v;
+                            else
+                                s->v2[(ind + a) * nb + b] = v;
+                        }
+            }
+    free(cmu);
+    free(hu);
+}
+
 /* Potential grids.
  *  pot_kind 0: calc_Vlmlpmp_NoQuad linRotCartSphHvFullVPrepRoutines.cpp:134-184
  *              (point function per (P,a,b), clamp V >= ceiling)
@@ -319,6 +383,32 @@ static void build_vgrid(bso_state *s)
                             s->v2[(ind + a) * nb + b] = v2;
                         }
                 }
+        return;
+    }
+    if (p->pot_kind == 3) {
+        /* CoulombPotential Coulomb_Routines.cpp:6-31: K q1 q2 / |CM - 0| */
+        for (int i = 0; i < nx; i++)
+            for (int j = 0; j < ny; j++)
+                for (int k = 0; k < nz; k++) {
+                    double x = s->gx[i], y = s->gy[j], z = s->gz[k];
+                    double d2 = 0.;
+                    d2 += x * x;
+                    d2 += y * y;
+                    d2 += z * z;
+                    double v = BSO_K_CONST * p->coul_q_centre * p->coul_q_mol / sqrt(d2);
+                    if (fabs(v) > fabs(p->coul_ceil)) v = p->coul_ceil;
+                    if (v >= ceil_v) v = ceil_v;
+                    size_t ind = (((size_t)i * ny + j) * nz + k) * na;
+                    for (int a = 0; a < na; a++)
+                        for (int b = 0; b < nb; b++) {
+                            s->v1[(ind + a) * nb + b] = v;
+                            s->v2[(ind + a) * nb + b] = v;
+                        }
+                }
+        return;
+    }
+    if (p->pot_kind == 2) {
+        build_vgrid_universe(s);
         return;
     }
     long long P = s->P;
@@ -432,7 +522,7 @@ bso_state *bso_prepare_external(const bso_problem *p, const double *vblk) { retu
 static bso_state *prepare(const bso_problem *p, const double *vblk_ext)
 {
     if (p->nx <= 0 || p->ny <= 0 || p->nz <= 0 || p->lmax < 0 || p->na <= 0 || p->nb <= 0) return NULL;
-    if (p->sym < 0 || p->sym > 2 || p->pot_kind < 0 || p->pot_kind > 1) return NULL;
+    if (p->sym < 0 || p->sym > 2 || p->pot_kind < 0 || p->pot_kind > 3) return NULL;
     bso_state *s = (bso_state *)calloc(1, sizeof(bso_state));
     s->p = *p;
     s->nx = p->nx;

@@ -77,7 +77,9 @@ typedef struct bso_problem {
     double rot_const;             /* B, kJ/mol */
     double ceiling;               /* V >= ceiling -> ceiling, kJ/mol */
     int sym;                      /* 0 All, 1 Even, 2 Odd */
-    int pot_kind;                 /* 0 isotropic harmonic oscillator, 1 Alavi cage */
+    int pot_kind;                 /* 0 isotropic harmonic oscillator, 1 Alavi cage (exact H
+                                     positions), 2 Alavi cage on a potential universe grid,
+                                     3 Coulomb point charge at the origin */
     double iho_factor;            /* 0.5 m w^2 */
     double iho_ceil;              /* |V| > |ceil| -> ceil */
     int nsites;
@@ -87,6 +89,11 @@ typedef struct bso_problem {
     double q_h2_h, q_h2_cm;       /* Alavi charges (e) */
     double h2_half_bond;          /* nm, H offset from the H2 centre */
     double lj_a, lj_b;            /* A/r^12 - B/r^6 on the O sites, kJ nm^n/mol */
+    /* pot_kind 2 (grid-mapped cage): potential universe */
+    int pnx, pny, pnz;
+    double px_max, py_max, pz_max;
+    /* pot_kind 3 (Coulomb): charges at the origin and on the molecule, |V| cap */
+    double coul_q_centre, coul_q_mol, coul_ceil;
 } bso_problem;
 
 typedef struct bso_state bso_state;

@@ -46,6 +46,23 @@ def cage_case(nx, lmax, na, nb, sym="All", x_max=0.4, sites=None, lj=(1.364e-06,
     return spec, kw
 
 
+def cage_grid_case(nx, lmax, na, nb, sym="All", universe=(20, 20, 20, 0.48, 0.48, 0.48), **kw):
+    """(oracle spec, calculator kwargs) for the grid-mapped cage potential."""
+    spec, ckw = cage_case(nx, lmax, na, nb, sym, **kw)
+    spec = dict(spec, pot_kind=2, universe=tuple(universe))
+    ckw = dict(ckw, potential="cage_grid", universe=tuple(universe))
+    return spec, ckw
+
+
+def coulomb_case(nx, lmax, na, nb, x_max=0.4, charges=(1.0, -0.5, -200.0), sym="All", ceiling=18.0):
+    """(oracle spec, calculator kwargs) for a point charge at the origin."""
+    spec, ckw = iho_case(nx, lmax, na, nb, x_max, sym=sym, ceiling=ceiling)
+    spec = dict(spec, pot_kind=3, coulomb=tuple(charges))
+    ckw = {k: v for k, v in ckw.items() if k not in ("iho_factor", "iho_ceil")}
+    ckw = dict(ckw, potential="coulomb", coulomb=tuple(charges))
+    return spec, ckw
+
+
 def iho_case(nx, lmax, na, nb, x_max, omega_cm1=100.0, mass=2.016, rot=0.709648713019339, sym="All",
              ceiling=1.0e6):
     from bsl_amd import physics

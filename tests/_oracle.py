@@ -26,6 +26,9 @@ class Problem(C.Structure):
         ("site_pos", C.c_void_p), ("site_q", C.c_void_p), ("site_type", C.c_char_p),
         ("q_h2_h", C.c_double), ("q_h2_cm", C.c_double), ("h2_half_bond", C.c_double),
         ("lj_a", C.c_double), ("lj_b", C.c_double),
+        ("pnx", C.c_int), ("pny", C.c_int), ("pnz", C.c_int),
+        ("px_max", C.c_double), ("py_max", C.c_double), ("pz_max", C.c_double),
+        ("coul_q_centre", C.c_double), ("coul_q_mol", C.c_double), ("coul_ceil", C.c_double),
     ]
 
 
@@ -106,7 +109,12 @@ class Oracle:
         p.iho_factor = float(spec.get("iho_factor", 0.0))
         p.iho_ceil = float(spec.get("iho_ceil", 0.0))
         self._keep = []
-        if spec["pot_kind"] == 1:
+        if spec["pot_kind"] == 2:
+            p.pnx, p.pny, p.pnz = [int(x) for x in spec["universe"][:3]]
+            p.px_max, p.py_max, p.pz_max = [float(x) for x in spec["universe"][3:]]
+        if spec["pot_kind"] == 3:
+            p.coul_q_centre, p.coul_q_mol, p.coul_ceil = [float(x) for x in spec["coulomb"]]
+        if spec["pot_kind"] in (1, 2):
             pos = np.ascontiguousarray(spec["site_pos"], dtype=np.float64).ravel()
             q = np.ascontiguousarray(spec["site_q"], dtype=np.float64)
             typ = C.create_string_buffer(bytes(spec["site_type"]), len(spec["site_type"]))

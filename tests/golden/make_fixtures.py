@@ -6,6 +6,7 @@ Fixtures (data only: inputs and expected outputs):
   sII_unitCelltest_CM_EA_ZYZ.xyz  unit cell of the -30.2331 kJ/mol known answer
   runs.json                       decks and boundstates.txt of the saved runs
                                   Compilation/SmallBasisTest2_2013-09-03_{18_36_00,19_06_47}
+                                  (ContGrid) and _2013-08-29_{19_35_47,19_42_01} (grid-mapped)
   point_energies.json             point energies of the reference's compiled
                                   site builders and pair sums (TIP4P Coulomb in
                                   the sII unit cell, SPC/E Coulomb + LJ in cage 11)
@@ -25,7 +26,9 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 REPO = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, os.path.join(REPO, "tests"))
 
-RUNS = ["SmallBasisTest2_2013-09-03_18_36_00", "SmallBasisTest2_2013-09-03_19_06_47"]
+RUNS = ["SmallBasisTest2_2013-09-03_18_36_00", "SmallBasisTest2_2013-09-03_19_06_47",
+        # grid-mapped calculator (linRotCartSph_NoQuad_Alavi_SPCE, 20^3 potential universe)
+        "SmallBasisTest2_2013-08-29_19_35_47", "SmallBasisTest2_2013-08-29_19_42_01"]
 
 
 def parse_deck(path):

@@ -44,7 +44,7 @@ pytestmark = pytest.mark.skipif(not os.path.exists(DRIVER), reason="driver not b
 
 def test_unknown_calculator_rejected(tmp_path):
     t = str(tmp_path)
-    r = run([lanczos_deck(t, calc="linRotCartSph_NoQuad_Alavi_SPCE"), hv_deck(t)], t)
+    r = run([lanczos_deck(t, calc="linRotCartSph_Alavi_SPCE"), hv_deck(t)], t)
     assert r.returncode == 1 and "not recognized" in r.stderr
 
 
@@ -111,3 +111,30 @@ def test_driver_reproduces_saved_run(tmp_path, path):
     vec = np.loadtxt(os.path.join(d, "eigVecFile.txt"), skiprows=4)
     assert vec.shape == (16000, 3)
     assert abs(np.linalg.norm(vec[:, 1]) - 1.0) < 1e-6
+
+
+def hv_deck_grid(tmp, run):
+    shutil.copyfile(os.path.join(cs.GOLDEN, "sII_smallCage11_CM_EA_ZYZ.xyz"), os.path.join(tmp, "cage.xyz"))
+    vals = list(run["hv_deck"])
+    vals[17] = "cage.xyz"
+    vals[18] = "DriverTest"
+    p = os.path.join(tmp, "hv.txt")
+    write_deck(p, vals)
+    return p
+
+
+@pytest.mark.gpu
+def test_driver_reference_string_grid_mapped(tmp_path):
+    """The reference's own calculator string linRotCartSph_NoQuad_Alavi_SPCE
+    and decks of Compilation/SmallBasisTest2_2013-08-29_19_35_47 through the
+    driver: the saved run's converged levels."""
+    t = str(tmp_path)
+    run_fx = json.load(open(os.path.join(cs.GOLDEN, "runs.json")))["SmallBasisTest2_2013-08-29_19_35_47"]
+    lz = run_fx["lanczos_deck"]
+    r = run([lanczos_deck(t, niter=int(lz[0]), calc=lz[5]), hv_deck_grid(t, run_fx)], t)
+    assert r.returncode == 0, r.stderr[-2000:]
+    d = os.path.join(t, [x for x in os.listdir(t) if x.startswith("DriverTest_")][0])
+    got = _read_boundstates(d)
+    ref = np.array(run_fx["boundstates"])
+    for e in ref[:3, 0]:
+        assert np.min(np.abs(got[:, 0] - e)) / abs(e) < 1e-10

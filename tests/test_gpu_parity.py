@@ -321,3 +321,44 @@ def test_quad_eigenvalues_match_noquad():
     assert len(conv) >= 3
     for e in conv:
         assert np.min(np.abs(eq - e)) / abs(e) < 1e-9
+
+
+# ---------------------------------------------- grid-mapped and Coulomb ----
+
+@pytest.mark.parametrize("sym,basis", [("All", "full"), ("Odd", "reduced")])
+def test_grid_mapped_potential_and_hv(sym, basis):
+    """Potential universe lookups (linRotCartSph_NoQuad_Alavi_SPCE) and Hv."""
+    spec, kw = cs.cage_grid_case(5, 3, 8, 9, sym, universe=(17, 19, 23, 0.44, 0.5, 0.41))
+    o = O.Oracle(spec)
+    c = B.Calculator(basis=basis, **kw)
+    g1, g2 = c.vgrid()
+    r1, r2 = o.vgrid()
+    assert _rel(g1, r1) < 1e-13 and _rel(g2, r2) < 1e-13
+    v = np.random.default_rng(41).uniform(-1, 1, o.N)
+    assert _rel(c.hv(v), o.hv(v)) < 1e-12
+
+
+@pytest.mark.parametrize("name,nlev", [("SmallBasisTest2_2013-08-29_19_35_47", 3),
+                                       ("SmallBasisTest2_2013-08-29_19_42_01", 8)])
+def test_saved_grid_mapped_runs(name, nlev):
+    run = json.load(open(os.path.join(cs.GOLDEN, "runs.json")))[name]
+    hv = run["hv_deck"]
+    _, kw = cs.cage_grid_case(int(hv[0]), int(hv[6]), int(hv[7]), int(hv[8]), hv[21], x_max=float(hv[1]),
+                              mass=float(hv[15]), rot=float(hv[16]), ceiling=float(hv[19]),
+                              universe=(int(hv[9]), int(hv[11]), int(hv[13]), float(hv[10]), float(hv[12]),
+                                        float(hv[14])))
+    c = B.Calculator(**kw)
+    ev, err = B.lancbis(*c.lanczos(int(run["lanczos_deck"][0])), -10.0, 10.0)
+    ref = np.array(run["boundstates"])
+    assert np.max(np.abs(ev[:nlev] - ref[:nlev, 0]) / np.abs(ref[:nlev, 0])) < 1e-10
+
+
+def test_coulomb_potential_and_hv():
+    spec, kw = cs.coulomb_case(6, 2, 6, 6)
+    o = O.Oracle(spec)
+    c = B.Calculator(**kw)
+    g1, _ = c.vgrid()
+    r1, _ = o.vgrid()
+    assert _rel(g1, r1) < 1e-14
+    v = np.random.default_rng(42).uniform(-1, 1, o.N)
+    assert _rel(c.hv(v), o.hv(v)) < 1e-12

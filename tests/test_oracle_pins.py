@@ -361,6 +361,93 @@ def test_saved_run_100_steps_ground_state():
     assert abs(ev[1] - ref[1, 0]) < 1e-10 * abs(ref[1, 0])
 
 
+def _grid_run_case(name):
+    run, spec = _run_case(name)
+    hv = run["hv_deck"]
+    spec = dict(spec, pot_kind=2, universe=(int(hv[9]), int(hv[11]), int(hv[13]), float(hv[10]), float(hv[12]),
+                                            float(hv[14])))
+    return run, spec
+
+
+@pytest.mark.parametrize("name,nlev", [("SmallBasisTest2_2013-08-29_19_35_47", 3),
+                                       ("SmallBasisTest2_2013-08-29_19_42_01", 8)])
+def test_saved_grid_mapped_runs(name, nlev):
+    """The grid-mapped calculator (linRotCartSph_NoQuad_Alavi_SPCE: centre and
+    H potentials on a 20^3 universe, nearest-point lookups) reproduces the
+    reference's saved runs (Compilation/<name>/boundstates.txt)."""
+    run, spec = _grid_run_case(name)
+    assert run["lanczos_deck"][5] == "linRotCartSph_NoQuad_Alavi_SPCE"
+    o = O.Oracle(spec)
+    ev, err = O.lancbis(*o.lanczos(int(run["lanczos_deck"][0])), -10.0, 10.0)
+    ref = np.array(run["boundstates"])
+    assert np.max(np.abs(ev[:nlev] - ref[:nlev, 0]) / np.abs(ref[:nlev, 0])) < 1e-11
+
+
+def test_grid_mapped_lookups_recomputed():
+    """Nearest-universe-point lookups recomputed with numpy (universe sizes
+    and lengths differ per axis, so an axis mix-up or a wrong rounding rule
+    fails), with pair sums at the universe points."""
+    sites = cs.cage11_sites()
+    pos, q, typ, lj = sites
+    spec, _ = cs.cage_case(4, 1, 3, 4, "All", sites=sites)
+    un, um = (7, 6, 5), (0.52, 0.61, 0.47)
+    spec = dict(spec, pot_kind=2, universe=un + um)
+    o = O.Oracle(spec)
+    v1, v2 = o.vgrid()
+    g = o.grid(0)
+    ct, wt, cp, wp = o.quad()
+    K = 138.9354846111096
+    du = [um[d] / (un[d] - 1) for d in range(3)]
+
+    def idx(c, d):
+        x = (c - (-0.5 * um[d])) / du[d]
+        i = int(math.floor(x + 0.5)) if x >= 0 else -int(math.floor(-x + 0.5))  # half away from zero
+        return min(max(i, 0), un[d] - 1)
+
+    def upoint(ix):
+        return np.array([ix[d] * du[d] - um[d] / 2.0 for d in range(3)])
+
+    rng = np.random.default_rng(1)
+    for _ in range(20):
+        i, j, k = rng.integers(0, 4, 3)
+        a, b = rng.integers(0, 3), rng.integers(0, 4)
+        c = np.array([g[i], g[j], g[k]])
+        th, ph = np.arccos(ct[a]), np.arccos(cp[b])
+        for half, phi in ((0, ph), (1, 2 * np.pi - ph)):
+            e = np.array([np.sin(th) * np.cos(phi), np.sin(th) * np.sin(phi), np.cos(th)])
+            u0 = upoint([idx(c[d], d) for d in range(3)])
+            r = np.linalg.norm(pos[lj == 1] - u0, axis=1)
+            v = np.sum(K * -0.9864 * q / np.linalg.norm(pos - u0, axis=1)) + np.sum(1.364e-6 / r ** 12 - 1.533e-3 / r ** 6)
+            for sgn in (1.0, -1.0):
+                uh = upoint([idx(c[d] + e[d] * sgn * 0.03707, d) for d in range(3)])
+                v += np.sum(K * 0.4932 * q / np.linalg.norm(pos - uh, axis=1))
+            v = min(v, 18.0)
+            got = (v1 if half == 0 else v2)[i, j, k, a, b]
+            assert abs(got - v) < 1e-12 * max(1, abs(v))
+
+
+def test_coulomb_potential_closed_form():
+    """linRotCartSph_NoQuad_Coulomb: V = K q1 q2 / |r| capped at |V| <= |cap|,
+    orientation independent, so the V blocks are V(r) times the identity."""
+    from bsl_amd import physics
+    spec, _ = cs.iho_case(4, 2, 6, 6, x_max=0.4)
+    spec = dict(spec, pot_kind=3, coulomb=(1.0, -0.5, -200.0), ceiling=18.0)
+    o = O.Oracle(spec)
+    v1, v2 = o.vgrid()
+    g = o.grid(0)
+    for i, x in enumerate(g):
+        for j, y in enumerate(o.grid(1)):
+            for k, z in enumerate(o.grid(2)):
+                v = physics.K_COULOMB * 1.0 * -0.5 / np.sqrt(x * x + y * y + z * z)
+                v = -200.0 if abs(v) > 200.0 else v
+                assert np.all(v1[i, j, k] == v1[i, j, k].flat[0])
+                assert abs(v1[i, j, k].flat[0] - min(v, 18.0)) <= 1e-13 * abs(v)
+    vb = o.vblocks()
+    nn = vb.shape[-1]
+    off = vb - np.einsum("ijkn,nm->ijknm", np.einsum("ijknn->ijkn", vb), np.eye(nn))
+    assert np.max(np.abs(off)) < 1e-12 * np.max(np.abs(vb))
+
+
 def test_reference_calculator_vectors():
     """V blocks and Hv of the reference's own compiled NoQuad ContGrid
     calculator (committed fixture): identical to the last bit."""

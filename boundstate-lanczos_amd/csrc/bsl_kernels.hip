@@ -269,17 +269,208 @@ __global__ __launch_bounds__(kWG) void k_kin_z_mfma(const double *__restrict__ X
         }
 }
 
+// Even axis lengths: T is symmetric Toeplitz and centrosymmetric, so with
+// mirror index n-1-i the sums and differences v(i) +- v(n-1-i), i < n/2, are
+// mapped separately by the n/2 x n/2 matrices t(|i - i2|) +- t(n-1-i-i2):
+// half the multiply-adds of the full contraction.
+constexpr int kCompLd = 33;
+
+// x and y axes on the four x/y parity components of the slab,
+// v_pq(j, i) = v(j, i) + p v(j, i') + q v(j', i) + pq v(j', i'), j, i < n/2;
+// wave w owns the 16 x 16 tile (w/2, w%2) of all four components, so each lane
+// holds the four transformed components of its elements and recombines them
+// into the four mirror points.
+template <typename T>
+__global__ __launch_bounds__(kWG) void k_kin_xy_half(const double *__restrict__ X, double *__restrict__ R,
+                                                     const double *__restrict__ tx, const double *__restrict__ ty,
+                                                     const double *__restrict__ rot, const double *__restrict__ mask,
+                                                     const Scal *__restrict__ sc, int nx, int ny, int nz)
+{
+    typedef typename KinAcc<T>::type acc_t;
+    extern __shared__ __align__(16) unsigned char kin_lds[];
+    T *comp = (T *)kin_lds;                // [4][32][kCompLd]
+    T *rx = comp + 4 * 32 * kCompLd;       // [129]
+    T *ry = rx + 129;                      // [129]
+    const int k = blockIdx.x, n = blockIdx.y;
+    const double sx = sc->sx, sr = sc->sr;
+    const size_t base = ((size_t)n * nz + k) * (size_t)ny * nx;
+    const int mx = nx >> 1, my = ny >> 1;
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int l16 = lane & 15, kq = lane >> 4;
+    const int ta = wid >> 1, tb = wid & 1;
+    const int i = 16 * tb + l16;  // this lane's component column
+    // R of this lane's 4 items x 4 mirror points, loaded with the slab
+    double rp[4][4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const int j = 16 * ta + kin_row<T>(kq, q);
+        const bool ok = j < my && i < mx;
+        const int jj[2] = {j, ny - 1 - j}, ii[2] = {i, nx - 1 - i};
+#pragma unroll
+        for (int m = 0; m < 4; m++)
+            rp[q][m] = ok ? R[base + (size_t)jj[m >> 1] * nx + ii[m & 1]] : 0.0;
+    }
+    // the four parity components, zero padded to 32 x 32
+    for (int t = threadIdx.x; t < 32 * 32; t += blockDim.x) {
+        const int j = t >> 5, c = t & 31;
+        double v00 = 0, v01 = 0, v10 = 0, v11 = 0;
+        if (j < my && c < mx) {
+            const double *Xp = X + base;
+            v00 = Xp[(size_t)j * nx + c];
+            v01 = Xp[(size_t)j * nx + (nx - 1 - c)];
+            v10 = Xp[(size_t)(ny - 1 - j) * nx + c];
+            v11 = Xp[(size_t)(ny - 1 - j) * nx + (nx - 1 - c)];
+        }
+        const double a = v00 + v01, b = v00 - v01, cc = v10 + v11, d = v10 - v11;
+        comp[(0 * 32 + j) * kCompLd + c] = (T)(sx * (a + cc));  // p = +, q = +
+        comp[(1 * 32 + j) * kCompLd + c] = (T)(sx * (b + d));   // p = -, q = +
+        comp[(2 * 32 + j) * kCompLd + c] = (T)(sx * (a - cc));  // p = +, q = -
+        comp[(3 * 32 + j) * kCompLd + c] = (T)(sx * (b - d));   // p = -, q = -
+    }
+    for (int t = threadIdx.x; t < 129; t += blockDim.x) {
+        const int d = abs(t - 64);
+        rx[t] = d < nx ? (T)tx[d] : (T)0;
+        ry[t] = d < ny ? (T)ty[d] : (T)0;
+    }
+    __syncthreads();
+    acc_t acc[4];
+#pragma unroll
+    for (int c = 0; c < 4; c++) acc[c] = acc_t{0, 0, 0, 0};
+    const int jr = 16 * ta + l16;  // A-operand row of this lane
+    const int kx = (mx + 3) >> 2, ky = (my + 3) >> 2;
+    for (int s = 0; s < kx; s++) {
+        const int i2 = 4 * s + kq;
+        const T t1 = rx[64 + i2 - i], t2 = rx[64 + (nx - 1 - i2 - i)];
+        const T bp = t1 + t2, bm = t1 - t2;
+#pragma unroll
+        for (int c = 0; c < 4; c++)
+            acc[c] = kmfma(comp[(c * 32 + jr) * kCompLd + i2], (c & 1) ? bm : bp, acc[c]);
+    }
+    for (int s = 0; s < ky; s++) {
+        const int j2 = 4 * s + kq;
+        const T t1 = ry[64 + jr - j2], t2 = ry[64 + (ny - 1 - jr - j2)];
+        const T ap = t1 + t2, am = t1 - t2;
+#pragma unroll
+        for (int c = 0; c < 4; c++)
+            acc[c] = kmfma((c & 2) ? am : ap, comp[(c * 32 + j2) * kCompLd + i], acc[c]);
+    }
+    const double rn = rot[n], mn = mask[n];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const int j = 16 * ta + kin_row<T>(kq, q);
+        if (j >= my || i >= mx) continue;
+        const double d0 = (double)acc[0][q], d1 = (double)acc[1][q], d2 = (double)acc[2][q], d3 = (double)acc[3][q];
+        const double c0 = (double)comp[(0 * 32 + j) * kCompLd + i], c1 = (double)comp[(1 * 32 + j) * kCompLd + i];
+        const double c2 = (double)comp[(2 * 32 + j) * kCompLd + i], c3 = (double)comp[(3 * 32 + j) * kCompLd + i];
+        // mirror point m = (y mirrored) * 2 + (x mirrored): signs p^mx q^my
+        const double o[4] = {0.25 * ((d0 + d1) + (d2 + d3)), 0.25 * ((d0 - d1) + (d2 - d3)),
+                             0.25 * ((d0 + d1) - (d2 + d3)), 0.25 * ((d0 - d1) - (d2 - d3))};
+        const double v[4] = {0.25 * ((c0 + c1) + (c2 + c3)), 0.25 * ((c0 - c1) + (c2 - c3)),
+                             0.25 * ((c0 + c1) - (c2 + c3)), 0.25 * ((c0 - c1) - (c2 - c3))};
+        const int jj[2] = {j, ny - 1 - j}, ii[2] = {i, nx - 1 - i};
+#pragma unroll
+        for (int m = 0; m < 4; m++)
+            R[base + (size_t)jj[m >> 1] * nx + ii[m & 1]] = sr * rp[q][m] + mn * (o[m] + rn * v[m]);
+    }
+}
+
+// z axis of 64 columns with the z parity split: s = v(k) + v(k'), d = v(k) - v(k'),
+// k < nz/2; wave w owns columns 16w.. of both components, every k tile.
+template <typename T>
+__global__ __launch_bounds__(kWG) void k_kin_z_half(const double *__restrict__ X, double *__restrict__ R,
+                                                    const double *__restrict__ tz, const double *__restrict__ mask,
+                                                    const Scal *__restrict__ sc, int nxy, int nz)
+{
+    typedef typename KinAcc<T>::type acc_t;
+    extern __shared__ __align__(16) unsigned char kin_lds[];
+    T *cs = (T *)kin_lds;                  // [32][kSlabLd]: sums
+    T *cd = cs + 32 * kSlabLd;             // [32][kSlabLd]: differences
+    T *rz = cd + 32 * kSlabLd;             // [129]
+    const int c0 = blockIdx.x * 64, n = blockIdx.y;
+    const double sx = sc->sx;
+    const size_t base = (size_t)n * nz * nxy;
+    const int mz = nz >> 1;
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int l16 = lane & 15, kq = lane >> 4;
+    const int cw = 16 * wid + l16, c = c0 + cw;
+    double rp[2][4][2];
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int kk = 16 * a + kin_row<T>(kq, q);
+            const bool ok = kk < mz && c < nxy;
+            rp[a][q][0] = ok ? R[base + (size_t)kk * nxy + c] : 0.0;
+            rp[a][q][1] = ok ? R[base + (size_t)(nz - 1 - kk) * nxy + c] : 0.0;
+        }
+    for (int t = threadIdx.x; t < 32 * 64; t += blockDim.x) {
+        const int kk = t >> 6, cc = t & 63;
+        double v0 = 0, v1 = 0;
+        if (kk < mz && c0 + cc < nxy) {
+            v0 = X[base + (size_t)kk * nxy + c0 + cc];
+            v1 = X[base + (size_t)(nz - 1 - kk) * nxy + c0 + cc];
+        }
+        cs[kk * kSlabLd + cc] = (T)(sx * (v0 + v1));
+        cd[kk * kSlabLd + cc] = (T)(sx * (v0 - v1));
+    }
+    for (int t = threadIdx.x; t < 129; t += blockDim.x) {
+        const int d = abs(t - 64);
+        rz[t] = d < nz ? (T)tz[d] : (T)0;
+    }
+    __syncthreads();
+    acc_t as[2], ad[2];
+#pragma unroll
+    for (int a = 0; a < 2; a++) as[a] = ad[a] = acc_t{0, 0, 0, 0};
+    const int kz = (mz + 3) >> 2;
+    for (int s = 0; s < kz; s++) {
+        const int kp = 4 * s + kq;
+        const T bs = cs[kp * kSlabLd + cw], bd = cd[kp * kSlabLd + cw];
+#pragma unroll
+        for (int a = 0; a < 2; a++) {
+            const int kr = 16 * a + l16;
+            const T t1 = rz[64 + kr - kp], t2 = rz[64 + (nz - 1 - kr - kp)];
+            as[a] = kmfma(t1 + t2, bs, as[a]);
+            ad[a] = kmfma(t1 - t2, bd, ad[a]);
+        }
+    }
+    const double mn = mask[n];
+    if (c >= nxy) return;
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int kk = 16 * a + kin_row<T>(kq, q);
+            if (kk >= mz) continue;
+            const double S = (double)as[a][q], D = (double)ad[a][q];
+            R[base + (size_t)kk * nxy + c] = rp[a][q][0] + mn * (0.5 * (S + D));
+            R[base + (size_t)(nz - 1 - kk) * nxy + c] = rp[a][q][1] + mn * (0.5 * (S - D));
+        }
+}
+
 template <typename T>
 static int launch_kinetic_t(const DevOperator &op, const double *X, double *R, Scal *sc, hipStream_t s)
 {
-    const size_t lds_xy = sizeof(T) * ((size_t)64 * kSlabLd + 2 * 129);
-    hipLaunchKernelGGL(k_kin_xy_mfma<T>, dim3(op.nz, op.nk), dim3(kWG), lds_xy, s, X, R, op.tx, op.ty, op.rot,
-                       op.mask, sc, op.nx, op.ny, op.nz);
+    // even axes use the half-size (centrosymmetric) contractions
+    if (op.nx % 2 == 0 && op.ny % 2 == 0) {
+        const size_t lds = sizeof(T) * ((size_t)4 * 32 * kCompLd + 2 * 129);
+        hipLaunchKernelGGL(k_kin_xy_half<T>, dim3(op.nz, op.nk), dim3(kWG), lds, s, X, R, op.tx, op.ty, op.rot,
+                           op.mask, sc, op.nx, op.ny, op.nz);
+    } else {
+        const size_t lds = sizeof(T) * ((size_t)64 * kSlabLd + 2 * 129);
+        hipLaunchKernelGGL(k_kin_xy_mfma<T>, dim3(op.nz, op.nk), dim3(kWG), lds, s, X, R, op.tx, op.ty, op.rot,
+                           op.mask, sc, op.nx, op.ny, op.nz);
+    }
     BSL_HIP(hipGetLastError());
     const int nxy = op.nx * op.ny;
-    const size_t lds_z = sizeof(T) * ((size_t)64 * kSlabLd + 129);
-    hipLaunchKernelGGL(k_kin_z_mfma<T>, dim3((nxy + 63) / 64, op.nk), dim3(kWG), lds_z, s, X, R, op.tz, op.mask,
-                       sc, nxy, op.nz);
+    if (op.nz % 2 == 0) {
+        const size_t lds = sizeof(T) * ((size_t)2 * 32 * kSlabLd + 129);
+        hipLaunchKernelGGL(k_kin_z_half<T>, dim3((nxy + 63) / 64, op.nk), dim3(kWG), lds, s, X, R, op.tz, op.mask,
+                           sc, nxy, op.nz);
+    } else {
+        const size_t lds = sizeof(T) * ((size_t)64 * kSlabLd + 129);
+        hipLaunchKernelGGL(k_kin_z_mfma<T>, dim3((nxy + 63) / 64, op.nk), dim3(kWG), lds, s, X, R, op.tz,
+                           op.mask, sc, nxy, op.nz);
+    }
     BSL_HIP(hipGetLastError());
     return BSL_OK;
 }

@@ -288,6 +288,7 @@ extern "C" int bsl_destroy(bsl_ctx *c)
     dev_free(o.q_rl);
     dev_free(o.partial);
     dev_free(o.partial_b);
+    dev_free(o.partial_c);
     dev_free(c->d_gx);
     dev_free(c->d_gy);
     dev_free(c->d_gz);
@@ -406,6 +407,7 @@ extern "C" int bsl_create(const bsl_params *pp, bsl_ctx **out)
     op.npartial = (int)std::max<int64_t>(op.nalpha, 4096);
     if ((rc = dev_alloc(&op.partial, (size_t)op.npartial))) return rc;
     if ((rc = dev_alloc(&op.partial_b, (size_t)op.npartial))) return rc;
+    if ((rc = dev_alloc(&op.partial_c, (size_t)op.npartial))) return rc;
 
     // ---- device: potential grid and V blocks ----
     BSL_HIP(hipMalloc((void **)&c->d_vgrid, sizeof(double) * (size_t)c->P * c->Q));
@@ -464,10 +466,13 @@ extern "C" int bsl_create(const bsl_params *pp, bsl_ctx **out)
         op.npartial = std::max<int>(op.npartial, (int)op.nalpha + 1);
         dev_free(op.partial);
         dev_free(op.partial_b);
+        dev_free(op.partial_c);
         if ((rc = dev_alloc(&op.partial, (size_t)op.npartial))) return rc;
         if ((rc = dev_alloc(&op.partial_b, (size_t)op.npartial))) return rc;
+        if ((rc = dev_alloc(&op.partial_c, (size_t)op.npartial))) return rc;
         c->op64.partial = op.partial;
         c->op64.partial_b = op.partial_b;
+        c->op64.partial_c = op.partial_c;
         c->op64.npartial = op.npartial;
     } else if (p.precision == BSL_PREC_FP64) {
         op.vpk = c->op64.vpk;
@@ -577,7 +582,7 @@ int lanczos_step(bsl_ctx *c, const DevOperator &op, int j, hipEvent_t *ev)
     int rc = launch_hv(op, c->d_X, c->d_R, c->d_scal, c->stream, ev);
     if (rc) return rc;
     (void)j;  // the device step counter indexes the coefficient outputs
-    rc = launch_update(op, c->d_X, c->d_R, c->d_scal, c->d_alpha, c->d_beta, c->d_beta2, c->stream);
+    rc = launch_finalize(op, c->d_scal, c->d_alpha, c->d_beta, c->d_beta2, c->stream);
     if (rc) return rc;
     if (ev) BSL_HIP(hipEventRecord(ev[3], c->stream));
     std::swap(c->d_X, c->d_R);
@@ -705,7 +710,7 @@ extern "C" int bsl_ritz(bsl_ctx *c, int32_t niter, const double *v0, int32_t sta
     rc = ensure_coeffs(c, niter);
     if (!rc) rc = start_lanczos(c, v0, start, seed);
     for (int j = 1; j <= niter && !rc; j++) {
-        rc = launch_ritz_accum(d_ritz, c->d_X, c->d_scal, d_coef + (size_t)(j - 1) * ngood, c->n_kry, ngood,
+        rc = launch_ritz_accum(d_ritz, c->d_X, c->d_R, c->d_scal, d_coef + (size_t)(j - 1) * ngood, c->n_kry, ngood,
                                c->stream);
         if (!rc) rc = lanczos_step(c, c->op, j, nullptr);
     }

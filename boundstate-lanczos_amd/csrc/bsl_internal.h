@@ -15,11 +15,16 @@ namespace bsl {
 
 // recurrence scalars kept in device memory so that the Lanczos loop never
 // synchronises with the host: the actual Krylov vectors are sx*X and sr*R.
+// Scalars of the recurrence, kept on the device.  At the start of a step
+// the first vector slot X holds r~ = H v_j - beta_j v_{j-1} (alpha not yet
+// removed) and the second slot R holds v_j; the x-y kinetic kernel forms
+// v_{j+1} = ca X + cb R (= (r~ - alpha v_j) / beta) and R = cr R (= -beta v_j)
+// on the fly, so no separate update pass is needed.
 struct Scal {
-    double sx, sr;
+    double ca, cb, cr;
     double alpha, beta, beta2;
     long long step;  // completed Lanczos steps: alpha/beta outputs are indexed by it
-    double pad[2];
+    double pad;
 };
 
 // error reporting (thread-local message + code)
@@ -112,30 +117,34 @@ struct DevOperator {
     int qmode = 0;                // 0 FP64 MFMA, 1 FP32 MFMA, 2 BF16x3 MFMA
     int q_npts = 0;               // grid points per workgroup of the quadrature kernel
     int *q_rl = nullptr;          // [16][9]: per m (+lmax) the count, then the carried planes
-    // reduction scratch: `partial` holds the nalpha per-workgroup partials of
-    // alpha written by the potential kernel, partial_b those of |r|^2
-    double *partial = nullptr, *partial_b = nullptr;
+    // reduction scratch: the potential kernel writes nalpha per-workgroup
+    // partials of v.R (partial), |R|^2 (partial_b) and |v|^2 (partial_c)
+    double *partial = nullptr, *partial_b = nullptr, *partial_c = nullptr;
     int npartial = 0;
     int64_t nalpha = 0;
 };
 
-// one Hv: R = sr*R + mask*(H (sx*X)); the potential kernel leaves the
-// nalpha partial sums of alpha = (sx X).R in op.partial
-int launch_hv(const DevOperator &op, const double *X, double *R, Scal *sc, hipStream_t s,
+// one Hv: v = ca X + cb R is formed by the x-y kinetic kernel and written
+// to X; R = cr R + mask*(H v); the potential kernel leaves the nalpha
+// partial sums of v.R, |R|^2, |v|^2 in op.partial, partial_b, partial_c
+int launch_hv(const DevOperator &op, double *X, double *R, Scal *sc, hipStream_t s,
               hipEvent_t *ev /* 3 events or null */);
 // the potential kernel alone (used by the autotuner)
 int launch_potential(const DevOperator &op, const double *X, double *R, Scal *sc, hipStream_t s);
-// alpha from the partials, R -= alpha (sx X), beta^2 = |R|^2 and the scalars
-// of the swap; alpha_out[step], beta_out[step + 1], beta2_out[step + 1] with
-// step the device counter of completed steps, which advances by one
-int launch_update(const DevOperator &op, const double *X, double *R, Scal *sc, double *alpha_out, double *beta_out,
-                  double *beta2_out, hipStream_t s);
-int launch_init_scal(Scal *sc, double sx, double sr, hipStream_t s);
+// alpha = v.R, beta^2 = |R|^2 - 2 alpha v.R + alpha^2 |v|^2 from the
+// potential kernel's three partial lists, and the scalars of the next step;
+// alpha_out[step], beta_out[step + 1], beta2_out[step + 1] with step the
+// device counter of completed steps, which advances by one
+int launch_finalize(const DevOperator &op, Scal *sc, double *alpha_out, double *beta_out, double *beta2_out,
+                    hipStream_t s);
+// ca = scale of X in the vector the next Hv reads, cb = 0, cr = scale of R
+int launch_init_scal(Scal *sc, double ca, double cr, hipStream_t s);
 int launch_start_vector(double *X, int64_t n, int64_t P, const int *plane_ref, int kind, uint64_t seed, Scal *sc,
                         double *partial, int npartial, hipStream_t s);
 int launch_zero(double *p, int64_t n, hipStream_t s);
-int launch_ritz_accum(double *ritz, const double *X, const Scal *sc, const double *coef, int64_t n, int ngood,
-                      hipStream_t s);
+// ritz[g] += coef[g] (ca X + cb R): the Lanczos vector of the current step
+int launch_ritz_accum(double *ritz, const double *X, const double *R, const Scal *sc, const double *coef, int64_t n,
+                      int ngood, hipStream_t s);
 
 // prep kernels
 struct CageDev {

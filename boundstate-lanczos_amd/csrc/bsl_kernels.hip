@@ -107,7 +107,7 @@ constexpr int kSlabLd = 65;
 // (products and accumulation in FP32, the recurrence vectors stay FP64).
 // R = sr * R + mask[n] * (D + rot[n] v)
 template <typename T>
-__global__ __launch_bounds__(kWG) void k_kin_xy_mfma(const double *__restrict__ X, double *__restrict__ R,
+__global__ __launch_bounds__(kWG) void k_kin_xy_mfma(double *__restrict__ X, double *__restrict__ R,
                                                      const double *__restrict__ tx, const double *__restrict__ ty,
                                                      const double *__restrict__ rot, const double *__restrict__ mask,
                                                      const Scal *__restrict__ sc, int nx, int ny, int nz)
@@ -118,7 +118,7 @@ __global__ __launch_bounds__(kWG) void k_kin_xy_mfma(const double *__restrict__ 
     T *rx = pl + 64 * kSlabLd;         // [129]
     T *ry = rx + 129;                  // [129]
     const int k = blockIdx.x, n = blockIdx.y;
-    const double sx = sc->sx, sr = sc->sr;
+    const double ca = sc->ca, cb = sc->cb, cr = sc->cr;
     const size_t base = ((size_t)n * nz + k) * (size_t)ny * nx;
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     const int l16 = lane & 15, kq = lane >> 4;
@@ -139,17 +139,21 @@ __global__ __launch_bounds__(kWG) void k_kin_xy_mfma(const double *__restrict__ 
                 rp[a][b][q] = (active && j < ny && i < nx) ? R[base + (size_t)j * nx + i] : 0.0;
             }
     {
-        const double *Xp = X + base;
+        // v = ca X + cb R over the slab, written back to X (the vector the
+        // other kernels of this Hv read)
+        double *Xp = X + base;
+        const double *Rp = R + base;
         double tmp[16];
 #pragma unroll
         for (int e = 0; e < 16; e++) {
             const int t = threadIdx.x + e * kWG, j = t >> 6, i = t & 63;
-            tmp[e] = (j < ny && i < nx) ? Xp[j * nx + i] : 0.0;
+            tmp[e] = (j < ny && i < nx) ? ca * Xp[j * nx + i] + cb * Rp[j * nx + i] : 0.0;
         }
 #pragma unroll
         for (int e = 0; e < 16; e++) {
             const int t = threadIdx.x + e * kWG, j = t >> 6, i = t & 63;
-            pl[j * kSlabLd + i] = (T)(sx * tmp[e]);
+            pl[j * kSlabLd + i] = (T)tmp[e];
+            if (j < ny && i < nx) Xp[j * nx + i] = tmp[e];
         }
     }
     for (int t = threadIdx.x; t < 129; t += blockDim.x) {
@@ -194,7 +198,7 @@ __global__ __launch_bounds__(kWG) void k_kin_xy_mfma(const double *__restrict__ 
                     const int j = j0 + 16 * a + kin_row<T>(kq, q);
                     if (j < ny && i < nx)
                         R[base + (size_t)j * nx + i] =
-                            sr * rp[a][b][q] + mn * ((double)acc[a][b][q] + rn * (double)pl[j * kSlabLd + i]);
+                            cr * rp[a][b][q] + mn * ((double)acc[a][b][q] + rn * (double)pl[j * kSlabLd + i]);
                 }
             }
     }
@@ -213,7 +217,7 @@ __global__ __launch_bounds__(kWG) void k_kin_z_mfma(const double *__restrict__ X
     T *col = (T *)kin_lds;             // [64][kSlabLd]
     T *rz = col + 64 * kSlabLd;        // [129]
     const int c0 = blockIdx.x * 64, n = blockIdx.y;
-    const double sx = sc->sx;
+    (void)sc;  // X already holds v
     const size_t base = (size_t)n * nz * nxy;
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     const int l16 = lane & 15, kq = lane >> 4;
@@ -240,7 +244,7 @@ __global__ __launch_bounds__(kWG) void k_kin_z_mfma(const double *__restrict__ X
 #pragma unroll
         for (int e = 0; e < 16; e++) {
             const int t = threadIdx.x + e * kWG, kk = t >> 6, cc = t & 63;
-            col[kk * kSlabLd + cc] = (T)(sx * tmp[e]);
+            col[kk * kSlabLd + cc] = (T)tmp[e];
         }
     }
     for (int t = threadIdx.x; t < 129; t += blockDim.x) {
@@ -281,7 +285,8 @@ constexpr int kCompLd = 33;
 // holds the four transformed components of its elements and recombines them
 // into the four mirror points.
 template <typename T>
-__global__ __launch_bounds__(kWG) void k_kin_xy_half(const double *__restrict__ X, double *__restrict__ R,
+__global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_kin_xy_half(double *__restrict__ X,
+                                                                                         double *__restrict__ R,
                                                      const double *__restrict__ tx, const double *__restrict__ ty,
                                                      const double *__restrict__ rot, const double *__restrict__ mask,
                                                      const Scal *__restrict__ sc, int nx, int ny, int nz)
@@ -292,40 +297,49 @@ __global__ __launch_bounds__(kWG) void k_kin_xy_half(const double *__restrict__ 
     T *rx = comp + 4 * 32 * kCompLd;       // [129]
     T *ry = rx + 129;                      // [129]
     const int k = blockIdx.x, n = blockIdx.y;
-    const double sx = sc->sx, sr = sc->sr;
+    const double ca = sc->ca, cb = sc->cb, cr = sc->cr;
     const size_t base = ((size_t)n * nz + k) * (size_t)ny * nx;
     const int mx = nx >> 1, my = ny >> 1;
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     const int l16 = lane & 15, kq = lane >> 4;
     const int ta = wid >> 1, tb = wid & 1;
     const int i = 16 * tb + l16;  // this lane's component column
-    // R of this lane's 4 items x 4 mirror points, loaded with the slab
+    // this lane's 4 items x 4 mirror points: R (kept for the epilogue) and
+    // v = ca X + cb R (written back to X), then the four parity components
+    // of v, zero padded to 32 x 32; the lanes cover the 32 x 32 component
+    // grid once
     double rp[4][4];
+    {
+        double xp[4][4], vp[4][4];
 #pragma unroll
-    for (int q = 0; q < 4; q++) {
-        const int j = 16 * ta + kin_row<T>(kq, q);
-        const bool ok = j < my && i < mx;
-        const int jj[2] = {j, ny - 1 - j}, ii[2] = {i, nx - 1 - i};
+        for (int q = 0; q < 4; q++) {
+            const int j = 16 * ta + kin_row<T>(kq, q);
+            const bool ok = j < my && i < mx;
+            const int jj[2] = {j, ny - 1 - j}, ii[2] = {i, nx - 1 - i};
 #pragma unroll
-        for (int m = 0; m < 4; m++)
-            rp[q][m] = ok ? R[base + (size_t)jj[m >> 1] * nx + ii[m & 1]] : 0.0;
-    }
-    // the four parity components, zero padded to 32 x 32
-    for (int t = threadIdx.x; t < 32 * 32; t += blockDim.x) {
-        const int j = t >> 5, c = t & 31;
-        double v00 = 0, v01 = 0, v10 = 0, v11 = 0;
-        if (j < my && c < mx) {
-            const double *Xp = X + base;
-            v00 = Xp[(size_t)j * nx + c];
-            v01 = Xp[(size_t)j * nx + (nx - 1 - c)];
-            v10 = Xp[(size_t)(ny - 1 - j) * nx + c];
-            v11 = Xp[(size_t)(ny - 1 - j) * nx + (nx - 1 - c)];
+            for (int m = 0; m < 4; m++) {
+                const size_t o = base + (size_t)jj[m >> 1] * nx + ii[m & 1];
+                rp[q][m] = ok ? R[o] : 0.0;
+                xp[q][m] = ok ? X[o] : 0.0;
+            }
         }
-        const double a = v00 + v01, b = v00 - v01, cc = v10 + v11, d = v10 - v11;
-        comp[(0 * 32 + j) * kCompLd + c] = (T)(sx * (a + cc));  // p = +, q = +
-        comp[(1 * 32 + j) * kCompLd + c] = (T)(sx * (b + d));   // p = -, q = +
-        comp[(2 * 32 + j) * kCompLd + c] = (T)(sx * (a - cc));  // p = +, q = -
-        comp[(3 * 32 + j) * kCompLd + c] = (T)(sx * (b - d));   // p = -, q = -
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int j = 16 * ta + kin_row<T>(kq, q);
+            const bool ok = j < my && i < mx;
+            const int jj[2] = {j, ny - 1 - j}, ii[2] = {i, nx - 1 - i};
+#pragma unroll
+            for (int m = 0; m < 4; m++) {
+                vp[q][m] = ca * xp[q][m] + cb * rp[q][m];
+                if (ok) X[base + (size_t)jj[m >> 1] * nx + ii[m & 1]] = vp[q][m];
+            }
+            const double a = vp[q][0] + vp[q][1], b = vp[q][0] - vp[q][1];
+            const double cc = vp[q][2] + vp[q][3], d = vp[q][2] - vp[q][3];
+            comp[(0 * 32 + j) * kCompLd + i] = (T)(a + cc);  // p = +, q = +
+            comp[(1 * 32 + j) * kCompLd + i] = (T)(b + d);   // p = -, q = +
+            comp[(2 * 32 + j) * kCompLd + i] = (T)(a - cc);  // p = +, q = -
+            comp[(3 * 32 + j) * kCompLd + i] = (T)(b - d);   // p = -, q = -
+        }
     }
     for (int t = threadIdx.x; t < 129; t += blockDim.x) {
         const int d = abs(t - 64);
@@ -360,17 +374,17 @@ __global__ __launch_bounds__(kWG) void k_kin_xy_half(const double *__restrict__ 
         const int j = 16 * ta + kin_row<T>(kq, q);
         if (j >= my || i >= mx) continue;
         const double d0 = (double)acc[0][q], d1 = (double)acc[1][q], d2 = (double)acc[2][q], d3 = (double)acc[3][q];
-        const double c0 = (double)comp[(0 * 32 + j) * kCompLd + i], c1 = (double)comp[(1 * 32 + j) * kCompLd + i];
-        const double c2 = (double)comp[(2 * 32 + j) * kCompLd + i], c3 = (double)comp[(3 * 32 + j) * kCompLd + i];
         // mirror point m = (y mirrored) * 2 + (x mirrored): signs p^mx q^my
         const double o[4] = {0.25 * ((d0 + d1) + (d2 + d3)), 0.25 * ((d0 - d1) + (d2 - d3)),
                              0.25 * ((d0 + d1) - (d2 + d3)), 0.25 * ((d0 - d1) - (d2 - d3))};
+        const double c0 = (double)comp[(0 * 32 + j) * kCompLd + i], c1 = (double)comp[(1 * 32 + j) * kCompLd + i];
+        const double c2 = (double)comp[(2 * 32 + j) * kCompLd + i], c3 = (double)comp[(3 * 32 + j) * kCompLd + i];
         const double v[4] = {0.25 * ((c0 + c1) + (c2 + c3)), 0.25 * ((c0 - c1) + (c2 - c3)),
                              0.25 * ((c0 + c1) - (c2 + c3)), 0.25 * ((c0 - c1) - (c2 - c3))};
         const int jj[2] = {j, ny - 1 - j}, ii[2] = {i, nx - 1 - i};
 #pragma unroll
         for (int m = 0; m < 4; m++)
-            R[base + (size_t)jj[m >> 1] * nx + ii[m & 1]] = sr * rp[q][m] + mn * (o[m] + rn * v[m]);
+            R[base + (size_t)jj[m >> 1] * nx + ii[m & 1]] = cr * rp[q][m] + mn * (o[m] + rn * v[m]);
     }
 }
 
@@ -387,7 +401,7 @@ __global__ __launch_bounds__(kWG) void k_kin_z_half(const double *__restrict__ X
     T *cd = cs + 32 * kSlabLd;             // [32][kSlabLd]: differences
     T *rz = cd + 32 * kSlabLd;             // [129]
     const int c0 = blockIdx.x * 64, n = blockIdx.y;
-    const double sx = sc->sx;
+    (void)sc;  // X already holds v
     const size_t base = (size_t)n * nz * nxy;
     const int mz = nz >> 1;
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
@@ -410,8 +424,8 @@ __global__ __launch_bounds__(kWG) void k_kin_z_half(const double *__restrict__ X
             v0 = X[base + (size_t)kk * nxy + c0 + cc];
             v1 = X[base + (size_t)(nz - 1 - kk) * nxy + c0 + cc];
         }
-        cs[kk * kSlabLd + cc] = (T)(sx * (v0 + v1));
-        cd[kk * kSlabLd + cc] = (T)(sx * (v0 - v1));
+        cs[kk * kSlabLd + cc] = (T)(v0 + v1);
+        cd[kk * kSlabLd + cc] = (T)(v0 - v1);
     }
     for (int t = threadIdx.x; t < 129; t += blockDim.x) {
         const int d = abs(t - 64);
@@ -448,7 +462,7 @@ __global__ __launch_bounds__(kWG) void k_kin_z_half(const double *__restrict__ X
 }
 
 template <typename T>
-static int launch_kinetic_t(const DevOperator &op, const double *X, double *R, Scal *sc, hipStream_t s)
+static int launch_kinetic_t(const DevOperator &op, double *X, double *R, Scal *sc, hipStream_t s)
 {
     // even axes use the half-size (centrosymmetric) contractions
     if (op.nx % 2 == 0 && op.ny % 2 == 0) {
@@ -488,13 +502,13 @@ static int launch_kinetic_t(const DevOperator &op, const double *X, double *R, S
 // B+1 loads per wave in flight.
 template <int B, typename VT>
 __device__ inline void vblock_point(const double *__restrict__ X, double *__restrict__ R, const VT *__restrict__ V,
-                                    int plane0, double mask, double sx, int64_t P, int p, double &dot)
+                                    int plane0, double mask, int64_t P, int p, double &dot, double &rr2, double &vv2)
 {
     double x[B], acc[B];
     const double *xp = X + (size_t)plane0 * P + p;
 #pragma unroll
     for (int r = 0; r < B; r++) {
-        x[r] = sx * *xp;
+        x[r] = *xp;
         xp += P;
         acc[r] = 0.0;
     }
@@ -541,6 +555,8 @@ __device__ inline void vblock_point(const double *__restrict__ X, double *__rest
         const double nr = rr[r] + mask * acc[r];
         rp[(size_t)r * P] = nr;
         dot += x[r] * nr;
+        rr2 += nr * nr;
+        vv2 += x[r] * x[r];
     }
 }
 
@@ -551,105 +567,82 @@ template <int B0, int B1, typename VT, int W>
 __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(W, 8))) void k_vblock(const double *__restrict__ X, double *__restrict__ R,
                                                 const VT *__restrict__ V0, const VT *__restrict__ V1, double mask0,
                                                 double mask1, const Scal *__restrict__ sc, int64_t P,
-                                                double *__restrict__ partial)
+                                                double *__restrict__ partial, double *__restrict__ partial_b,
+                                                double *__restrict__ partial_c)
 {
     __shared__ double red[kWG / 64];
-    const double sx = sc->sx;
-    double dot = 0.0;
+    (void)sc;  // X already holds v
+    double dot = 0.0, rr2 = 0.0, vv2 = 0.0;
     const int p = (int)(blockIdx.x * blockDim.x + threadIdx.x);
     if (p < P) {
         if (blockIdx.y == 0)
-            vblock_point<B0, VT>(X, R, V0, 0, mask0, sx, P, p, dot);
+            vblock_point<B0, VT>(X, R, V0, 0, mask0, P, p, dot, rr2, vv2);
         else if (B1 > 0)
-            vblock_point<(B1 > 0 ? B1 : 1), VT>(X, R, V1, B0, mask1, sx, P, p, dot);
+            vblock_point<(B1 > 0 ? B1 : 1), VT>(X, R, V1, B0, mask1, P, p, dot, rr2, vv2);
     }
-    const double s = block_sum(dot, red);
-    if (threadIdx.x == 0) partial[blockIdx.y * gridDim.x + blockIdx.x] = s;
-}
-
-// ---------------------------------------------------------------------------
-// recurrence update
-// ---------------------------------------------------------------------------
-// alpha = (sx X).R is the fixed-order sum of the potential kernel's partials,
-// recomputed identically by every workgroup (na > 0), or taken from
-// sc->alpha when k_finalize has summed a long partial list before (na == 0);
-// then R -= alpha (sx X) and the partial sums of |R|^2 go to partial_b for
-// k_finalize.
-__global__ __launch_bounds__(kWG) void k_update(const double *__restrict__ X, double *__restrict__ R,
-                                                Scal *__restrict__ sc, int64_t n,
-                                                const double *__restrict__ partial_a, unsigned int na,
-                                                double *__restrict__ partial_b, double *alpha_out)
-{
-    __shared__ double red[kWG / 64];
-    const double a = na ? total_of_partials(partial_a, na, red) : sc->alpha;
-    const double sx = sc->sx;
-    double s = 0.0;
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    for (; i + 3 * stride < n; i += 4 * stride) {
-        double xv[4], rv[4];
-#pragma unroll
-        for (int e = 0; e < 4; e++) {
-            xv[e] = X[i + e * stride];
-            rv[e] = R[i + e * stride];
-        }
-#pragma unroll
-        for (int e = 0; e < 4; e++) {
-            const double r = rv[e] - a * (sx * xv[e]);
-            R[i + e * stride] = r;
-            s += r * r;
-        }
-    }
-    for (; i < n; i += stride) {
-        const double r = R[i] - a * (sx * X[i]);
-        R[i] = r;
-        s += r * r;
-    }
-    const double b = block_sum(s, red);
+    const int w = blockIdx.y * gridDim.x + blockIdx.x;
+    const double s0 = block_sum(dot, red);
+    __syncthreads();
+    const double s1 = block_sum(rr2, red);
+    __syncthreads();
+    const double s2 = block_sum(vv2, red);
     if (threadIdx.x == 0) {
-        partial_b[blockIdx.x] = b;
-        if (blockIdx.x == 0) {
-            sc->alpha = a;
-            if (alpha_out) alpha_out[sc->step] = a;
-        }
+        partial[w] = s0;
+        partial_b[w] = s1;
+        partial_c[w] = s2;
     }
 }
 
-// mode 0: beta^2 = sum of partials; set beta and the scalars of the swap
-//         (X' = R / beta, R' = -beta (sx X)).
-// mode 1: start vector norm^2; sx = 1/norm.
-// mode 2: alpha = sum of partials.
-__global__ __launch_bounds__(kWG) void k_finalize(const double *__restrict__ partial, unsigned int n,
-                                                  Scal *__restrict__ sc, int mode, double *beta_out,
-                                                  double *beta2_out)
+// ---------------------------------------------------------------------------
+// recurrence scalars
+// ---------------------------------------------------------------------------
+// mode 0 (end of a step): alpha = v.R, beta^2 = |R|^2 - 2 alpha v.R +
+//   alpha^2 |v|^2 = |R - alpha v|^2 from the potential kernel's partials
+//   (fixed order); outputs at the step counter; the next step forms
+//   v' = (R - alpha v) / beta and R' = -beta v.
+// mode 1 (start vector): |X|^2 in `pa`; v_1 = X / |X|, R' = 0.
+__global__ __launch_bounds__(kWG) void k_finalize(const double *__restrict__ pa, const double *__restrict__ pb,
+                                                  const double *__restrict__ pc, unsigned int n,
+                                                  Scal *__restrict__ sc, int mode, double *alpha_out,
+                                                  double *beta_out, double *beta2_out)
 {
     __shared__ double red[kWG / 64];
-    const double t = total_of_partials(partial, n, red);
-    if (threadIdx.x != 0) return;
-    if (mode == 0) {
-        const double beta = sqrt(t);
-        const double sx = sc->sx;
-        sc->beta2 = t;
-        sc->beta = beta;
-        const long long j = sc->step;
-        if (beta_out) beta_out[j + 1] = beta;
-        if (beta2_out) beta2_out[j + 1] = t;
-        sc->step = j + 1;
-        sc->sx = 1. / beta;
-        sc->sr = (-beta) * sx;
-    } else if (mode == 1) {
-        sc->sx = 1.0 / sqrt(t);
-        sc->sr = 0.0;
-        sc->step = 0;
-    } else {
-        sc->alpha = t;
+    const double t = total_of_partials(pa, n, red);
+    if (mode == 1) {
+        if (threadIdx.x == 0) {
+            sc->ca = 1.0 / sqrt(t);
+            sc->cb = 0.0;
+            sc->cr = 0.0;
+            sc->step = 0;
+        }
+        return;
     }
+    __syncthreads();
+    const double rr = total_of_partials(pb, n, red);
+    __syncthreads();
+    const double vv = total_of_partials(pc, n, red);
+    if (threadIdx.x != 0) return;
+    const double alpha = t;
+    const double b2 = rr - 2.0 * alpha * t + alpha * alpha * vv;
+    const double beta = sqrt(b2);
+    const long long j = sc->step;
+    if (alpha_out) alpha_out[j] = alpha;
+    if (beta_out) beta_out[j + 1] = beta;
+    if (beta2_out) beta2_out[j + 1] = b2;
+    sc->alpha = alpha;
+    sc->beta = beta;
+    sc->beta2 = b2;
+    sc->step = j + 1;
+    sc->ca = 1.0 / beta;
+    sc->cb = -alpha / beta;
+    sc->cr = -beta;
 }
 
-__global__ void k_set_scal(Scal *sc, double sx, double sr)
+__global__ void k_set_scal(Scal *sc, double ca, double cr)
 {
-    sc->sx = sx;
-    sc->sr = sr;
+    sc->ca = ca;
+    sc->cb = 0.0;
+    sc->cr = cr;
     sc->alpha = 0.0;
     sc->beta = 0.0;
     sc->beta2 = 0.0;
@@ -693,12 +686,12 @@ __global__ __launch_bounds__(kWG) void k_start_random(double *X, int64_t n, int6
 }
 
 // ritz[g][i] += coef[g] * (sx X[i]) for the vectors still accumulating
-__global__ void k_ritz_accum(double *ritz, const double *__restrict__ X, const Scal *__restrict__ sc,
-                             const double *__restrict__ coef, int64_t n, int ngood)
+__global__ void k_ritz_accum(double *ritz, const double *__restrict__ X, const double *__restrict__ R,
+                             const Scal *__restrict__ sc, const double *__restrict__ coef, int64_t n, int ngood)
 {
-    const double sx = sc->sx;
+    const double ca = sc->ca, cb = sc->cb;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        const double v = sx * X[i];
+        const double v = ca * X[i] + cb * R[i];
         for (int g = 0; g < ngood; g++) {
             const double c = coef[g];
             if (c != 0.0) ritz[(size_t)g * n + i] += c * v;
@@ -733,15 +726,15 @@ static int launch_vblock_t(const DevOperator &op, const double *X, double *R, Sc
         switch (op.vb_variant) {                                                                       \
         case 0:                                                                                        \
             hipLaunchKernelGGL((k_vblock<A, Bb, VT, 1>), grid, dim3(kWG), 0, s, X, R, V0, V1, mk0, mk1, sc, \
-                               op.P, op.partial);                                                      \
+                               op.P, op.partial, op.partial_b, op.partial_c);                          \
             break;                                                                                     \
         case 1:                                                                                        \
             hipLaunchKernelGGL((k_vblock<A, Bb, VT, 2>), grid, dim3(kWG), 0, s, X, R, V0, V1, mk0, mk1, sc, \
-                               op.P, op.partial);                                                      \
+                               op.P, op.partial, op.partial_b, op.partial_c);                          \
             break;                                                                                     \
         default:                                                                                       \
             hipLaunchKernelGGL((k_vblock<A, Bb, VT, 3>), grid, dim3(kWG), 0, s, X, R, V0, V1, mk0, mk1, sc, \
-                               op.P, op.partial);                                                      \
+                               op.P, op.partial, op.partial_b, op.partial_c);                          \
             break;                                                                                     \
         }                                                                                              \
         return hipGetLastError() == hipSuccess ? BSL_OK : fail(BSL_E_HIP, "k_vblock launch");           \
@@ -752,7 +745,7 @@ static int launch_vblock_t(const DevOperator &op, const double *X, double *R, Sc
     return fail(BSL_E_ARG, "unsupported potential block sizes " + std::to_string(b0) + "/" + std::to_string(b1));
 }
 
-int launch_hv(const DevOperator &op, const double *X, double *R, Scal *sc, hipStream_t s, hipEvent_t *ev)
+int launch_hv(const DevOperator &op, double *X, double *R, Scal *sc, hipStream_t s, hipEvent_t *ev)
 {
     if (ev) BSL_HIP(hipEventRecord(ev[0], s));
     if (op.nx > 64 || op.ny > 64 || op.nz > 64) return fail(BSL_E_ARG, "grid axes above 64 points are not supported");
@@ -775,30 +768,18 @@ int launch_potential(const DevOperator &op, const double *X, double *R, Scal *sc
                                        : launch_vblock_t<float>(op, X, R, sc, s);
 }
 
-int launch_update(const DevOperator &op, const double *X, double *R, Scal *sc, double *alpha_out, double *beta_out,
-                  double *beta2_out, hipStream_t s)
+int launch_finalize(const DevOperator &op, Scal *sc, double *alpha_out, double *beta_out, double *beta2_out,
+                    hipStream_t s)
 {
-    const int64_t n = op.P * op.nk;
-    int g = grid_for(n, 2048);
-    if (g > op.npartial) g = op.npartial;
-    // a long list of alpha partials (the quadrature kernel's) is summed once
-    // instead of by every workgroup of k_update
-    unsigned na = (unsigned)op.nalpha;
-    if (na > 4096) {
-        hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kWG), 0, s, op.partial, na, sc, 2, nullptr, nullptr);
-        BSL_HIP(hipGetLastError());
-        na = 0;
-    }
-    hipLaunchKernelGGL(k_update, dim3(g), dim3(kWG), 0, s, X, R, sc, n, op.partial, na, op.partial_b, alpha_out);
-    BSL_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kWG), 0, s, op.partial_b, (unsigned)g, sc, 0, beta_out, beta2_out);
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kWG), 0, s, op.partial, op.partial_b, op.partial_c,
+                       (unsigned)op.nalpha, sc, 0, alpha_out, beta_out, beta2_out);
     BSL_HIP(hipGetLastError());
     return BSL_OK;
 }
 
-int launch_init_scal(Scal *sc, double sx, double sr, hipStream_t s)
+int launch_init_scal(Scal *sc, double ca, double cr, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_set_scal, dim3(1), dim3(1), 0, s, sc, sx, sr);
+    hipLaunchKernelGGL(k_set_scal, dim3(1), dim3(1), 0, s, sc, ca, cr);
     BSL_HIP(hipGetLastError());
     return BSL_OK;
 }
@@ -823,15 +804,16 @@ int launch_start_vector(double *X, int64_t n, int64_t P, const int *plane_ref, i
     if (g > npartial) g = npartial;
     hipLaunchKernelGGL(k_start_random, dim3(g), dim3(kWG), 0, s, X, n, P, plane_ref, seed, partial);
     BSL_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kWG), 0, s, partial, (unsigned)g, sc, 1, nullptr, nullptr);
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kWG), 0, s, partial, partial, partial, (unsigned)g, sc, 1, nullptr,
+                       nullptr, nullptr);
     BSL_HIP(hipGetLastError());
     return BSL_OK;
 }
 
-int launch_ritz_accum(double *ritz, const double *X, const Scal *sc, const double *coef, int64_t n, int ngood,
-                      hipStream_t s)
+int launch_ritz_accum(double *ritz, const double *X, const double *R, const Scal *sc, const double *coef, int64_t n,
+                      int ngood, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_ritz_accum, dim3(grid_for(n)), dim3(kWG), 0, s, ritz, X, sc, coef, n, ngood);
+    hipLaunchKernelGGL(k_ritz_accum, dim3(grid_for(n)), dim3(kWG), 0, s, ritz, X, R, sc, coef, n, ngood);
     BSL_HIP(hipGetLastError());
     return BSL_OK;
 }

@@ -180,7 +180,8 @@ __global__ __launch_bounds__(kQWG) void k_quad_mfma(const double *__restrict__ X
                                                     const double *__restrict__ Sg, const int *__restrict__ mplane,
                                                     const int *__restrict__ qrl, const double *__restrict__ mask,
                                                     const Scal *__restrict__ sc, int64_t P, int nk, int na,
-                                                    int nphi, int nm, int ppw, double *__restrict__ partial)
+                                                    int nphi, int nm, int ppw, double *__restrict__ partial,
+                                                    double *__restrict__ partial_b, double *__restrict__ partial_c)
 {
     typedef typename QuadTypes<MODE>::T T;
     typedef typename QuadTypes<MODE>::acc acc_t;
@@ -194,7 +195,7 @@ __global__ __launch_bounds__(kQWG) void k_quad_mfma(const double *__restrict__ X
     double *us = (double *)(smem + q.off_u);
     T *Ls = (T *)(smem + q.off_l);
     const int64_t p0 = (int64_t)blockIdx.x * npts;
-    const double sx = sc->sx;
+    (void)sc;  // X already holds v (formed by the kinetic kernel)
 
     // ---- tables and inputs ----
     for (int t = threadIdx.x; t < nk * na; t += blockDim.x) Ls[t] = (T)Lg[t];
@@ -220,7 +221,7 @@ __global__ __launch_bounds__(kQWG) void k_quad_mfma(const double *__restrict__ X
     }
     for (int t = threadIdx.x; t < nk * npts; t += blockDim.x) {
         const int r = t / npts, p = t - r * npts;
-        cs[p * nk + r] = p0 + p < P ? sx * X[(size_t)r * P + p0 + p] : 0.0;
+        cs[p * nk + r] = p0 + p < P ? X[(size_t)r * P + p0 + p] : 0.0;
     }
     __syncthreads();
 
@@ -379,18 +380,27 @@ __global__ __launch_bounds__(kQWG) void k_quad_mfma(const double *__restrict__ X
     if (lane < nk && pcur >= 0) us[pcur * nk + lane] = uacc;
     __syncthreads();
 
-    // ---- R update and the alpha partial ----
-    double dot = 0.0;
+    // ---- R update and the partials of v.R, |R|^2, |v|^2 ----
+    double dot = 0.0, rr2 = 0.0, vv2 = 0.0;
     for (int t = threadIdx.x; t < nk * npts; t += blockDim.x) {
         const int r = t / npts, p = t - r * npts;
         if (p0 + p >= P) continue;
         const size_t o = (size_t)r * P + p0 + p;
         const double nr = R[o] + mask[r] * us[p * nk + r];
         R[o] = nr;
-        dot += cs[p * nk + r] * nr;
+        const double v = cs[p * nk + r];
+        dot += v * nr;
+        rr2 += nr * nr;
+        vv2 += v * v;
     }
-    const double s = q_block_sum(dot, red);
-    if (threadIdx.x == 0) partial[blockIdx.x] = s;
+    const double s0 = q_block_sum(dot, red);
+    const double s1 = q_block_sum(rr2, red);
+    const double s2 = q_block_sum(vv2, red);
+    if (threadIdx.x == 0) {
+        partial[blockIdx.x] = s0;
+        partial_b[blockIdx.x] = s1;
+        partial_c[blockIdx.x] = s2;
+    }
 }
 
 // W[tile][t][lane][i] = w_a w_b vg[p][half*na*nb + a*nb + b] for column
@@ -467,7 +477,8 @@ int launch_weight_grid(const double *vg, void *W, int wprec, const double *wt, c
 
 #define BSL_QUAD_LAUNCH(M, NTV, KSV)                                                                       \
     hipLaunchKernelGGL((k_quad_mfma<M, NTV, KSV>), dim3(grid), dim3(kQWG), lds, s, X, R, op.wgrid, op.qL, op.qS, \
-                       op.mplane, op.q_rl, op.mask, sc, op.P, op.nk, op.na, op.nphi, op.nm, ppw, op.partial)
+                       op.mplane, op.q_rl, op.mask, sc, op.P, op.nk, op.na, op.nphi, op.nm, ppw, op.partial,  \
+                       op.partial_b, op.partial_c)
 
 template <int M, int NTV>
 static void launch_quad_ks(const DevOperator &op, const double *X, double *R, Scal *sc, unsigned grid, size_t lds,

@@ -232,13 +232,20 @@ def test_fp32_levels_within_target():
 
 def test_reduced_basis_drops_spurious_zero():
     """para (Even) block: the reduced basis reproduces the masked full-basis
-    levels without the spurious 0 of the masked subspace."""
+    levels without the spurious 0 of the masked subspace.
+
+    The masked full-basis operator is not symmetric (masked rows are zero,
+    their columns are not), so where its 0 lands is set by rounding alone:
+    the same 200-step recurrence on this case puts it at 1e-7 (the oracle),
+    2e-7, 1e-6 or 1.5e-5 depending only on how alpha and beta are summed.
+    The check is the dichotomy: a level within 1e-3 of 0 in the full basis,
+    none in the reduced basis."""
     spec, kw = cs.cage_case(6, 2, 6, 6, "Even")
     full = B.Calculator(**kw)
     red = B.Calculator(basis="reduced", **kw)
     ef, rf = B.lancbis(*full.lanczos(200), -10.0, 10.0)
     er, rr = B.lancbis(*red.lanczos(200), -10.0, 10.0)
-    assert np.min(np.abs(ef)) < 1e-5          # the masked subspace's zero
+    assert np.min(np.abs(ef)) < 1e-3          # the masked subspace's zero
     assert np.min(np.abs(er)) > 1e-3
     conv = _converged(ef, rf)
     conv = conv[np.abs(conv) > 1e-6]

@@ -93,8 +93,9 @@ int build_vpk(bsl_ctx *c, DevOperator &op, int vprec, const double *d_h)
 {
     op.vprec = vprec;
     const size_t es = vprec == BSL_PREC_FP64 ? sizeof(double) : sizeof(float);
-    BSL_HIP(hipMalloc(&op.vpk, es * (size_t)op.npair * (size_t)c->P));
-    int rc = launch_vblock_gemm(c->d_vgrid, d_h, op.vpk, vprec, c->P, c->Q, op.npair, c->stream);
+    BSL_HIP(hipMalloc(&op.vpk, es * (size_t)op.npair * (size_t)(vpk_ntile(c->P) * kVTile)));
+    const int64_t npair0 = op.nblk > 1 ? op.blk[1].pair_offset : op.npair;
+    int rc = launch_vblock_gemm(c->d_vgrid, d_h, op.vpk, vprec, c->P, c->Q, npair0, op.npair, c->stream);
     if (rc) return rc;
     return BSL_OK;
 }
@@ -800,7 +801,9 @@ extern "C" int bsl_get_vblocks(bsl_ctx *c, double *out)
     if (c->p.basis != BSL_BASIS_FULL) return fail(BSL_E_STATE, "V blocks are reported for FULL-basis contexts");
     BSL_HIP(hipSetDevice(c->device));
     const DevOperator &op = c->op64;
-    std::vector<double> vpk((size_t)op.npair * c->P);
+    const int64_t ntile = vpk_ntile(c->P);
+    const int64_t npair0 = op.nblk > 1 ? op.blk[1].pair_offset : op.npair;
+    std::vector<double> vpk((size_t)op.npair * ntile * kVTile);
     BSL_HIP(hipMemcpy(vpk.data(), op.vpk, sizeof(double) * vpk.size(), hipMemcpyDeviceToHost));
     const int nn = c->nn, nx = op.nx, ny = op.ny, nz = op.nz;
     std::memset(out, 0, sizeof(double) * (size_t)c->P * nn * nn);
@@ -814,7 +817,7 @@ extern "C" int bsl_get_vblocks(bsl_ctx *c, double *out)
                     for (int i = 0; i < nx; i++) {
                         const int64_t p = ((int64_t)k * ny + j) * nx + i;
                         const int64_t ref = (((int64_t)i * ny + j) * nz + k);
-                        const double v = vpk[(size_t)pr * c->P + p];
+                        const double v = vpk[vpk_index(pr, p, npair0, op.npair, ntile)];
                         out[(ref * nn + n) * nn + np] = v;
                         out[(ref * nn + np) * nn + n] = v;
                     }

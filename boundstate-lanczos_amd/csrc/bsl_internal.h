@@ -84,6 +84,21 @@ inline std::vector<std::pair<int, int>> block_pair_order(int B)
     return out;
 }
 
+// Packed V layout.  The grid points are cut into tiles of kVTile points (the
+// potential kernel's workgroup, one point per thread); the pairs of parity
+// block b are stored tile by tile, [tile][pair of b][kVTile], block 0 first,
+// so that a workgroup streams one contiguous region instead of one stripe
+// per pair.  npair0 = pairs of block 0, ntile = ceil(P / kVTile); the last
+// tile's points beyond P are padding and never read.
+constexpr int kVTile = 256;
+__host__ __device__ inline int64_t vpk_ntile(int64_t P) { return (P + kVTile - 1) / kVTile; }
+__host__ __device__ inline size_t vpk_index(int64_t pair, int64_t p, int64_t npair0, int64_t npair, int64_t ntile)
+{
+    const bool b1 = pair >= npair0;
+    const int64_t off = b1 ? npair0 : 0, nb = b1 ? npair - npair0 : npair0;
+    return (size_t)(off * ntile * kVTile + ((p / kVTile) * nb + (pair - off)) * kVTile + p % kVTile);
+}
+
 // a parity block of the potential: `size` consecutive carried planes from `plane0`
 struct Block {
     int size = 0;
@@ -100,7 +115,7 @@ struct DevOperator {
     int kprec = 0;                // BSL_PREC_FP64 or BSL_PREC_FP32: kinetic contractions
     double *tx = nullptr, *ty = nullptr, *tz = nullptr;  // Toeplitz rows
     double *rot = nullptr, *mask = nullptr;              // per carried plane
-    void *vpk = nullptr;          // packed V blocks [pair][P] (double or float)
+    void *vpk = nullptr;          // packed V blocks in the tiled layout of vpk_index (double or float)
     int nblk = 0;
     Block blk[2];
     int *plane_ref = nullptr;     // reference lm index of each carried plane
@@ -170,8 +185,8 @@ int launch_vgrid_mapped(double *vg, int64_t P, int Q, const double *gx, const do
 int launch_vgrid_coulomb(double *vg, int64_t P, int Q, const double *gx, const double *gy, const double *gz, int nx,
                          int ny, double q1, double q2, double v_ceil, double ceiling, hipStream_t s);
 // Vpk[pair][P] = sum_q Vg[P][q] * H[q][pair]
-int launch_vblock_gemm(const double *vg, const double *h, void *vpk, int vprec, int64_t P, int Q, int64_t npair,
-                       hipStream_t s);
+int launch_vblock_gemm(const double *vg, const double *h, void *vpk, int vprec, int64_t P, int Q, int64_t npair0,
+                       int64_t npair, hipStream_t s);
 // quadrature calculator (bsl_quad.hip)
 // weighted quadrature grid in the MFMA accumulator layout of the quadrature
 // kernel (npts grid points per workgroup)

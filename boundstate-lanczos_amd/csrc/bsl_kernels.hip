@@ -391,7 +391,8 @@ __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
 // z axis of 64 columns with the z parity split: s = v(k) + v(k'), d = v(k) - v(k'),
 // k < nz/2; wave w owns columns 16w.. of both components, every k tile.
 template <typename T>
-__global__ __launch_bounds__(kWG) void k_kin_z_half(const double *__restrict__ X, double *__restrict__ R,
+__global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_kin_z_half(
+    const double *__restrict__ X, double *__restrict__ R,
                                                     const double *__restrict__ tz, const double *__restrict__ mask,
                                                     const Scal *__restrict__ sc, int nxy, int nz)
 {
@@ -417,15 +418,24 @@ __global__ __launch_bounds__(kWG) void k_kin_z_half(const double *__restrict__ X
             rp[a][q][0] = ok ? R[base + (size_t)kk * nxy + c] : 0.0;
             rp[a][q][1] = ok ? R[base + (size_t)(nz - 1 - kk) * nxy + c] : 0.0;
         }
-    for (int t = threadIdx.x; t < 32 * 64; t += blockDim.x) {
-        const int kk = t >> 6, cc = t & 63;
-        double v0 = 0, v1 = 0;
-        if (kk < mz && c0 + cc < nxy) {
-            v0 = X[base + (size_t)kk * nxy + c0 + cc];
-            v1 = X[base + (size_t)(nz - 1 - kk) * nxy + c0 + cc];
+    {
+        // the 32 x 64 mirror pairs of the tile, all loads issued before the
+        // LDS stores (8 pairs per thread at 256 threads)
+        static_assert(32 * 64 == 8 * kWG, "z tile staging assumes 256 threads");
+        double v0[8], v1[8];
+#pragma unroll
+        for (int e = 0; e < 8; e++) {
+            const int t = threadIdx.x + e * kWG, kk = t >> 6, cc = t & 63;
+            const bool ok = kk < mz && c0 + cc < nxy;
+            v0[e] = ok ? X[base + (size_t)kk * nxy + c0 + cc] : 0.0;
+            v1[e] = ok ? X[base + (size_t)(nz - 1 - kk) * nxy + c0 + cc] : 0.0;
         }
-        cs[kk * kSlabLd + cc] = (T)(v0 + v1);
-        cd[kk * kSlabLd + cc] = (T)(v0 - v1);
+#pragma unroll
+        for (int e = 0; e < 8; e++) {
+            const int t = threadIdx.x + e * kWG, kk = t >> 6, cc = t & 63;
+            cs[kk * kSlabLd + cc] = (T)(v0[e] + v1[e]);
+            cd[kk * kSlabLd + cc] = (T)(v0[e] - v1[e]);
+        }
     }
     for (int t = threadIdx.x; t < 129; t += blockDim.x) {
         const int d = abs(t - 64);
@@ -492,8 +502,8 @@ static int launch_kinetic_t(const DevOperator &op, double *X, double *R, Scal *s
 // ---------------------------------------------------------------------------
 // potential term: one thread per (grid point, parity block)
 // ---------------------------------------------------------------------------
-// R[n_r][P] += mask * sum_c V_P[r][c] (sx X[n_c][P]) over the packed upper
-// triangle; then the contribution of these planes to alpha = (sx X).R.
+// R[n_r][P] += mask * sum_c V_P[r][c] X[n_c][P] over the packed upper
+// triangle; then the contributions of these planes to v.R, |R|^2, |v|^2.
 // One thread per grid point.  The packed upper triangle of a B x B block is
 // stored row pair by row pair: row r (columns r..B-1) followed by row
 // B-1-r (columns B-1-r..B-1), for r = 0 .. (B-1)/2, so that each row pair
@@ -512,7 +522,8 @@ __device__ inline void vblock_point(const double *__restrict__ X, double *__rest
         xp += P;
         acc[r] = 0.0;
     }
-    const VT *vp = V + p;
+    // this point's tile of the block: B(B+1)/2 pairs of kVTile points
+    const VT *vp = V + (size_t)(p / kVTile) * (B * (B + 1) / 2) * kVTile + p % kVTile;
 #pragma unroll
     for (int r = 0; r <= (B - 1) / 2; r++) {
         const int r2 = B - 1 - r;
@@ -521,13 +532,13 @@ __device__ inline void vblock_point(const double *__restrict__ X, double *__rest
 #pragma unroll
         for (int c = r; c < B; c++) {
             vv[k++] = *vp;
-            vp += P;
+            vp += kVTile;
         }
         if (r2 != r) {
 #pragma unroll
             for (int c = r2; c < B; c++) {
                 vv[k++] = *vp;
-                vp += P;
+                vp += kVTile;
             }
         }
         k = 0;
@@ -574,8 +585,11 @@ __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(W, 8))) voi
     (void)sc;  // X already holds v
     double dot = 0.0, rr2 = 0.0, vv2 = 0.0;
     const int p = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    // the larger block's workgroups are dispatched first (blockIdx.y = 0), so
+    // that the shorter ones fill the tail of the launch
+    const int blk = (B1 > B0) ? (int)(gridDim.y - 1 - blockIdx.y) : (int)blockIdx.y;
     if (p < P) {
-        if (blockIdx.y == 0)
+        if (blk == 0)
             vblock_point<B0, VT>(X, R, V0, 0, mask0, P, p, dot, rr2, vv2);
         else if (B1 > 0)
             vblock_point<(B1 > 0 ? B1 : 1), VT>(X, R, V1, B0, mask1, P, p, dot, rr2, vv2);
@@ -714,8 +728,10 @@ template <typename VT>
 static int launch_vblock_t(const DevOperator &op, const double *X, double *R, Scal *sc, hipStream_t s)
 {
     const VT *V = (const VT *)op.vpk;
-    const VT *V0 = V + op.blk[0].pair_offset * op.P;
-    const VT *V1 = op.nblk > 1 ? V + op.blk[1].pair_offset * op.P : V0;
+    static_assert(kVTile == kWG, "one potential-kernel workgroup per V tile");
+    const int64_t ptile = vpk_ntile(op.P) * kVTile;
+    const VT *V0 = V + op.blk[0].pair_offset * ptile;
+    const VT *V1 = op.nblk > 1 ? V + op.blk[1].pair_offset * ptile : V0;
     const double mk0 = op.blk[0].mask, mk1 = op.nblk > 1 ? op.blk[1].mask : 0.0;
     dim3 grid((unsigned)((op.P + kWG - 1) / kWG), (unsigned)op.nblk);
     if ((int64_t)grid.x * grid.y > op.npartial || (int64_t)grid.x * grid.y != op.nalpha)
@@ -1094,8 +1110,10 @@ int launch_vgrid_coulomb(double *vg, int64_t P, int Q, const double *gx, const d
 // in LDS, a 4x4 register tile per thread.
 template <typename VT>
 __global__ __launch_bounds__(kWG) void k_vblock_gemm(const double *__restrict__ vg, const double *__restrict__ h,
-                                                     VT *__restrict__ vpk, int64_t P, int Q, int64_t npair)
+                                                     VT *__restrict__ vpk, int64_t P, int Q, int64_t npair0,
+                                                     int64_t npair)
 {
+    const int64_t ntile = vpk_ntile(P);
     __shared__ double As[16][64 + 1];  // [kk][point]
     __shared__ double Bs[16][64 + 1];  // [kk][pair]
     const int tm = threadIdx.x & 15, tn = threadIdx.x >> 4;
@@ -1135,19 +1153,19 @@ __global__ __launch_bounds__(kWG) void k_vblock_gemm(const double *__restrict__ 
 #pragma unroll
         for (int r = 0; r < 4; r++) {
             const int64_t p = p0 + tm * 4 + r;
-            if (p < P) vpk[(size_t)pair * P + p] = (VT)acc[r][c];
+            if (p < P) vpk[vpk_index(pair, p, npair0, npair, ntile)] = (VT)acc[r][c];
         }
     }
 }
 
-int launch_vblock_gemm(const double *vg, const double *h, void *vpk, int vprec, int64_t P, int Q, int64_t npair,
-                       hipStream_t s)
+int launch_vblock_gemm(const double *vg, const double *h, void *vpk, int vprec, int64_t P, int Q, int64_t npair0,
+                       int64_t npair, hipStream_t s)
 {
     dim3 grid((unsigned)((P + 63) / 64), (unsigned)((npair + 63) / 64));
     if (vprec == BSL_PREC_FP64)
-        hipLaunchKernelGGL(k_vblock_gemm<double>, grid, dim3(kWG), 0, s, vg, h, (double *)vpk, P, Q, npair);
+        hipLaunchKernelGGL(k_vblock_gemm<double>, grid, dim3(kWG), 0, s, vg, h, (double *)vpk, P, Q, npair0, npair);
     else
-        hipLaunchKernelGGL(k_vblock_gemm<float>, grid, dim3(kWG), 0, s, vg, h, (float *)vpk, P, Q, npair);
+        hipLaunchKernelGGL(k_vblock_gemm<float>, grid, dim3(kWG), 0, s, vg, h, (float *)vpk, P, Q, npair0, npair);
     BSL_HIP(hipGetLastError());
     return BSL_OK;
 }

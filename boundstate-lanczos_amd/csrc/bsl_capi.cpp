@@ -580,12 +580,9 @@ int start_lanczos(bsl_ctx *c, const double *v0, int start, uint64_t seed)
 // one Lanczos step j (1-based) on the current X/R; swaps the buffers
 int lanczos_step(bsl_ctx *c, const DevOperator &op, int j, hipEvent_t *ev)
 {
-    int rc = launch_hv(op, c->d_X, c->d_R, c->d_scal, c->stream, ev);
-    if (rc) return rc;
     (void)j;  // the device step counter indexes the coefficient outputs
-    rc = launch_finalize(op, c->d_scal, c->d_alpha, c->d_beta, c->d_beta2, c->stream);
+    int rc = launch_step(op, c->d_X, c->d_R, c->d_scal, c->d_alpha, c->d_beta, c->d_beta2, c->stream, ev);
     if (rc) return rc;
-    if (ev) BSL_HIP(hipEventRecord(ev[3], c->stream));
     std::swap(c->d_X, c->d_R);
     return BSL_OK;
 }

@@ -144,6 +144,9 @@ struct DevOperator {
 // partial sums of v.R, |R|^2, |v|^2 in op.partial, partial_b, partial_c
 int launch_hv(const DevOperator &op, double *X, double *R, Scal *sc, hipStream_t s,
               hipEvent_t *ev /* 3 events or null */);
+// one Lanczos step: launch_hv, then launch_finalize (event 4 after it)
+int launch_step(const DevOperator &op, double *X, double *R, Scal *sc, double *alpha_out, double *beta_out,
+                double *beta2_out, hipStream_t s, hipEvent_t *ev /* 4 events or null */);
 // the potential kernel alone (used by the autotuner)
 int launch_potential(const DevOperator &op, const double *X, double *R, Scal *sc, hipStream_t s);
 // alpha = v.R, beta^2 = |R|^2 - 2 alpha v.R + alpha^2 |v|^2 from the

@@ -68,6 +68,53 @@ __device__ inline double total_of_partials(const double *__restrict__ partial, u
     return tot;
 }
 
+// End of a Lanczos step: alpha = v.R, beta^2 = |R|^2 - 2 alpha v.R +
+// alpha^2 |v|^2 = |R - alpha v|^2 from the potential kernel's three partial
+// lists (fixed order); outputs at the step counter; the next step forms
+// v' = (R - alpha v) / beta and R' = -beta v.  Called by the whole workgroup.
+__device__ inline void finalize_step(const double *__restrict__ pa, const double *__restrict__ pb,
+                                     const double *__restrict__ pc, unsigned int n, Scal *sc, double *alpha_out,
+                                     double *beta_out, double *beta2_out, double *red)
+{
+    // the three lists are read together (one memory round trip), each summed
+    // in the order of total_of_partials
+    double sa[4] = {0.0, 0.0, 0.0, 0.0}, sb[4] = {0.0, 0.0, 0.0, 0.0}, sv[4] = {0.0, 0.0, 0.0, 0.0};
+    const unsigned int st = blockDim.x;
+    unsigned int i = threadIdx.x;
+    for (; i + 3 * st < n; i += 4 * st)
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+            sa[e] += pa[i + e * st];
+            sb[e] += pb[i + e * st];
+            sv[e] += pc[i + e * st];
+        }
+    for (; i < n; i += st) {
+        sa[0] += pa[i];
+        sb[0] += pb[i];
+        sv[0] += pc[i];
+    }
+    const double t = block_sum((sa[0] + sa[1]) + (sa[2] + sa[3]), red);
+    __syncthreads();
+    const double rr = block_sum((sb[0] + sb[1]) + (sb[2] + sb[3]), red);
+    __syncthreads();
+    const double vv = block_sum((sv[0] + sv[1]) + (sv[2] + sv[3]), red);
+    if (threadIdx.x != 0) return;
+    const double alpha = t;
+    const double b2 = rr - 2.0 * alpha * t + alpha * alpha * vv;
+    const double beta = sqrt(b2);
+    const long long j = sc->step;
+    if (alpha_out) alpha_out[j] = alpha;
+    if (beta_out) beta_out[j + 1] = beta;
+    if (beta2_out) beta2_out[j + 1] = b2;
+    sc->alpha = alpha;
+    sc->beta = beta;
+    sc->beta2 = b2;
+    sc->step = j + 1;
+    sc->ca = 1.0 / beta;
+    sc->cb = -alpha / beta;
+    sc->cr = -beta;
+}
+
 // ---------------------------------------------------------------------------
 // kinetic terms on MFMA (v_mfma_f64_16x16x4f64): the contractions are small
 // GEMMs with the slab (or column block) staged in LDS, zero padded to 64,
@@ -575,14 +622,12 @@ __device__ inline void vblock_point(const double *__restrict__ X, double *__rest
 // [0, B0), block 1 planes [B0, B0 + B1).  The symmetry mask is uniform
 // within a parity block.
 template <int B0, int B1, typename VT, int W>
-__global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(W, 8))) void k_vblock(const double *__restrict__ X, double *__restrict__ R,
-                                                const VT *__restrict__ V0, const VT *__restrict__ V1, double mask0,
-                                                double mask1, const Scal *__restrict__ sc, int64_t P,
-                                                double *__restrict__ partial, double *__restrict__ partial_b,
-                                                double *__restrict__ partial_c)
+__global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(W, 8))) void k_vblock(
+    const double *__restrict__ X, double *__restrict__ R, const VT *__restrict__ V0, const VT *__restrict__ V1,
+    double mask0, double mask1, int64_t P, double *__restrict__ partial, double *__restrict__ partial_b,
+    double *__restrict__ partial_c)
 {
     __shared__ double red[kWG / 64];
-    (void)sc;  // X already holds v
     double dot = 0.0, rr2 = 0.0, vv2 = 0.0;
     const int p = (int)(blockIdx.x * blockDim.x + threadIdx.x);
     // the larger block's workgroups are dispatched first (blockIdx.y = 0), so
@@ -610,10 +655,7 @@ __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(W, 8))) voi
 // ---------------------------------------------------------------------------
 // recurrence scalars
 // ---------------------------------------------------------------------------
-// mode 0 (end of a step): alpha = v.R, beta^2 = |R|^2 - 2 alpha v.R +
-//   alpha^2 |v|^2 = |R - alpha v|^2 from the potential kernel's partials
-//   (fixed order); outputs at the step counter; the next step forms
-//   v' = (R - alpha v) / beta and R' = -beta v.
+// mode 0 (end of a step): finalize_step.
 // mode 1 (start vector): |X|^2 in `pa`; v_1 = X / |X|, R' = 0.
 __global__ __launch_bounds__(kWG) void k_finalize(const double *__restrict__ pa, const double *__restrict__ pb,
                                                   const double *__restrict__ pc, unsigned int n,
@@ -621,35 +663,17 @@ __global__ __launch_bounds__(kWG) void k_finalize(const double *__restrict__ pa,
                                                   double *beta_out, double *beta2_out)
 {
     __shared__ double red[kWG / 64];
-    const double t = total_of_partials(pa, n, red);
-    if (mode == 1) {
-        if (threadIdx.x == 0) {
-            sc->ca = 1.0 / sqrt(t);
-            sc->cb = 0.0;
-            sc->cr = 0.0;
-            sc->step = 0;
-        }
+    if (mode == 0) {
+        finalize_step(pa, pb, pc, n, sc, alpha_out, beta_out, beta2_out, red);
         return;
     }
-    __syncthreads();
-    const double rr = total_of_partials(pb, n, red);
-    __syncthreads();
-    const double vv = total_of_partials(pc, n, red);
-    if (threadIdx.x != 0) return;
-    const double alpha = t;
-    const double b2 = rr - 2.0 * alpha * t + alpha * alpha * vv;
-    const double beta = sqrt(b2);
-    const long long j = sc->step;
-    if (alpha_out) alpha_out[j] = alpha;
-    if (beta_out) beta_out[j + 1] = beta;
-    if (beta2_out) beta2_out[j + 1] = b2;
-    sc->alpha = alpha;
-    sc->beta = beta;
-    sc->beta2 = b2;
-    sc->step = j + 1;
-    sc->ca = 1.0 / beta;
-    sc->cb = -alpha / beta;
-    sc->cr = -beta;
+    const double t = total_of_partials(pa, n, red);
+    if (threadIdx.x == 0) {
+        sc->ca = 1.0 / sqrt(t);
+        sc->cb = 0.0;
+        sc->cr = 0.0;
+        sc->step = 0;
+    }
 }
 
 __global__ void k_set_scal(Scal *sc, double ca, double cr)
@@ -725,7 +749,7 @@ static int grid_for(int64_t n, int cap = 4096)
 }
 
 template <typename VT>
-static int launch_vblock_t(const DevOperator &op, const double *X, double *R, Scal *sc, hipStream_t s)
+static int launch_vblock_t(const DevOperator &op, const double *X, double *R, hipStream_t s)
 {
     const VT *V = (const VT *)op.vpk;
     static_assert(kVTile == kWG, "one potential-kernel workgroup per V tile");
@@ -741,15 +765,15 @@ static int launch_vblock_t(const DevOperator &op, const double *X, double *R, Sc
     if (b0 == A && b1 == Bb) {                                                                         \
         switch (op.vb_variant) {                                                                       \
         case 0:                                                                                        \
-            hipLaunchKernelGGL((k_vblock<A, Bb, VT, 1>), grid, dim3(kWG), 0, s, X, R, V0, V1, mk0, mk1, sc, \
+            hipLaunchKernelGGL((k_vblock<A, Bb, VT, 1>), grid, dim3(kWG), 0, s, X, R, V0, V1, mk0, mk1,     \
                                op.P, op.partial, op.partial_b, op.partial_c);                          \
             break;                                                                                     \
         case 1:                                                                                        \
-            hipLaunchKernelGGL((k_vblock<A, Bb, VT, 2>), grid, dim3(kWG), 0, s, X, R, V0, V1, mk0, mk1, sc, \
+            hipLaunchKernelGGL((k_vblock<A, Bb, VT, 2>), grid, dim3(kWG), 0, s, X, R, V0, V1, mk0, mk1,     \
                                op.P, op.partial, op.partial_b, op.partial_c);                          \
             break;                                                                                     \
         default:                                                                                       \
-            hipLaunchKernelGGL((k_vblock<A, Bb, VT, 3>), grid, dim3(kWG), 0, s, X, R, V0, V1, mk0, mk1, sc, \
+            hipLaunchKernelGGL((k_vblock<A, Bb, VT, 3>), grid, dim3(kWG), 0, s, X, R, V0, V1, mk0, mk1,     \
                                op.P, op.partial, op.partial_b, op.partial_c);                          \
             break;                                                                                     \
         }                                                                                              \
@@ -761,6 +785,13 @@ static int launch_vblock_t(const DevOperator &op, const double *X, double *R, Sc
     return fail(BSL_E_ARG, "unsupported potential block sizes " + std::to_string(b0) + "/" + std::to_string(b1));
 }
 
+int launch_potential(const DevOperator &op, const double *X, double *R, Scal *sc, hipStream_t s)
+{
+    return op.calc == BSL_CALC_QUAD        ? launch_quad_potential(op, X, R, sc, s)
+           : op.vprec == BSL_PREC_FP64 ? launch_vblock_t<double>(op, X, R, s)
+                                       : launch_vblock_t<float>(op, X, R, s);
+}
+
 int launch_hv(const DevOperator &op, double *X, double *R, Scal *sc, hipStream_t s, hipEvent_t *ev)
 {
     if (ev) BSL_HIP(hipEventRecord(ev[0], s));
@@ -769,19 +800,10 @@ int launch_hv(const DevOperator &op, double *X, double *R, Scal *sc, hipStream_t
                                         : launch_kinetic_t<float>(op, X, R, sc, s);
     if (krc) return krc;
     if (ev) BSL_HIP(hipEventRecord(ev[1], s));
-    int rc = op.calc == BSL_CALC_QUAD        ? launch_quad_potential(op, X, R, sc, s)
-             : op.vprec == BSL_PREC_FP64 ? launch_vblock_t<double>(op, X, R, sc, s)
-                                         : launch_vblock_t<float>(op, X, R, sc, s);
+    int rc = launch_potential(op, X, R, sc, s);
     if (rc) return rc;
     if (ev) BSL_HIP(hipEventRecord(ev[2], s));
     return BSL_OK;
-}
-
-int launch_potential(const DevOperator &op, const double *X, double *R, Scal *sc, hipStream_t s)
-{
-    return op.calc == BSL_CALC_QUAD        ? launch_quad_potential(op, X, R, sc, s)
-           : op.vprec == BSL_PREC_FP64 ? launch_vblock_t<double>(op, X, R, sc, s)
-                                       : launch_vblock_t<float>(op, X, R, sc, s);
 }
 
 int launch_finalize(const DevOperator &op, Scal *sc, double *alpha_out, double *beta_out, double *beta2_out,
@@ -790,6 +812,16 @@ int launch_finalize(const DevOperator &op, Scal *sc, double *alpha_out, double *
     hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kWG), 0, s, op.partial, op.partial_b, op.partial_c,
                        (unsigned)op.nalpha, sc, 0, alpha_out, beta_out, beta2_out);
     BSL_HIP(hipGetLastError());
+    return BSL_OK;
+}
+
+int launch_step(const DevOperator &op, double *X, double *R, Scal *sc, double *alpha_out, double *beta_out,
+                double *beta2_out, hipStream_t s, hipEvent_t *ev)
+{
+    int rc = launch_hv(op, X, R, sc, s, ev);
+    if (!rc) rc = launch_finalize(op, sc, alpha_out, beta_out, beta2_out, s);
+    if (rc) return rc;
+    if (ev) BSL_HIP(hipEventRecord(ev[3], s));
     return BSL_OK;
 }
 

@@ -578,13 +578,13 @@ __device__ inline void vblock_point(const double *__restrict__ X, double *__rest
         int k = 0;
 #pragma unroll
         for (int c = r; c < B; c++) {
-            vv[k++] = *vp;
+            vv[k++] = __builtin_nontemporal_load(vp);
             vp += kVTile;
         }
         if (r2 != r) {
 #pragma unroll
             for (int c = r2; c < B; c++) {
-                vv[k++] = *vp;
+                vv[k++] = __builtin_nontemporal_load(vp);
                 vp += kVTile;
             }
         }

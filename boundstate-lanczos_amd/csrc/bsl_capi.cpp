@@ -942,8 +942,8 @@ extern "C" int bsl_step_bytes(const bsl_ctx *c, double *out)
         pot_flops = 2.0 * (double)c->P * pair_fma;
     }
     out[0] = pot_bytes + 3.0 * vec;  // potential kernel: V + read X, read and write R
-    out[1] = 6.0 * vec;              // kinetic kernels (xy and z): read X, read and write R each
-    out[2] = 3.0 * vec;              // update kernel
+    out[1] = 7.0 * vec;              // kinetic kernels: x-y reads X, R and writes X, R; z reads X, R, writes R
+    out[2] = 0.0;                    // recurrence update: folded into the kinetic and potential kernels
     out[3] = pot_bytes + 6.0 * vec;  // fused-step minimum: V + the six vector passes of the recurrence
     out[4] = pot_flops + 2.0 * (double)(op.nx + op.ny + op.nz) * (double)c->n_kry + 4.0 * (double)c->n_kry;
     return BSL_OK;

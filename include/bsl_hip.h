@@ -172,14 +172,17 @@ int bsl_get_vgrid(bsl_ctx *ctx, double *v1, double *v2);
 /* Timing of the device recurrence: runs `steps` Lanczos steps from the
  * random start after `warmup` untimed ones.  ms[0] = total time of the timed
  * steps (HIP events on the context stream, no host synchronisation inside);
- * ms[1], ms[2], ms[3] = average per-step duration of the potential-block
- * kernel, the two kinetic kernels and the recurrence update kernel, from
- * events recorded around each of them in a further min(steps, 64) steps. */
+ * ms[1], ms[2], ms[3] = average per-step duration of the potential kernel,
+ * the two kinetic kernels and the step finalize (alpha, beta and the scalars
+ * of the next step), from events recorded around each of them in a further
+ * min(steps, 64) steps. */
 int bsl_time_steps(bsl_ctx *ctx, int32_t warmup, int32_t steps, double *ms);
 
 /* Bytes each kernel of one Lanczos step must move at minimum (algorithmic
- * bytes): out[0] potential term, out[1] kinetic term, out[2] recurrence
- * update, out[3] total; out[4] FLOPs of one Hv. */
+ * bytes): out[0] potential kernel (V + read X, read and write R), out[1]
+ * kinetic kernels (seven vector passes), out[2] recurrence update (0: it is
+ * folded into the kinetic and potential kernels), out[3] minimum of a step
+ * (V + six vector passes); out[4] FLOPs of one Hv. */
 int bsl_step_bytes(const bsl_ctx *ctx, double *out);
 
 // Hellmann-Feynman gradients: grad = sum_k g[k] <psi_k| dV/dtheta |psi_k>

@@ -463,7 +463,8 @@ extern "C" int bsl_create(const bsl_params *pp, bsl_ctx **out)
             qrl[(size_t)mpl[r] * 9 + 1 + cnt++] = r;
         }
         if ((rc = dev_upload(&op.q_rl, qrl))) return rc;
-        op.nalpha = (c->P + op.q_npts - 1) / op.q_npts;
+        if ((rc = quad_grid(op, c->device, &op.q_grid))) return rc;
+        op.nalpha = op.q_grid;
         op.npartial = std::max<int>(op.npartial, (int)op.nalpha + 1);
         dev_free(op.partial);
         dev_free(op.partial_b);

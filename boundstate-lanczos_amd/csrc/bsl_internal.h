@@ -130,7 +130,8 @@ struct DevOperator {
     int *mplane = nullptr;        // m + lmax of each carried plane
     int na = 0, nphi = 0, nm = 0;
     int qmode = 0;                // 0 FP64 MFMA, 1 FP32 MFMA, 2 BF16x3 MFMA
-    int q_npts = 0;               // grid points per workgroup of the quadrature kernel
+    int q_npts = 0;               // grid points per group of the quadrature kernel
+    int q_grid = 0;               // its persistent workgroups (resident ones, at most the groups)
     int *q_rl = nullptr;          // [16][9]: per m (+lmax) the count, then the carried planes
     // reduction scratch: the potential kernel writes nalpha per-workgroup
     // partials of v.R (partial), |R|^2 (partial_b) and |v|^2 (partial_c)
@@ -210,6 +211,9 @@ int launch_hf_grad(const double *om, int64_t P, int Q, const double *gx, const d
                    int ny, const double *off, const double *site4, const int *lj, int ns, double q_h, double q_cm,
                    double lj_a, double lj_b, int npb, double *partial, double *out, hipStream_t s);
 int quad_points_per_group(int na, int nk);
+// workgroups of the persistent quadrature kernel: its resident workgroups
+// per CU times the CUs of `device`, at most the number of point groups
+int quad_grid(const DevOperator &op, int device, int *grid);
 int64_t quad_tiles(int64_t P, int na, int npts);
 int quad_phi_tiles(int nphi);
 int launch_quad_potential(const DevOperator &op, const double *X, double *R, Scal *sc, hipStream_t s);

@@ -180,8 +180,9 @@ __global__ __launch_bounds__(kQWG) void k_quad_mfma(const double *__restrict__ X
                                                     const double *__restrict__ Sg, const int *__restrict__ mplane,
                                                     const int *__restrict__ qrl, const double *__restrict__ mask,
                                                     const Scal *__restrict__ sc, int64_t P, int nk, int na,
-                                                    int nphi, int nm, int ppw, double *__restrict__ partial,
-                                                    double *__restrict__ partial_b, double *__restrict__ partial_c)
+                                                    int nphi, int nm, int ppw, int64_t ngroups,
+                                                    double *__restrict__ partial, double *__restrict__ partial_b,
+                                                    double *__restrict__ partial_c)
 {
     typedef typename QuadTypes<MODE>::T T;
     typedef typename QuadTypes<MODE>::acc acc_t;
@@ -194,7 +195,6 @@ __global__ __launch_bounds__(kQWG) void k_quad_mfma(const double *__restrict__ X
     double *cs = (double *)smem;
     double *us = (double *)(smem + q.off_u);
     T *Ls = (T *)(smem + q.off_l);
-    const int64_t p0 = (int64_t)blockIdx.x * npts;
     (void)sc;  // X already holds v (formed by the kinetic kernel)
 
     // ---- tables and inputs ----
@@ -219,18 +219,12 @@ __global__ __launch_bounds__(kQWG) void k_quad_mfma(const double *__restrict__ X
             Ss[ph * q.sp + mi] = (ph < nphi && mi < nm) ? (T)Sg[(size_t)mi * nphi + ph] : (T)0;
         }
     }
-    for (int t = threadIdx.x; t < nk * npts; t += blockDim.x) {
-        const int r = t / npts, p = t - r * npts;
-        cs[p * nk + r] = p0 + p < P ? X[(size_t)r * P + p0 + p] : 0.0;
-    }
-    __syncthreads();
 
-    // ---- per wave: its ppw points, tw column tiles ----
+    // ---- per wave: ppw points of each group, tw column tiles ----
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     const int l16 = lane & 15, kq = lane >> 4;
     const int tw = ppw * na / 16;
-    const int pw0 = wid * ppw;  // first point of this wave within the workgroup
-    const int64_t tile0 = (int64_t)blockIdx.x * kQW * tw + (int64_t)wid * tw;
+    const int pw0 = wid * ppw;  // first point of this wave within the group
     const T *W = (const T *)Wv + (size_t)lane * 4;
     T *xs = (T *)(smem + q.off_x) + wid * 16 * q.fm;
     const int mr = lane < nk ? mplane[lane] : 0;
@@ -238,14 +232,14 @@ __global__ __launch_bounds__(kQWG) void k_quad_mfma(const double *__restrict__ X
     double uacc = 0.0;
 
     // One column tile.  Each phi tile's W register tile is refilled with the
-    // next column tile's W right after its use (prefetch: std::true_type); the scheduling barrier keeps
-    // the load there (the compiler would otherwise sink it to the end of the
-    // tile), so its latency overlaps the rest of this tile's transforms.
-    auto tile = [&](int n, acc_t(&wc)[NT], auto prefetch) {
+    // W of the next tile (Wn: the next tile of this wave, or its first tile in
+    // the workgroup's next group) right after its use (prefetch:
+    // std::true_type); the scheduling barrier keeps the load there (the
+    // compiler would otherwise sink it to the end of the tile), so its latency
+    // overlaps the rest of this tile's transforms.
+    auto tile = [&](int n, acc_t(&wc)[NT], const T *Wn, auto prefetch) {
         const int colw = 16 * n + l16;  // column within the wave
         const int pc = pw0 + colw / na, ac = colw % na;
-        // next tile's W (the last tile reloads itself: no read past the grid)
-        const T *Wn = W + (size_t)((tile0 + n + 1) * NT) * 256;  // read only when prefetching
         // Legendre forward of this lane's column for every m (uniform loops
         // over the planes of each m), then the B operand rows of this lane
         double f[KF];
@@ -371,27 +365,51 @@ __global__ __launch_bounds__(kQWG) void k_quad_mfma(const double *__restrict__ X
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     };
-    acc_t wb[NT];
-#pragma unroll
-    for (int t = 0; t < NT; t++) wb[t] = load_w<MODE>(W + (size_t)(tile0 * NT + t) * 256);
-    // the last tile has no successor to prefetch
-    for (int n = 0; n + 1 < tw; n++) tile(n, wb, std::true_type());
-    tile(tw - 1, wb, std::false_type());
-    if (lane < nk && pcur >= 0) us[pcur * nk + lane] = uacc;
-    __syncthreads();
-
-    // ---- R update and the partials of v.R, |R|^2, |v|^2 ----
+    // The workgroup is persistent: it takes the groups blockIdx.x,
+    // blockIdx.x + gridDim.x, ... (npts points each); the tables above are
+    // loaded once, and the W prefetch runs on across group boundaries.
     double dot = 0.0, rr2 = 0.0, vv2 = 0.0;
-    for (int t = threadIdx.x; t < nk * npts; t += blockDim.x) {
-        const int r = t / npts, p = t - r * npts;
-        if (p0 + p >= P) continue;
-        const size_t o = (size_t)r * P + p0 + p;
-        const double nr = R[o] + mask[r] * us[p * nk + r];
-        R[o] = nr;
-        const double v = cs[p * nk + r];
-        dot += v * nr;
-        rr2 += nr * nr;
-        vv2 += v * v;
+    acc_t wb[NT];
+    int64_t g = blockIdx.x;
+    if (g < ngroups) {
+        const int64_t t0 = g * kQW * tw + (int64_t)wid * tw;
+#pragma unroll
+        for (int t = 0; t < NT; t++) wb[t] = load_w<MODE>(W + (size_t)(t0 * NT + t) * 256);
+    }
+    for (; g < ngroups; g += gridDim.x) {
+        const int64_t p0 = g * npts;
+        const int64_t tile0 = g * kQW * tw + (int64_t)wid * tw;
+        const bool more = g + gridDim.x < ngroups;  // uniform over the workgroup
+        for (int t = threadIdx.x; t < nk * npts; t += blockDim.x) {
+            const int r = t / npts, p = t - r * npts;
+            cs[p * nk + r] = p0 + p < P ? X[(size_t)r * P + p0 + p] : 0.0;
+        }
+        __syncthreads();
+        pcur = -1;
+        uacc = 0.0;
+        // one call site: the successor of the group's last tile is the wave's
+        // first tile of the next group, or (none left) the tile itself
+        const int64_t next0 = more ? (g + gridDim.x) * kQW * tw + (int64_t)wid * tw : tile0 + tw - 1;
+        for (int n = 0; n < tw; n++) {
+            const int64_t nt = n + 1 < tw ? tile0 + n + 1 : next0;
+            tile(n, wb, W + (size_t)(nt * NT) * 256, std::true_type());
+        }
+        if (lane < nk && pcur >= 0) us[pcur * nk + lane] = uacc;
+        __syncthreads();
+
+        // ---- R update and the partials of v.R, |R|^2, |v|^2 ----
+        for (int t = threadIdx.x; t < nk * npts; t += blockDim.x) {
+            const int r = t / npts, p = t - r * npts;
+            if (p0 + p >= P) continue;
+            const size_t o = (size_t)r * P + p0 + p;
+            const double nr = R[o] + mask[r] * us[p * nk + r];
+            R[o] = nr;
+            const double v = cs[p * nk + r];
+            dot += v * nr;
+            rr2 += nr * nr;
+            vv2 += v * v;
+        }
+        __syncthreads();  // cs and us are rewritten by the next group
     }
     const double s0 = q_block_sum(dot, red);
     const double s1 = q_block_sum(rr2, red);
@@ -476,13 +494,23 @@ int launch_weight_grid(const double *vg, void *W, int wprec, const double *wt, c
 }
 
 #define BSL_QUAD_LAUNCH(M, NTV, KSV)                                                                       \
-    hipLaunchKernelGGL((k_quad_mfma<M, NTV, KSV>), dim3(grid), dim3(kQWG), lds, s, X, R, op.wgrid, op.qL, op.qS, \
-                       op.mplane, op.q_rl, op.mask, sc, op.P, op.nk, op.na, op.nphi, op.nm, ppw, op.partial,  \
-                       op.partial_b, op.partial_c)
+    do {                                                                                                   \
+        if (occ) {                                                                                         \
+            *occ = 0;                                                                                      \
+            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(occ, (const void *)k_quad_mfma<M, NTV, KSV>,   \
+                                                             kQWG, lds) != hipSuccess)                     \
+                *occ = 0;                                                                                  \
+        } else {                                                                                           \
+            hipLaunchKernelGGL((k_quad_mfma<M, NTV, KSV>), dim3(grid), dim3(kQWG), lds, s, X, R, op.wgrid,  \
+                               op.qL, op.qS, op.mplane, op.q_rl, op.mask, sc, op.P, op.nk, op.na, op.nphi,   \
+                               op.nm, ppw, ngroups, op.partial, op.partial_b, op.partial_c);               \
+        }                                                                                                  \
+    } while (0)
 
+// with occ != null: only the resident workgroups per CU of the instance
 template <int M, int NTV>
 static void launch_quad_ks(const DevOperator &op, const double *X, double *R, Scal *sc, unsigned grid, size_t lds,
-                           int ppw, int ks, hipStream_t s)
+                           int ppw, int64_t ngroups, int ks, hipStream_t s, int *occ)
 {
     if constexpr (M == 2) {
         BSL_QUAD_LAUNCH(M, NTV, 4);
@@ -499,14 +527,14 @@ static void launch_quad_ks(const DevOperator &op, const double *X, double *R, Sc
 
 template <int M>
 static void launch_quad_mode(const DevOperator &op, const double *X, double *R, Scal *sc, unsigned grid, size_t lds,
-                             int ppw, int ks, hipStream_t s)
+                             int ppw, int64_t ngroups, int ks, hipStream_t s, int *occ)
 {
     switch (quad_phi_tiles(op.nphi)) {
-    case 1: launch_quad_ks<M, 1>(op, X, R, sc, grid, lds, ppw, ks, s); break;
-    case 2: launch_quad_ks<M, 2>(op, X, R, sc, grid, lds, ppw, ks, s); break;
-    case 4: launch_quad_ks<M, 4>(op, X, R, sc, grid, lds, ppw, ks, s); break;
-    case 9: launch_quad_ks<M, 9>(op, X, R, sc, grid, lds, ppw, ks, s); break;
-    default: launch_quad_ks<M, 16>(op, X, R, sc, grid, lds, ppw, ks, s); break;
+    case 1: launch_quad_ks<M, 1>(op, X, R, sc, grid, lds, ppw, ngroups, ks, s, occ); break;
+    case 2: launch_quad_ks<M, 2>(op, X, R, sc, grid, lds, ppw, ngroups, ks, s, occ); break;
+    case 4: launch_quad_ks<M, 4>(op, X, R, sc, grid, lds, ppw, ngroups, ks, s, occ); break;
+    case 9: launch_quad_ks<M, 9>(op, X, R, sc, grid, lds, ppw, ngroups, ks, s, occ); break;
+    default: launch_quad_ks<M, 16>(op, X, R, sc, grid, lds, ppw, ngroups, ks, s, occ); break;
     }
 }
 
@@ -517,23 +545,43 @@ int quad_kf(int nm, int mode)
     return mode == 2 ? 16 : 4 * (ks < 2 ? 2 : ks);
 }
 
-int launch_quad_potential(const DevOperator &op, const double *X, double *R, Scal *sc, hipStream_t s)
+// launch (occ == null) or report the resident workgroups per CU (occ)
+static int quad_dispatch(const DevOperator &op, const double *X, double *R, Scal *sc, hipStream_t s, int *occ)
 {
     const int npts = op.q_npts, ppw = npts / kQW;
-    const unsigned grid = (unsigned)((op.P + npts - 1) / npts);
-    if ((int64_t)grid > op.npartial || (int64_t)grid != op.nalpha)
+    const int64_t ngroups = (op.P + npts - 1) / npts;
+    const unsigned grid = (unsigned)op.q_grid;
+    if (!occ && (grid == 0 || (int64_t)grid > op.npartial || (int64_t)grid != op.nalpha))
         return fail(BSL_E_STATE, "partial buffer does not match the quadrature kernel grid");
     const int kf = quad_kf(op.nm, op.qmode), ntp = quad_phi_tiles(op.nphi);
     const size_t lds = quad_lds(op.nk, op.na, npts, kf, op.qmode, 16 * ntp).bytes;
     if (op.nm > 16 || op.nk > 64 || op.nphi > 256 || npts % kQW || (ppw * op.na) % 16 || lds > 65536)
         return fail(BSL_E_ARG, "quadrature kernel: unsupported sizes");
     if (op.qmode == 0)
-        launch_quad_mode<0>(op, X, R, sc, grid, lds, ppw, kf / 4, s);
+        launch_quad_mode<0>(op, X, R, sc, grid, lds, ppw, ngroups, kf / 4, s, occ);
     else if (op.qmode == 1)
-        launch_quad_mode<1>(op, X, R, sc, grid, lds, ppw, kf / 4, s);
+        launch_quad_mode<1>(op, X, R, sc, grid, lds, ppw, ngroups, kf / 4, s, occ);
     else
-        launch_quad_mode<2>(op, X, R, sc, grid, lds, ppw, kf / 4, s);
-    BSL_HIP(hipGetLastError());
+        launch_quad_mode<2>(op, X, R, sc, grid, lds, ppw, ngroups, kf / 4, s, occ);
+    if (!occ) BSL_HIP(hipGetLastError());
+    return BSL_OK;
+}
+
+int launch_quad_potential(const DevOperator &op, const double *X, double *R, Scal *sc, hipStream_t s)
+{
+    return quad_dispatch(op, X, R, sc, s, nullptr);
+}
+
+int quad_grid(const DevOperator &op, int device, int *grid)
+{
+    int per_cu = 0, cus = 0;
+    int rc = quad_dispatch(op, nullptr, nullptr, nullptr, nullptr, &per_cu);
+    if (rc) return rc;
+    BSL_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
+    if (per_cu < 1) per_cu = 1;
+    const int64_t ngroups = (op.P + op.q_npts - 1) / op.q_npts;
+    const int64_t g = (int64_t)per_cu * (cus > 0 ? cus : 1);
+    *grid = (int)(g < ngroups ? g : ngroups);
     return BSL_OK;
 }
 

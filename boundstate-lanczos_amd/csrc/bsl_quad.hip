@@ -98,7 +98,7 @@ __host__ __device__ inline int acc_row(int kq, int i)
 // points (ppw*na a multiple of 16: whole column tiles per wave).
 struct QuadLds {
     int sp, fm, bp;               // row strides (odd, against bank conflicts)
-    size_t off_s, off_b, off_x, off_u, off_l, bytes;
+    size_t off_s, off_b, off_x, off_u, off_l, off_sel, bytes;
 };
 
 constexpr int kQW = kQWG / 64;
@@ -133,6 +133,9 @@ __host__ __device__ inline QuadLds quad_lds(int nk, int na, int npts, int kf, in
     o = (o + 15) & ~(size_t)15;
     q.off_l = o;
     o += (size_t)es * nk * na;                     // L [nk][na]
+    o = (o + 15) & ~(size_t)15;
+    q.off_sel = o;
+    o += (size_t)es * ((nk + 3) / 4) * 64;         // m selection, A operand [K step][lane]
     q.bytes = o;
     return q;
 }
@@ -199,6 +202,19 @@ __global__ __launch_bounds__(kQWG) void k_quad_mfma(const double *__restrict__ X
 
     // ---- tables and inputs ----
     for (int t = threadIdx.x; t < nk * na; t += blockDim.x) Ls[t] = (T)Lg[t];
+    // A operand of the forward Legendre MFMA: Sel[row][r] = 1 where carried
+    // plane r has m = msel(row), for K step s4 (r = 4 s4 + lane/16) and row
+    // lane%16.  msel orders the rows so that accumulator item i of lane group
+    // kq holds the m the forward phi operand of that lane needs (m = 4i + kq
+    // for the 16x16x4 instructions, m = 4kq + i for the BF16 K = 16 block).
+    const int nks = (nk + 3) >> 2;
+    T *Sel = (T *)(smem + q.off_sel);
+    for (int t = threadIdx.x; t < nks * 64; t += blockDim.x) {
+        const int r = 4 * (t >> 6) + ((t & 63) >> 4), row = t & 15;
+        const int msel = (MODE == 1) ? 4 * (row & 3) + (row >> 2) : row;  // FP64 D rows kq + 4i, FP32 4kq + i
+        Sel[t] = (r < nk && mplane[r] == msel) ? (T)1 : (T)0;
+    }
+    (void)qrl;
     if constexpr (MODE == 2) {
         short *fh = (short *)(smem + q.off_s), *fl = fh + 16 * NT * 16;
         short *bh = (short *)(smem + q.off_b), *bl = bh + 16 * q.bp;
@@ -240,20 +256,15 @@ __global__ __launch_bounds__(kQWG) void k_quad_mfma(const double *__restrict__ X
     auto tile = [&](int n, acc_t(&wc)[NT], const T *Wn, auto prefetch) {
         const int colw = 16 * n + l16;  // column within the wave
         const int pc = pw0 + colw / na, ac = colw % na;
-        // Legendre forward of this lane's column for every m (uniform loops
-        // over the planes of each m), then the B operand rows of this lane
-        double f[KF];
-#pragma unroll
-        for (int m = 0; m < KF; m++) {
-            double acc = 0.0;
-            if (m < nm) {
-                const int cnt = qrl[m * 9];
-                for (int u = 0; u < cnt; u++) {
-                    const int r = qrl[m * 9 + 1 + u];
-                    acc += (double)Ls[r * na + ac] * cs[pc * nk + r];
-                }
-            }
-            f[m] = acc;
+        // Legendre forward on MFMA: f(m, col) = sum_r Sel[m][r] L_r(a) c_r(p)
+        // over the carried planes, the B operand (k = r, column l16) being
+        // this lane's own products; item i of lane group kq ends up holding
+        // the m of the forward phi operand (see Sel)
+        acc_t fa = {0, 0, 0, 0};
+        for (int s4 = 0; s4 < nks; s4++) {
+            const int r = 4 * s4 + kq;
+            const double prod = r < nk ? (double)Ls[r * na + ac] * cs[pc * nk + r] : 0.0;
+            fa = mfma16(Sel[s4 * 64 + lane], (T)prod, fa);
         }
         acc_t acc2 = {0, 0, 0, 0};
         if constexpr (MODE == 2) {
@@ -262,13 +273,9 @@ __global__ __launch_bounds__(kQWG) void k_quad_mfma(const double *__restrict__ X
             s4v bfh, bfl;
 #pragma unroll
             for (int i = 0; i < 4; i++) {
-                // row 4*kq + i of the 16-row K block (rows >= KF are zero)
-                double v = 0.0;
-#pragma unroll
-                for (int g = 0; g < 4; g++)
-                    if (4 * g + i < KF) v = kq == g ? f[4 * g + i] : v;
+                // row 4*kq + i of the 16-row K block (rows >= nm are zero)
                 short h, l;
-                bf16_split((float)v, h, l);
+                bf16_split(fa[i], h, l);
                 bfh[i] = h;
                 bfl[i] = l;
             }
@@ -301,14 +308,9 @@ __global__ __launch_bounds__(kQWG) void k_quad_mfma(const double *__restrict__ X
             }
         } else {
             const T *Ss = (const T *)(smem + q.off_s);
-            T bF[KS];
+            T bF[KS];  // m = 4s + kq of this lane's column
 #pragma unroll
-            for (int s = 0; s < KS; s++) {
-                double v = f[4 * s];
-#pragma unroll
-                for (int g = 1; g < 4; g++) v = kq == g ? f[4 * s + g] : v;
-                bF[s] = (T)v;
-            }
+            for (int s = 0; s < KS; s++) bF[s] = fa[s];
 #pragma unroll
             for (int t = 0; t < NT; t++) {
                 acc_t g = {0, 0, 0, 0};

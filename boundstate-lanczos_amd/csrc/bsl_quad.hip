@@ -462,11 +462,13 @@ int quad_phi_tiles(int nphi)
 int quad_points_per_group(int na, int nk)
 {
     // each wave owns ppw points with ppw*na a multiple of 16 (whole column
-    // tiles), at least 4 tiles per wave so that the W prefetch has a next tile
+    // tiles), at least 4 tiles and 4 points per wave (16 points per group:
+    // 128-byte segments of each plane in the group's X and R accesses)
     int g = 16;
     while (na % g) g >>= 1;
     const int unit = 16 / g, tiles_per_unit = na / g;
     int k = (4 + tiles_per_unit - 1) / tiles_per_unit;
+    while (unit * k < 4) k++;
     while (k > 1 && quad_lds(nk, na, kQW * unit * k, 16, 0, 256).bytes > 65536) k--;
     return kQW * unit * k;
 }

@@ -77,6 +77,34 @@ def test_hv_reduced_matches_oracle(sym):
     assert _rel(got, ref) < 1e-12
 
 
+@pytest.mark.parametrize("dims", [(6, 5, 4), (8, 6, 5), (6, 8, 7), (7, 7, 6)])
+def test_hv_anisotropic_grids_match_oracle(dims):
+    """Axis lengths and box lengths that differ per axis (the deck's nx, ny,
+    nz and x/y/z_max, linRotCartSphHvFullVPrepRoutines.cpp:248-321,
+    362-382), odd and even: every pairing of the full-size and the half-size
+    (centrosymmetric) kinetic contractions for x-y and for z."""
+    nx, ny, nz = dims
+    spec, kw = cs.cage_case(nx, 2, 6, 6, "All")
+    spec = dict(spec, ny=ny, nz=nz, y_max=0.35, z_max=0.45)
+    kw = dict(kw, ny=ny, nz=nz, y_max=0.35, z_max=0.45)
+    o, c = O.Oracle(spec), B.Calculator(**kw)
+    v = np.random.default_rng(21).uniform(-1, 1, o.N)
+    u0 = np.random.default_rng(22).uniform(-1, 1, o.N)
+    ref = u0.copy()
+    O.lib.bso_hv(o.h, v, ref)
+    assert _rel(c.hv(v, u0), ref) < 1e-12
+
+
+@pytest.mark.parametrize("n", [64, 63])
+def test_hv_largest_axis_matches_oracle(n):
+    """64 points per axis, the largest the kinetic kernels stage (and 63, its
+    odd neighbour); 65 is rejected at creation (test_host)."""
+    spec, kw = cs.cage_case(n, 1, 4, 4, "All", x_max=0.8)
+    o, c = O.Oracle(spec), B.Calculator(**kw)
+    v = np.random.default_rng(23).uniform(-1, 1, o.N)
+    assert _rel(c.hv(v), o.hv(v)) < 1e-12
+
+
 def test_hv_iho_matches_oracle():
     spec, kw = cs.iho_case(9, 2, 6, 8, x_max=0.4)
     o, c = O.Oracle(spec), B.Calculator(**kw)

@@ -65,6 +65,78 @@ def quad_leg(B, args):
     return out
 
 
+def adjoint_leg(B, args):
+    """Hellmann-Feynman gradients at a 41^3 grid (lmax 5, 40 x 70, same cage):
+    lowest 10 levels from an FP32-Hv run (Loop 1, Cullum-Willoughby, Loop 2
+    Ritz vectors recomputed, FP64 Rayleigh refinement), gradients summed over
+    degenerate multiplets, checked against central differences (h = 1e-4
+    relative) of the refined levels and against the gradient of an FP64-Hv
+    run."""
+    from bsl_amd import cage, physics
+    pos, q, lj = cage.synthetic_cage(0)
+    lj_a, lj_b = cage.baseline_lj()
+    base = dict(nx=args.adjoint_n, x_max=args.x_max, lmax=5, n_theta=40, n_phi=70, mass=2.016,
+                rot_const=physics.cm1_to_kjmol(59.322), ceiling=18.0, potential="cage", sites=(pos, q, lj),
+                lj_a=lj_a, lj_b=lj_b, q_h2_h=physics.Q_H2_H, q_h2_cm=physics.Q_H2_CM)
+    niter, count = args.adjoint_steps, 10
+
+    def run(kw, grad):
+        with B.Calculator(**kw) as c:
+            t0 = time.perf_counter()
+            e, psi, res = c.levels(niter, count)
+            t_lev = time.perf_counter() - t0
+            if not grad:
+                return e
+            groups = B.multiplets(e, tol=1e-6)
+            t0 = time.perf_counter()
+            gr = [c.hf_gradient(psi[gi], np.ones(len(gi))) for gi in groups]
+            t_grad = time.perf_counter() - t0
+            return e, res, groups, gr, t_lev, t_grad
+
+    e32, res, groups, g32, t_lev, t_grad = run(dict(base, precision="fp32"), True)
+    _, _, groups64, g64, _, _ = run(dict(base, precision="fp64"), True)
+
+    def entry(g, name, idx):
+        if name == "site_q":
+            return float(g["site_q"][idx])
+        if name == "site_pos":
+            return float(g["site_pos"].reshape(-1)[idx])
+        return float(g[name])
+
+    names = [("q_h2_h", None), ("q_h2_cm", None), ("lj_a", None), ("lj_b", None), ("site_q", 0), ("site_pos", 0)]
+    fd_err, fp64_diff = 0.0, 0.0
+    for name, idx in names:
+        x = base[name] if idx is None else (q[idx] if name == "site_q" else pos.reshape(-1)[idx])
+        h = 1e-4 * abs(x)
+        vals = []
+        for sgn in (1.0, -1.0):
+            kw = dict(base, precision="fp32")
+            if name == "site_q":
+                qq = q.copy()
+                qq[idx] += sgn * h
+                kw["sites"] = (pos, qq, lj)
+            elif name == "site_pos":
+                pp = pos.copy()
+                pp.reshape(-1)[idx] += sgn * h
+                kw["sites"] = (pp, q, lj)
+            else:
+                kw[name] = x + sgn * h
+            vals.append(run(kw, False))
+        for gi, g, g_64 in zip(groups, g32, g64):
+            fd = (np.sum(vals[0][gi]) - np.sum(vals[1][gi])) / (2 * h)
+            an = entry(g, name, idx)
+            fd_err = max(fd_err, abs(an - fd) / max(abs(fd), 1e-3 / abs(x)))
+            fp64_diff = max(fp64_diff, abs(an - entry(g_64, name, idx)) / max(abs(entry(g_64, name, idx)), 1e-12))
+    out = {"grid": args.adjoint_n, "levels": count, "multiplets": len(groups), "steps": niter,
+           "levels_s": t_lev, "gradient_ms": 1e3 * t_grad, "max_residual": float(np.max(res)),
+           "fd_parameters": [n if i is None else "%s[%d]" % (n, i) for n, i in names],
+           "max_rel_err_vs_fd": fd_err, "max_rel_diff_vs_fp64_gradient": fp64_diff,
+           "same_multiplets_as_fp64": [list(g) for g in groups] == [list(g) for g in groups64]}
+    print("adjoint %d^3: levels %.2f s, gradients %.1f ms, max rel err vs FD %.2e, vs FP64 gradient %.2e"
+          % (args.adjoint_n, t_lev, 1e3 * t_grad, fd_err, fp64_diff), file=sys.stderr, flush=True)
+    return out
+
+
 def batch_members(args, calculator, precision, n=32):
     """16 synthetic cages (seeds 0-15) x {para (even l), ortho (odd l)} on
     grids cycled through 41^3, 47^3, 54^3 (same box, lmax 5, 40 x 70)."""
@@ -182,10 +254,11 @@ def eig_error(B, c64, c32, niter):
     t0 = time.perf_counter()
     e64d, _ = B.lancbis(*coef, -10.0, 10.0, device=True)
     t_dev = time.perf_counter() - t0
-    e32, r32 = B.lancbis(*c32.lanczos(niter, start="random", seed=0), -10.0, 10.0)
+    e32, r32 = B.lancbis(*c32.lanczos(niter, start="random", seed=0), -10.0, 10.0, mult_tol=c32.mult_tol)
     low = e64[:20]
     d = [np.min(np.abs(e32 - e)) / physics.KJMOL_PER_CM1 for e in low]
-    tri = {"host_s": t_host, "device_s": t_dev, "good_levels": int(len(e64)),
+    tri = {"host_s": t_host, "device_s": t_dev, "good_levels": int(len(e64)), "mixed_mult_tol_kjmol": c32.mult_tol,
+           "good_levels_mixed": int(len(e32)),
            "max_abs_diff_kjmol": float(np.max(np.abs(e64 - e64d))) if len(e64) == len(e64d) else None}
     return float(np.max(d)), len(low), [float(x) for x in low[:5]], tri
 
@@ -197,7 +270,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--n", type=int, default=54)
     ap.add_argument("--x-max", type=float, default=0.8)
-    ap.add_argument("--eig-steps", type=int, default=3000)
+    ap.add_argument("--eig-steps", type=int, default=10000)
     ap.add_argument("--cpu-steps", type=int, default=30)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-eig", action="store_true")
@@ -206,6 +279,8 @@ def main():
                     help="lockstep iterations timed for the 32-member batch (0 skips the batch leg)")
     ap.add_argument("--quad-steps", type=int, default=200,
                     help="steps timed per quadrature-calculator precision (0 skips the quadrature leg)")
+    ap.add_argument("--adjoint-n", type=int, default=41, help="grid of the adjoint leg (0 skips it)")
+    ap.add_argument("--adjoint-steps", type=int, default=4000)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain copy kernel over 1 GiB of doubles three times (the reference point "
                          "for the HBM byte counters of rocprofv3 --pmc) and exit")
@@ -260,6 +335,13 @@ def main():
 
     quad = quad_leg(B, args) if args.quad_steps > 0 else None
     batch = batch_leg(B, args) if args.batch_steps > 0 else None
+    adj = None
+    if args.adjoint_n > 0:
+        try:
+            adj = adjoint_leg(B, args)
+        except B.BslError as e:
+            adj = {"error": str(e)}
+            print("adjoint leg: %s" % e, file=sys.stderr, flush=True)
 
     cpu = None
     if not args.no_cpu:
@@ -292,6 +374,7 @@ def main():
             "per_step_ms": {"potential": ms[1], "kinetic": ms[2], "finalize": ms[3]},
             "quadrature": quad,
             "batch_32": batch,
+            "adjoint": adj,
             "algorithmic_bytes_per_step": sb[3],
             "step_bytes_roofline_frac": sb[3] / (ms[0] / args.steps * 1e-3) / 1e9 / HBM_PEAK_GBS,
         },

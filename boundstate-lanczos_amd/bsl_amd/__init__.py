@@ -97,6 +97,12 @@ class Calculator:
         check(lib().bsl_rayleigh(self.h, len(y), y.ravel(), e, r))
         return e, r
 
+    @property
+    def mult_tol(self):
+        """Multiplicity tolerance of the eigensolve for this operator: 0 (the
+        reference's BISEC) for FP64, MIXED_MULT_TOL for the mixed precisions."""
+        return 0.0 if self.params.precision == PREC["fp64"] else MIXED_MULT_TOL
+
     def levels(self, niter, count, emin=-1.0e4, emax=1.0e4, start="random", seed=0, tol=1e-8):
         """The `count` lowest converged good levels: Loop 1, Cullum-Willoughby,
         Tm eigenvectors, then Loop 2 rebuilds their Ritz vectors by
@@ -105,12 +111,12 @@ class Calculator:
         each level.  Returns (energies, unit Ritz vectors (count, N),
         residual norms)."""
         a, b, b2 = self.lanczos(niter, start=start, seed=seed)
-        ev, er = lancbis(a, b, b2, emin, emax)
+        ev, er = lancbis(a, b, b2, emin, emax, mult_tol=self.mult_tol)
         order = np.argsort(ev)[:count]
         if len(order) < count or not np.all(np.abs(er[order]) < tol):
             raise BslError(0, "the %d lowest levels are not all converged after %d steps" % (count, niter))
         sel = ev[order]
-        evtr, _ = trivec(a, b, b2, sel)
+        evtr, _ = trivec(a, b, b2, sel, mult_tol=self.mult_tol)
         psi = self.ritz(niter, evtr, count, start=start, seed=seed)
         psi /= np.linalg.norm(psi, axis=1, keepdims=True)
         e, r = self.rayleigh(psi)
@@ -269,26 +275,39 @@ def multiplets(energies, tol=1e-6):
     return groups
 
 
-def lancbis(alpha, beta, beta2, emin, emax, device=False):
+# Multiplicity tolerance (kJ/mol) of the eigensolve for the mixed-precision
+# operators (BSL_MIXED_MULT_TOL of bsl_hip.h): their rounding spreads the
+# ghost copies of a converged level by ~1e-8 relative (3e-8 kJ/mol at
+# 54^3 x 36, FP32), far beyond FP64's BISEC width; 1e-6 kJ/mol (8e-5 cm^-1)
+# merges them and stays two orders below the 0.01 cm^-1 accuracy of the
+# mixed levels.
+MIXED_MULT_TOL = 1.0e-6
+
+
+def lancbis(alpha, beta, beta2, emin, emax, device=False, mult_tol=0.0):
     """Good eigenvalues and error estimates (lancbis.cpp:5-105); device=True
-    runs the Sturm bisections and inverse iterations on the GPU."""
+    runs the Sturm bisections and inverse iterations on the GPU.  mult_tol
+    > 0 merges eigenvalues of T_m closer than that into one multiple
+    eigenvalue (bsl_lancbis_tol: for the coefficients of a mixed-precision
+    operator, see MIXED_MULT_TOL); 0 is the reference's BISEC."""
     niter = len(alpha)
     ev, er = np.zeros(niter), np.zeros(niter)
     ng = C.c_int32()
-    fn = lib().bsl_lancbis_device if device else lib().bsl_lancbis
+    fn = lib().bsl_lancbis_device_tol if device else lib().bsl_lancbis_tol
     check(fn(niter, np.ascontiguousarray(alpha, dtype=np.float64),
-                            np.ascontiguousarray(beta, dtype=np.float64),
-                            np.ascontiguousarray(beta2, dtype=np.float64), emin, emax, ev, er, C.byref(ng)))
+             np.ascontiguousarray(beta, dtype=np.float64),
+             np.ascontiguousarray(beta2, dtype=np.float64), emin, emax, float(mult_tol), ev, er, C.byref(ng)))
     return ev[: ng.value].copy(), er[: ng.value].copy()
 
 
-def trivec(alpha, beta, beta2, evals):
-    """Tm eigenvectors, column-major [niter][ngood] flattened (trivec.f)."""
+def trivec(alpha, beta, beta2, evals, mult_tol=0.0):
+    """Tm eigenvectors, column-major [niter][ngood] flattened (trivec.f);
+    mult_tol as given to lancbis for these eigenvalues (bsl_trivec_tol)."""
     niter, ng = len(alpha), len(evals)
     evtr = np.zeros(niter * max(ng, 1))
     mamin = np.zeros(max(ng, 1), dtype=np.int32)
-    check(lib().bsl_trivec(niter, np.ascontiguousarray(alpha, dtype=np.float64),
-                           np.ascontiguousarray(beta, dtype=np.float64),
-                           np.ascontiguousarray(beta2, dtype=np.float64),
-                           np.ascontiguousarray(evals, dtype=np.float64), ng, evtr, mamin))
+    check(lib().bsl_trivec_tol(niter, np.ascontiguousarray(alpha, dtype=np.float64),
+                               np.ascontiguousarray(beta, dtype=np.float64),
+                               np.ascontiguousarray(beta2, dtype=np.float64),
+                               np.ascontiguousarray(evals, dtype=np.float64), ng, float(mult_tol), evtr, mamin))
     return evtr[: niter * ng], mamin[:ng].copy()

@@ -23,7 +23,8 @@ EXPORTS = (
     "bsl_hv_device", "bsl_lanczos", "bsl_lancbis", "bsl_trivec", "bsl_ritz", "bsl_rayleigh",
     "bsl_water_sites", "bsl_get_vblocks", "bsl_get_vgrid", "bsl_time_steps", "bsl_step_bytes",
     "bsl_potential_kernel", "bsl_hf_gradient", "bsl_batch_create", "bsl_batch_destroy", "bsl_batch_size",
-    "bsl_batch_member", "bsl_batch_lanczos", "bsl_batch_time_steps", "bsl_lancbis_device",
+    "bsl_batch_member", "bsl_batch_lanczos", "bsl_batch_time_steps", "bsl_lancbis_device", "bsl_lancbis_tol",
+    "bsl_lancbis_device_tol", "bsl_trivec_tol",
 )
 
 
@@ -70,7 +71,12 @@ def load():
                                 C.POINTER(C.c_int32)]
     lib.bsl_lancbis_device.argtypes = [C.c_int32, _dp, _dp, _dp, C.c_double, C.c_double, _dp, _dp,
                                        C.POINTER(C.c_int32)]
+    lib.bsl_lancbis_tol.argtypes = [C.c_int32, _dp, _dp, _dp, C.c_double, C.c_double, C.c_double, _dp, _dp,
+                                    C.POINTER(C.c_int32)]
+    lib.bsl_lancbis_device_tol.argtypes = [C.c_int32, _dp, _dp, _dp, C.c_double, C.c_double, C.c_double, _dp, _dp,
+                                           C.POINTER(C.c_int32)]
     lib.bsl_trivec.argtypes = [C.c_int32, _dp, _dp, _dp, _dp, C.c_int32, _dp, _ip]
+    lib.bsl_trivec_tol.argtypes = [C.c_int32, _dp, _dp, _dp, _dp, C.c_int32, C.c_double, _dp, _ip]
     lib.bsl_ritz.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_int32, C.c_uint64, _dp, C.c_int32, _dp]
     lib.bsl_rayleigh.argtypes = [C.c_void_p, C.c_int32, _dp, _dp, _dp]
     lib.bsl_water_sites.argtypes = [C.c_int32, C.c_int32, _dp, _dp, _dp, _ip]

@@ -286,7 +286,6 @@ extern "C" int bsl_destroy(bsl_ctx *c)
     dev_free(o.qL);
     dev_free(o.qS);
     dev_free(o.mplane);
-    dev_free(o.q_rl);
     dev_free(o.partial);
     dev_free(o.partial_b);
     dev_free(o.partial_c);
@@ -456,13 +455,6 @@ extern "C" int bsl_create(const bsl_params *pp, bsl_ctx **out)
         if ((rc = dev_upload(&op.qL, qL))) return rc;
         if ((rc = dev_upload(&op.qS, qS))) return rc;
         if ((rc = dev_upload(&op.mplane, mpl))) return rc;
-        std::vector<int> qrl(16 * 9, 0);
-        for (int r = 0; r < nk; r++) {
-            int &cnt = qrl[(size_t)mpl[r] * 9];
-            if (cnt >= 8) return fail(BSL_E_ARG, "quadrature kernel: more than 8 carried planes share one m");
-            qrl[(size_t)mpl[r] * 9 + 1 + cnt++] = r;
-        }
-        if ((rc = dev_upload(&op.q_rl, qrl))) return rc;
         if ((rc = quad_grid(op, c->device, &op.q_grid))) return rc;
         op.nalpha = op.q_grid;
         op.npartial = std::max<int>(op.npartial, (int)op.nalpha + 1);
@@ -647,12 +639,36 @@ extern "C" int bsl_lanczos(bsl_ctx *c, int32_t niter, const double *v0, int32_t 
     return BSL_OK;
 }
 
+extern "C" int bsl_lancbis_tol(int32_t niter, const double *alpha, const double *beta, const double *beta2,
+                               double emin, double emax, double mult_tol, double *eval, double *err,
+                               int32_t *ngood)
+{
+    if (!alpha || !beta || !beta2 || !eval || !err || !ngood) return fail(BSL_E_ARG, "null argument");
+    if (!(mult_tol >= 0.0)) return fail(BSL_E_ARG, "lancbis: mult_tol must be >= 0");
+    int ng = 0;
+    int rc = cw_lancbis(niter, alpha, beta, beta2, emin, emax, mult_tol, eval, err, &ng);
+    *ngood = ng;
+    return rc;
+}
+
 extern "C" int bsl_lancbis(int32_t niter, const double *alpha, const double *beta, const double *beta2,
                            double emin, double emax, double *eval, double *err, int32_t *ngood)
 {
-    if (!alpha || !beta || !beta2 || !eval || !err || !ngood) return fail(BSL_E_ARG, "null argument");
+    return bsl_lancbis_tol(niter, alpha, beta, beta2, emin, emax, 0.0, eval, err, ngood);
+}
+
+extern "C" int bsl_lancbis_device_tol(int32_t niter, const double *alpha, const double *beta, const double *beta2,
+                                      double emin, double emax, double mult_tol, double *eval, double *err,
+                                      int32_t *ngood)
+{
+    (void)beta2;
+    if (!alpha || !beta || !eval || !err || !ngood) return fail(BSL_E_ARG, "null argument");
+    if (!(mult_tol >= 0.0)) return fail(BSL_E_ARG, "lancbis: mult_tol must be >= 0");
+    hipStream_t s = nullptr;
+    BSL_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
     int ng = 0;
-    int rc = cw_lancbis(niter, alpha, beta, beta2, emin, emax, eval, err, &ng);
+    int rc = cw_lancbis_device(niter, alpha, beta, emin, emax, mult_tol, s, eval, err, &ng);
+    (void)hipStreamDestroy(s);
     *ngood = ng;
     return rc;
 }
@@ -660,25 +676,24 @@ extern "C" int bsl_lancbis(int32_t niter, const double *alpha, const double *bet
 extern "C" int bsl_lancbis_device(int32_t niter, const double *alpha, const double *beta, const double *beta2,
                                   double emin, double emax, double *eval, double *err, int32_t *ngood)
 {
-    (void)beta2;
-    if (!alpha || !beta || !eval || !err || !ngood) return fail(BSL_E_ARG, "null argument");
-    hipStream_t s = nullptr;
-    BSL_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
-    int ng = 0;
-    int rc = cw_lancbis_device(niter, alpha, beta, emin, emax, s, eval, err, &ng);
-    (void)hipStreamDestroy(s);
-    *ngood = ng;
+    return bsl_lancbis_device_tol(niter, alpha, beta, beta2, emin, emax, 0.0, eval, err, ngood);
+}
+
+extern "C" int bsl_trivec_tol(int32_t niter, const double *alpha, const double *beta, const double *beta2,
+                              const double *eval, int32_t ngood, double mult_tol, double *evtr, int32_t *mamin)
+{
+    if (!alpha || !beta || !beta2 || (ngood > 0 && (!eval || !evtr || !mamin))) return fail(BSL_E_ARG, "null argument");
+    if (!(mult_tol >= 0.0)) return fail(BSL_E_ARG, "trivec: mult_tol must be >= 0");
+    std::vector<int> mm(std::max(ngood, 1));
+    int rc = cw_trivec(niter, alpha, beta, beta2, eval, ngood, mult_tol, evtr, mm.data());
+    for (int i = 0; i < ngood; i++) mamin[i] = mm[i];
     return rc;
 }
 
 extern "C" int bsl_trivec(int32_t niter, const double *alpha, const double *beta, const double *beta2,
                           const double *eval, int32_t ngood, double *evtr, int32_t *mamin)
 {
-    if (!alpha || !beta || !beta2 || (ngood > 0 && (!eval || !evtr || !mamin))) return fail(BSL_E_ARG, "null argument");
-    std::vector<int> mm(std::max(ngood, 1));
-    int rc = cw_trivec(niter, alpha, beta, beta2, eval, ngood, evtr, mm.data());
-    for (int i = 0; i < ngood; i++) mamin[i] = mm[i];
-    return rc;
+    return bsl_trivec_tol(niter, alpha, beta, beta2, eval, ngood, 0.0, evtr, mamin);
 }
 
 extern "C" int bsl_ritz(bsl_ctx *c, int32_t niter, const double *v0, int32_t start, uint64_t seed,

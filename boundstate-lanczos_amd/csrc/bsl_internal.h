@@ -13,9 +13,8 @@
 
 namespace bsl {
 
-// recurrence scalars kept in device memory so that the Lanczos loop never
-// synchronises with the host: the actual Krylov vectors are sx*X and sr*R.
-// Scalars of the recurrence, kept on the device.  At the start of a step
+// Scalars of the recurrence, kept in device memory so that the Lanczos loop
+// never synchronises with the host.  At the start of a step
 // the first vector slot X holds r~ = H v_j - beta_j v_{j-1} (alpha not yet
 // removed) and the second slot R holds v_j; the x-y kinetic kernel forms
 // v_{j+1} = ca X + cb R (= (r~ - alpha v_j) / beta) and R = cr R (= -beta v_j)
@@ -55,17 +54,20 @@ void kinetic_row(double x_max, int n, double mass, std::vector<double> &row, std
 // Cullum-Willoughby post-processing (bsl_tridiag.cpp)
 // ---------------------------------------------------------------------------
 // device Sturm bisection and error estimates (bsl_tridiag_dev.hip)
+// per-index output of k_tri_bisect: x1, xl, xu, nu, kl, jl, ku, ju, klm, kum
+constexpr int kTriOut = 10;
 int launch_tri_bisect(const double *al, const double *be, int m, double lo, double hi, int above_hi, int mt,
-                      double ep0, double ep1, double *out, hipStream_t s);
+                      double ep0, double ep1, double epm, double *out, hipStream_t s);
 int launch_tri_inverr(const double *al, const double *be, int m, const double *xs, int nx, const double *g,
                       double eps3, double eps4, double beta_m1, int maxit, double *sv, double *sm, unsigned char *sf,
                       double *err, hipStream_t s);
-int cw_lancbis_device(int niter, const double *alpha, const double *beta, double emin, double emax,
+// mtol: lower bound of BISEC's multiplicity interval half-width (0 = BISEC)
+int cw_lancbis_device(int niter, const double *alpha, const double *beta, double emin, double emax, double mtol,
                       hipStream_t stream, double *eval, double *err, int *ngood);
 int cw_lancbis(int niter, const double *alpha, const double *beta, const double *beta2, double emin, double emax,
-               double *eval, double *err, int *ngood);
+               double mtol, double *eval, double *err, int *ngood);
 int cw_trivec(int niter, const double *alpha, const double *beta, const double *beta2, const double *eval,
-              int ngood, double *evtr, int *mamin);
+              int ngood, double mtol, double *evtr, int *mamin);
 
 // ---------------------------------------------------------------------------
 // device-side launchers (bsl_kernels.hip)
@@ -132,7 +134,6 @@ struct DevOperator {
     int qmode = 0;                // 0 FP64 MFMA, 1 FP32 MFMA, 2 BF16x3 MFMA
     int q_npts = 0;               // grid points per group of the quadrature kernel
     int q_grid = 0;               // its persistent workgroups (resident ones, at most the groups)
-    int *q_rl = nullptr;          // [16][9]: per m (+lmax) the count, then the carried planes
     // reduction scratch: the potential kernel writes nalpha per-workgroup
     // partials of v.R (partial), |R|^2 (partial_b) and |v|^2 (partial_c)
     double *partial = nullptr, *partial_b = nullptr, *partial_c = nullptr;

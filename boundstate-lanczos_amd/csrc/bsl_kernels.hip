@@ -152,7 +152,8 @@ constexpr int kSlabLd = 65;
 // + sum_jp t_y(|j - jp|) v[jp][i], both into the same accumulator tiles;
 // each wave owns a 2 x 2 block of 16 x 16 output tiles.  T: FP64, or FP32
 // (products and accumulation in FP32, the recurrence vectors stay FP64).
-// R = sr * R + mask[n] * (D + rot[n] v)
+// v = ca X + cb R is formed while the slab is staged and written back to X;
+// R = cr R + mask[n] * (D + rot[n] v)
 template <typename T>
 __global__ __launch_bounds__(kWG) void k_kin_xy_mfma(double *__restrict__ X, double *__restrict__ R,
                                                      const double *__restrict__ tx, const double *__restrict__ ty,
@@ -723,7 +724,8 @@ __global__ __launch_bounds__(kWG) void k_start_random(double *X, int64_t n, int6
     if (threadIdx.x == 0) partial[blockIdx.x] = b;
 }
 
-// ritz[g][i] += coef[g] * (sx X[i]) for the vectors still accumulating
+// ritz[g][i] += coef[g] * (ca X[i] + cb R[i]) (the Lanczos vector the next
+// step forms) for the vectors still accumulating
 __global__ void k_ritz_accum(double *ritz, const double *__restrict__ X, const double *__restrict__ R,
                              const Scal *__restrict__ sc, const double *__restrict__ coef, int64_t n, int ngood)
 {
@@ -843,7 +845,7 @@ int launch_start_vector(double *X, int64_t n, int64_t P, const int *plane_ref, i
                         double *partial, int npartial, hipStream_t s)
 {
     if (kind == BSL_START_UNIFORM) {
-        // X = 1, sx = 1/sqrt(N): sx*X is the reference's 1./sqrt((double)N) exactly
+        // X = 1, ca = 1/sqrt(N): ca*X is the reference's 1./sqrt((double)N) exactly
         hipLaunchKernelGGL(k_fill, dim3(grid_for(n)), dim3(kWG), 0, s, X, n, 1.0);
         BSL_HIP(hipGetLastError());
         return launch_init_scal(sc, 1. / std::sqrt((double)n), 0.0, s);

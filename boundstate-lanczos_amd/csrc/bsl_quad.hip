@@ -172,8 +172,7 @@ __device__ inline typename QuadTypes<MODE>::acc load_w(const typename QuadTypes<
 
 // W: [tile][NT][64][4] (tile = global 16-column tile, NT phi tiles, the
 // ones beyond nphi zero; see k_weight_grid); Lg: [nk][na]; Sg: [nm][nphi];
-// mplane[r]: m + lmax of carried plane r; qrl[m*9]: number of carried planes
-// of m, qrl[m*9 + 1 + u]: those planes.  MODE 0: FP64 MFMA; 1: FP32 MFMA;
+// mplane[r]: m + lmax of carried plane r.  MODE 0: FP64 MFMA; 1: FP32 MFMA;
 // 2: BF16x3 MFMA (W in FP32).  KS: forward K steps of 4 m (m padded to
 // 4*KS with zeros).  W of the next column tile is prefetched into NT register
 // tiles while the current one is transformed.
@@ -181,7 +180,7 @@ template <int MODE, int NT, int KS>
 __global__ __launch_bounds__(kQWG) void k_quad_mfma(const double *__restrict__ X, double *__restrict__ R,
                                                     const void *__restrict__ Wv, const double *__restrict__ Lg,
                                                     const double *__restrict__ Sg, const int *__restrict__ mplane,
-                                                    const int *__restrict__ qrl, const double *__restrict__ mask,
+                                                    const double *__restrict__ mask,
                                                     const Scal *__restrict__ sc, int64_t P, int nk, int na,
                                                     int nphi, int nm, int ppw, int64_t ngroups,
                                                     double *__restrict__ partial, double *__restrict__ partial_b,
@@ -214,7 +213,6 @@ __global__ __launch_bounds__(kQWG) void k_quad_mfma(const double *__restrict__ X
         const int msel = (MODE == 1) ? 4 * (row & 3) + (row >> 2) : row;  // FP64 D rows kq + 4i, FP32 4kq + i
         Sel[t] = (r < nk && mplane[r] == msel) ? (T)1 : (T)0;
     }
-    (void)qrl;
     if constexpr (MODE == 2) {
         short *fh = (short *)(smem + q.off_s), *fl = fh + 16 * NT * 16;
         short *bh = (short *)(smem + q.off_b), *bl = bh + 16 * q.bp;
@@ -506,7 +504,7 @@ int launch_weight_grid(const double *vg, void *W, int wprec, const double *wt, c
                 *occ = 0;                                                                                  \
         } else {                                                                                           \
             hipLaunchKernelGGL((k_quad_mfma<M, NTV, KSV>), dim3(grid), dim3(kQWG), lds, s, X, R, op.wgrid,  \
-                               op.qL, op.qS, op.mplane, op.q_rl, op.mask, sc, op.P, op.nk, op.na, op.nphi,   \
+                               op.qL, op.qS, op.mplane, op.mask, sc, op.P, op.nk, op.na, op.nphi,   \
                                op.nm, ppw, ngroups, op.partial, op.partial_b, op.partial_c);               \
         }                                                                                                  \
     } while (0)

@@ -75,8 +75,14 @@ void sturm_backward(const Tri &t, double x, int *count, int *last_negative)
 
 // BISEC (bisec.f:4-225): distinct eigenvalues of T in (lo, hi], ascending,
 // with multiplicities (0 = spurious).  Returns -1 when the Sturm budget is
-// exceeded (the reference STOPs).
-int bisect(Tri &t, double lo, double hi, std::vector<double> &vs, std::vector<int> &mp)
+// exceeded (the reference STOPs).  mtol > 0 (no reference counterpart)
+// widens the interval [xl - ep0, xu + ep0] in which T's eigenvalues count as
+// one multiple eigenvalue to at least mtol, while the spurious test of a
+// simple eigenvalue keeps BISEC's ep0: with a reduced-precision operator the
+// ghost copies of a converged level are spread by its rounding rather than
+// by FP64's, and at BISEC's FP64-sized width each copy would be a simple
+// eigenvalue of both T and T-hat, i.e. spurious.  mtol = 0 is BISEC itself.
+int bisect(Tri &t, double lo, double hi, double mtol, std::vector<double> &vs, std::vector<int> &mp)
 {
     const int m = t.m;
     double tkmax = 0.0;
@@ -84,6 +90,7 @@ int bisect(Tri &t, double lo, double hi, std::vector<double> &vs, std::vector<in
     const double ttol = kEps2 * tkmax;
     const double scale = (double)(m + 1000);
     const double ep0 = scale * ttol, ep1 = std::sqrt(scale) * ttol;
+    const double epm = std::fmax(ep0, mtol);  // multiplicity width
     t.be[m] = 0.0;  // BETA(MEV+1) is taken as zero during the search
     for (int i = 0; i <= m; i++) t.b2[i] = t.be[i] * t.be[i];
 
@@ -117,22 +124,30 @@ int bisect(Tri &t, double lo, double hi, std::vector<double> &vs, std::vector<in
         vs.push_back(x1);
         // spurious test: compare T and T-hat counts just outside the interval
         int kl, jl, ku, ju;
-        sturm_backward(t, xl - ep0, &kl, &jl);
+        sturm_backward(t, xl - epm, &kl, &jl);
         kl -= above_hi;
         jl = kl - jl;
-        x1 = xu + ep0;
+        x1 = xu + epm;
         sturm_backward(t, x1, &ku, &ju);
         ku -= above_hi;
         ju = ku - ju;
         int mult, knew;
         if (kl - ku - 1 == 0) {
+            if (epm > ep0) {
+                // simple at the multiplicity width: T-hat counts at BISEC's own
+                int c0, l0;
+                sturm_backward(t, xl - ep0, &c0, &l0);
+                jl = c0 - above_hi - l0;
+                sturm_backward(t, xu + ep0, &c0, &l0);
+                ju = c0 - above_hi - l0;
+            }
             mult = (ju < jl) ? 0 : 1;
             knew = k - 1;
         } else {
             if (ku != nu) {
                 int change;
                 do {
-                    x1 = x1 + ep0;
+                    x1 = x1 + epm;
                     const int nev = cnt(x1);
                     change = ku - nev;
                     ku = nev;
@@ -259,7 +274,7 @@ void error_estimates(Tri &t, double beta_m1, const std::vector<double> &vs, cons
 }  // namespace
 
 int cw_lancbis(int niter, const double *alpha, const double *beta, const double *beta2, double emin, double emax,
-               double *eval, double *err, int *ngood)
+               double mtol, double *eval, double *err, int *ngood)
 {
     (void)beta2;  // BISEC recomputes BETA**2 (bisec.f:91-92)
     if (niter <= 0) return fail(BSL_E_ARG, "lancbis: niter must be positive");
@@ -268,7 +283,7 @@ int cw_lancbis(int niter, const double *alpha, const double *beta, const double 
     const double beta_m1 = beta[niter];
     std::vector<double> vs, es;
     std::vector<int> mp;
-    if (bisect(t, emin, emax, vs, mp) < 0)
+    if (bisect(t, emin, emax, mtol, vs, mp) < 0)
         return fail(BSL_E_STURM, "BISEC: estimated number of Sturm sequences exceeds its limit");
     t.be[niter] = beta_m1;
     isolate(vs, mp);
@@ -290,7 +305,7 @@ int cw_lancbis(int niter, const double *alpha, const double *beta, const double 
 // BISEC and ISOEV stay here.  Eigenvalues agree with cw_lancbis to the
 // bisection tolerance (each index is bisected from the full interval rather
 // than from the bounds the previous indices refined).
-int cw_lancbis_device(int niter, const double *alpha, const double *beta, double emin, double emax,
+int cw_lancbis_device(int niter, const double *alpha, const double *beta, double emin, double emax, double mtol,
                       hipStream_t stream, double *eval, double *err, int *ngood)
 {
     if (niter <= 0) return fail(BSL_E_ARG, "lancbis: niter must be positive");
@@ -302,6 +317,7 @@ int cw_lancbis_device(int niter, const double *alpha, const double *beta, double
     for (int i = 1; i <= m; i++) tkmax = std::fmax(std::fmax(std::fabs(t.a(i)), t.b(i)), tkmax);
     const double ttol = kEps2 * tkmax, scale = (double)(m + 1000);
     const double ep0 = scale * ttol, ep1 = std::sqrt(scale) * ttol;
+    const double epm = std::fmax(ep0, mtol);  // multiplicity width, as in bisect()
     t.be[m] = 0.0;
     for (int i = 0; i <= m; i++) t.b2[i] = t.be[i] * t.be[i];
     const int above_hi = sturm_forward(t, emax);
@@ -321,15 +337,15 @@ int cw_lancbis_device(int niter, const double *alpha, const double *beta, double
     };
     hipError_t e = hipMalloc(&d_al, sizeof(double) * m);
     if (e == hipSuccess) e = hipMalloc(&d_be, sizeof(double) * (m + 1));
-    if (e == hipSuccess) e = hipMalloc(&d_out, sizeof(double) * 8 * (size_t)mt);
+    if (e == hipSuccess) e = hipMalloc(&d_out, sizeof(double) * kTriOut * (size_t)mt);
     if (e == hipSuccess) e = hipMemcpyAsync(d_al, alpha, sizeof(double) * m, hipMemcpyHostToDevice, stream);
     if (e == hipSuccess) e = hipMemcpyAsync(d_be, bev.data(), sizeof(double) * (m + 1), hipMemcpyHostToDevice, stream);
     if (e != hipSuccess) {
         cleanup();
         return hip_fail(e, "lancbis device buffers", __FILE__, __LINE__);
     }
-    int rc = launch_tri_bisect(d_al, d_be, m, emin, emax, above_hi, mt, ep0, ep1, d_out, stream);
-    std::vector<double> res(8 * (size_t)mt);
+    int rc = launch_tri_bisect(d_al, d_be, m, emin, emax, above_hi, mt, ep0, ep1, epm, d_out, stream);
+    std::vector<double> res(kTriOut * (size_t)mt);
     if (!rc) {
         e = hipMemcpyAsync(res.data(), d_out, sizeof(double) * res.size(), hipMemcpyDeviceToHost, stream);
         if (e == hipSuccess) e = hipStreamSynchronize(stream);
@@ -345,23 +361,23 @@ int cw_lancbis_device(int niter, const double *alpha, const double *beta, double
     std::vector<int> mp;
     int k = mt;
     while (k > 0) {
-        const double *r = res.data() + 8 * (size_t)(k - 1);
+        const double *r = res.data() + kTriOut * (size_t)(k - 1);
         double x1 = r[0];
         const double xu = r[2];
-        const int nu = (int)r[3], kl = (int)r[4], jl = (int)r[5];
-        int ku = (int)r[6];
-        const int ju = (int)r[7];
+        const int nu = (int)r[3], jl = (int)r[5], ju = (int)r[7];  // T-hat counts at ep0
+        const int kl = (int)r[8];                                   // T counts at epm
+        int ku = (int)r[9];
         vs.push_back(x1);
         int mult, knew;
         if (kl - ku - 1 == 0) {
             mult = (ju < jl) ? 0 : 1;
             knew = k - 1;
         } else {
-            x1 = xu + ep0;
+            x1 = xu + epm;
             if (ku != nu) {
                 int change;
                 do {
-                    x1 = x1 + ep0;
+                    x1 = x1 + epm;
                     const int nev = cnt(x1);
                     change = ku - nev;
                     ku = nev;
@@ -437,13 +453,17 @@ int cw_lancbis_device(int niter, const double *alpha, const double *beta, double
 namespace {
 // STURMI (sturmi.f:4-116): smallest orders at which x first appears (mk1)
 // and appears twice (mk2, or m).  Returns false when x is not an eigenvalue.
-bool sturm_orders(const Tri &t, const double *b2, double x, int *mk1, int *mk2)
+// mtol > 0 (no reference counterpart) widens the window x +- tol to at least
+// mtol, the multiplicity width of bisect(): the value of a merged cluster of
+// spread ghost copies is then found from the order at which its first copy
+// converged, not only the order at which the copy bisect() returned did.
+bool sturm_orders(const Tri &t, const double *b2, double x, double mtol, int *mk1, int *mk2)
 {
     const int m = t.m;
     double tkmax = 0.0;
     for (int i = 1; i <= m; i++) tkmax = std::fmax(std::fmax(std::fabs(t.a(i)), t.b(i)), tkmax);
     const double stutol = 5.0 * ((double)(m + 1000) * (kEps2 * tkmax));
-    const double tol = std::fmax(std::fabs(x) * 1e-10, stutol);
+    const double tol = std::fmax(std::fmax(std::fabs(x) * 1e-10, stutol), mtol);
     const double lo = x - tol, hi = x + tol;
     double d1 = 1.0, d2 = 1.0;
     int c1 = 0, c2 = 0, found = 0, diff = 0, i;
@@ -473,11 +493,54 @@ bool sturm_orders(const Tri &t, const double *b2, double x, int *mk1, int *mk2)
     return diff != 0;
 }
 
-// INVERM (inverm.f:4-142): eigenvector of T_order for x; returns |y_order|
-void tm_eigenvector(const Tri &t, double *be, int order, double x, const std::vector<double> &g, double *v,
-                    std::vector<double> &mult, double *error, bool *converged)
+// Eigenvalue of the leading order x order block of T nearest to x within
+// [x - w, x + w] (Sturm bisection), or x when the window holds none.
+double nearest_eigenvalue(const Tri &t, const double *be, int order, double x, double w)
+{
+    // eigenvalues of T_order below s
+    auto below = [&](double s) {
+        int c = 0;
+        double d = 1.0;
+        for (int i = 1; i <= order; i++) {
+            const double b = i > 1 ? be[i - 1] : 0.0;
+            const double q = (d != 0.0) ? (b * b) / d : std::fabs(b) / kEps2;
+            d = t.a(i) - s - q;
+            if (d < 0.0) c++;
+        }
+        return c;
+    };
+    // the k-th eigenvalue (0-based, ascending) inside (lo, hi]
+    auto kth = [&](int k, double lo, double hi) {
+        for (int it = 0; it < 200 && hi - lo > kEps2 * (std::fabs(lo) + std::fabs(hi)); it++) {
+            const double mid = 0.5 * (lo + hi);
+            if (below(mid) > k) hi = mid; else lo = mid;
+        }
+        return 0.5 * (lo + hi);
+    };
+    const int c_lo = below(x - w), c_x = below(x), c_hi = below(x + w);
+    double best = x, dist = 2.0 * w + 1.0;
+    if (c_x > c_lo) {  // the largest one below x
+        const double e = kth(c_x - 1, x - w, x);
+        if (x - e < dist) dist = x - e, best = e;
+    }
+    if (c_hi > c_x) {  // the smallest one above x
+        const double e = kth(c_x, x, x + w);
+        if (e - x < dist) dist = e - x, best = e;
+    }
+    return best;
+}
+
+// INVERM (inverm.f:4-142): eigenvector of T_order for x; returns |y_order|.
+// shift_w > 0 (no reference counterpart, the multiplicity width of a
+// mixed-precision eigensolve): the shift is T_order's own eigenvalue nearest
+// to x within shift_w, because x, one copy of a merged cluster, can lie
+// farther from the copy T_order holds than the inverse iteration's
+// convergence test (a 1-norm of 1 from eps4) tolerates.
+void tm_eigenvector(const Tri &t, double *be, int order, double x, double shift_w, const std::vector<double> &g,
+                    double *v, std::vector<double> &mult, double *error, bool *converged)
 {
     const int maxit = 10;
+    if (shift_w > 0.0) x = nearest_eigenvalue(t, be, order, x, shift_w);
     const double keep = be[order];
     be[order] = 0.0;
     double tsum = std::fabs(t.a(1));
@@ -499,7 +562,7 @@ void tm_eigenvector(const Tri &t, double *be, int order, double x, const std::ve
 // reference, entries of a column beyond the chosen order keep the values of
 // larger trial orders (INVERM writes only its own order).
 int cw_trivec(int niter, const double *alpha, const double *beta, const double *beta2, const double *eval,
-              int ngood, double *evtr, int *mamin)
+              int ngood, double mtol, double *evtr, int *mamin)
 {
     if (niter <= 0 || ngood < 0) return fail(BSL_E_ARG, "trivec: bad sizes");
     Tri t{niter, alpha, std::vector<double>(niter + 2, 0.0), std::vector<double>()};
@@ -512,8 +575,9 @@ int cw_trivec(int niter, const double *alpha, const double *beta, const double *
     for (int ig = 0; ig < ngood; ig++) {
         mamin[ig] = 0;
         int mk1, mk2;
-        if (!sturm_orders(t, b2.data(), eval[ig], &mk1, &mk2))
+        if (!sturm_orders(t, b2.data(), eval[ig], mtol, &mk1, &mk2))
             return fail(BSL_E_NOTFOUND, "STURMI: eigenvalue not found");
+        const double shift_w = mtol > 0.0 ? std::fmax(mtol, std::fabs(eval[ig]) * 1e-10) : 0.0;
         int order, last, step;
         if (mk2 == niter) {
             order = std::min((5 * mk1) / 4 + 1, niter);
@@ -530,7 +594,7 @@ int cw_trivec(int niter, const double *alpha, const double *beta, const double *
         for (int tries = 1;; tries++) {
             double e;
             bool conv;
-            tm_eigenvector(t, t.be.data(), order, eval[ig], g, col, mult, &e, &conv);
+            tm_eigenvector(t, t.be.data(), order, eval[ig], shift_w, g, col, mult, &e, &conv);
             any = any || conv;
             if (e <= best) {
                 best = e;
@@ -542,7 +606,7 @@ int cw_trivec(int niter, const double *alpha, const double *beta, const double *
         if (!any) return fail(BSL_E_NOCONV, "TRIVEC: inverse iteration never converged");
         double e;
         bool conv;
-        tm_eigenvector(t, t.be.data(), mamin[ig], eval[ig], g, col, mult, &e, &conv);
+        tm_eigenvector(t, t.be.data(), mamin[ig], eval[ig], shift_w, g, col, mult, &e, &conv);
     }
     return BSL_OK;
 }

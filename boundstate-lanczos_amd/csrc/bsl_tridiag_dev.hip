@@ -54,9 +54,13 @@ __device__ void sturm_backward_d(const double *__restrict__ al, const double *__
 }
 
 // thread k (1-based, counted from the top of (lo, hi]): bisection, then the
-// counts of the spurious test.  out[k-1][0..7] = x1, xl, xu, nu, kl, jl, ku, ju.
+// counts of the spurious test.  out[k-1][0..9] = x1, xl, xu, nu, kl, jl, ku,
+// ju (T and T-hat counts at xl - ep0 and xu + ep0), klm, kum (T counts at
+// xl - epm and xu + epm, the multiplicity width; equal to kl, ku when
+// epm = ep0).
 __global__ void k_tri_bisect(const double *__restrict__ al, const double *__restrict__ be, int m, double lo,
-                             double hi, int above_hi, int mt, double ep0, double ep1, double *__restrict__ out)
+                             double hi, int above_hi, int mt, double ep0, double ep1, double epm,
+                             double *__restrict__ out)
 {
     const int k = blockIdx.x * blockDim.x + threadIdx.x + 1;
     if (k > mt) return;
@@ -81,7 +85,15 @@ __global__ void k_tri_bisect(const double *__restrict__ al, const double *__rest
     sturm_backward_d(al, be, m, xu + ep0, &ku, &ju);
     ku -= above_hi;
     ju = ku - ju;
-    double *o = out + (size_t)(k - 1) * 8;
+    int klm = kl, kum = ku;
+    if (epm > ep0) {
+        int l;
+        sturm_backward_d(al, be, m, xl - epm, &klm, &l);
+        klm -= above_hi;
+        sturm_backward_d(al, be, m, xu + epm, &kum, &l);
+        kum -= above_hi;
+    }
+    double *o = out + (size_t)(k - 1) * kTriOut;
     o[0] = x1;
     o[1] = xl;
     o[2] = xu;
@@ -90,6 +102,8 @@ __global__ void k_tri_bisect(const double *__restrict__ al, const double *__rest
     o[5] = jl;
     o[6] = ku;
     o[7] = ju;
+    o[8] = klm;
+    o[9] = kum;
 }
 
 // INVERR's inverse iteration for eigenvalue xs[j], thread j: (T - x) y = v
@@ -163,11 +177,11 @@ __global__ void k_tri_inverr(const double *__restrict__ al, const double *__rest
 }  // namespace
 
 int launch_tri_bisect(const double *al, const double *be, int m, double lo, double hi, int above_hi, int mt,
-                      double ep0, double ep1, double *out, hipStream_t s)
+                      double ep0, double ep1, double epm, double *out, hipStream_t s)
 {
     if (mt <= 0) return BSL_OK;
     hipLaunchKernelGGL(k_tri_bisect, dim3((mt + 63) / 64), dim3(64), 0, s, al, be, m, lo, hi, above_hi, mt, ep0,
-                       ep1, out);
+                       ep1, epm, out);
     BSL_HIP(hipGetLastError());
     return BSL_OK;
 }

@@ -146,8 +146,8 @@ int main(int argc, char **argv)
     // ---- good eigenvalues (:251-295) ----
     std::vector<double> eval(niter), evalerr(niter);
     int32_t ngood = 0;
-    check(bsl_lancbis(niter, alpha.data(), beta.data(), beta2.data(), emin, emax, eval.data(), evalerr.data(),
-                      &ngood),
+    check(bsl_lancbis_tol(niter, alpha.data(), beta.data(), beta2.data(), emin, emax,
+                          bsl_plugin_mult_tol(general_data), eval.data(), evalerr.data(), &ngood),
           "lancbis");
     cout << endl << "Number of Good Eigenvalues is: " << ngood << endl;
     if (ngood > 0) {
@@ -181,7 +181,8 @@ int main(int argc, char **argv)
     // ---- Tm eigenvectors and lv (:329-364) ----
     std::vector<double> evtr((size_t)niter * std::max(ngood, 1));
     std::vector<int32_t> mamin(std::max(ngood, 1));
-    check(bsl_trivec(niter, alpha.data(), beta.data(), beta2.data(), eval.data(), ngood, evtr.data(), mamin.data()),
+    check(bsl_trivec_tol(niter, alpha.data(), beta.data(), beta2.data(), eval.data(), ngood,
+                         bsl_plugin_mult_tol(general_data), evtr.data(), mamin.data()),
           "trivec");
     {
         std::ofstream lv((dir + "/lv").c_str());

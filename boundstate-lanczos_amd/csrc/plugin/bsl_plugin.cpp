@@ -161,6 +161,11 @@ bsl_ctx *bsl_plugin_context(generalStor *general_data)
     return reinterpret_cast<bslPluginStor *>(general_data)->ctx;
 }
 
+double bsl_plugin_mult_tol(generalStor *general_data)
+{
+    return reinterpret_cast<bslPluginStor *>(general_data)->precision == BSL_PREC_FP64 ? 0.0 : BSL_MIXED_MULT_TOL;
+}
+
 void Hv_Prep_linRotCartSph_HIP(int argc, char **argv, generalStor *general_data, lanczosStor *lanczos)
 {
     bslPluginStor *st = reinterpret_cast<bslPluginStor *>(general_data);

@@ -138,10 +138,34 @@ int bsl_lancbis(int32_t niter, const double *alpha, const double *beta, const do
 int bsl_lancbis_device(int32_t niter, const double *alpha, const double *beta, const double *beta2, double emin,
                        double emax, double *eval, double *err, int32_t *ngood);
 
+/* bsl_lancbis and bsl_lancbis_device with the half-width of BISEC's
+ * multiplicity interval (EP0, bisec.f:66-95) raised to at least mult_tol
+ * (energy units of alpha): eigenvalues of T_m closer than that are one
+ * multiple, hence good, eigenvalue.  For the mixed-precision calculators,
+ * whose operator rounding spreads the ghost copies of a converged level by
+ * more than FP64's EP0, so that BISEC would find each copy in T-hat as well
+ * and drop the level as spurious.  mult_tol = 0 is bsl_lancbis itself.  No
+ * reference counterpart (the reference runs an FP64 operator only). */
+/* The multiplicity tolerance the mixed-precision calculators use (kJ/mol):
+ * their rounding spreads the ghost copies of a converged level by ~1e-8
+ * relative; 1e-6 kJ/mol (8e-5 cm^-1) merges them and stays two orders below
+ * the 0.01 cm^-1 accuracy of mixed levels. */
+#define BSL_MIXED_MULT_TOL 1.0e-6
+int bsl_lancbis_tol(int32_t niter, const double *alpha, const double *beta, const double *beta2, double emin,
+                    double emax, double mult_tol, double *eval, double *err, int32_t *ngood);
+int bsl_lancbis_device_tol(int32_t niter, const double *alpha, const double *beta, const double *beta2,
+                           double emin, double emax, double mult_tol, double *eval, double *err, int32_t *ngood);
+
 /* Tm eigenvectors (lanczosvectors, lanczosEigenValuesVectors.cpp:509-561 ->
  * TRIVEC trivec.f:4-95).  evtr column-major [niter][ngood]. */
 int bsl_trivec(int32_t niter, const double *alpha, const double *beta, const double *beta2, const double *eval,
                int32_t ngood, double *evtr, int32_t *mamin);
+/* bsl_trivec with STURMI's window (sturmi.f:4-116) widened to at least
+ * mult_tol, for the values bsl_lancbis_tol returned with the same mult_tol:
+ * the order search then starts where the first of the merged copies
+ * converged.  mult_tol = 0 is bsl_trivec itself.  No reference counterpart. */
+int bsl_trivec_tol(int32_t niter, const double *alpha, const double *beta, const double *beta2, const double *eval,
+                   int32_t ngood, double mult_tol, double *evtr, int32_t *mamin);
 
 /* Lanczos loop 2 (lanczosEigenValuesVectors.cpp:371-469): recomputes the
  * Krylov vectors from the same start and accumulates the Ritz vectors on

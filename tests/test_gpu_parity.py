@@ -258,6 +258,38 @@ def test_fp32_levels_within_target():
         assert d < 0.01, (e, d)
 
 
+def test_mixed_levels_match_fp64_one_to_one():
+    """A long FP32 run (3000 steps on a 3456-dimensional operator, many ghost
+    copies): with the mixed calculators' multiplicity tolerance every
+    converged FP64 level has an FP32 counterpart within 0.01 cm^-1 (no copy
+    dropped as spurious), and a second counterpart only where a copy is still
+    converging, which its own error estimate (>= 1e-3) marks.  Without the
+    tolerance the same coefficients report the ghost copies as hundreds of
+    extra levels."""
+    from bsl_amd import physics
+    spec, kw = cs.cage_case(6, 3, 8, 8, "All")
+    c64 = B.Calculator(**kw)
+    c32 = B.Calculator(precision="fp32", **kw)
+    assert c64.mult_tol == 0.0 and c32.mult_tol == B.MIXED_MULT_TOL
+    e64, r64 = B.lancbis(*c64.lanczos(3000, start="random", seed=5), -10.0, 10.0)
+    e32, r32 = B.lancbis(*c32.lanczos(3000, start="random", seed=5), -10.0, 10.0, mult_tol=c32.mult_tol)
+    conv = e64[np.abs(r64) < 1e-8]
+    assert len(conv) >= 20
+    extra = 0
+    for e in conv:
+        near = np.abs(e32 - e) / physics.KJMOL_PER_CM1 < 0.01
+        assert near.sum() >= 1, e
+        if near.sum() > 1:
+            err = np.sort(np.abs(r32[near]))
+            assert np.all(err[1:] >= 1e-3), (e, e32[near], r32[near])
+            extra += near.sum() - 1
+    assert extra <= 0.02 * len(conv)
+    e0, _ = B.lancbis(*c32.lanczos(3000, start="random", seed=5), -10.0, 10.0)
+    assert len(e0) > 1.5 * len(e64)  # BISEC's FP64 width keeps the spread copies apart
+    ed, _ = B.lancbis(*c32.lanczos(3000, start="random", seed=5), -10.0, 10.0, mult_tol=c32.mult_tol, device=True)
+    assert len(ed) == len(e32) and np.max(np.abs(ed - e32)) < 1e-10
+
+
 def test_reduced_basis_drops_spurious_zero():
     """para (Even) block: the reduced basis reproduces the masked full-basis
     levels without the spurious 0 of the masked subspace.

@@ -83,6 +83,102 @@ def test_lancbis_on_a_real_lanczos_run_identical():
     assert np.array_equal(e1, e2) and np.array_equal(r1, r2)
 
 
+def test_lancbis_tol_zero_is_lancbis():
+    """bsl_lancbis_tol with mult_tol = 0 is the reference's BISEC (bitwise
+    the oracle's restatement)."""
+    spec, _ = cs.cage_case(5, 2, 5, 5, "All")
+    a, b, b2 = O.Oracle(spec).lanczos(300)
+    e1, r1 = O.lancbis(a, b, b2, -10.0, 10.0)
+    e2, r2 = B.lancbis(a, b, b2, -10.0, 10.0, mult_tol=0.0)
+    assert np.array_equal(e1, e2) and np.array_equal(r1, r2)
+    from bsl_amd._lib import lib
+    import ctypes as C
+    ng = C.c_int32()
+    ev, er = np.zeros(len(a)), np.zeros(len(a))
+    assert lib().bsl_lancbis_tol(len(a), a, b, b2, -10.0, 10.0, -1.0, ev, er, C.byref(ng)) != 0
+
+
+def _dense_operator(spec):
+    o = O.Oracle(spec)
+    h = np.zeros((o.N, o.N))
+    e = np.zeros(o.N)
+    for j in range(o.N):
+        e[j] = 1.0
+        h[:, j] = o.hv(e)
+        e[j] = 0.0
+    return 0.5 * (h + h.T)
+
+
+def _noisy_lanczos(h, niter, noise, seed):
+    """Plain Lanczos whose Hv carries a relative rounding of `noise` per
+    application (what a reduced-precision operator does), FP64 otherwise."""
+    rng = np.random.default_rng(seed)
+    n = h.shape[0]
+    v, vp, beta = np.ones(n) / np.sqrt(n), np.zeros(n), 0.0
+    a, b = np.zeros(niter), np.zeros(niter + 1)
+    for j in range(niter):
+        u = h @ v
+        u = u + noise * np.abs(u) * rng.standard_normal(n)
+        r = u - beta * vp
+        a[j] = v @ r
+        r -= a[j] * v
+        beta = np.linalg.norm(r)
+        b[j + 1] = beta
+        vp, v = v, r / beta
+    return a, b
+
+
+def test_lancbis_tol_merges_noisy_ghosts():
+    """With a noisy operator (relative 1e-7 per Hv) the ghost copies of the
+    converged levels are spread beyond BISEC's FP64 width: mult_tol = 0
+    reports them as extra good levels; mult_tol = 1e-6 (the mixed
+    calculators' tolerance) returns exactly the operator's eigenvalues in the
+    window, each within 1e-5."""
+    h = _dense_operator(cs.cage_case(4, 2, 6, 6, "All")[0])
+    w = np.linalg.eigvalsh(h)
+    inside = w[(w > -10.0) & (w <= 10.0)]
+    a, b = _noisy_lanczos(h, 3 * h.shape[0], 1e-7, 0)
+    e0, _ = B.lancbis(a, b, b * b, -10.0, 10.0)
+    assert len(e0) > 2 * len(inside)  # the failure mode, for the record
+    e1, _ = B.lancbis(a, b, b * b, -10.0, 10.0, mult_tol=B.MIXED_MULT_TOL)
+    assert len(e1) == len(inside)
+    assert np.max(np.abs(np.sort(e1) - inside)) < 1e-5
+
+
+def test_trivec_tol_finds_merged_levels():
+    """TRIVEC for the merged values of a noisy operator (relative 3e-8 per
+    Hv: the copies of a level spread well inside the mixed tolerance).  With
+    STURMI's FP64 window some values are found only at orders where several
+    copies coexist and the inverse iteration never converges; with the same
+    mult_tol as the eigensolve every value gets a T_MA eigenvector (shifted
+    by T_MA's own copy), with a residual at rounding level and a Rayleigh
+    quotient within the tolerance of the value."""
+    h = _dense_operator(cs.cage_case(4, 2, 6, 6, "All")[0])
+    failed = 0
+    for seed in range(4):
+        a, b = _noisy_lanczos(h, 3 * h.shape[0], 3e-8, seed)
+        x, _ = B.lancbis(a, b, b * b, -10.0, 10.0, mult_tol=B.MIXED_MULT_TOL)
+        x = x[:30]
+        for v in x:
+            try:
+                B.trivec(a, b, b * b, np.array([v]))
+            except B.BslError:
+                failed += 1
+        evtr, mam = B.trivec(a, b, b * b, x, mult_tol=B.MIXED_MULT_TOL)
+        m = len(a)
+        for g in range(len(x)):
+            k = mam[g]
+            assert 0 < k <= m
+            y = evtr[g * m: g * m + k]
+            ty = a[:k] * y
+            ty[:-1] += b[1:k] * y[1:]
+            ty[1:] += b[1:k] * y[:-1]
+            lam = y @ ty / (y @ y)
+            assert np.linalg.norm(ty - lam * y) / np.linalg.norm(y) < 1e-10 * np.max(np.abs(a))
+            assert abs(lam - x[g]) < 2 * B.MIXED_MULT_TOL
+    assert failed > 0  # the failure mode, for the record
+
+
 def test_water_sites_identical_to_oracle():
     cm_ea = cs.cage11_cm_ea()
     pos_o, q_o, typ = O.build_sites(cm_ea, 0)

@@ -100,8 +100,9 @@ class Calculator:
     @property
     def mult_tol(self):
         """Multiplicity tolerance of the eigensolve for this operator: 0 (the
-        reference's BISEC) for FP64, MIXED_MULT_TOL for the mixed precisions."""
-        return 0.0 if self.params.precision == PREC["fp64"] else MIXED_MULT_TOL
+        reference's BISEC) for FP64, MIXED_MULT_TOL for FP32 and
+        BF16X3_MULT_TOL for BF16x3."""
+        return {PREC["fp64"]: 0.0, PREC["fp32"]: MIXED_MULT_TOL}.get(self.params.precision, BF16X3_MULT_TOL)
 
     def levels(self, niter, count, emin=-1.0e4, emax=1.0e4, start="random", seed=0, tol=1e-8):
         """The `count` lowest converged good levels: Loop 1, Cullum-Willoughby,
@@ -275,13 +276,16 @@ def multiplets(energies, tol=1e-6):
     return groups
 
 
-# Multiplicity tolerance (kJ/mol) of the eigensolve for the mixed-precision
-# operators (BSL_MIXED_MULT_TOL of bsl_hip.h): their rounding spreads the
-# ghost copies of a converged level by ~1e-8 relative (3e-8 kJ/mol at
-# 54^3 x 36, FP32), far beyond FP64's BISEC width; 1e-6 kJ/mol (8e-5 cm^-1)
-# merges them and stays two orders below the 0.01 cm^-1 accuracy of the
-# mixed levels.
+# Multiplicity tolerances (kJ/mol) of the eigensolve for the mixed-precision
+# operators (BSL_MIXED_MULT_TOL, BSL_BF16X3_MULT_TOL of bsl_hip.h): their
+# rounding spreads the ghost copies of a converged level far beyond FP64's
+# BISEC width.  FP32: ~1e-8 relative (3e-8 kJ/mol at 54^3 x 36); 1e-6 kJ/mol
+# (8e-5 cm^-1) merges them, two orders below the 0.01 cm^-1 accuracy of the
+# mixed levels.  BF16x3 (two-term bf16 splits, ~16 significant bits): ~1e-6
+# relative (6.6e-6 kJ/mol measured); 3e-5 kJ/mol (2.5e-3 cm^-1), the level
+# accuracy of that precision.
 MIXED_MULT_TOL = 1.0e-6
+BF16X3_MULT_TOL = 3.0e-5
 
 
 def lancbis(alpha, beta, beta2, emin, emax, device=False, mult_tol=0.0):

@@ -163,7 +163,8 @@ bsl_ctx *bsl_plugin_context(generalStor *general_data)
 
 double bsl_plugin_mult_tol(generalStor *general_data)
 {
-    return reinterpret_cast<bslPluginStor *>(general_data)->precision == BSL_PREC_FP64 ? 0.0 : BSL_MIXED_MULT_TOL;
+    const int prec = reinterpret_cast<bslPluginStor *>(general_data)->precision;
+    return prec == BSL_PREC_FP64 ? 0.0 : prec == BSL_PREC_BF16X3 ? BSL_BF16X3_MULT_TOL : BSL_MIXED_MULT_TOL;
 }
 
 void Hv_Prep_linRotCartSph_HIP(int argc, char **argv, generalStor *general_data, lanczosStor *lanczos)

@@ -63,5 +63,6 @@ void Hv_linRotCartSph_HIP(int argc, char **argv, generalStor *general_data, lanc
 // the prepared context behind a storage struct from bsl_HvInterfaceSetup
 bsl_ctx *bsl_plugin_context(generalStor *general_data);
 // multiplicity tolerance of the eigensolve for the calculator's operator:
-// 0 (the reference's BISEC) for FP64, BSL_MIXED_MULT_TOL for the mixed ones
+// 0 (the reference's BISEC) for FP64, BSL_MIXED_MULT_TOL for FP32 and
+// BSL_BF16X3_MULT_TOL for BF16x3
 double bsl_plugin_mult_tol(generalStor *general_data);

@@ -146,11 +146,14 @@ int bsl_lancbis_device(int32_t niter, const double *alpha, const double *beta, c
  * more than FP64's EP0, so that BISEC would find each copy in T-hat as well
  * and drop the level as spurious.  mult_tol = 0 is bsl_lancbis itself.  No
  * reference counterpart (the reference runs an FP64 operator only). */
-/* The multiplicity tolerance the mixed-precision calculators use (kJ/mol):
- * their rounding spreads the ghost copies of a converged level by ~1e-8
- * relative; 1e-6 kJ/mol (8e-5 cm^-1) merges them and stays two orders below
- * the 0.01 cm^-1 accuracy of mixed levels. */
+/* The multiplicity tolerances the mixed-precision calculators use (kJ/mol).
+ * FP32: its rounding spreads the ghost copies of a converged level by ~1e-8
+ * relative; 1e-6 kJ/mol (8e-5 cm^-1) merges them, two orders below the
+ * 0.01 cm^-1 accuracy of mixed levels.  BF16x3 (two-term bf16 splits, about
+ * 16 significant bits): copies spread by ~1e-6 relative; 3e-5 kJ/mol
+ * (2.5e-3 cm^-1), the level accuracy of that precision. */
 #define BSL_MIXED_MULT_TOL 1.0e-6
+#define BSL_BF16X3_MULT_TOL 3.0e-5
 int bsl_lancbis_tol(int32_t niter, const double *alpha, const double *beta, const double *beta2, double emin,
                     double emax, double mult_tol, double *eval, double *err, int32_t *ngood);
 int bsl_lancbis_device_tol(int32_t niter, const double *alpha, const double *beta, const double *beta2,

@@ -113,6 +113,28 @@ def test_driver_reproduces_saved_run(tmp_path, path):
     assert abs(np.linalg.norm(vec[:, 1]) - 1.0) < 1e-6
 
 
+@pytest.mark.gpu
+@pytest.mark.parametrize("calc", ["linRotCartSph_HIP_NoQuad_ContGrid_Alavi_SPCE_FP32",
+                                  "linRotCartSph_HIP_Quad_ContGrid_Alavi_SPCE_BF16X3"])
+def test_driver_mixed_precision_long_run(tmp_path, calc):
+    """The saved run's decks through the mixed-precision calculators for 1000
+    steps (ghost copies of the converged levels exist by then): the driver's
+    eigensolve uses the mixed multiplicity tolerance, so each of the saved
+    run's 10 lowest levels appears once in boundstates.txt, within
+    0.01 cm^-1."""
+    from bsl_amd import physics
+    t = str(tmp_path)
+    r = run([lanczos_deck(t, niter=1000, calc=calc), hv_deck(t)], t)
+    assert r.returncode == 0, r.stderr[-2000:]
+    d = os.path.join(t, [x for x in os.listdir(t) if x.startswith("DriverTest_")][0])
+    got = _read_boundstates(d)[:, 0]
+    ref = np.array(json.load(open(os.path.join(cs.GOLDEN, "runs.json")))
+                   ["SmallBasisTest2_2013-09-03_19_06_47"]["boundstates"])[:10, 0]
+    for e in ref:
+        near = np.abs(got - e) / physics.KJMOL_PER_CM1 < 0.01
+        assert near.sum() == 1, (e, got[near])
+
+
 def hv_deck_grid(tmp, run):
     shutil.copyfile(os.path.join(cs.GOLDEN, "sII_smallCage11_CM_EA_ZYZ.xyz"), os.path.join(tmp, "cage.xyz"))
     vals = list(run["hv_deck"])

@@ -372,6 +372,7 @@ def main():
             "eig_err_vs_fp64": eig,
             "prep_s": {"fp64": prep64, "fp32": prep32},
             "per_step_ms": {"potential": ms[1], "kinetic": ms[2], "finalize": ms[3]},
+            "hv_per_s": 1e3 / (ms[1] + ms[2]),
             "quadrature": quad,
             "batch_32": batch,
             "adjoint": adj,

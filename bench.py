@@ -271,7 +271,7 @@ def main():
     ap.add_argument("--n", type=int, default=54)
     ap.add_argument("--x-max", type=float, default=0.8)
     ap.add_argument("--eig-steps", type=int, default=10000)
-    ap.add_argument("--cpu-steps", type=int, default=30)
+    ap.add_argument("--cpu-steps", type=int, default=40)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-eig", action="store_true")
     ap.add_argument("--no-mixed", action="store_true", help="skip the FP32-potential context")

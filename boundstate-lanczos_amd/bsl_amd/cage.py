@@ -28,15 +28,17 @@ def read_xyz(path):
     return np.array(rows, dtype=np.float64)
 
 
-def sites_from_cm_ea(cm_ea):
-    """SPC/E sites (positions nm, charges e, LJ flags) built by the library
-    (bsl_water_sites) from CM/Euler rows."""
+def sites_from_cm_ea(cm_ea, model=0):
+    """Water sites (positions nm, charges e, LJ flags) built by the library
+    (bsl_water_sites) from CM/Euler rows: model 0 SPC/E (3 sites per
+    molecule), 1 TIP4P (4 sites)."""
     cm_ea = np.ascontiguousarray(cm_ea, dtype=np.float64)
     nmol = cm_ea.shape[0]
-    pos = np.zeros(nmol * 9)
-    q = np.zeros(nmol * 3)
-    lj = np.zeros(nmol * 3, dtype=np.int32)
-    check(lib().bsl_water_sites(0, nmol, cm_ea.ravel(), pos, q, lj))
+    ns = 3 if model == 0 else 4
+    pos = np.zeros(nmol * ns * 3)
+    q = np.zeros(nmol * ns)
+    lj = np.zeros(nmol * ns, dtype=np.int32)
+    check(lib().bsl_water_sites(model, nmol, cm_ea.ravel(), pos, q, lj))
     return pos.reshape(-1, 3), q, lj
 
 

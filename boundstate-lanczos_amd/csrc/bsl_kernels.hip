@@ -148,6 +148,21 @@ __device__ inline int kin_row(int kq, int q)
 
 constexpr int kSlabLd = 65;
 
+// XCD-aware order of a (tile, plane) grid: workgroups are dealt to the 8 XCDs
+// round-robin by linear id, so neighbouring tiles would run on different XCDs
+// and fetch the cache lines they share twice; each XCD gets a contiguous range
+// of the (plane-major) tile sequence instead.  A bijection of the grid (the
+// identity when its size is not a multiple of 8).
+constexpr int kXcds = 8;
+__device__ inline void xcd_tile(int &tile, int &plane)
+{
+    const int nt = (int)gridDim.x, total = nt * (int)gridDim.y;
+    const int id = (int)(blockIdx.x + gridDim.x * blockIdx.y);
+    const int lin = (total % kXcds == 0) ? (id % kXcds) * (total / kXcds) + id / kXcds : id;
+    tile = lin % nt;
+    plane = lin / nt;
+}
+
 // x and y axes of one (k, n) slab: D[j][i] = sum_ip v[j][ip] t_x(|ip - i|)
 // + sum_jp t_y(|j - jp|) v[jp][i], both into the same accumulator tiles;
 // each wave owns a 2 x 2 block of 16 x 16 output tiles.  T: FP64, or FP32
@@ -264,7 +279,9 @@ __global__ __launch_bounds__(kWG) void k_kin_z_mfma(const double *__restrict__ X
     extern __shared__ __align__(16) unsigned char kin_lds[];
     T *col = (T *)kin_lds;             // [64][kSlabLd]
     T *rz = col + 64 * kSlabLd;        // [129]
-    const int c0 = blockIdx.x * 64, n = blockIdx.y;
+    int tile, n;
+    xcd_tile(tile, n);
+    const int c0 = tile * 64;
     (void)sc;  // X already holds v
     const size_t base = (size_t)n * nz * nxy;
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
@@ -449,7 +466,9 @@ __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
     T *cs = (T *)kin_lds;                  // [32][kSlabLd]: sums
     T *cd = cs + 32 * kSlabLd;             // [32][kSlabLd]: differences
     T *rz = cd + 32 * kSlabLd;             // [129]
-    const int c0 = blockIdx.x * 64, n = blockIdx.y;
+    int tile, n;
+    xcd_tile(tile, n);
+    const int c0 = tile * 64;
     (void)sc;  // X already holds v
     const size_t base = (size_t)n * nz * nxy;
     const int mz = nz >> 1;

@@ -328,6 +328,38 @@ def test_quad_fp64_hv_matches_oracle(sym, basis):
     assert _rel(c.hv(v), o.hv(v)) < 1e-12
 
 
+def _tip4p_case(nx, lmax, na, nb, sym="All"):
+    pos, q, typ = O.build_sites(cs.cage11_cm_ea(), 1)
+    lj = np.array([1 if t == ord("O") else 0 for t in typ], dtype=np.int32)
+    return cs.cage_case(nx, lmax, na, nb, sym, sites=(pos, q, typ, lj))
+
+
+@pytest.mark.parametrize("case", ["tip4p", "cage_grid", "coulomb", "iho", "anisotropic"])
+def test_quad_fp64_other_potentials_match_oracle(case):
+    """The quadrature calculator over every potential grid the library builds
+    (TIP4P waters, the grid-mapped universe, the Coulomb charge, the
+    oscillator) and on an anisotropic grid, against the oracle's NoQuad Hv;
+    the NoQuad calculator on the same operator too."""
+    if case == "tip4p":
+        spec, kw = _tip4p_case(5, 3, 8, 9)
+    elif case == "cage_grid":
+        spec, kw = cs.cage_grid_case(5, 3, 8, 9, "All", universe=(17, 19, 23, 0.44, 0.5, 0.41))
+    elif case == "coulomb":
+        spec, kw = cs.coulomb_case(6, 2, 6, 7)
+    elif case == "iho":
+        spec, kw = cs.iho_case(7, 3, 8, 9, x_max=0.4)
+    else:
+        spec, kw = cs.cage_case(6, 3, 8, 9, "All")
+        spec = dict(spec, ny=5, nz=7, y_max=0.35, z_max=0.45)
+        kw = dict(kw, ny=5, nz=7, y_max=0.35, z_max=0.45)
+    o = O.Oracle(spec)
+    v = np.random.default_rng(43).uniform(-1, 1, o.N)
+    ref = o.hv(v)
+    for calc in ("quad", "noquad"):
+        with B.Calculator(calculator=calc, **kw) as c:
+            assert _rel(c.hv(v), ref) < 1e-12, calc
+
+
 def test_quad_fp32_hv_close_to_oracle():
     spec, kw = cs.cage_case(5, 3, 8, 9, "All")
     o = O.Oracle(spec)

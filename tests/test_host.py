@@ -179,10 +179,15 @@ def test_trivec_tol_finds_merged_levels():
     assert failed > 0  # the failure mode, for the record
 
 
-def test_water_sites_identical_to_oracle():
+@pytest.mark.parametrize("model", [0, 1])
+def test_water_sites_identical_to_oracle(model):
+    """SPC/E (getSPCEatoms, SPCE_AH2_EngRoutines.cpp:5-108) and TIP4P
+    (getTIP4Patoms, TIP4P_AH2_EngRoutines.cpp) sites from the same CM/Euler
+    rows, bitwise the oracle's."""
     cm_ea = cs.cage11_cm_ea()
-    pos_o, q_o, typ = O.build_sites(cm_ea, 0)
-    pos, q, lj = B.cage.sites_from_cm_ea(cm_ea)
+    pos_o, q_o, typ = O.build_sites(cm_ea, model)
+    pos, q, lj = B.cage.sites_from_cm_ea(cm_ea, model)
+    assert len(q) == len(cm_ea) * (3 if model == 0 else 4)
     assert np.array_equal(pos, pos_o) and np.array_equal(q, q_o)
     assert list(lj) == [1 if t == ord("O") else 0 for t in typ]
 

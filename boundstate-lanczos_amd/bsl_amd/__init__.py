@@ -10,7 +10,7 @@ import ctypes as C
 import numpy as np
 
 from . import cage, physics  # noqa: F401
-from ._lib import BASIS, CALC, POT, PREC, START, SYM, BslError, Params, check, lib  # noqa: F401
+from ._lib import BASIS, CALC, POT, PREC, START, SYM, VFORM, BslError, Params, check, lib  # noqa: F401
 
 
 class Calculator:
@@ -158,6 +158,21 @@ class Calculator:
         out = np.zeros(5)
         check(lib().bsl_step_bytes(self.h, out))
         return out
+
+    @property
+    def potential_form(self):
+        """"blocks" (packed <lm|V|l'm'> blocks per point, as the reference
+        stores them) or "harmonics" (their even-L harmonic coefficients,
+        expanded by the kernel through the Gaunt coefficients)."""
+        r = lib().bsl_get_potential_form(self.h)
+        if r < 0:
+            check(r)
+        return {v: k for k, v in VFORM.items()}[r]
+
+    @potential_form.setter
+    def potential_form(self, form):
+        """Rebuilds the potential in the other form (clears the Krylov vectors)."""
+        check(lib().bsl_set_potential_form(self.h, VFORM[form]))
 
     def potential_kernel(self):
         """Name of the potential kernel (and autotuned variant) in use."""

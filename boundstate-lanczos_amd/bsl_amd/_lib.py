@@ -17,6 +17,7 @@ CALC = {"noquad": 0, "quad": 1}
 PREC = {"fp64": 0, "fp32": 1, "bf16x3": 2}
 POT = {"iho": 0, "cage": 1, "cage_grid": 2, "coulomb": 3}
 START = {"uniform": 0, "random": 1}
+VFORM = {"blocks": 0, "harmonics": 1}
 
 EXPORTS = (
     "bsl_create", "bsl_destroy", "bsl_last_error", "bsl_basis_size", "bsl_krylov_size", "bsl_hv",
@@ -24,7 +25,7 @@ EXPORTS = (
     "bsl_water_sites", "bsl_get_vblocks", "bsl_get_vgrid", "bsl_time_steps", "bsl_step_bytes",
     "bsl_potential_kernel", "bsl_hf_gradient", "bsl_batch_create", "bsl_batch_destroy", "bsl_batch_size",
     "bsl_batch_member", "bsl_batch_lanczos", "bsl_batch_time_steps", "bsl_lancbis_device", "bsl_lancbis_tol",
-    "bsl_lancbis_device_tol", "bsl_trivec_tol",
+    "bsl_lancbis_device_tol", "bsl_trivec_tol", "bsl_get_potential_form", "bsl_set_potential_form",
 )
 
 
@@ -85,6 +86,8 @@ def load():
     lib.bsl_time_steps.argtypes = [C.c_void_p, C.c_int32, C.c_int32, _dp]
     lib.bsl_step_bytes.argtypes = [C.c_void_p, _dp]
     lib.bsl_potential_kernel.argtypes = [C.c_void_p, C.c_char_p, C.c_int32]
+    lib.bsl_get_potential_form.argtypes = [C.c_void_p]
+    lib.bsl_set_potential_form.argtypes = [C.c_void_p, C.c_int32]
     lib.bsl_hf_gradient.argtypes = [C.c_void_p, C.c_int32, _dp, _dp, _dp]
     lib.bsl_batch_create.argtypes = [C.c_int32, C.c_void_p, C.c_void_p]
     lib.bsl_batch_destroy.argtypes = [C.c_void_p]

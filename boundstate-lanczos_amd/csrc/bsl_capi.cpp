@@ -88,16 +88,141 @@ int check_params(const bsl_params &p)
     return BSL_OK;
 }
 
-// Build a DevOperator's potential part (packed blocks) from the grid
-int build_vpk(bsl_ctx *c, DevOperator &op, int vprec, const double *d_h)
+// Does every parity block fit the harmonic-form kernels (l <= kMaxLb, planes
+// in the order of its term list)?
+bool harmonic_fits(const bsl_ctx *c, const DevOperator &op)
+{
+    for (int g = 0; g < op.nblk; g++) {
+        const Block &B = op.blk[g];
+        if (B.lb > gaunt::kMaxLb || B.size != gaunt::block_size(B.lb, B.par)) return false;
+        for (int k = 0; k < B.size; k++) {
+            const int n = c->carried[B.plane0 + k];
+            if (c->tab.l[n] != gaunt::block_l(B.lb, B.par, k) || c->tab.m[n] != gaunt::block_m(B.lb, B.par, k))
+                return false;
+        }
+    }
+    return true;
+}
+
+// Angular table of a potential form, [q][column] with ncol columns:
+// BLOCKS H[q][pair] = w_a w_b L_n(a) S_m(b) L_n'(a) S_m'(b), q = (half, a, b)
+// (calc_Vlmlpmp_NoQuad linRotCartSphHvFullVPrepRoutines.cpp:116-131);
+// HARMONICS H[q][c] = w_a w_b Yh_c(a, b) over the even-L harmonics of the
+// largest block.
+void potential_table(const bsl_ctx *c, const DevOperator &op, int form, std::vector<double> &h, int64_t *ncol)
+{
+    const Tables &t = c->tab;
+    const int na = t.na, nb = t.nb, lmax = t.lmax;
+    if (form == BSL_VFORM_HARMONICS) {
+        int lb = 0;
+        for (int g = 0; g < op.nblk; g++) lb = std::max(lb, op.blk[g].lb);
+        harmonic_table(t, lb, h);
+        *ncol = gaunt::ncoef(lb);
+        return;
+    }
+    const int64_t pairs = op.npair;
+    h.assign((size_t)c->Q * pairs, 0.0);
+    for (int half = 0; half < 2; half++) {
+        const std::vector<double> &S = half == 0 ? t.S1 : t.S2;
+        for (int a = 0; a < na; a++)
+            for (int b = 0; b < nb; b++) {
+                const size_t q = (size_t)half * na * nb + (size_t)a * nb + b;
+                for (int g = 0; g < op.nblk; g++) {
+                    const Block &B = op.blk[g];
+                    int64_t pr = B.pair_offset;
+                    for (const auto &rc : block_pair_order(B.size)) {
+                        const int n = c->carried[B.plane0 + rc.first], np = c->carried[B.plane0 + rc.second];
+                        const int m = t.m[n], mp = t.m[np];
+                        h[q * pairs + pr] = t.wt[a] * t.wp[b] * t.L[(size_t)n * na + a] *
+                                            S[(size_t)(m + lmax) * nb + b] * t.L[(size_t)np * na + a] *
+                                            S[(size_t)(mp + lmax) * nb + b];
+                        pr++;
+                    }
+                }
+            }
+    }
+    *ncol = pairs;
+}
+
+// Build a DevOperator's potential (form, precision) from the grid d_vg and
+// its angular table d_h of ncol columns
+int build_vpk(bsl_ctx *c, DevOperator &op, int form, int vprec, const double *d_vg, const double *d_h, int64_t ncol)
 {
     op.vprec = vprec;
+    op.vform = form;
     const size_t es = vprec == BSL_PREC_FP64 ? sizeof(double) : sizeof(float);
-    BSL_HIP(hipMalloc(&op.vpk, es * (size_t)op.npair * (size_t)(vpk_ntile(c->P) * kVTile)));
-    const int64_t npair0 = op.nblk > 1 ? op.blk[1].pair_offset : op.npair;
-    int rc = launch_vblock_gemm(c->d_vgrid, d_h, op.vpk, vprec, c->P, c->Q, npair0, op.npair, c->stream);
-    if (rc) return rc;
-    return BSL_OK;
+    const int64_t ntile = vpk_ntile(c->P);
+    BSL_HIP(hipMalloc(&op.vpk, es * (size_t)ncol * (size_t)(ntile * kVTile)));
+    int64_t col0 = ncol;  // columns of the first layout region
+    if (form == BSL_VFORM_HARMONICS) {
+        op.ncoef = (int)ncol;
+        for (int g = 0; g < op.nblk; g++) {
+            std::vector<double> gv;
+            double excluded = 0.0;
+            gaunt_values(op.blk[g].lb, op.blk[g].par, gv, &excluded);
+            if ((int64_t)gv.size() != gaunt::count_terms(op.blk[g].lb, op.blk[g].par) || excluded > 1e-12)
+                return fail(BSL_E_STATE, "harmonic term list does not match the block's couplings");
+            int rc = dev_upload(&op.gc[g], gv);
+            if (rc) return rc;
+        }
+    } else {
+        op.ncoef = 0;
+        col0 = op.nblk > 1 ? op.blk[1].pair_offset : op.npair;
+    }
+    if (op.calc != BSL_CALC_QUAD) op.nalpha = form == BSL_VFORM_HARMONICS ? ntile : ntile * op.nblk;
+    return launch_vblock_gemm(d_vg, d_h, op.vpk, vprec, c->P, c->Q, col0, ncol, c->stream);
+}
+
+// release the potential of op (and of op64 unless it shares op's)
+void free_potentials(bsl_ctx *c)
+{
+    DevOperator *ops[2] = {&c->op, &c->op64};
+    const bool shared = c->op64.vpk == c->op.vpk;
+    for (int i = 0; i < 2; i++) {
+        DevOperator &o = *ops[i];
+        if (!(i == 1 && shared)) {
+            dev_free(o.vpk);
+            dev_free(o.gc[0]);
+            dev_free(o.gc[1]);
+        }
+        o.vpk = nullptr;
+        o.gc[0] = o.gc[1] = nullptr;
+    }
+}
+
+// op (NoQuad, FP64) reads op64's potential
+void share_potential(DevOperator &op, const DevOperator &op64)
+{
+    op.vpk = op64.vpk;
+    op.vprec = op64.vprec;
+    op.vform = op64.vform;
+    op.ncoef = op64.ncoef;
+    op.gc[0] = op64.gc[0];
+    op.gc[1] = op64.gc[1];
+    op.nalpha = op64.nalpha;
+}
+
+// The potential of both operators in `form` from the grid d_vg: op64 in
+// FP64, op in the context precision (shared when that is FP64).  A
+// quadrature context's op has no potential blocks.
+int build_potentials(bsl_ctx *c, int form, const double *d_vg)
+{
+    std::vector<double> h;
+    int64_t ncol = 0;
+    potential_table(c, c->op64, form, h, &ncol);
+    double *d_h = nullptr;
+    int rc = dev_upload(&d_h, h);
+    if (!rc) rc = build_vpk(c, c->op64, form, BSL_PREC_FP64, d_vg, d_h, ncol);
+    if (!rc && c->op.calc != BSL_CALC_QUAD) {
+        if (c->p.precision == BSL_PREC_FP64)
+            share_potential(c->op, c->op64);
+        else
+            rc = build_vpk(c, c->op, form, BSL_PREC_FP32, d_vg, d_h, ncol);
+    }
+    const hipError_t e = hipStreamSynchronize(c->stream);
+    dev_free(d_h);
+    if (!rc && e != hipSuccess) rc = hip_fail(e, "build_potentials", __FILE__, __LINE__);
+    return rc;
 }
 
 // H offsets of the quadrature nodes (q = half*na*nb + a*nb + b; the second
@@ -238,7 +363,8 @@ int autotune(bsl_ctx *c)
         }
         float best = 1e30f;
         int pick = 1;
-        for (int v = 0; v < 3 && !rc; v++) {
+        const int nvar = ops[o]->vform == BSL_VFORM_HARMONICS ? 2 : 3;
+        for (int v = 0; v < nvar && !rc; v++) {
             ops[o]->vb_variant = v;
             float ms = 0;
             rc = time_potential(c, *ops[o], &ms);
@@ -269,12 +395,7 @@ extern "C" int bsl_destroy(bsl_ctx *c)
     if (!c) return BSL_OK;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
-    DevOperator *ops[2] = {&c->op, &c->op64};
-    for (int i = 0; i < 2; i++) {
-        DevOperator &o = *ops[i];
-        if (i == 1 && c->op64.vpk == c->op.vpk) break;  // shared with op
-        dev_free(o.vpk);
-    }
+    free_potentials(c);
     DevOperator &o = c->op;
     dev_free(o.tx);
     if (o.ty != o.tx) dev_free(o.ty);
@@ -354,6 +475,9 @@ extern "C" int bsl_create(const bsl_params *pp, bsl_ctx **out)
         if (b.size == 0) continue;
         b.pair_offset = pairs;
         b.mask = (p.basis == BSL_BASIS_FULL && !in_sym(parity)) ? 0.0 : 1.0;
+        b.par = parity;
+        b.lb = 0;
+        for (int k = b.plane0; k < (int)c->carried.size(); k++) b.lb = std::max(b.lb, t.l[c->carried[k]]);
         pairs += (int64_t)b.size * (b.size + 1) / 2;
         op.blk[op.nblk++] = b;
     }
@@ -368,30 +492,8 @@ extern "C" int bsl_create(const bsl_params *pp, bsl_ctx **out)
         c->mask.push_back(p.basis == BSL_BASIS_FULL ? (in_sym(t.l[n]) ? 1.0 : 0.0) : 1.0);
     }
 
-    // H[q][pair] = w_a w_b L_n(a) S_m(b) L_n'(a) S_m'(b), q = (half, a, b)
-    // (calc_Vlmlpmp_NoQuad linRotCartSphHvFullVPrepRoutines.cpp:116-131)
     const int na = t.na, nb = t.nb, lmax = t.lmax;
     c->Q = 2 * na * nb;
-    std::vector<double> h((size_t)c->Q * pairs);
-    for (int half = 0; half < 2; half++) {
-        const std::vector<double> &S = half == 0 ? t.S1 : t.S2;
-        for (int a = 0; a < na; a++)
-            for (int b = 0; b < nb; b++) {
-                const size_t q = (size_t)half * na * nb + (size_t)a * nb + b;
-                for (int g = 0; g < op.nblk; g++) {
-                    const Block &B = op.blk[g];
-                    int64_t pr = B.pair_offset;
-                    for (const auto &rc : block_pair_order(B.size)) {
-                        const int n = c->carried[B.plane0 + rc.first], np = c->carried[B.plane0 + rc.second];
-                        const int m = t.m[n], mp = t.m[np];
-                        h[q * pairs + pr] = t.wt[a] * t.wp[b] * t.L[(size_t)n * na + a] *
-                                            S[(size_t)(m + lmax) * nb + b] * t.L[(size_t)np * na + a] *
-                                            S[(size_t)(mp + lmax) * nb + b];
-                        pr++;
-                    }
-                }
-            }
-    }
 
     // ---- device: tables ----
     if ((rc = dev_upload(&op.tx, rowx))) return rc;
@@ -412,14 +514,14 @@ extern "C" int bsl_create(const bsl_params *pp, bsl_ctx **out)
     // ---- device: potential grid and V blocks ----
     BSL_HIP(hipMalloc((void **)&c->d_vgrid, sizeof(double) * (size_t)c->P * c->Q));
     if ((rc = make_vgrid(c.get(), c->d_vgrid))) return rc;
-    double *d_h = nullptr;
-    if ((rc = dev_upload(&d_h, h))) return rc;
     c->op64 = op;
     c->op64.vpk = nullptr;
     c->op64.kprec = BSL_PREC_FP64;
     // mixed precisions contract the kinetic term in FP32 as well
     op.kprec = p.precision == BSL_PREC_FP64 ? BSL_PREC_FP64 : BSL_PREC_FP32;
-    if ((rc = build_vpk(c.get(), c->op64, BSL_PREC_FP64, d_h))) return rc;
+    if (p.calculator == BSL_CALC_QUAD) op.calc = BSL_CALC_QUAD;
+    const int form = harmonic_fits(c.get(), op) ? BSL_VFORM_HARMONICS : BSL_VFORM_BLOCKS;
+    if ((rc = build_potentials(c.get(), form, c->d_vgrid))) return rc;
     if (p.calculator == BSL_CALC_QUAD) {
         // weighted grid W[P][a][half*nb + b] and the transform tables
         op.calc = BSL_CALC_QUAD;
@@ -468,14 +570,8 @@ extern "C" int bsl_create(const bsl_params *pp, bsl_ctx **out)
         c->op64.partial_b = op.partial_b;
         c->op64.partial_c = op.partial_c;
         c->op64.npartial = op.npartial;
-    } else if (p.precision == BSL_PREC_FP64) {
-        op.vpk = c->op64.vpk;
-        op.vprec = BSL_PREC_FP64;
-    } else {
-        if ((rc = build_vpk(c.get(), op, BSL_PREC_FP32, d_h))) return rc;
     }
     BSL_HIP(hipStreamSynchronize(c->stream));
-    dev_free(d_h);
     dev_free(c->d_vgrid);
     c->d_vgrid = nullptr;
 
@@ -815,29 +911,83 @@ extern "C" int bsl_get_vblocks(bsl_ctx *c, double *out)
     BSL_HIP(hipSetDevice(c->device));
     const DevOperator &op = c->op64;
     const int64_t ntile = vpk_ntile(c->P);
+    const bool harm = op.vform == BSL_VFORM_HARMONICS;
+    const int64_t ncol = harm ? op.ncoef : op.npair;
     const int64_t npair0 = op.nblk > 1 ? op.blk[1].pair_offset : op.npair;
-    std::vector<double> vpk((size_t)op.npair * ntile * kVTile);
+    std::vector<double> vpk((size_t)ncol * ntile * kVTile);
     BSL_HIP(hipMemcpy(vpk.data(), op.vpk, sizeof(double) * vpk.size(), hipMemcpyDeviceToHost));
     const int nn = c->nn, nx = op.nx, ny = op.ny, nz = op.nz;
     std::memset(out, 0, sizeof(double) * (size_t)c->P * nn * nn);
+    auto put = [&](int n, int np, const std::vector<double> &v /* [P], device point order */) {
+        for (int k = 0; k < nz; k++)
+            for (int j = 0; j < ny; j++)
+                for (int i = 0; i < nx; i++) {
+                    const int64_t p = ((int64_t)k * ny + j) * nx + i;
+                    const int64_t ref = (((int64_t)i * ny + j) * nz + k);
+                    out[(ref * nn + n) * nn + np] = v[p];
+                    out[(ref * nn + np) * nn + n] = v[p];
+                }
+    };
+    std::vector<double> v((size_t)c->P);
     for (int g = 0; g < op.nblk; g++) {
         const Block &B = op.blk[g];
-        int64_t pr = B.pair_offset;
-        for (const auto &rc : block_pair_order(B.size)) {
-            const int n = c->carried[B.plane0 + rc.first], np = c->carried[B.plane0 + rc.second];
-            for (int k = 0; k < nz; k++)
-                for (int j = 0; j < ny; j++)
-                    for (int i = 0; i < nx; i++) {
-                        const int64_t p = ((int64_t)k * ny + j) * nx + i;
-                        const int64_t ref = (((int64_t)i * ny + j) * nz + k);
-                        const double v = vpk[vpk_index(pr, p, npair0, op.npair, ntile)];
-                        out[(ref * nn + n) * nn + np] = v;
-                        out[(ref * nn + np) * nn + n] = v;
+        if (harm) {
+            // V[i][j] = sum over the pair's terms of G h[c], the kernel's sum
+            std::vector<double> gv;
+            double excluded = 0.0;
+            gaunt_values(B.lb, B.par, gv, &excluded);
+            size_t t = 0;
+            for (int i = 0; i < B.size; i++)
+                for (int j = i; j < B.size; j++) {
+                    std::fill(v.begin(), v.end(), 0.0);
+                    for (int cc = 0; cc < gaunt::ncoef(B.lb); cc++) {
+                        if (!gaunt::coupled(gaunt::block_l(B.lb, B.par, i), gaunt::block_m(B.lb, B.par, i),
+                                            gaunt::block_l(B.lb, B.par, j), gaunt::block_m(B.lb, B.par, j),
+                                            gaunt::coef_L(cc), gaunt::coef_M(cc)))
+                            continue;
+                        const double gt = gv[t++];
+                        for (int64_t p = 0; p < c->P; p++)
+                            v[p] += gt * vpk[(size_t)((p / kVTile) * ncol + cc) * kVTile + p % kVTile];
                     }
-            pr++;
+                    put(c->carried[B.plane0 + i], c->carried[B.plane0 + j], v);
+                }
+        } else {
+            int64_t pr = B.pair_offset;
+            for (const auto &rc : block_pair_order(B.size)) {
+                for (int64_t p = 0; p < c->P; p++) v[p] = vpk[vpk_index(pr, p, npair0, op.npair, ntile)];
+                put(c->carried[B.plane0 + rc.first], c->carried[B.plane0 + rc.second], v);
+                pr++;
+            }
         }
     }
     return BSL_OK;
+}
+
+extern "C" int bsl_get_potential_form(const bsl_ctx *c)
+{
+    if (!c) return fail(BSL_E_ARG, "null argument");
+    return c->op64.vform;
+}
+
+extern "C" int bsl_set_potential_form(bsl_ctx *c, int32_t form)
+{
+    if (!c) return fail(BSL_E_ARG, "null argument");
+    if (form != BSL_VFORM_BLOCKS && form != BSL_VFORM_HARMONICS) return fail(BSL_E_ARG, "potential form not recognised");
+    if (form == c->op64.vform) return BSL_OK;
+    if (form == BSL_VFORM_HARMONICS && !harmonic_fits(c, c->op64))
+        return fail(BSL_E_ARG, "the harmonic form needs parity blocks of l <= " + std::to_string(gaunt::kMaxLb));
+    BSL_HIP(hipSetDevice(c->device));
+    BSL_HIP(hipStreamSynchronize(c->stream));
+    double *d_vg = nullptr;
+    int rc = dev_alloc(&d_vg, (size_t)c->P * c->Q);
+    if (!rc) rc = make_vgrid(c, d_vg);
+    if (!rc) {
+        free_potentials(c);
+        rc = build_potentials(c, form, d_vg);
+    }
+    dev_free(d_vg);
+    if (!rc) rc = autotune(c);
+    return rc;
 }
 
 extern "C" int bsl_get_vgrid(bsl_ctx *c, double *v1, double *v2)
@@ -952,9 +1102,15 @@ extern "C" int bsl_step_bytes(const bsl_ctx *c, double *out)
         pot_flops = (double)c->P * (4.0 * op.na * op.nk + 4.0 * op.na * op.nphi * op.nm + op.na * op.nphi);
     } else {
         const double es = op.vprec == BSL_PREC_FP64 ? 8.0 : 4.0;
-        pot_bytes = es * (double)op.npair * (double)c->P;
+        const bool harm = op.vform == BSL_VFORM_HARMONICS;
+        pot_bytes = es * (double)(harm ? op.ncoef : op.npair) * (double)c->P;
+        // per point: B^2 multiply-adds of the block product, plus one per
+        // harmonic term in the harmonic form
         double pair_fma = 0.0;
-        for (int g = 0; g < op.nblk; g++) pair_fma += (double)op.blk[g].size * op.blk[g].size;
+        for (int g = 0; g < op.nblk; g++) {
+            pair_fma += (double)op.blk[g].size * op.blk[g].size;
+            if (harm) pair_fma += (double)gaunt::count_terms(op.blk[g].lb, op.blk[g].par);
+        }
         pot_flops = 2.0 * (double)c->P * pair_fma;
     }
     out[0] = pot_bytes + 3.0 * vec;  // potential kernel: V + read X, read and write R
@@ -972,6 +1128,10 @@ extern "C" int bsl_potential_kernel(const bsl_ctx *c, char *name, int32_t len)
     char buf[128];
     if (op.calc == BSL_CALC_QUAD)
         std::snprintf(buf, sizeof buf, "k_quad_mfma<%d>", op.qmode);
+    else if (op.vform == BSL_VFORM_HARMONICS)
+        std::snprintf(buf, sizeof buf, "k_vgaunt<%d, %d, %d, %d, %s, %d>", op.blk[0].lb, op.blk[0].par,
+                      op.nblk > 1 ? op.blk[1].lb : -1, op.nblk > 1 ? op.blk[1].par : 0,
+                      op.vprec == BSL_PREC_FP64 ? "double" : "float", op.vb_variant + 1);
     else
         std::snprintf(buf, sizeof buf, "k_vblock<%d, %d, %s, %d>", op.blk[0].size, op.nblk > 1 ? op.blk[1].size : 0,
                       op.vprec == BSL_PREC_FP64 ? "double" : "float", op.vb_variant + 1);

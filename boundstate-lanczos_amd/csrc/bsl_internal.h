@@ -9,6 +9,7 @@
 #include <utility>
 #include <vector>
 
+#include "bsl_gaunt.h"
 #include "bsl_hip.h"
 
 namespace bsl {
@@ -49,6 +50,13 @@ struct Tables {
 void build_tables(int lmax, int na, int nb, Tables &t);
 // first row of the (symmetric Toeplitz) sinc-DVR kinetic matrix and the grid
 void kinetic_row(double x_max, int n, double mass, std::vector<double> &row, std::vector<double> &grid);
+// harmonic-form angular table hh[q][c] = w_a w_b Yh_c(node q), q = (half, a, b)
+// as in the V-block table, c over the ncoef(lb) even-L real harmonics
+void harmonic_table(const Tables &t, int lb, std::vector<double> &hh);
+// values of the harmonic term list of parity block (lb, par), in list order,
+// from an exact quadrature; *excluded = largest |G| among the (i, j, c) the
+// selection rules leave out (rounding level when the rules hold)
+void gaunt_values(int lb, int par, std::vector<double> &g, double *excluded);
 
 // ---------------------------------------------------------------------------
 // Cullum-Willoughby post-processing (bsl_tridiag.cpp)
@@ -107,6 +115,7 @@ struct Block {
     int plane0 = 0;
     int64_t pair_offset = 0;      // first packed pair of this block
     double mask = 1.0;            // symmetry mask of its planes (uniform within a block)
+    int lb = 0, par = 0;          // largest l and the l parity of its planes
 };
 
 struct DevOperator {
@@ -117,7 +126,12 @@ struct DevOperator {
     int kprec = 0;                // BSL_PREC_FP64 or BSL_PREC_FP32: kinetic contractions
     double *tx = nullptr, *ty = nullptr, *tz = nullptr;  // Toeplitz rows
     double *rot = nullptr, *mask = nullptr;              // per carried plane
-    void *vpk = nullptr;          // packed V blocks in the tiled layout of vpk_index (double or float)
+    void *vpk = nullptr;          // potential of the points (double or float): packed V blocks in the
+                                  // tiled layout of vpk_index (vform 0) or harmonic coefficients
+                                  // [tile][ncoef][kVTile] (vform 1)
+    int vform = 0;                // BSL_VFORM_*
+    int ncoef = 0;                // harmonic columns per point (vform 1)
+    double *gc[2] = {nullptr, nullptr};  // per block: the values of its harmonic term list (vform 1)
     int nblk = 0;
     Block blk[2];
     int *plane_ref = nullptr;     // reference lm index of each carried plane

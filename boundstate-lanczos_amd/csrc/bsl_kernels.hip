@@ -673,6 +673,109 @@ __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(W, 8))) voi
 }
 
 // ---------------------------------------------------------------------------
+// potential term, harmonic form: one thread per (grid point, parity block)
+// ---------------------------------------------------------------------------
+// The block of a point is rebuilt from its harmonic coefficients h[c] as the
+// sparse sum V[i][j] = sum_c G[t] h[c] over the block's term list (one run
+// of terms per packed pair), each element applied to both of its entries
+// at once.  The list is a compile-time constant, so h, x and acc stay in
+// registers; the coefficients G[t] are uniform across the wave (scalar
+// loads at constant offsets).
+template <class Pat, int T>
+__device__ __forceinline__ void gaunt_term(const double *__restrict__ g, const double (&h)[Pat::NC],
+                                           const double (&x)[Pat::B], double (&acc)[Pat::B], double &e)
+{
+    constexpr gaunt::Term tm = Pat::list.t[T];
+    if constexpr (tm.first)
+        e = g[T] * h[tm.c];
+    else
+        e = fma(g[T], h[tm.c], e);
+    if constexpr (tm.last) {
+        acc[tm.i] = fma(e, x[tm.j], acc[tm.i]);
+        if constexpr (tm.i != tm.j) acc[tm.j] = fma(e, x[tm.i], acc[tm.j]);
+    }
+}
+
+template <class Pat, int... T>
+__device__ __forceinline__ void gaunt_terms(const double *__restrict__ g, const double (&h)[Pat::NC],
+                                            const double (&x)[Pat::B], double (&acc)[Pat::B],
+                                            std::integer_sequence<int, T...>)
+{
+    double e = 0.0;
+    (gaunt_term<Pat, T>(g, h, x, acc, e), ...);
+}
+
+// H is [tile][column][kVTile] with nc columns per point; a block of lb < the
+// context's largest reads the leading ncoef(lb) columns.
+template <int LB, int PAR, typename VT>
+__device__ inline void gaunt_point(const double *__restrict__ X, double *__restrict__ R, const VT *__restrict__ H,
+                                   const double *__restrict__ G, int nc, int plane0, double mask, int64_t P, int p,
+                                   double &dot, double &rr2, double &vv2)
+{
+    using Pat = gaunt::Pattern<LB, PAR>;
+    constexpr int B = Pat::B, NC = Pat::NC;
+    double x[B], acc[B], h[NC];
+    const double *xp = X + (size_t)plane0 * P + p;
+#pragma unroll
+    for (int r = 0; r < B; r++) {
+        x[r] = xp[(size_t)r * P];
+        acc[r] = 0.0;
+    }
+    const VT *hp = H + (size_t)(p / kVTile) * nc * kVTile + p % kVTile;
+#pragma unroll
+    for (int c = 0; c < NC; c++) h[c] = (double)hp[(size_t)c * kVTile];
+    gaunt_terms<Pat>(G, h, x, acc, std::make_integer_sequence<int, Pat::NT>{});
+    double *rp = R + (size_t)plane0 * P + p;
+    double rr[B];
+#pragma unroll
+    for (int r = 0; r < B; r++) rr[r] = rp[(size_t)r * P];
+#pragma unroll
+    for (int r = 0; r < B; r++) {
+        const double nr = rr[r] + mask * acc[r];
+        rp[(size_t)r * P] = nr;
+        dot += x[r] * nr;
+        rr2 += nr * nr;
+        vv2 += x[r] * x[r];
+    }
+}
+
+// NB parity blocks per workgroup: threads [0, kWG) take block 0 of the
+// workgroup's kVTile points, threads [kWG, 2 kWG) block 1 of the same points,
+// so that the coefficient columns both blocks read are fetched once.
+// Block 0 occupies planes [0, plane1), block 1 planes [plane1, ...).
+template <int LB0, int PAR0, int LB1, int PAR1, typename VT, int W>
+__global__ __launch_bounds__((LB1 >= 0 ? 2 : 1) * kWG) __attribute__((amdgpu_waves_per_eu(W, 8))) void k_vgaunt(
+    const double *__restrict__ X, double *__restrict__ R, const VT *__restrict__ H, const double *__restrict__ G0,
+    const double *__restrict__ G1, int nc, int plane1, double mask0, double mask1, int64_t P,
+    double *__restrict__ partial, double *__restrict__ partial_b, double *__restrict__ partial_c)
+{
+    __shared__ double red[2 * kWG / 64];
+    double dot = 0.0, rr2 = 0.0, vv2 = 0.0;
+    const int half = (int)threadIdx.x / kWG;
+    const int p = (int)blockIdx.x * kWG + ((int)threadIdx.x & (kWG - 1));
+    if (p < P) {
+        if constexpr (LB1 >= 0) {
+            if (half == 0)
+                gaunt_point<LB0, PAR0, VT>(X, R, H, G0, nc, 0, mask0, P, p, dot, rr2, vv2);
+            else
+                gaunt_point<LB1, PAR1, VT>(X, R, H, G1, nc, plane1, mask1, P, p, dot, rr2, vv2);
+        } else {
+            gaunt_point<LB0, PAR0, VT>(X, R, H, G0, nc, 0, mask0, P, p, dot, rr2, vv2);
+        }
+    }
+    const double s0 = block_sum(dot, red);
+    __syncthreads();
+    const double s1 = block_sum(rr2, red);
+    __syncthreads();
+    const double s2 = block_sum(vv2, red);
+    if (threadIdx.x == 0) {
+        partial[blockIdx.x] = s0;
+        partial_b[blockIdx.x] = s1;
+        partial_c[blockIdx.x] = s2;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // recurrence scalars
 // ---------------------------------------------------------------------------
 // mode 0 (end of a step): finalize_step.
@@ -806,11 +909,43 @@ static int launch_vblock_t(const DevOperator &op, const double *X, double *R, hi
     return fail(BSL_E_ARG, "unsupported potential block sizes " + std::to_string(b0) + "/" + std::to_string(b1));
 }
 
+template <typename VT>
+static int launch_vgaunt_t(const DevOperator &op, const double *X, double *R, hipStream_t s)
+{
+    const VT *H = (const VT *)op.vpk;
+    dim3 grid((unsigned)vpk_ntile(op.P));
+    const dim3 threads((unsigned)(op.nblk * kWG));
+    if ((int64_t)grid.x > op.npartial || (int64_t)grid.x != op.nalpha)
+        return fail(BSL_E_STATE, "partial buffer does not match the potential kernel grid");
+    const Block &b0 = op.blk[0], &b1 = op.nblk > 1 ? op.blk[1] : op.blk[0];
+    const double mk1 = op.nblk > 1 ? b1.mask : 0.0;
+    const int l0 = b0.lb, p0 = b0.par, l1 = op.nblk > 1 ? b1.lb : -1, p1 = op.nblk > 1 ? b1.par : 0;
+#define BSL_VG(A, PA, Bb, PB)                                                                              \
+    if (l0 == A && p0 == PA && l1 == Bb && (Bb < 0 || p1 == PB)) {                                         \
+        if (op.vb_variant == 0)                                                                            \
+            hipLaunchKernelGGL((k_vgaunt<A, PA, Bb, PB, VT, 1>), grid, threads, 0, s, X, R, H, op.gc[0],       \
+                               op.gc[1], op.ncoef, b1.plane0, b0.mask, mk1, op.P, op.partial, op.partial_b,    \
+                               op.partial_c);                                                              \
+        else                                                                                               \
+            hipLaunchKernelGGL((k_vgaunt<A, PA, Bb, PB, VT, 2>), grid, threads, 0, s, X, R, H, op.gc[0],       \
+                               op.gc[1], op.ncoef, b1.plane0, b0.mask, mk1, op.P, op.partial, op.partial_b,    \
+                               op.partial_c);                                                              \
+        return hipGetLastError() == hipSuccess ? BSL_OK : fail(BSL_E_HIP, "k_vgaunt launch");             \
+    }
+    BSL_VG(4, 0, 5, 1) BSL_VG(4, 0, 3, 1) BSL_VG(2, 0, 3, 1) BSL_VG(2, 0, 1, 1) BSL_VG(0, 0, 1, 1)
+    BSL_VG(0, 0, -1, 0) BSL_VG(1, 1, -1, 0) BSL_VG(2, 0, -1, 0) BSL_VG(3, 1, -1, 0) BSL_VG(4, 0, -1, 0)
+    BSL_VG(5, 1, -1, 0)
+#undef BSL_VG
+    return fail(BSL_E_ARG, "no harmonic-form potential kernel for blocks of l <= " + std::to_string(l0) + "/" +
+                               std::to_string(l1));
+}
+
 int launch_potential(const DevOperator &op, const double *X, double *R, Scal *sc, hipStream_t s)
 {
-    return op.calc == BSL_CALC_QUAD        ? launch_quad_potential(op, X, R, sc, s)
-           : op.vprec == BSL_PREC_FP64 ? launch_vblock_t<double>(op, X, R, s)
-                                       : launch_vblock_t<float>(op, X, R, s);
+    if (op.calc == BSL_CALC_QUAD) return launch_quad_potential(op, X, R, sc, s);
+    if (op.vform == BSL_VFORM_HARMONICS)
+        return op.vprec == BSL_PREC_FP64 ? launch_vgaunt_t<double>(op, X, R, s) : launch_vgaunt_t<float>(op, X, R, s);
+    return op.vprec == BSL_PREC_FP64 ? launch_vblock_t<double>(op, X, R, s) : launch_vblock_t<float>(op, X, R, s);
 }
 
 int launch_hv(const DevOperator &op, double *X, double *R, Scal *sc, hipStream_t s, hipEvent_t *ev)

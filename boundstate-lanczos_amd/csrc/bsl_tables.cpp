@@ -6,6 +6,7 @@
 // tables agree with it to the last bit.
 #include <cfloat>
 #include <cmath>
+#include <algorithm>
 #include <cstdlib>
 
 #include "bsl_internal.h"
@@ -149,6 +150,82 @@ void kinetic_row(double x_max, int n, double mass, std::vector<double> &row, std
         row[d] = v;
     }
     for (int i = 0; i < n; i++) grid[i] = (double)i * dx - x_max / 2.0;
+}
+
+// harmonic columns of a block of lb: L = 0, 2, .., 2 lb (bsl_gaunt.h order)
+static void harmonic_columns(int lb, std::vector<int> &Ls, std::vector<int> &Ms)
+{
+    Ls.clear();
+    Ms.clear();
+    for (int c = 0; c < gaunt::ncoef(lb); c++) {
+        Ls.push_back(gaunt::coef_L(c));
+        Ms.push_back(gaunt::coef_M(c));
+    }
+}
+
+void harmonic_table(const Tables &t, int lb, std::vector<double> &hh)
+{
+    std::vector<int> Ls, Ms;
+    harmonic_columns(lb, Ls, Ms);
+    const int nc = (int)Ls.size(), na = t.na, nb = t.nb;
+    std::vector<double> lrow(nc);
+    hh.assign((size_t)2 * na * nb * nc, 0.0);
+    for (int a = 0; a < na; a++) {
+        legendre_row(2 * lb, Ls, Ms, t.ct[a], lrow.data());
+        for (int half = 0; half < 2; half++)
+            for (int b = 0; b < nb; b++) {
+                // the nodes of the V-block table (S1 and S2 of build_tables)
+                const double phi = half == 0 ? std::acos(t.cp[b]) : 2.0 * kPi - std::acos(t.cp[b]);
+                const size_t q = (size_t)half * na * nb + (size_t)a * nb + b;
+                for (int c = 0; c < nc; c++) hh[q * nc + c] = t.wt[a] * t.wp[b] * lrow[c] * trig(Ms[c], phi);
+            }
+    }
+}
+
+// G[i][j][c] = integral of Y_i Y_j Yh_c over the sphere with the functions of
+// the tables above: Gauss-Legendre in cos(theta) with 2 lb + 2 nodes (exact
+// to degree 4 lb + 3) and 4 lb + 4 equally spaced phi nodes (exact for
+// frequencies below 4 lb + 4); the integrands have degree at most 4 lb.
+void gaunt_values(int lb, int par, std::vector<double> &g, double *excluded)
+{
+    const int B = gaunt::block_size(lb, par), nc = gaunt::ncoef(lb);
+    std::vector<int> ls, ms, Ls, Ms;
+    for (int k = 0; k < B; k++) {
+        ls.push_back(gaunt::block_l(lb, par, k));
+        ms.push_back(gaunt::block_m(lb, par, k));
+    }
+    harmonic_columns(lb, Ls, Ms);
+    const int nx = 2 * lb + 2, nphi = 4 * lb + 4;
+    std::vector<double> x, w;
+    gauss_legendre(nx, x, w);
+    std::vector<double> full((size_t)B * B * nc, 0.0), yb(B), yc(nc), lb_row(B), lc_row(nc);
+    for (int a = 0; a < nx; a++) {
+        legendre_row(2 * lb, ls, ms, x[a], lb_row.data());
+        legendre_row(2 * lb, Ls, Ms, x[a], lc_row.data());
+        for (int f = 0; f < nphi; f++) {
+            const double phi = kPi * (2.0 * f + 1.0) / (double)nphi, wq = w[a] * 2.0 * kPi / (double)nphi;
+            for (int k = 0; k < B; k++) yb[k] = lb_row[k] * trig(ms[k], phi);
+            for (int c = 0; c < nc; c++) yc[c] = lc_row[c] * trig(Ms[c], phi);
+            for (int i = 0; i < B; i++)
+                for (int j = 0; j < B; j++) {
+                    const double yy = wq * yb[i] * yb[j];
+                    double *o = &full[((size_t)i * B + j) * nc];
+                    for (int c = 0; c < nc; c++) o[c] += yy * yc[c];
+                }
+        }
+    }
+    g.clear();
+    double ex = 0.0;
+    for (int i = 0; i < B; i++)
+        for (int j = i; j < B; j++)
+            for (int c = 0; c < nc; c++) {
+                const double v = full[((size_t)i * B + j) * nc + c];
+                if (gaunt::coupled(ls[i], ms[i], ls[j], ms[j], Ls[c], Ms[c]))
+                    g.push_back(v);
+                else
+                    ex = std::max(ex, std::fabs(v));
+            }
+    *excluded = ex;
 }
 
 }  // namespace bsl

@@ -44,6 +44,17 @@ enum { BSL_BASIS_FULL = 0, BSL_BASIS_REDUCED = 1 };
  * transforms around the resident potential grid, evaluated on MFMA
  * (the on-the-fly linRotCartSph calculator, with its summation corrected). */
 enum { BSL_CALC_NOQUAD = 0, BSL_CALC_QUAD = 1 };
+/* How a NOQUAD context holds its per-point potential.  BLOCKS: the packed
+ * <lm|V|l'm'> blocks themselves (B(B+1)/2 values per point and parity
+ * block).  HARMONICS: the even-L real-harmonic coefficients
+ * vh_LM(P) = sum_q w_q Y_LM(q) V(P, q) on the same quadrature (66 values per
+ * point for lmax 5), from which the potential kernel rebuilds each block as
+ * <lm|V|l'm'> = sum_LM G(lm, l'm', LM) vh_LM with the Gaunt coefficients G of
+ * the real harmonics.  The two are the same matrices, exactly: the product
+ * of two harmonics of l and l' is a finite sum of harmonics of L <= l + l',
+ * at every node of any quadrature; they differ by rounding only.  HARMONICS
+ * is the default wherever it applies (every parity block of l <= 5). */
+enum { BSL_VFORM_BLOCKS = 0, BSL_VFORM_HARMONICS = 1 };
 /* Hv arithmetic; the recurrence, its dot products and norms are always FP64 */
 enum { BSL_PREC_FP64 = 0, BSL_PREC_FP32 = 1, BSL_PREC_BF16X3 = 2 };
 /* IHO: isotropic harmonic oscillator (IsoHarmOsc_Routines.cpp:6-79).
@@ -196,6 +207,16 @@ int bsl_water_sites(int32_t model, int32_t nmol, const double *cm_ea, double *po
 int bsl_get_vblocks(bsl_ctx *ctx, double *out);
 int bsl_get_vgrid(bsl_ctx *ctx, double *v1, double *v2);
 
+/* The BSL_VFORM_* a NOQUAD context's potential kernel reads, and a switch
+ * between the two (the potential is rebuilt from a regenerated grid and the
+ * kernel variant re-tuned; the Krylov vectors are cleared, so call it before
+ * a run).  BLOCKS is the representation HvPrep_Internal_NoQuad builds
+ * (linRotCartSphHvFullVPrepRoutines.cpp:229-528); HARMONICS has no
+ * reference counterpart.  For the quadrature calculator these act on the
+ * FP64 NoQuad operator the context keeps for its FP64 passes. */
+int bsl_get_potential_form(const bsl_ctx *ctx);
+int bsl_set_potential_form(bsl_ctx *ctx, int32_t form);
+
 /* Timing of the device recurrence: runs `steps` Lanczos steps from the
  * random start after `warmup` untimed ones.  ms[0] = total time of the timed
  * steps (HIP events on the context stream, no host synchronisation inside);
@@ -242,8 +263,10 @@ int bsl_batch_lanczos(bsl_batch *b, int32_t niter, int32_t start, uint64_t seed,
 int bsl_batch_time_steps(bsl_batch *b, int32_t warmup, int32_t steps, int32_t use_graph, double *ms);
 
 // Name (with template arguments, as a profiler prints it) of the potential
-// kernel the context launches, e.g. "k_vblock<15, 21, double, 1>"; the last
-// argument is the occupancy variant picked by the autotuner at bsl_create.
+// kernel the context launches, e.g. "k_vgaunt<4, 0, 5, 1, double, 2>" (harmonic
+// form: largest l and parity of each block) or "k_vblock<15, 21, double, 1>"
+// (block form: block sizes); the last argument is the occupancy variant
+// picked by the autotuner at bsl_create.
 // No reference counterpart (measurement support).
 int bsl_potential_kernel(const bsl_ctx *ctx, char *name, int32_t len);
 

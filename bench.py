@@ -4,14 +4,16 @@ Workload (one GPU): Alavi H2 in a synthetic 5^12 cage of 20 rigid SPC/E
 waters (seed 0), sinc-DVR grid of 54 points per axis on a box of x_max
 (default 0.8 nm, i.e. [-0.4, 0.4] nm, spacing 0.0151 nm), tesseral
 harmonics l <= 5 (36 lm, full basis), 40 x 70 (theta, phi) quadrature,
-NoQuad potential blocks, FP64 Hv and FP64 recurrence.  A step is one Hv plus
-one update of the three-term recurrence (folded into the Hv kernels plus a
-one-workgroup finalize); inputs (potential blocks, Krylov
-vectors) are resident in HBM when the timed region starts.
+NoQuad potential (its per-point blocks held as even-L harmonic coefficients
+and expanded through the Gaunt coefficients in the kernel), FP64 Hv and FP64
+recurrence.  A step is one Hv plus one update of the three-term recurrence
+(folded into the Hv kernels plus a one-workgroup finalize); inputs
+(potential coefficients, Krylov vectors) are resident in HBM when the timed
+region starts.
 
 Also reported: the FP32-potential ("mixed") step rate and its ratio to FP64,
 the error of the mixed run's lowest levels against the FP64 run (cm^-1), the
-roofline of the dominant (potential-block) kernel and the CPU oracle timed
+roofline of the dominant (potential) kernel and the CPU oracle timed
 on this host on a bounded sample of the same operator.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
@@ -364,8 +366,8 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": "Alavi H2 in a synthetic 5^12 SPC/E cage (seed 0); %d^3 sinc-DVR grid, x_max %.3g nm; "
-                        "lmax 5 (36 lm, full basis); 40x70 quadrature; NoQuad potential blocks; FP64 Hv and "
-                        "recurrence" % (args.n, args.x_max),
+                        "lmax 5 (36 lm, full basis); 40x70 quadrature; NoQuad potential (%s form); FP64 Hv and "
+                        "recurrence" % (args.n, args.x_max, c64.potential_form),
             "basis_size": int(c64.N),
             "mixed_fp32_steps_per_s": steps_s32,
             "mixed_vs_fp64": steps_s32 / steps_s if steps_s32 else None,

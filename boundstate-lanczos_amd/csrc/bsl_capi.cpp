@@ -169,7 +169,12 @@ int build_vpk(bsl_ctx *c, DevOperator &op, int form, int vprec, const double *d_
         op.ncoef = 0;
         col0 = op.nblk > 1 ? op.blk[1].pair_offset : op.npair;
     }
-    if (op.calc != BSL_CALC_QUAD) op.nalpha = form == BSL_VFORM_HARMONICS ? ntile : ntile * op.nblk;
+    if (op.calc != BSL_CALC_QUAD) {
+        const Block &b0 = op.blk[0], &b1 = op.blk[op.nblk > 1 ? 1 : 0];
+        const bool split = harmonic_split(op.nblk, b0.lb, b0.par, b1.lb, b1.par, b1.plane0, b0.size);
+        op.nalpha = form == BSL_VFORM_HARMONICS ? harmonic_workgroups(c->P, split) : ntile * op.nblk;
+        if (op.nalpha > op.npartial) return fail(BSL_E_STATE, "partial buffer too small for the potential kernel");
+    }
     return launch_vblock_gemm(d_vg, d_h, op.vpk, vprec, c->P, c->Q, col0, ncol, c->stream);
 }
 
@@ -1128,6 +1133,11 @@ extern "C" int bsl_potential_kernel(const bsl_ctx *c, char *name, int32_t len)
     char buf[128];
     if (op.calc == BSL_CALC_QUAD)
         std::snprintf(buf, sizeof buf, "k_quad_mfma<%d>", op.qmode);
+    else if (op.vform == BSL_VFORM_HARMONICS &&
+             harmonic_split(op.nblk, op.blk[0].lb, op.blk[0].par, op.nblk > 1 ? op.blk[1].lb : -1,
+                            op.nblk > 1 ? op.blk[1].par : 0, op.blk[op.nblk > 1 ? 1 : 0].plane0, op.blk[0].size))
+        std::snprintf(buf, sizeof buf, "k_vgaunt_split<%s, %d>", op.vprec == BSL_PREC_FP64 ? "double" : "float",
+                      op.vb_variant + 3);
     else if (op.vform == BSL_VFORM_HARMONICS)
         std::snprintf(buf, sizeof buf, "k_vgaunt<%d, %d, %d, %d, %s, %d>", op.blk[0].lb, op.blk[0].par,
                       op.nblk > 1 ? op.blk[1].lb : -1, op.nblk > 1 ? op.blk[1].par : 0,

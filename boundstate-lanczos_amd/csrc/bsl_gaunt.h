@@ -77,11 +77,20 @@ constexpr bool coupled(int l1, int m1, int l2, int m2, int L, int M)
 
 // One term of the block's expansion: V[i][j] += G * vh[c].  Terms are listed
 // pair by pair (i <= j, i outer, j inner), columns ascending; `first` and
-// `last` mark the ends of a pair's run.
+// `last` mark the ends of a pair's run; g is the term's place in the
+// block's full list (where its value is stored).
 struct Term {
-    int i = 0, j = 0, c = 0;
+    int i = 0, j = 0, c = 0, g = 0;
     bool first = false, last = false;
 };
+
+// Pair types of a sub-list: both planes cos-type (m >= 0), both sin-type
+// (m < 0), or one of each.  Same-type pairs couple through the M >= 0
+// harmonics only and cross pairs through M < 0 only (an even number of
+// sines), so the sub-lists read disjoint coefficient columns.
+enum { kAll = 0, kCC = 1, kSS = 2, kCross = 3 };
+constexpr int pair_type(int mi, int mj) { return (mi < 0) != (mj < 0) ? kCross : (mi < 0 ? kSS : kCC); }
+constexpr bool in_list(int type, int mi, int mj) { return type == kAll || pair_type(mi, mj) == type; }
 
 // (l, m) of the block's planes and (L, M) of its harmonic columns
 struct Shape {
@@ -105,49 +114,58 @@ struct Shape {
     }
 };
 
-constexpr int count_terms(int lb, int par)
+constexpr int count_terms(int lb, int par, int type = kAll)
 {
     const Shape s(lb, par);
     int n = 0;
     for (int i = 0; i < s.B; i++)
         for (int j = i; j < s.B; j++)
-            for (int c = 0; c < s.NC; c++)
-                if (coupled(s.l[i], s.m[i], s.l[j], s.m[j], s.L[c], s.M[c])) n++;
+            if (in_list(type, s.m[i], s.m[j]))
+                for (int c = 0; c < s.NC; c++)
+                    if (coupled(s.l[i], s.m[i], s.l[j], s.m[j], s.L[c], s.M[c])) n++;
     return n;
 }
 
-// the block's term list in order
+// the block's term list (or the sub-list of one pair type) in order, with
+// the rows and columns it touches
 template <int NT>
 struct TermList {
     Term t[NT > 0 ? NT : 1];
-    constexpr TermList(int lb, int par) : t{}
+    bool row[64] = {}, col[128] = {};
+    constexpr TermList(int lb, int par, int type) : t{}
     {
         const Shape s(lb, par);
-        int n = 0;
+        int n = 0, g = 0;
         for (int i = 0; i < s.B; i++)
             for (int j = i; j < s.B; j++) {
+                const bool take = in_list(type, s.m[i], s.m[j]);
                 const int f = n;
                 for (int c = 0; c < s.NC; c++)
                     if (coupled(s.l[i], s.m[i], s.l[j], s.m[j], s.L[c], s.M[c])) {
-                        t[n].i = i;
-                        t[n].j = j;
-                        t[n].c = c;
-                        t[n].first = n == f;
-                        n++;
+                        if (take) {
+                            t[n].i = i;
+                            t[n].j = j;
+                            t[n].c = c;
+                            t[n].g = g;
+                            t[n].first = n == f;
+                            row[i] = row[j] = col[c] = true;
+                            n++;
+                        }
+                        g++;
                     }
                 if (n > f) t[n - 1].last = true;
             }
     }
 };
 
-template <int LB, int PAR>
+template <int LB, int PAR, int TYPE = kAll>
 struct Pattern {
     static constexpr int lb = LB;
     static constexpr int par = PAR;
     static constexpr int B = block_size(LB, PAR);
     static constexpr int NC = ncoef(LB);
-    static constexpr int NT = count_terms(LB, PAR);
-    static constexpr TermList<NT> list{LB, PAR};
+    static constexpr int NT = count_terms(LB, PAR, TYPE);
+    static constexpr TermList<NT> list{LB, PAR, TYPE};
 };
 
 // largest block held in harmonic form: its coefficients, vector entries and

@@ -687,9 +687,9 @@ __device__ __forceinline__ void gaunt_term(const double *__restrict__ g, const d
 {
     constexpr gaunt::Term tm = Pat::list.t[T];
     if constexpr (tm.first)
-        e = g[T] * h[tm.c];
+        e = g[tm.g] * h[tm.c];
     else
-        e = fma(g[T], h[tm.c], e);
+        e = fma(g[tm.g], h[tm.c], e);
     if constexpr (tm.last) {
         acc[tm.i] = fma(e, x[tm.j], acc[tm.i]);
         if constexpr (tm.i != tm.j) acc[tm.j] = fma(e, x[tm.i], acc[tm.j]);
@@ -761,6 +761,125 @@ __global__ __launch_bounds__((LB1 >= 0 ? 2 : 1) * kWG) __attribute__((amdgpu_wav
                 gaunt_point<LB1, PAR1, VT>(X, R, H, G1, nc, plane1, mask1, P, p, dot, rr2, vv2);
         } else {
             gaunt_point<LB0, PAR0, VT>(X, R, H, G0, nc, 0, mask0, P, p, dot, rr2, vv2);
+        }
+    }
+    const double s0 = block_sum(dot, red);
+    __syncthreads();
+    const double s1 = block_sum(rr2, red);
+    __syncthreads();
+    const double s2 = block_sum(vv2, red);
+    if (threadIdx.x == 0) {
+        partial[blockIdx.x] = s0;
+        partial_b[blockIdx.x] = s1;
+        partial_c[blockIdx.x] = s2;
+    }
+}
+
+// One sub-list of a block for the wave's 64 points: the x entries and
+// coefficient columns it touches are loaded, its terms accumulated, and the
+// rows it touches written to part[row][lane] (LDS).  Every index is a
+// compile-time constant (index-sequence expansions), so x, acc and h stay in
+// registers.  Lanes past the last point load the last point (in bounds,
+// unconditional loads issue back to back) and are never written back.
+template <class Pat, int R_>
+__device__ __forceinline__ void sub_load_x(double (&x)[Pat::B], double (&acc)[Pat::B], const double *__restrict__ xp,
+                                           int64_t P)
+{
+    acc[R_] = 0.0;
+    if constexpr (Pat::list.row[R_])
+        x[R_] = xp[(size_t)R_ * P];
+    else
+        x[R_] = 0.0;
+}
+template <class Pat, typename VT, int C>
+__device__ __forceinline__ void sub_load_h(double (&h)[Pat::NC], const VT *__restrict__ hp)
+{
+    if constexpr (Pat::list.col[C])
+        h[C] = (double)hp[(size_t)C * kVTile];
+    else
+        h[C] = 0.0;
+}
+template <class Pat, int R_>
+__device__ __forceinline__ void sub_store(const double (&acc)[Pat::B], double (*part)[64], int lane)
+{
+    if constexpr (Pat::list.row[R_]) part[R_][lane] = acc[R_];
+}
+template <class Pat, typename VT, int... Rs, int... Cs>
+__device__ __forceinline__ void sub_run(const double *__restrict__ xp, const VT *__restrict__ hp,
+                                        const double *__restrict__ G, int64_t P, double (*part)[64], int lane,
+                                        std::integer_sequence<int, Rs...>, std::integer_sequence<int, Cs...>)
+{
+    double x[Pat::B], acc[Pat::B], h[Pat::NC];
+    (sub_load_x<Pat, Rs>(x, acc, xp, P), ...);
+    (sub_load_h<Pat, VT, Cs>(h, hp), ...);
+    gaunt_terms<Pat>(G, h, x, acc, std::make_integer_sequence<int, Pat::NT>{});
+    (sub_store<Pat, Rs>(acc, part, lane), ...);
+}
+template <class Pat, typename VT>
+__device__ __forceinline__ void gaunt_sublist(const double *__restrict__ X, const VT *__restrict__ hp,
+                                              const double *__restrict__ G, int plane0, int64_t P, int p,
+                                              double (*part)[64], int lane)
+{
+    sub_run<Pat, VT>(X + (size_t)plane0 * P + p, hp, G, P, part, lane, std::make_integer_sequence<int, Pat::B>{},
+                     std::make_integer_sequence<int, Pat::NC>{});
+}
+
+// The lmax-5 pair of blocks (l = 0, 2, 4 and l = 1, 3, 5; 15 + 21 planes,
+// 1597 terms) split over four waves per 64 points by pair type, so that a
+// wave holds only the coefficient columns and rows of its sub-lists:
+//   wave 0: odd cross (695 multiply-adds), wave 1: odd cos-cos (512),
+//   wave 2: odd sin-sin + even sin-sin (505), wave 3: even cos-cos + even cross (551).
+// Each row of a block gets one same-type partial (cos-cos for m >= 0 rows,
+// sin-sin for m < 0 rows) and one cross partial, exchanged through LDS; after
+// the barrier each wave finishes 9 of the 36 rows as R += mask (same + cross).
+template <typename VT, int W>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, 8))) void k_vgaunt_split(
+    const double *__restrict__ X, double *__restrict__ R, const VT *__restrict__ H, const double *__restrict__ G0,
+    const double *__restrict__ G1, int nc, int plane1, double mask0, double mask1, int64_t P,
+    double *__restrict__ partial, double *__restrict__ partial_b, double *__restrict__ partial_c)
+{
+    constexpr int B0 = gaunt::block_size(4, 0), B1 = gaunt::block_size(5, 1), NR = B0 + B1;
+    static_assert(NR % 4 == 0, "rows are shared out evenly over the four waves");
+    __shared__ double part[2][NR][64];  // [same-type | cross][even rows, then odd rows][lane]
+    __shared__ double red[4];
+    const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
+    const int lane = (int)threadIdx.x & 63;
+    const int p = (int)blockIdx.x * 64 + lane;
+    const bool live = p < P;
+    const int pl = live ? p : (int)(P - 1);
+    const VT *hp = H + (size_t)(pl / kVTile) * nc * kVTile + pl % kVTile;
+    if (wave == 0) {
+        gaunt_sublist<gaunt::Pattern<5, 1, gaunt::kCross>, VT>(X, hp, G1, plane1, P, pl, &part[1][B0], lane);
+    } else if (wave == 1) {
+        gaunt_sublist<gaunt::Pattern<5, 1, gaunt::kCC>, VT>(X, hp, G1, plane1, P, pl, &part[0][B0], lane);
+    } else if (wave == 2) {
+        gaunt_sublist<gaunt::Pattern<5, 1, gaunt::kSS>, VT>(X, hp, G1, plane1, P, pl, &part[0][B0], lane);
+        gaunt_sublist<gaunt::Pattern<4, 0, gaunt::kSS>, VT>(X, hp, G0, 0, P, pl, &part[0][0], lane);
+    } else {
+        gaunt_sublist<gaunt::Pattern<4, 0, gaunt::kCC>, VT>(X, hp, G0, 0, P, pl, &part[0][0], lane);
+        gaunt_sublist<gaunt::Pattern<4, 0, gaunt::kCross>, VT>(X, hp, G0, 0, P, pl, &part[1][0], lane);
+    }
+    __syncthreads();
+    // rows [9 w, 9 w + 9) of (even rows | odd rows); planes are contiguous
+    double dot = 0.0, rr2 = 0.0, vv2 = 0.0;
+    if (live) {
+        constexpr int NW = NR / 4;
+        const int r0 = wave * NW;
+        double rr[NW], xx[NW];
+#pragma unroll
+        for (int k = 0; k < NW; k++) {
+            rr[k] = R[(size_t)(r0 + k) * P + p];
+            xx[k] = X[(size_t)(r0 + k) * P + p];
+        }
+#pragma unroll
+        for (int k = 0; k < NW; k++) {
+            const int r = r0 + k;
+            const double mk = r < B0 ? mask0 : mask1;
+            const double nr = rr[k] + mk * (part[0][r][lane] + part[1][r][lane]);
+            R[(size_t)r * P + p] = nr;
+            dot += xx[k] * nr;
+            rr2 += nr * nr;
+            vv2 += xx[k] * xx[k];
         }
     }
     const double s0 = block_sum(dot, red);
@@ -913,13 +1032,24 @@ template <typename VT>
 static int launch_vgaunt_t(const DevOperator &op, const double *X, double *R, hipStream_t s)
 {
     const VT *H = (const VT *)op.vpk;
-    dim3 grid((unsigned)vpk_ntile(op.P));
-    const dim3 threads((unsigned)(op.nblk * kWG));
-    if ((int64_t)grid.x > op.npartial || (int64_t)grid.x != op.nalpha)
-        return fail(BSL_E_STATE, "partial buffer does not match the potential kernel grid");
     const Block &b0 = op.blk[0], &b1 = op.nblk > 1 ? op.blk[1] : op.blk[0];
     const double mk1 = op.nblk > 1 ? b1.mask : 0.0;
     const int l0 = b0.lb, p0 = b0.par, l1 = op.nblk > 1 ? b1.lb : -1, p1 = op.nblk > 1 ? b1.par : 0;
+    const bool split = harmonic_split(op.nblk, l0, p0, l1, p1, b1.plane0, b0.size);
+    dim3 grid((unsigned)harmonic_workgroups(op.P, split));
+    if ((int64_t)grid.x > op.npartial || (int64_t)grid.x != op.nalpha)
+        return fail(BSL_E_STATE, "partial buffer does not match the potential kernel grid");
+    if (split) {
+        if (op.ncoef != gaunt::ncoef(5)) return fail(BSL_E_STATE, "coefficient columns do not match the blocks");
+        if (op.vb_variant == 0)
+            hipLaunchKernelGGL((k_vgaunt_split<VT, 3>), grid, dim3(256), 0, s, X, R, H, op.gc[0], op.gc[1],
+                               op.ncoef, b1.plane0, b0.mask, mk1, op.P, op.partial, op.partial_b, op.partial_c);
+        else
+            hipLaunchKernelGGL((k_vgaunt_split<VT, 4>), grid, dim3(256), 0, s, X, R, H, op.gc[0], op.gc[1],
+                               op.ncoef, b1.plane0, b0.mask, mk1, op.P, op.partial, op.partial_b, op.partial_c);
+        return hipGetLastError() == hipSuccess ? BSL_OK : fail(BSL_E_HIP, "k_vgaunt_split launch");
+    }
+    const dim3 threads((unsigned)(op.nblk * kWG));
 #define BSL_VG(A, PA, Bb, PB)                                                                              \
     if (l0 == A && p0 == PA && l1 == Bb && (Bb < 0 || p1 == PB)) {                                         \
         if (op.vb_variant == 0)                                                                            \

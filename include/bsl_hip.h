@@ -263,9 +263,11 @@ int bsl_batch_lanczos(bsl_batch *b, int32_t niter, int32_t start, uint64_t seed,
 int bsl_batch_time_steps(bsl_batch *b, int32_t warmup, int32_t steps, int32_t use_graph, double *ms);
 
 // Name (with template arguments, as a profiler prints it) of the potential
-// kernel the context launches, e.g. "k_vgaunt<4, 0, 5, 1, double, 2>" (harmonic
-// form: largest l and parity of each block) or "k_vblock<15, 21, double, 1>"
-// (block form: block sizes); the last argument is the occupancy variant
+// kernel the context launches: "k_vgaunt_split<double, 4>" (harmonic form of
+// the lmax-5 pair of blocks, split over four waves per 64 points),
+// "k_vgaunt<4, 0, 3, 1, double, 2>" (other harmonic-form shapes: largest l and
+// parity of each block) or "k_vblock<15, 21, double, 1>" (block form: block
+// sizes); the last argument is the occupancy (waves per SIMD) of the variant
 // picked by the autotuner at bsl_create.
 // No reference counterpart (measurement support).
 int bsl_potential_kernel(const bsl_ctx *ctx, char *name, int32_t len);

@@ -39,7 +39,7 @@ def test_harmonic_hv_matches_oracle_and_blocks(lmax, sym, basis):
     o = O.Oracle(spec)
     with B.Calculator(basis=basis, **kw) as c:
         assert c.potential_form == "harmonics"
-        assert c.potential_kernel().startswith("k_vgaunt<")
+        assert c.potential_kernel().startswith("k_vgaunt")
         v = np.random.default_rng(100 + lmax).uniform(-1, 1, o.N)
         u0 = np.random.default_rng(200 + lmax).uniform(-1, 1, o.N)
         ref = u0.copy()
@@ -74,12 +74,15 @@ def test_harmonic_vblocks_match_oracle():
 
 def test_harmonic_form_at_production_quadrature():
     """lmax 5 on the 40 x 70 quadrature of the 54^3 workload, both blocks
-    and a ragged last tile (5^3 = 125 points), other potentials."""
-    for spec, kw in (cs.cage_case(5, 5, 40, 70, "All"), cs.cage_grid_case(5, 5, 12, 14, "Odd"),
-                     cs.iho_case(5, 5, 10, 12, x_max=0.4), cs.coulomb_case(5, 4, 9, 10)):
+    and ragged last workgroups (5^3 = 125 points: 61 of the second 64-point
+    group; 7^3 = 343), other potentials."""
+    cases = ((cs.cage_case(5, 5, 40, 70, "All"), "k_vgaunt_split<"), (cs.cage_case(7, 5, 12, 13, "Odd"), "k_vgaunt_split<"),
+             (cs.cage_grid_case(5, 5, 12, 14, "Odd"), "k_vgaunt_split<"),
+             (cs.iho_case(5, 5, 10, 12, x_max=0.4), "k_vgaunt_split<"), (cs.coulomb_case(5, 4, 9, 10), "k_vgaunt<"))
+    for (spec, kw), kernel in cases:
         o = O.Oracle(spec)
         with B.Calculator(**kw) as c:
-            assert c.potential_form == "harmonics"
+            assert c.potential_form == "harmonics" and c.potential_kernel().startswith(kernel)
             v = np.random.default_rng(7).uniform(-1, 1, o.N)
             assert _rel(c.hv(v), o.hv(v)) < 1e-12
 

@@ -815,6 +815,27 @@ __device__ __forceinline__ void sub_run(const double *__restrict__ xp, const VT 
     gaunt_terms<Pat>(G, h, x, acc, std::make_integer_sequence<int, Pat::NT>{});
     (sub_store<Pat, Rs>(acc, part, lane), ...);
 }
+// Two sub-lists in one wave sharing the coefficient columns: B's columns
+// must be among A's (B of a smaller or equal lb: its columns are a prefix).
+template <class PA, class PB, typename VT, int... RAs, int... RBs, int... Cs>
+__device__ __forceinline__ void sub_run2(const double *__restrict__ xpa, const double *__restrict__ xpb,
+                                         const VT *__restrict__ hp, const double *__restrict__ GA,
+                                         const double *__restrict__ GB, int64_t P, double (*parta)[64],
+                                         double (*partb)[64], int lane, std::integer_sequence<int, RAs...>,
+                                         std::integer_sequence<int, RBs...>, std::integer_sequence<int, Cs...>)
+{
+    static_assert(PB::NC <= PA::NC, "the second sub-list reads a prefix of the first one's columns");
+    double xa[PA::B], acca[PA::B], xb[PB::B], accb[PB::B], h[PA::NC];
+    (sub_load_x<PA, RAs>(xa, acca, xpa, P), ...);
+    (sub_load_x<PB, RBs>(xb, accb, xpb, P), ...);
+    (sub_load_h<PA, VT, Cs>(h, hp), ...);
+    gaunt_terms<PA>(GA, h, xa, acca, std::make_integer_sequence<int, PA::NT>{});
+    gaunt_terms<PB>(GB, reinterpret_cast<const double(&)[PB::NC]>(h), xb, accb,
+                    std::make_integer_sequence<int, PB::NT>{});
+    (sub_store<PA, RAs>(acca, parta, lane), ...);
+    (sub_store<PB, RBs>(accb, partb, lane), ...);
+}
+
 template <class Pat, typename VT>
 __device__ __forceinline__ void gaunt_sublist(const double *__restrict__ X, const VT *__restrict__ hp,
                                               const double *__restrict__ G, int plane0, int64_t P, int p,
@@ -853,8 +874,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, 8))) voi
     } else if (wave == 1) {
         gaunt_sublist<gaunt::Pattern<5, 1, gaunt::kCC>, VT>(X, hp, G1, plane1, P, pl, &part[0][B0], lane);
     } else if (wave == 2) {
-        gaunt_sublist<gaunt::Pattern<5, 1, gaunt::kSS>, VT>(X, hp, G1, plane1, P, pl, &part[0][B0], lane);
-        gaunt_sublist<gaunt::Pattern<4, 0, gaunt::kSS>, VT>(X, hp, G0, 0, P, pl, &part[0][0], lane);
+        // the even sin-sin columns (M >= 0, L <= 8) are among the odd ones
+        using PA = gaunt::Pattern<5, 1, gaunt::kSS>;
+        using PB = gaunt::Pattern<4, 0, gaunt::kSS>;
+        static_assert(gaunt::ncoef(4) <= gaunt::ncoef(5), "column prefix");
+        sub_run2<PA, PB, VT>(X + (size_t)plane1 * P + pl, X + pl, hp, G1, G0, P, &part[0][B0], &part[0][0], lane,
+                             std::make_integer_sequence<int, PA::B>{}, std::make_integer_sequence<int, PB::B>{},
+                             std::make_integer_sequence<int, PA::NC>{});
     } else {
         gaunt_sublist<gaunt::Pattern<4, 0, gaunt::kCC>, VT>(X, hp, G0, 0, P, pl, &part[0][0], lane);
         gaunt_sublist<gaunt::Pattern<4, 0, gaunt::kCross>, VT>(X, hp, G0, 0, P, pl, &part[1][0], lane);

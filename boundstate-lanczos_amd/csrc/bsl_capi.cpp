@@ -171,7 +171,7 @@ int build_vpk(bsl_ctx *c, DevOperator &op, int form, int vprec, const double *d_
     }
     if (op.calc != BSL_CALC_QUAD) {
         const Block &b0 = op.blk[0], &b1 = op.blk[op.nblk > 1 ? 1 : 0];
-        const bool split = harmonic_split(op.nblk, b0.lb, b0.par, b1.lb, b1.par, b1.plane0, b0.size);
+        const int split = harmonic_split(op.nblk, b0.lb, b0.par, op.nblk > 1 ? b1.lb : -1, b1.par, b1.plane0, b0.size);
         op.nalpha = form == BSL_VFORM_HARMONICS ? harmonic_workgroups(c->P, split) : ntile * op.nblk;
         if (op.nalpha > op.npartial) return fail(BSL_E_STATE, "partial buffer too small for the potential kernel");
     }
@@ -1135,9 +1135,14 @@ extern "C" int bsl_potential_kernel(const bsl_ctx *c, char *name, int32_t len)
         std::snprintf(buf, sizeof buf, "k_quad_mfma<%d>", op.qmode);
     else if (op.vform == BSL_VFORM_HARMONICS &&
              harmonic_split(op.nblk, op.blk[0].lb, op.blk[0].par, op.nblk > 1 ? op.blk[1].lb : -1,
-                            op.nblk > 1 ? op.blk[1].par : 0, op.blk[op.nblk > 1 ? 1 : 0].plane0, op.blk[0].size))
+                            op.nblk > 1 ? op.blk[1].par : 0, op.blk[op.nblk > 1 ? 1 : 0].plane0,
+                            op.blk[0].size) == 1)
         std::snprintf(buf, sizeof buf, "k_vgaunt_split<%s, %d>", op.vprec == BSL_PREC_FP64 ? "double" : "float",
                       op.vb_variant + 3);
+    else if (op.vform == BSL_VFORM_HARMONICS &&
+             harmonic_split(op.nblk, op.blk[0].lb, op.blk[0].par, -1, 0, 0, op.blk[0].size) == 2)
+        std::snprintf(buf, sizeof buf, "k_vgaunt_split1<%d, %d, %s, %d>", op.blk[0].lb, op.blk[0].par,
+                      op.vprec == BSL_PREC_FP64 ? "double" : "float", op.vb_variant + 3);
     else if (op.vform == BSL_VFORM_HARMONICS)
         std::snprintf(buf, sizeof buf, "k_vgaunt<%d, %d, %d, %d, %s, %d>", op.blk[0].lb, op.blk[0].par,
                       op.nblk > 1 ? op.blk[1].lb : -1, op.nblk > 1 ? op.blk[1].par : 0,

@@ -168,6 +168,19 @@ struct Pattern {
     static constexpr TermList<NT> list{LB, PAR, TYPE};
 };
 
+// Split kernels take every row of a block from one same-type partial
+// (cos-cos rows m >= 0, sin-sin rows m < 0) and one cross partial: each row
+// must appear in the cross sub-list and in its same-type sub-list.
+template <int LB, int PAR>
+constexpr bool split_covers_rows()
+{
+    for (int r = 0; r < block_size(LB, PAR); r++) {
+        if (!Pattern<LB, PAR, kCross>::list.row[r]) return false;
+        if (!Pattern<LB, PAR, kCC>::list.row[r] && !Pattern<LB, PAR, kSS>::list.row[r]) return false;
+    }
+    return true;
+}
+
 // largest block held in harmonic form: its coefficients, vector entries and
 // accumulators stay in registers (66 + 2 x 21 doubles at lb = 5)
 constexpr int kMaxLb = 5;

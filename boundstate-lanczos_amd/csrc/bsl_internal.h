@@ -109,14 +109,19 @@ __host__ __device__ inline size_t vpk_index(int64_t pair, int64_t p, int64_t npa
     return (size_t)(off * ntile * kVTile + ((p / kVTile) * nb + (pair - off)) * kVTile + p % kVTile);
 }
 
-// Workgroups (= partial sums) of the harmonic-form potential kernel: the
-// lmax-5 pair of blocks is split over four waves per 64 points, every other
-// shape takes one point per thread and block (kVTile points per workgroup).
-inline bool harmonic_split(int nblk, int lb0, int par0, int lb1, int par1, int plane1, int size0)
+// Harmonic-form potential kernel of an operator: 1 = the lmax-5 pair of
+// blocks split over four waves per 64 points (k_vgaunt_split), 2 = a single
+// lmax-5 block of the reduced basis (l = 0, 2, 4 or 1, 3, 5) split over three
+// (k_vgaunt_split1), 0 = one point per thread and block (k_vgaunt).  The
+// workgroups (= partial sums) are 64 points for the split kernels and
+// kVTile points otherwise.
+inline int harmonic_split(int nblk, int lb0, int par0, int lb1, int par1, int plane1, int size0)
 {
-    return nblk == 2 && lb0 == 4 && par0 == 0 && lb1 == 5 && par1 == 1 && plane1 == size0;
+    if (nblk == 2 && lb0 == 4 && par0 == 0 && lb1 == 5 && par1 == 1 && plane1 == size0) return 1;
+    if (nblk == 1 && ((lb0 == 5 && par0 == 1) || (lb0 == 4 && par0 == 0))) return 2;
+    return 0;
 }
-inline int64_t harmonic_workgroups(int64_t P, bool split) { return split ? (P + 63) / 64 : vpk_ntile(P); }
+inline int64_t harmonic_workgroups(int64_t P, int split) { return split ? (P + 63) / 64 : vpk_ntile(P); }
 
 // a parity block of the potential: `size` consecutive carried planes from `plane0`
 struct Block {

@@ -861,6 +861,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, 8))) voi
 {
     constexpr int B0 = gaunt::block_size(4, 0), B1 = gaunt::block_size(5, 1), NR = B0 + B1;
     static_assert(NR % 4 == 0, "rows are shared out evenly over the four waves");
+    static_assert(gaunt::split_covers_rows<4, 0>() && gaunt::split_covers_rows<5, 1>(),
+                  "every row needs a same-type and a cross partial");
     __shared__ double part[2][NR][64];  // [same-type | cross][even rows, then odd rows][lane]
     __shared__ double red[4];
     const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
@@ -906,6 +908,67 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, 8))) voi
             dot += xx[k] * nr;
             rr2 += nr * nr;
             vv2 += xx[k] * xx[k];
+        }
+    }
+    const double s0 = block_sum(dot, red);
+    __syncthreads();
+    const double s1 = block_sum(rr2, red);
+    __syncthreads();
+    const double s2 = block_sum(vv2, red);
+    if (threadIdx.x == 0) {
+        partial[blockIdx.x] = s0;
+        partial_b[blockIdx.x] = s1;
+        partial_c[blockIdx.x] = s2;
+    }
+}
+
+// One parity block of the reduced basis (para l = 0, 2, 4 or ortho
+// l = 1, 3, 5) split the same way over three waves per 64 points: wave 0 the
+// cross pairs, wave 1 cos-cos, wave 2 sin-sin; then each wave finishes
+// ceil(B/3) rows as R += mask (same + cross).
+template <int LB, int PAR, typename VT, int W>
+__global__ __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(W, 8))) void k_vgaunt_split1(
+    const double *__restrict__ X, double *__restrict__ R, const VT *__restrict__ H, const double *__restrict__ G0,
+    int nc, double mask0, int64_t P, double *__restrict__ partial, double *__restrict__ partial_b,
+    double *__restrict__ partial_c)
+{
+    constexpr int B = gaunt::block_size(LB, PAR), NW = (B + 2) / 3;
+    static_assert(gaunt::split_covers_rows<LB, PAR>(), "every row needs a same-type and a cross partial");
+    __shared__ double part[2][B][64];  // [same-type | cross][row][lane]
+    __shared__ double red[3];
+    const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
+    const int lane = (int)threadIdx.x & 63;
+    const int p = (int)blockIdx.x * 64 + lane;
+    const bool live = p < P;
+    const int pl = live ? p : (int)(P - 1);
+    const VT *hp = H + (size_t)(pl / kVTile) * nc * kVTile + pl % kVTile;
+    if (wave == 0)
+        gaunt_sublist<gaunt::Pattern<LB, PAR, gaunt::kCross>, VT>(X, hp, G0, 0, P, pl, &part[1][0], lane);
+    else if (wave == 1)
+        gaunt_sublist<gaunt::Pattern<LB, PAR, gaunt::kCC>, VT>(X, hp, G0, 0, P, pl, &part[0][0], lane);
+    else
+        gaunt_sublist<gaunt::Pattern<LB, PAR, gaunt::kSS>, VT>(X, hp, G0, 0, P, pl, &part[0][0], lane);
+    __syncthreads();
+    double dot = 0.0, rr2 = 0.0, vv2 = 0.0;
+    if (live) {
+        const int r0 = wave * NW;
+        double rr[NW], xx[NW];
+#pragma unroll
+        for (int k = 0; k < NW; k++) {
+            const int r = r0 + k < B ? r0 + k : B - 1;
+            rr[k] = R[(size_t)r * P + p];
+            xx[k] = X[(size_t)r * P + p];
+        }
+#pragma unroll
+        for (int k = 0; k < NW; k++) {
+            const int r = r0 + k;
+            if (r < B) {
+                const double nr = rr[k] + mask0 * (part[0][r][lane] + part[1][r][lane]);
+                R[(size_t)r * P + p] = nr;
+                dot += xx[k] * nr;
+                rr2 += nr * nr;
+                vv2 += xx[k] * xx[k];
+            }
         }
     }
     const double s0 = block_sum(dot, red);
@@ -1061,11 +1124,27 @@ static int launch_vgaunt_t(const DevOperator &op, const double *X, double *R, hi
     const Block &b0 = op.blk[0], &b1 = op.nblk > 1 ? op.blk[1] : op.blk[0];
     const double mk1 = op.nblk > 1 ? b1.mask : 0.0;
     const int l0 = b0.lb, p0 = b0.par, l1 = op.nblk > 1 ? b1.lb : -1, p1 = op.nblk > 1 ? b1.par : 0;
-    const bool split = harmonic_split(op.nblk, l0, p0, l1, p1, b1.plane0, b0.size);
+    const int split = harmonic_split(op.nblk, l0, p0, l1, p1, b1.plane0, b0.size);
     dim3 grid((unsigned)harmonic_workgroups(op.P, split));
     if ((int64_t)grid.x > op.npartial || (int64_t)grid.x != op.nalpha)
         return fail(BSL_E_STATE, "partial buffer does not match the potential kernel grid");
-    if (split) {
+    if (split == 2) {
+        if (op.ncoef != gaunt::ncoef(l0)) return fail(BSL_E_STATE, "coefficient columns do not match the block");
+#define BSL_VG1(A, PA)                                                                                      \
+    if (l0 == A && p0 == PA) {                                                                              \
+        if (op.vb_variant == 0)                                                                             \
+            hipLaunchKernelGGL((k_vgaunt_split1<A, PA, VT, 3>), grid, dim3(192), 0, s, X, R, H, op.gc[0],       \
+                               op.ncoef, b0.mask, op.P, op.partial, op.partial_b, op.partial_c);            \
+        else                                                                                                \
+            hipLaunchKernelGGL((k_vgaunt_split1<A, PA, VT, 4>), grid, dim3(192), 0, s, X, R, H, op.gc[0],       \
+                               op.ncoef, b0.mask, op.P, op.partial, op.partial_b, op.partial_c);            \
+        return hipGetLastError() == hipSuccess ? BSL_OK : fail(BSL_E_HIP, "k_vgaunt_split1 launch");       \
+    }
+        BSL_VG1(5, 1) BSL_VG1(4, 0)
+#undef BSL_VG1
+        return fail(BSL_E_STATE, "no single-block split kernel for this block");
+    }
+    if (split == 1) {
         if (op.ncoef != gaunt::ncoef(5)) return fail(BSL_E_STATE, "coefficient columns do not match the blocks");
         if (op.vb_variant == 0)
             hipLaunchKernelGGL((k_vgaunt_split<VT, 3>), grid, dim3(256), 0, s, X, R, H, op.gc[0], op.gc[1],

@@ -78,7 +78,11 @@ def test_harmonic_form_at_production_quadrature():
     group; 7^3 = 343), other potentials."""
     cases = ((cs.cage_case(5, 5, 40, 70, "All"), "k_vgaunt_split<"), (cs.cage_case(7, 5, 12, 13, "Odd"), "k_vgaunt_split<"),
              (cs.cage_grid_case(5, 5, 12, 14, "Odd"), "k_vgaunt_split<"),
-             (cs.iho_case(5, 5, 10, 12, x_max=0.4), "k_vgaunt_split<"), (cs.coulomb_case(5, 4, 9, 10), "k_vgaunt<"))
+             (cs.iho_case(5, 5, 10, 12, x_max=0.4), "k_vgaunt_split<"), (cs.coulomb_case(5, 4, 9, 10), "k_vgaunt<"),
+             ((lambda sp, k: (sp, dict(k, basis="reduced")))(*cs.cage_case(7, 5, 12, 13, "Odd")),
+              "k_vgaunt_split1<5, 1"),
+             ((lambda sp, k: (sp, dict(k, basis="reduced")))(*cs.cage_case(5, 5, 40, 70, "Even")),
+              "k_vgaunt_split1<4, 0"))
     for (spec, kw), kernel in cases:
         o = O.Oracle(spec)
         with B.Calculator(**kw) as c:

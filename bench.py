@@ -277,7 +277,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-eig", action="store_true")
     ap.add_argument("--no-mixed", action="store_true", help="skip the FP32-potential context")
-    ap.add_argument("--batch-steps", type=int, default=20,
+    ap.add_argument("--batch-steps", type=int, default=100,
                     help="lockstep iterations timed for the 32-member batch (0 skips the batch leg)")
     ap.add_argument("--quad-steps", type=int, default=200,
                     help="steps timed per quadrature-calculator precision (0 skips the quadrature leg)")

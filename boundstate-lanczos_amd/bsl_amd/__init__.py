@@ -181,6 +181,21 @@ class Calculator:
         return buf.value.decode()
 
 
+def gaunt_table(lb, par):
+    """Harmonic-form term list of parity block (lb, par) (bsl_gaunt_table):
+    (ijc [nt, 3] int32: row i, column j of the block, coefficient column c;
+    g [nt]: the Gaunt value)."""
+    nt = lib().bsl_gaunt_table(lb, par, 0, None, None)
+    if nt < 0:
+        check(-nt)
+    ijc = np.zeros((nt, 3), dtype=np.int32)
+    g = np.zeros(nt)
+    r = lib().bsl_gaunt_table(lb, par, nt, ijc.ctypes.data, g.ctypes.data)
+    if r != nt:
+        raise BslError(7, "bsl_gaunt_table returned %d for %d terms" % (r, nt))
+    return ijc, g
+
+
 def make_params(*, nx, ny=None, nz=None, x_max, y_max=None, z_max=None, lmax, n_theta, n_phi, mass, rot_const,
                 ceiling, symmetry="All", basis="full", calculator="noquad", precision="fp64", potential="iho",
                 iho_factor=0.0, iho_ceil=0.0, sites=None, q_h2_h=physics.Q_H2_H, q_h2_cm=physics.Q_H2_CM,

@@ -26,6 +26,7 @@ EXPORTS = (
     "bsl_potential_kernel", "bsl_hf_gradient", "bsl_batch_create", "bsl_batch_destroy", "bsl_batch_size",
     "bsl_batch_member", "bsl_batch_lanczos", "bsl_batch_time_steps", "bsl_lancbis_device", "bsl_lancbis_tol",
     "bsl_lancbis_device_tol", "bsl_trivec_tol", "bsl_get_potential_form", "bsl_set_potential_form",
+    "bsl_gaunt_table",
 )
 
 
@@ -88,6 +89,7 @@ def load():
     lib.bsl_potential_kernel.argtypes = [C.c_void_p, C.c_char_p, C.c_int32]
     lib.bsl_get_potential_form.argtypes = [C.c_void_p]
     lib.bsl_set_potential_form.argtypes = [C.c_void_p, C.c_int32]
+    lib.bsl_gaunt_table.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p]
     lib.bsl_hf_gradient.argtypes = [C.c_void_p, C.c_int32, _dp, _dp, _dp]
     lib.bsl_batch_create.argtypes = [C.c_int32, C.c_void_p, C.c_void_p]
     lib.bsl_batch_destroy.argtypes = [C.c_void_p]

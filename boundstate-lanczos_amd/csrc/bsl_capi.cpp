@@ -968,6 +968,30 @@ extern "C" int bsl_get_vblocks(bsl_ctx *c, double *out)
     return BSL_OK;
 }
 
+extern "C" int bsl_gaunt_table(int32_t lb, int32_t par, int32_t cap, int32_t *ijc, double *g)
+{
+    if (lb < 0 || lb > 6 || (par != 0 && par != 1) || lb % 2 != par)
+        return -fail(BSL_E_ARG, "a parity block needs 0 <= lb <= 6 with lb of parity par");
+    std::vector<double> gv;
+    double excluded = 0.0;
+    gaunt_values(lb, par, gv, &excluded);
+    const int nt = (int)gv.size();
+    if (!ijc || !g || cap < nt) return nt;
+    int t = 0;
+    for (int i = 0; i < gaunt::block_size(lb, par); i++)
+        for (int j = i; j < gaunt::block_size(lb, par); j++)
+            for (int c = 0; c < gaunt::ncoef(lb); c++)
+                if (gaunt::coupled(gaunt::block_l(lb, par, i), gaunt::block_m(lb, par, i), gaunt::block_l(lb, par, j),
+                                   gaunt::block_m(lb, par, j), gaunt::coef_L(c), gaunt::coef_M(c))) {
+                    ijc[3 * t + 0] = i;
+                    ijc[3 * t + 1] = j;
+                    ijc[3 * t + 2] = c;
+                    g[t] = gv[t];
+                    t++;
+                }
+    return nt;
+}
+
 extern "C" int bsl_get_potential_form(const bsl_ctx *c)
 {
     if (!c) return fail(BSL_E_ARG, "null argument");

@@ -217,6 +217,15 @@ int bsl_get_vgrid(bsl_ctx *ctx, double *v1, double *v2);
 int bsl_get_potential_form(const bsl_ctx *ctx);
 int bsl_set_potential_form(bsl_ctx *ctx, int32_t form);
 
+/* The harmonic-form term list of parity block (lb, par): the planes are
+ * every l of parity par up to lb in the reference lm order, the columns the
+ * even-L real harmonics (L = 0, 2, .., 2 lb; M = -L .. L).  Term t adds
+ * g[t] * vh[ijc[3t + 2]] to <i|V|j> with i = ijc[3t], j = ijc[3t + 1]
+ * (i <= j).  Returns the number of terms, or minus a BSL_E_* code; writes
+ * them only if cap is at least that (host only, no device).  No reference counterpart
+ * (diagnostics of BSL_VFORM_HARMONICS). */
+int bsl_gaunt_table(int32_t lb, int32_t par, int32_t cap, int32_t *ijc, double *g);
+
 /* Timing of the device recurrence: runs `steps` Lanczos steps from the
  * random start after `warmup` untimed ones.  ms[0] = total time of the timed
  * steps (HIP events on the context stream, no host synchronisation inside);

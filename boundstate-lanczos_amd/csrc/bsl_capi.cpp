@@ -532,12 +532,18 @@ extern "C" int bsl_create(const bsl_params *pp, bsl_ctx **out)
         op.calc = BSL_CALC_QUAD;
         op.wprec = p.precision == BSL_PREC_FP64 ? BSL_PREC_FP64 : BSL_PREC_FP32;
         op.qmode = p.precision;
-        op.na = na;
+        // hemisphere fold: the kernel's columns are the nodes a < ceil(na/2)
+        const int nah = (na + 1) / 2;
+        op.na = nah;
         op.nphi = 2 * nb;
         op.nm = 2 * lmax + 1;
-        op.q_npts = quad_points_per_group(na, nk);
+        op.q_ne = 0;
+        for (int r = 0; r < nk; r++) op.q_ne += t.l[c->carried[r]] % 2 == 0 ? 1 : 0;
+        for (int r = 0; r < op.q_ne; r++)
+            if (t.l[c->carried[r]] % 2 != 0) return fail(BSL_E_STATE, "carried planes are not grouped by l parity");
+        op.q_npts = quad_points_per_group(nah, nk);
         const size_t es = op.wprec == BSL_PREC_FP64 ? sizeof(double) : sizeof(float);
-        const int64_t ntiles = quad_tiles(c->P, na, op.q_npts);
+        const int64_t ntiles = quad_tiles(c->P, nah, op.q_npts);
         BSL_HIP(hipMalloc(&op.wgrid, es * (size_t)ntiles * quad_phi_tiles(op.nphi) * 256));
         double *d_wt = nullptr, *d_wp = nullptr;
         if ((rc = dev_upload(&d_wt, t.wt))) return rc;
@@ -547,11 +553,11 @@ extern "C" int bsl_create(const bsl_params *pp, bsl_ctx **out)
         dev_free(d_wt);
         dev_free(d_wp);
         if (rc) return rc;
-        std::vector<double> qL((size_t)nk * na), qS((size_t)op.nm * op.nphi);
+        std::vector<double> qL((size_t)nk * nah), qS((size_t)op.nm * op.nphi);
         std::vector<int> mpl(nk);
         for (int r = 0; r < nk; r++) {
             const int n = c->carried[r];
-            for (int a = 0; a < na; a++) qL[(size_t)r * na + a] = t.L[(size_t)n * na + a];
+            for (int a = 0; a < nah; a++) qL[(size_t)r * nah + a] = t.L[(size_t)n * na + a];
             mpl[r] = t.m[n] + lmax;
         }
         for (int mi = 0; mi < op.nm; mi++)
@@ -1126,9 +1132,12 @@ extern "C" int bsl_step_bytes(const bsl_ctx *c, double *out)
     double pot_bytes, pot_flops;
     if (op.calc == BSL_CALC_QUAD) {
         const double es = op.wprec == BSL_PREC_FP64 ? 8.0 : 4.0;
-        pot_bytes = es * (double)c->P * (double)c->Q;
-        // two Legendre and two phi transforms plus the pointwise product
-        pot_flops = (double)c->P * (4.0 * op.na * op.nk + 4.0 * op.na * op.nphi * op.nm + op.na * op.nphi);
+        // the hemisphere's weighted grid (op.na = ceil(n_theta / 2) nodes)
+        pot_bytes = es * (double)c->P * (double)op.na * (double)op.nphi;
+        // two Legendre transforms, and per carried l parity two phi
+        // transforms and the pointwise product
+        const double npar = (op.q_ne > 0 && op.q_ne < op.nk) ? 2.0 : 1.0;
+        pot_flops = (double)c->P * (4.0 * op.na * op.nk + npar * (4.0 * op.na * op.nphi * op.nm + op.na * op.nphi));
     } else {
         const double es = op.vprec == BSL_PREC_FP64 ? 8.0 : 4.0;
         const bool harm = op.vform == BSL_VFORM_HARMONICS;

@@ -158,9 +158,10 @@ struct DevOperator {
     double *qL = nullptr;         // [nk][na] normalised Legendre of the carried planes
     double *qS = nullptr;         // [nm][nphi] tesseral trig factors, nphi = 2*nb
     int *mplane = nullptr;        // m + lmax of each carried plane
-    int na = 0, nphi = 0, nm = 0;
+    int na = 0, nphi = 0, nm = 0;  // na: theta nodes of the folded hemisphere, ceil(n_theta / 2)
     int qmode = 0;                // 0 FP64 MFMA, 1 FP32 MFMA, 2 BF16x3 MFMA
     int q_npts = 0;               // grid points per group of the quadrature kernel
+    int q_ne = 0;                 // carried planes of even l (the first q_ne)
     int q_grid = 0;               // its persistent workgroups (resident ones, at most the groups)
     // reduction scratch: the potential kernel writes nalpha per-workgroup
     // partials of v.R (partial), |R|^2 (partial_b) and |v|^2 (partial_c)

@@ -11,6 +11,19 @@
 // are corrected here).  The weighted potential grid W = w_a w_phi V stays
 // resident in HBM and is streamed once per Hv.
 //
+// Hemisphere fold.  The H2 potential is unchanged by inverting the molecular
+// axis, V(pi - theta, phi + pi) = V(theta, phi), both quadrature grids map
+// onto themselves under that inversion, and Y_lm(pi - theta, phi + pi) =
+// (-1)^l Y_lm(theta, phi).  So over the whole grid the products of two
+// harmonics of one l parity count twice what they count over one
+// hemisphere and those of opposite parity cancel: u = 2 [T(c_even)]_even +
+// 2 [T(c_odd)]_odd with T the transform chain above over the hemisphere's
+// nodes a < ceil(na/2) (the equator row of an odd na maps onto itself
+// through phi + pi and keeps weight 1).  The kernel carries the two parities
+// through separate phi transforms (the forward Legendre sum split by the
+// plane's parity, the backward one reading the plane's own) over half the
+// columns, and W holds the hemisphere only, with the factor 2 folded in.
+//
 // The two phi transforms are GEMMs batched over columns (P, a): with
 // columns as N and phi as M (forward) or K (backward), on MFMA 16x16x4
 // (FP64 or FP32, FP32 accumulation for FP32) or 16x16x16 BF16 (three split
@@ -126,7 +139,7 @@ __host__ __device__ inline QuadLds quad_lds(int nk, int na, int npts, int kf, in
     }
     o = (o + 15) & ~(size_t)15;
     q.off_x = o;
-    o += (size_t)es * kQW * 16 * q.fm;             // per-wave F' of one column tile [16][m]
+    o += (size_t)es * 2 * kQW * 16 * q.fm;         // per-wave F' of one column tile [16][m], both parities
     o = (o + 15) & ~(size_t)15;
     q.off_u = o;
     o += sizeof(double) * (size_t)npts * nk;       // u [npts][nk]
@@ -182,7 +195,7 @@ __global__ __launch_bounds__(kQWG) void k_quad_mfma(const double *__restrict__ X
                                                     const double *__restrict__ Sg, const int *__restrict__ mplane,
                                                     const double *__restrict__ mask,
                                                     const Scal *__restrict__ sc, int64_t P, int nk, int na,
-                                                    int nphi, int nm, int ppw, int64_t ngroups,
+                                                    int ne, int nphi, int nm, int ppw, int64_t ngroups,
                                                     double *__restrict__ partial, double *__restrict__ partial_b,
                                                     double *__restrict__ partial_c)
 {
@@ -240,7 +253,11 @@ __global__ __launch_bounds__(kQWG) void k_quad_mfma(const double *__restrict__ X
     const int tw = ppw * na / 16;
     const int pw0 = wid * ppw;  // first point of this wave within the group
     const T *W = (const T *)Wv + (size_t)lane * 4;
-    T *xs = (T *)(smem + q.off_x) + wid * 16 * q.fm;
+    T *xs = (T *)(smem + q.off_x) + wid * 16 * q.fm;          // even-l F'
+    T *xo = (T *)(smem + q.off_x) + (kQW + wid) * 16 * q.fm;  // odd-l F'
+    // both l parities carried (full basis, uniform): two transform chains;
+    // otherwise the one parity takes the first chain
+    const bool two = ne > 0 && ne < nk;
     const int mr = lane < nk ? mplane[lane] : 0;
     int pcur = -1;
     double uacc = 0.0;
@@ -257,84 +274,123 @@ __global__ __launch_bounds__(kQWG) void k_quad_mfma(const double *__restrict__ X
         // Legendre forward on MFMA: f(m, col) = sum_r Sel[m][r] L_r(a) c_r(p)
         // over the carried planes, the B operand (k = r, column l16) being
         // this lane's own products; item i of lane group kq ends up holding
-        // the m of the forward phi operand (see Sel)
-        acc_t fa = {0, 0, 0, 0};
+        // the m of the forward phi operand (see Sel).  Even-l planes
+        // (r < ne) feed fe, odd-l planes fo.
+        acc_t fe = {0, 0, 0, 0}, fo = {0, 0, 0, 0};
         for (int s4 = 0; s4 < nks; s4++) {
             const int r = 4 * s4 + kq;
             const double prod = r < nk ? (double)Ls[r * na + ac] * cs[pc * nk + r] : 0.0;
-            fa = mfma16(Sel[s4 * 64 + lane], (T)prod, fa);
+            const T sel = Sel[s4 * 64 + lane];
+            fe = mfma16(sel, (T)(two && r >= ne ? 0.0 : prod), fe);
+            if (two) fo = mfma16(sel, (T)(r >= ne ? prod : 0.0), fo);
         }
-        acc_t acc2 = {0, 0, 0, 0};
+        acc_t ae = {0, 0, 0, 0}, ao = {0, 0, 0, 0};
         if constexpr (MODE == 2) {
             const short *fh = (const short *)(smem + q.off_s), *fl = fh + 16 * NT * 16;
             const short *bh = (const short *)(smem + q.off_b), *bl = bh + 16 * q.bp;
-            s4v bfh, bfl;
+            s4v beh, bel, boh, bol;
 #pragma unroll
             for (int i = 0; i < 4; i++) {
                 // row 4*kq + i of the 16-row K block (rows >= nm are zero)
                 short h, l;
-                bf16_split(fa[i], h, l);
-                bfh[i] = h;
-                bfl[i] = l;
+                bf16_split(fe[i], h, l);
+                beh[i] = h;
+                bel[i] = l;
+                bf16_split(fo[i], h, l);
+                boh[i] = h;
+                bol[i] = l;
             }
 #pragma unroll
             for (int t = 0; t < NT; t++) {
                 const s4v ah = *(const s4v *)(fh + (16 * t + l16) * 16 + 4 * kq);
                 const s4v al = *(const s4v *)(fl + (16 * t + l16) * 16 + 4 * kq);
-                f4v g = {0.f, 0.f, 0.f, 0.f};
-                g = mfma_bf16(ah, bfh, g);
-                g = mfma_bf16(ah, bfl, g);
-                g = mfma_bf16(al, bfh, g);
-                const f4v h4 = g * wc[t];
+                f4v ge = {0.f, 0.f, 0.f, 0.f}, go = {0.f, 0.f, 0.f, 0.f};
+                ge = mfma_bf16(ah, beh, ge);
+                ge = mfma_bf16(ah, bel, ge);
+                ge = mfma_bf16(al, beh, ge);
+                if (two) {
+                    go = mfma_bf16(ah, boh, go);
+                    go = mfma_bf16(ah, bol, go);
+                    go = mfma_bf16(al, boh, go);
+                }
+                const f4v he = ge * wc[t], ho = go * wc[t];
                 if constexpr (decltype(prefetch)::value) {
                     wc[t] = load_w<MODE>(Wn + (size_t)t * 256);
                     __builtin_amdgcn_sched_barrier(kKeepLoads);
                 }
+                const s4v a2h = *(const s4v *)(bh + l16 * q.bp + 16 * t + 4 * kq);
+                const s4v a2l = *(const s4v *)(bl + l16 * q.bp + 16 * t + 4 * kq);
                 s4v hh, hl;
 #pragma unroll
                 for (int i = 0; i < 4; i++) {
                     short h, l;
-                    bf16_split(h4[i], h, l);
+                    bf16_split(he[i], h, l);
                     hh[i] = h;
                     hl[i] = l;
                 }
-                const s4v a2h = *(const s4v *)(bh + l16 * q.bp + 16 * t + 4 * kq);
-                const s4v a2l = *(const s4v *)(bl + l16 * q.bp + 16 * t + 4 * kq);
-                acc2 = mfma_bf16(a2h, hh, acc2);
-                acc2 = mfma_bf16(a2h, hl, acc2);
-                acc2 = mfma_bf16(a2l, hh, acc2);
+                ae = mfma_bf16(a2h, hh, ae);
+                ae = mfma_bf16(a2h, hl, ae);
+                ae = mfma_bf16(a2l, hh, ae);
+                if (two) {
+#pragma unroll
+                    for (int i = 0; i < 4; i++) {
+                        short h, l;
+                        bf16_split(ho[i], h, l);
+                        hh[i] = h;
+                        hl[i] = l;
+                    }
+                    ao = mfma_bf16(a2h, hh, ao);
+                    ao = mfma_bf16(a2h, hl, ao);
+                    ao = mfma_bf16(a2l, hh, ao);
+                }
             }
         } else {
             const T *Ss = (const T *)(smem + q.off_s);
-            T bF[KS];  // m = 4s + kq of this lane's column
+            T bFe[KS], bFo[KS];  // m = 4s + kq of this lane's column
 #pragma unroll
-            for (int s = 0; s < KS; s++) bF[s] = fa[s];
+            for (int s = 0; s < KS; s++) {
+                bFe[s] = fe[s];
+                bFo[s] = fo[s];
+            }
 #pragma unroll
             for (int t = 0; t < NT; t++) {
-                acc_t g = {0, 0, 0, 0};
+                acc_t ge = {0, 0, 0, 0}, go = {0, 0, 0, 0};
 #pragma unroll
-                for (int s = 0; s < KS; s++) g = mfma16(Ss[(16 * t + l16) * q.sp + 4 * s + kq], bF[s], g);
-                const acc_t h4 = g * wc[t];
+                for (int s = 0; s < KS; s++) ge = mfma16(Ss[(16 * t + l16) * q.sp + 4 * s + kq], bFe[s], ge);
+                if (two) {
+#pragma unroll
+                    for (int s = 0; s < KS; s++) go = mfma16(Ss[(16 * t + l16) * q.sp + 4 * s + kq], bFo[s], go);
+                }
+                const acc_t he = ge * wc[t], ho = go * wc[t];
                 if constexpr (decltype(prefetch)::value) {
                     wc[t] = load_w<MODE>(Wn + (size_t)t * 256);
                     __builtin_amdgcn_sched_barrier(kKeepLoads);
                 }
 #pragma unroll
-                for (int i = 0; i < 4; i++)
-                    acc2 = mfma16(Ss[(16 * t + acc_row<F64>(kq, i)) * q.sp + l16], h4[i], acc2);
+                for (int i = 0; i < 4; i++) ae = mfma16(Ss[(16 * t + acc_row<F64>(kq, i)) * q.sp + l16], he[i], ae);
+                if (two) {
+#pragma unroll
+                    for (int i = 0; i < 4; i++)
+                        ao = mfma16(Ss[(16 * t + acc_row<F64>(kq, i)) * q.sp + l16], ho[i], ao);
+                }
             }
         }
-        // F' of the tile -> per-wave LDS, then the Legendre backward of its
-        // 16 columns by lane r (one carried plane per lane)
+        // F' of the tile (both parities) -> per-wave LDS, then the Legendre
+        // backward of its 16 columns by lane r (one carried plane per lane,
+        // reading its own parity's F')
 #pragma unroll
         for (int i = 0; i < 4; i++) {
             const int m = acc_row<F64>(kq, i);
-            if (m < KF) xs[l16 * q.fm + m] = acc2[i];
+            if (m < KF) {
+                xs[l16 * q.fm + m] = ae[i];
+                xo[l16 * q.fm + m] = ao[i];
+            }
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         if (lane < nk) {
+            const T *xf = (two && lane >= ne) ? xo : xs;
             // independent products first, then a segmented sum over the
             // points the 16 columns belong to
             const int a0 = (16 * n) % na;
@@ -342,7 +398,7 @@ __global__ __launch_bounds__(kQWG) void k_quad_mfma(const double *__restrict__ X
             int aa = a0;
 #pragma unroll
             for (int c = 0; c < 16; c++) {
-                v[c] = (double)Ls[lane * na + aa] * (double)xs[c * q.fm + mr];
+                v[c] = (double)Ls[lane * na + aa] * (double)xf[c * q.fm + mr];
                 if (++aa == na) aa = 0;
             }
             int pp = pw0 + (16 * n) / na;
@@ -421,14 +477,17 @@ __global__ __launch_bounds__(kQWG) void k_quad_mfma(const double *__restrict__ X
     }
 }
 
-// W[tile][t][lane][i] = w_a w_b vg[p][half*na*nb + a*nb + b] for column
-// col = 16*tile + lane%16 = p*na + a and phi = 16*t + acc_row(lane/16, i) =
-// half*nb + b; zero outside the grid (padding columns and phi).  FP64 grids
-// feed the FP64 instruction, FP32 grids the FP32 and BF16 ones.
+// W[tile][t][lane][i] = f_a w_a w_b vg[p][half*na*nb + a*nb + b] for column
+// col = 16*tile + lane%16 = p*nah + a (the hemisphere's nodes a < nah =
+// ceil(na/2)) and phi = 16*t + acc_row(lane/16, i) = half*nb + b; f_a = 2,
+// or 1 on the equator row of an odd na (hemisphere fold); zero outside the
+// grid (padding columns and phi).  FP64 grids feed the FP64 instruction,
+// FP32 grids the FP32 and BF16 ones.
 template <typename WT>
 __global__ void k_weight_grid(const double *__restrict__ vg, WT *__restrict__ W, const double *__restrict__ wt,
                               const double *__restrict__ wp, int64_t P, int na, int nb, int64_t ntiles, int ntp)
 {
+    const int nah = (na + 1) / 2;
     const int nphi = 2 * nb;
     const int64_t total = ntiles * ntp * 256;
     for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
@@ -439,12 +498,13 @@ __global__ void k_weight_grid(const double *__restrict__ vg, WT *__restrict__ W,
         const int64_t tile = rest / ntp;
         const int64_t col = 16 * tile + (lane & 15);
         const int ph = 16 * t + acc_row<sizeof(WT) == 8>(lane >> 4, i);
-        const int64_t p = col / na;
-        const int a = (int)(col - p * na);
+        const int64_t p = col / nah;
+        const int a = (int)(col - p * nah);
         double v = 0.0;
         if (p < P && ph < nphi) {
             const int half = ph / nb, b = ph - half * nb;
-            v = wt[a] * wp[b] * vg[(size_t)p * 2 * na * nb + (size_t)half * na * nb + (size_t)a * nb + b];
+            const double fold = (na % 2 == 1 && a == nah - 1) ? 1.0 : 2.0;
+            v = fold * wt[a] * wp[b] * vg[(size_t)p * 2 * na * nb + (size_t)half * na * nb + (size_t)a * nb + b];
         }
         W[idx] = (WT)v;
     }
@@ -480,7 +540,7 @@ int64_t quad_tiles(int64_t P, int na, int npts)
 int launch_weight_grid(const double *vg, void *W, int wprec, const double *wt, const double *wp, int64_t P,
                        int na, int nb, int npts, hipStream_t s)
 {
-    const int64_t ntiles = quad_tiles(P, na, npts);
+    const int64_t ntiles = quad_tiles(P, (na + 1) / 2, npts);
     const int ntp = quad_phi_tiles(2 * nb);
     const int64_t total = ntiles * ntp * 256;
     int64_t g = (total + 255) / 256;
@@ -504,7 +564,7 @@ int launch_weight_grid(const double *vg, void *W, int wprec, const double *wt, c
                 *occ = 0;                                                                                  \
         } else {                                                                                           \
             hipLaunchKernelGGL((k_quad_mfma<M, NTV, KSV>), dim3(grid), dim3(kQWG), lds, s, X, R, op.wgrid,  \
-                               op.qL, op.qS, op.mplane, op.mask, sc, op.P, op.nk, op.na, op.nphi,   \
+                               op.qL, op.qS, op.mplane, op.mask, sc, op.P, op.nk, op.na, op.q_ne, op.nphi, \
                                op.nm, ppw, ngroups, op.partial, op.partial_b, op.partial_c);               \
         }                                                                                                  \
     } while (0)

@@ -59,8 +59,13 @@ def quad_leg(B, args):
                      "achieved_GBs": ach, "hbm_frac": ach / HBM_PEAK_GBS,
                      "flops_per_launch": sb[4] - 2.0 * 3 * args.n * c.N - 4.0 * c.N,
                      "tflops": (sb[4] - 2.0 * 3 * args.n * c.N - 4.0 * c.N) / (ms[1] * 1e-3) / 1e12}
-        print("quad %s: prep %.2f s, %.1f steps/s, potential %.3f ms (%.0f GB/s)"
-              % (prec, prep, sps, ms[1], ach), file=sys.stderr, flush=True)
+        # algorithmic FLOP/s against the dense MFMA peak of the instruction the
+        # mode runs on (BF16x3: three BF16 products per FP32 one)
+        peak = {"fp64": 78.6, "fp32": 157.3, "bf16x3": 2500.0 / 3.0}[prec]
+        out[prec]["mfma_peak_tflops"] = peak
+        out[prec]["mfma_frac"] = out[prec]["tflops"] / peak
+        print("quad %s: prep %.2f s, %.1f steps/s, potential %.3f ms (%.0f GB/s, %.1f TFLOP/s)"
+              % (prec, prep, sps, ms[1], ach, out[prec]["tflops"]), file=sys.stderr, flush=True)
         del c
     out["fp32_vs_fp64"] = out["fp32"]["steps_per_s"] / out["fp64"]["steps_per_s"]
     out["bf16x3_vs_fp64"] = out["bf16x3"]["steps_per_s"] / out["fp64"]["steps_per_s"]

@@ -222,32 +222,67 @@ def traffic_from_profiles(tag, kernel):
 
 
 def cpu_baseline(B, calc, args):
-    """The oracle (reference algorithm: serial kinetic term, OpenMP potential
-    term) on this host, a few steps of the same operator: the potential
-    blocks are taken from the GPU context so that the CPU does not redo the
-    preparation."""
+    """CPU time of the same operator on this host.  Where the reference's
+    own calculator was built (oracle/_ref, compiled from the reference
+    sources), that is timed: HvInterfaceSetup / HvPrep / Hv of
+    linRotCartSph_NoQuad_ContGrid_Alavi_SPCE on the same grid, basis, mass
+    and rotor constant, prepared on a 1 x 1 quadrature and then given the GPU
+    context's V blocks (so the CPU does not redo the 40 x 70 preparation),
+    one Hv per step.  The oracle (the same algorithm restated: serial kinetic
+    term, OpenMP potential term) is timed on full Lanczos steps beside it,
+    and the two Hv results are compared."""
     sys.path.insert(0, os.path.join(REPO, "tests"))
+    import tempfile
+
     import _oracle as O
     from bsl_amd import cage, physics
     pos, q, lj = cage.synthetic_cage(0)
     typ = bytes([ord("O") if f else ord("H") for f in lj])
     lj_a, lj_b = cage.baseline_lj()
+    mass, rot = 2.016, physics.cm1_to_kjmol(59.322)
     spec = dict(nx=args.n, ny=args.n, nz=args.n, x_max=args.x_max, y_max=args.x_max, z_max=args.x_max, lmax=5,
-                na=40, nb=70, mass=2.016, rot_const=physics.cm1_to_kjmol(59.322), ceiling=18.0, sym=0,
+                na=40, nb=70, mass=mass, rot_const=rot, ceiling=18.0, sym=0,
                 pot_kind=1, site_pos=pos, site_q=q, site_type=typ, q_h2_h=physics.Q_H2_H,
                 q_h2_cm=physics.Q_H2_CM, h2_half_bond=physics.H2_HALF_BOND, lj_a=lj_a, lj_b=lj_b)
-    vb = calc.vblocks()
-    o = O.Oracle(spec, vblocks=vb)
-    del vb
+    vb = np.ascontiguousarray(calc.vblocks().ravel())
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    o = O.Oracle(spec, vblocks=vb.reshape(args.n, args.n, args.n, 36, 36))
     steps = args.cpu_steps
     o.lanczos(1)  # warm-up (page faults, thread pool)
     t0 = time.perf_counter()
     o.lanczos(steps)
     dt = time.perf_counter() - t0
-    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return {"value": steps / dt, "unit": "steps/s", "cores": cores, "kind": "port",
-            "sample": "%d Lanczos steps of the same 54^3 x 36 operator (oracle Hv: serial kinetic term, "
-                      "OpenMP potential term, as the reference), %.1f s" % (steps, dt)}
+    port = {"value": steps / dt, "unit": "steps/s", "cores": cores, "kind": "port",
+            "sample": "%d Lanczos steps of the same %d^3 x 36 operator (oracle: serial kinetic term, OpenMP "
+                      "potential term, as the reference), %.1f s" % (steps, args.n, dt)}
+    if not O.ref_available() or args.ref_hv_calls <= 0:
+        return port
+    R = O.ref_lib()
+    with tempfile.TemporaryDirectory() as tmp:
+        deck = os.path.join(tmp, "hv.txt")
+        geo = os.path.join(REPO, "tests", "golden", "sII_smallCage11_CM_EA_ZYZ.xyz")  # prep only: V is set below
+        O.write_ref_hv_deck(deck, args.n, args.x_max, 5, 1, 1, mass, rot, geo, 18.0, "All")
+        n = R.ref_setup(b"linRotCartSph_NoQuad_ContGrid_Alavi_SPCE", deck.encode(), None)
+    if n != calc.N or R.ref_set_vblocks(vb, vb.size) != 0:
+        return port
+    del vb
+    v = np.random.default_rng(1).uniform(-1, 1, calc.N)
+    u = np.zeros(calc.N)
+    R.ref_hv(v, u)  # warm-up, and the parity check against the oracle
+    ref_diff = float(np.max(np.abs(u - o.hv(v))) / np.max(np.abs(u)))
+    k = args.ref_hv_calls
+    t0 = time.perf_counter()
+    for _ in range(k):
+        u[:] = 0.0
+        R.ref_hv(v, u)
+    dt = time.perf_counter() - t0
+    return {"value": k / dt, "unit": "steps/s", "cores": cores, "kind": "reference",
+            "sample": "%d Hv calls of the reference's own compiled calculator (oracle/_ref: "
+                      "linRotCartSph_NoQuad_ContGrid_Alavi_SPCE, serial kinetic term with its index reshuffles, "
+                      "OpenMP potential term) on the same %d^3 x 36 operator, its V blocks set from the GPU "
+                      "context, %.1f s; one Hv per step (the recurrence's seven serial N-length passes not "
+                      "included)" % (k, args.n, dt),
+            "ref_vs_oracle_hv_max_rel": ref_diff, "port": port}
 
 
 def eig_error(B, c64, c32, niter):
@@ -279,6 +314,8 @@ def main():
     ap.add_argument("--x-max", type=float, default=0.8)
     ap.add_argument("--eig-steps", type=int, default=10000)
     ap.add_argument("--cpu-steps", type=int, default=40)
+    ap.add_argument("--ref-hv-calls", type=int, default=32,
+                    help="Hv calls of the reference's compiled calculator timed for cpu_baseline (0: the oracle only)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-eig", action="store_true")
     ap.add_argument("--no-mixed", action="store_true", help="skip the FP32-potential context")
@@ -353,8 +390,11 @@ def main():
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(B, c64, args)
-        print("cpu baseline: %.4f steps/s on %d threads" % (cpu["value"], cpu["cores"]), file=sys.stderr,
-              flush=True)
+        print("cpu baseline (%s): %.4f steps/s on %d threads%s" % (
+            cpu["kind"], cpu["value"], cpu["cores"],
+            "; oracle %.4f steps/s, reference Hv vs oracle Hv %.1e" % (cpu["port"]["value"],
+                                                                      cpu["ref_vs_oracle_hv_max_rel"])
+            if "port" in cpu else ""), file=sys.stderr, flush=True)
 
     line = {
         "metric": "Lanczos steps/s at 54³×36 basis, mixed vs FP64 GPU; eig err cm⁻¹ vs FP64",

@@ -47,6 +47,17 @@ extern "C" __attribute__((visibility("default"))) int ref_vblocks(double *out, l
     return 0;
 }
 
+// replaces the V blocks of a prepared NoQuad calculator (same order): the
+// reference's Hv then applies the given potential with its own kinetic,
+// rotor and mask terms
+extern "C" __attribute__((visibility("default"))) int ref_set_vblocks(const double *in, long long count)
+{
+    interfaceStor *s = reinterpret_cast<interfaceStor *>(g_general);
+    if (!s->potential || !s->potential->fullPotential) return -1;
+    std::memcpy(s->potential->fullPotential, in, sizeof(double) * (size_t)count);
+    return 0;
+}
+
 // kinetic matrix of axis 0/1/2 and the grid
 extern "C" __attribute__((visibility("default"))) void ref_kin(int axis, double *kin, double *grid)
 {

@@ -273,8 +273,23 @@ def genran(seed, n):
 
 
 # --------------------------------------------------------------------------
-# reference harness (only in the build container)
+# reference harness (oracle/_ref: built from the reference sources in the
+# build container; the built library travels with the tree)
 # --------------------------------------------------------------------------
+
+def write_ref_hv_deck(path, nx, x_max, lmax, na, nb, mass, rot_const, geo, ceiling, sym):
+    """The reference's 22-token Hv deck (HvPrep_Internal_NoQuad,
+    linRotCartSphHvFullVPrepRoutines.cpp:248-321): a cubic grid; the
+    grid-mapped universe fields are unused by the ContGrid calculators but
+    must contain the box (:330-348)."""
+    u = max(0.48, float(x_max))
+    vals = [nx, x_max, nx, x_max, nx, x_max, lmax, na, nb, 2, u, 2, u, 2, u, mass, rot_const, geo,
+            "deck", ceiling, "FALSE", sym]
+    with open(path, "w") as f:
+        f.write("#\n#\n")
+        for i, v in enumerate(vals):
+            f.write("v%d= %s\n" % (i, repr(float(v)) if isinstance(v, float) else v))
+
 
 def ref_available():
     return os.path.exists(REF_SO)
@@ -287,6 +302,8 @@ def ref_lib():
     lib.ref_hv.argtypes = [_dp, _dp]
     lib.ref_vblocks.restype = C.c_int
     lib.ref_vblocks.argtypes = [_dp, C.c_longlong]
+    lib.ref_set_vblocks.restype = C.c_int
+    lib.ref_set_vblocks.argtypes = [_dp, C.c_longlong]
     lib.ref_kin.argtypes = [C.c_int, _dp, _dp]
     lib.ref_quad.argtypes = [_dp, _dp, _dp, _dp, _dp, _dp, _dp]
     lib.ref_spce_sites.restype = C.c_int

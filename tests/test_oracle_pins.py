@@ -483,3 +483,31 @@ def test_reference_build_bitwise(tmp_path, case):
     u = np.zeros(N)
     R.ref_hv(v, u)
     assert np.array_equal(u, o.hv(v))
+
+
+@pytest.mark.skipif(not O.ref_available(), reason="reference build (oracle/_ref) not present")
+def test_reference_hv_with_injected_blocks(tmp_path):
+    """The reference's Hv on blocks set from outside (ref_set_vblocks, which
+    the CPU baseline of bench.py uses to time the reference on the GPU
+    context's blocks): a reference prepared on a 1 x 1 quadrature, with the
+    oracle's blocks of a 6 x 7 quadrature set in, gives the oracle's Hv of
+    the 6 x 7 deck bitwise, and differs from it without them."""
+    nx, lmax, na, nb, sym = 5, 3, 6, 7, "All"
+    geo = os.path.join(cs.GOLDEN, "sII_smallCage11_CM_EA_ZYZ.xyz")
+    deck = str(tmp_path / "hv.txt")
+    O.write_ref_hv_deck(deck, nx, 0.4, lmax, 1, 1, cs.MASS, cs.B_ROT, geo, 18.0, sym)
+    R = O.ref_lib()
+    N = R.ref_setup(b"linRotCartSph_NoQuad_ContGrid_Alavi_SPCE", deck.encode(), None)
+    spec, _ = cs.cage_case(nx, lmax, na, nb, sym)
+    o = O.Oracle(spec)
+    assert N == o.N
+    v = np.random.default_rng(17).uniform(-1, 1, N)
+    u1 = np.zeros(N)
+    R.ref_hv(v, u1)
+    ref = o.hv(v)
+    assert not np.array_equal(u1, ref)
+    vb = np.ascontiguousarray(o.vblocks().ravel())
+    assert R.ref_set_vblocks(vb, vb.size) == 0
+    u = np.zeros(N)
+    R.ref_hv(v, u)
+    assert np.array_equal(u, ref)

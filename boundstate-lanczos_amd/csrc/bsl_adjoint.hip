@@ -22,12 +22,31 @@ namespace bsl {
 constexpr int kAWG = 256;
 constexpr double kCoulombK = 138.9354846111096;
 
-// omega[p][q] = [vg[p][q] < ceiling] w_q sum_k g_k (sum_r Y[r][q] psi_k[r][p])^2,
-// written over vg.  psi: [nvec][nk][P]; Y: [nk][Q]; one workgroup per point.
+// mask bit n = [vg[n] < ceiling], 64 nodes per word: one wave per word, the
+// wave's lanes read 64 consecutive entries (coalesced) and lane 0 stores the
+// ballot.  The loop bound is uniform over a wave, so every ballot is whole.
+__global__ __launch_bounds__(kAWG) void k_unclamped_mask(const double *__restrict__ vg, int64_t n, double ceiling,
+                                                         uint64_t *__restrict__ mask)
+{
+    const int lane = threadIdx.x & 63;
+    const int64_t nw = (n + 63) / 64;
+    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int64_t stride = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    for (int64_t w = wave; w < nw; w += stride) {
+        const int64_t idx = w * 64 + lane;
+        const bool keep = idx < n && vg[idx] < ceiling;
+        const uint64_t bits = __ballot(keep);
+        if (lane == 0) mask[w] = bits;
+    }
+}
+
+// om[p][q] = [unclamped bit p*Q + q] w_q sum_k g_k (sum_r Y[r][q] psi_k[r][p])^2.
+// psi: [nvec][nk][P]; Y: [nk][Q]; one workgroup per point.
 __global__ __launch_bounds__(kAWG) void k_hf_density(const double *__restrict__ psi, int nvec,
                                                      const double *__restrict__ g, const double *__restrict__ Y,
-                                                     const double *__restrict__ wq, double *vg, int64_t P, int nk,
-                                                     int Q, double ceiling)
+                                                     const double *__restrict__ wq,
+                                                     const uint64_t *__restrict__ unclamped, double *__restrict__ om,
+                                                     int64_t P, int nk, int Q)
 {
     extern __shared__ double cpsi[];  // [nvec][nk]
     const int64_t p = blockIdx.x;
@@ -38,17 +57,16 @@ __global__ __launch_bounds__(kAWG) void k_hf_density(const double *__restrict__ 
     __syncthreads();
     for (int q = threadIdx.x; q < Q; q += blockDim.x) {
         const size_t idx = (size_t)p * Q + q;
-        const double v = vg[idx];
-        double om = 0.0;
-        if (v < ceiling) {
+        double o = 0.0;
+        if ((unclamped[idx >> 6] >> (idx & 63)) & 1) {
             for (int k = 0; k < nvec; k++) {
                 double s = 0.0;
                 for (int r = 0; r < nk; r++) s += Y[(size_t)r * Q + q] * cpsi[k * nk + r];
-                om += g[k] * s * s;
+                o += g[k] * s * s;
             }
-            om *= wq[q];
+            o *= wq[q];
         }
-        vg[idx] = om;
+        om[idx] = o;
     }
 }
 
@@ -65,11 +83,33 @@ __device__ inline double a_block_sum(double v, double *lds)
     return s;  // valid in thread 0
 }
 
-// Per workgroup (npb consecutive points, threads over (point, q) pairs):
+// 1/sqrt(x) for x > 0: the hardware estimate (about 23 correct bits) and two
+// Newton steps y <- y (3/2 - x y^2 / 2), each of which squares the relative
+// error, so the result is within a few ulp of the rounded value.
+__device__ inline double a_rsqrt(double x)
+{
+    const double h = 0.5 * x;
+    double y = __builtin_amdgcn_rsq(x);
+    y = fma(y, fma(-h * y, y, 0.5), y);
+    y = fma(y, fma(-h * y, y, 0.5), y);
+    return y;
+}
+
+// Per workgroup (npb consecutive points):
 // partial[wg][4*s + {0,1,2,3}] = d/d(q_s, x_s, y_s, z_s), then
 // partial[wg][4*ns + {0,1,2,3}] = d/d(q_h2_h, q_h2_cm, A, B).
-// The H2 centre term only depends on the point: its weight is
-// omega_cm(p) = sum_q omega(p, q) (unclamped nodes only, as omega).
+// Only pairs (point, node) with omega != 0 contribute, and clamped nodes
+// (most of a box that reaches into the cage walls) have omega = 0, so the
+// workgroup compacts its pairs: each pass over kAWG pairs appends those with
+// omega != 0 (in pair order, by a ballot prefix) to an LDS stage of omega and
+// the two H positions; whenever kAWG are staged, thread t accumulates site
+// sb + t % 64 over a quarter (t / 64) of them.  Per site only two sums are
+// carried over the H atoms h of the pairs,
+//   S = sum omega / |h - s|,   G = sum omega (h - s) / |h - s|^3,
+// since d/dq_s = k q_h S, its share of d/dq_h = k q_s S and d/ds = k q_h q_s G;
+// the quarters combine in a fixed order.  The H2 centre term only depends on
+// the point: its weight is omega_cm(p) = sum_q omega(p, q).
+constexpr int kHfSites = 64;  // sites per pass
 __global__ __launch_bounds__(kAWG) void k_hf_grad(const double *__restrict__ om, int64_t P, int Q,
                                                   const double *__restrict__ gx, const double *__restrict__ gy,
                                                   const double *__restrict__ gz, int nx, int ny,
@@ -81,6 +121,9 @@ __global__ __launch_bounds__(kAWG) void k_hf_grad(const double *__restrict__ om,
     double *wcm = site + 4 * ns;
     int *lj = (int *)(wcm + npb);
     __shared__ double red[kAWG / 64];
+    __shared__ int wcnt[kAWG / 64];
+    __shared__ double2 st[4][2 * kAWG];              // staged pairs: (w, x1), (y1, z1), (x2, y2), (z2, -)
+    __shared__ double acc[4][4][kHfSites];           // per quarter: S, Gx, Gy, Gz of a site
     for (int t = threadIdx.x; t < 4 * ns; t += blockDim.x) site[t] = site_g[t];
     for (int t = threadIdx.x; t < ns; t += blockDim.x) lj[t] = lj_g[t];
     const int64_t p0 = (int64_t)blockIdx.x * npb;
@@ -93,69 +136,126 @@ __global__ __launch_bounds__(kAWG) void k_hf_grad(const double *__restrict__ om,
     }
     __syncthreads();
     const int npairs = np * Q;
-    double d_qh = 0.0, d_qcm = 0.0, d_a = 0.0, d_b = 0.0;
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int sl = threadIdx.x % kHfSites, quarter = threadIdx.x / kHfSites;
     double *out = partial + (size_t)blockIdx.x * (4 * ns + 4);
-    for (int s = 0; s < ns; s++) {
-        const double sx = site[4 * s], sy = site[4 * s + 1], sz = site[4 * s + 2], qs = site[4 * s + 3];
-        double dq = 0.0, dx = 0.0, dy = 0.0, dz = 0.0;
-        for (int t = threadIdx.x; t < npairs; t += blockDim.x) {
-            const int pl = t / Q, q = t - pl * Q;
-            const int64_t p = p0 + pl;
-            const double w = om[(size_t)p * Q + q];
-            if (w == 0.0) continue;
-            const int i = (int)(p % nx), j = (int)((p / nx) % ny), k = (int)(p / ((int64_t)nx * ny));
-            const double x = gx[i], y = gy[j], z = gz[k];
-            const double ox = off[3 * q], oy = off[3 * q + 1], oz = off[3 * q + 2];
+    double d_qh = 0.0, d_qcm = 0.0, d_a = 0.0, d_b = 0.0;
+    for (int sb = 0; sb < ns; sb += kHfSites) {
+        const int s = sb + sl;
+        const bool has = s < ns;
+        const double sx = has ? site[4 * s] : 0.0, sy = has ? site[4 * s + 1] : 0.0;
+        const double sz = has ? site[4 * s + 2] : 0.0, qs = has ? site[4 * s + 3] : 0.0;
+        double S = 0.0, Gx = 0.0, Gy = 0.0, Gz = 0.0;
+        // staged pairs [u0, u1) into this site's sums
+        auto eval = [&](int u0, int u1) {
+            for (int u = u0; u < u1; u++) {
+                const double2 a = st[0][u], b = st[1][u], c = st[2][u], d = st[3][u];
+                const double w = a.x;
+                const double e1x = a.y - sx, e1y = b.x - sy, e1z = b.y - sz;
+                const double e2x = c.x - sx, e2y = c.y - sy, e2z = d.x - sz;
+                const double i1 = a_rsqrt(e1x * e1x + e1y * e1y + e1z * e1z);
+                const double i2 = a_rsqrt(e2x * e2x + e2y * e2y + e2z * e2z);
+                const double w1 = w * i1, w2 = w * i2;
+                S += w1 + w2;
+                const double c1 = w1 * i1 * i1, c2 = w2 * i2 * i2;
+                Gx += c1 * e1x + c2 * e2x;
+                Gy += c1 * e1y + c2 * e2y;
+                Gz += c1 * e1z + c2 * e2z;
+            }
+        };
+        int fill = 0;  // staged pairs, the same in every thread
+        for (int t0 = 0; t0 < npairs; t0 += kAWG) {
+            const int t = t0 + (int)threadIdx.x;
+            int pl = 0, q = 0;
+            double w = 0.0;
+            if (t < npairs) {
+                pl = t / Q;
+                q = t - pl * Q;
+                w = om[(size_t)(p0 + pl) * Q + q];
+            }
+            const bool keep = w != 0.0;
+            const uint64_t bal = __ballot(keep);
+            if (lane == 0) wcnt[wid] = __popcll(bal);
+            __syncthreads();
+            int woff = 0, total = 0;
 #pragma unroll
-            for (int h = 0; h < 2; h++) {
-                const double sg = h ? -1.0 : 1.0;
-                const double ex = x + sg * ox - sx, ey = y + sg * oy - sy, ez = z + sg * oz - sz;
-                const double ir = 1.0 / sqrt(ex * ex + ey * ey + ez * ez);
-                const double ir3 = ir * ir * ir;
-                dq += w * kCoulombK * q_h * ir;
-                d_qh += w * kCoulombK * qs * ir;
-                const double c = w * kCoulombK * q_h * qs * ir3;
-                dx += c * ex;
-                dy += c * ey;
-                dz += c * ez;
+            for (int v = 0; v < kAWG / 64; v++) {
+                const int n = wcnt[v];
+                woff += v < wid ? n : 0;
+                total += n;
+            }
+            if (keep) {
+                // omega != 0 only at unclamped nodes, where no H atom sits on a site
+                const int slot = fill + woff + __popcll(bal & ((1ull << lane) - 1ull));
+                const int64_t p = p0 + pl;
+                const int i = (int)(p % nx), j = (int)((p / nx) % ny), k = (int)(p / ((int64_t)nx * ny));
+                const double ox = off[3 * q], oy = off[3 * q + 1], oz = off[3 * q + 2];
+                st[0][slot] = make_double2(w, gx[i] + ox);
+                st[1][slot] = make_double2(gy[j] + oy, gz[k] + oz);
+                st[2][slot] = make_double2(gx[i] - ox, gy[j] - oy);
+                st[3][slot] = make_double2(gz[k] - oz, 0.0);
+            }
+            fill += total;
+            __syncthreads();
+            if (fill >= kAWG) {
+                if (has) eval(quarter * (kAWG / 4), (quarter + 1) * (kAWG / 4));
+                __syncthreads();
+                const int rest = fill - kAWG;  // < kAWG: moved from [kAWG, fill) to [0, rest)
+                if ((int)threadIdx.x < rest)
+#pragma unroll
+                    for (int f = 0; f < 4; f++) st[f][threadIdx.x] = st[f][kAWG + threadIdx.x];
+                fill = rest;
+                __syncthreads();
             }
         }
-        // the centre term, by thread pl for point pl
-        if (threadIdx.x < np) {
-            const int pl = threadIdx.x;
-            const int64_t p = p0 + pl;
-            const double w = wcm[pl];
-            if (w != 0.0) {
+        if (has) {
+            const int part = (fill + 3) / 4;
+            const int u0 = quarter * part, u1 = u0 + part < fill ? u0 + part : fill;
+            eval(u0, u1);
+        }
+        acc[quarter][0][sl] = S;
+        acc[quarter][1][sl] = Gx;
+        acc[quarter][2][sl] = Gy;
+        acc[quarter][3][sl] = Gz;
+        __syncthreads();
+        // the quarters in order, then the centre term, by thread sl of quarter 0
+        if (quarter == 0 && has) {
+            double r[4];
+#pragma unroll
+            for (int e = 0; e < 4; e++) r[e] = ((acc[0][e][sl] + acc[1][e][sl]) + acc[2][e][sl]) + acc[3][e][sl];
+            double g0 = kCoulombK * q_h * r[0];
+            double g1 = kCoulombK * q_h * qs * r[1], g2 = kCoulombK * q_h * qs * r[2];
+            double g3 = kCoulombK * q_h * qs * r[3];
+            d_qh += kCoulombK * qs * r[0];
+            for (int pl = 0; pl < np; pl++) {
+                const int64_t p = p0 + pl;
+                const double w = wcm[pl];
+                if (w == 0.0) continue;
                 const int i = (int)(p % nx), j = (int)((p / nx) % ny), k = (int)(p / ((int64_t)nx * ny));
                 const double ex = gx[i] - sx, ey = gy[j] - sy, ez = gz[k] - sz;
                 const double r2 = ex * ex + ey * ey + ez * ez;
                 const double ir = 1.0 / sqrt(r2);
                 const double ir3 = ir * ir * ir;
-                dq += w * kCoulombK * q_cm * ir;
+                g0 += w * kCoulombK * q_cm * ir;
                 d_qcm += w * kCoulombK * qs * ir;
                 double c = w * kCoulombK * q_cm * qs * ir3;
                 if (lj[s]) {
-                    const double ir2 = ir * ir, ir6 = ir2 * ir2 * ir2, ir12 = ir6 * ir6;
-                    d_a += w * ir12;
-                    d_b -= w * ir6;
+                    const double i2 = ir * ir, i6 = i2 * i2 * i2, i12 = i6 * i6;
+                    d_a += w * i12;
+                    d_b -= w * i6;
                     // d/ds of A r^-12 - B r^-6 = (12 A r^-14 - 6 B r^-8) (x - s)
-                    c += w * (12.0 * lj_a * ir12 * ir2 - 6.0 * lj_b * ir6 * ir2);
+                    c += w * (12.0 * lj_a * i12 * i2 - 6.0 * lj_b * i6 * i2);
                 }
-                dx += c * ex;
-                dy += c * ey;
-                dz += c * ez;
+                g1 += c * ex;
+                g2 += c * ey;
+                g3 += c * ez;
             }
+            out[4 * s + 0] = g0;
+            out[4 * s + 1] = g1;
+            out[4 * s + 2] = g2;
+            out[4 * s + 3] = g3;
         }
-        dq = a_block_sum(dq, red);
-        dx = a_block_sum(dx, red);
-        dy = a_block_sum(dy, red);
-        dz = a_block_sum(dz, red);
-        if (threadIdx.x == 0) {
-            out[4 * s + 0] = dq;
-            out[4 * s + 1] = dx;
-            out[4 * s + 2] = dy;
-            out[4 * s + 3] = dz;
-        }
+        __syncthreads();
     }
     d_qh = a_block_sum(d_qh, red);
     d_qcm = a_block_sum(d_qcm, red);
@@ -169,23 +269,41 @@ __global__ __launch_bounds__(kAWG) void k_hf_grad(const double *__restrict__ om,
     }
 }
 
-// out[e] = sum over workgroups of partial[wg][e], in workgroup order
-__global__ void k_hf_reduce(const double *__restrict__ partial, int nwg, int nent, double *__restrict__ out)
+// out[e] = sum over workgroups of partial[wg][e]: one workgroup per entry,
+// thread t sums rows t, t + kAWG, ... in order, then a fixed tree
+__global__ __launch_bounds__(kAWG) void k_hf_reduce(const double *__restrict__ partial, int nwg, int nent,
+                                                    double *__restrict__ out)
 {
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= nent) return;
+    __shared__ double red[kAWG];
+    const int e = blockIdx.x;
     double s = 0.0;
-    for (int w = 0; w < nwg; w++) s += partial[(size_t)w * nent + e];
-    out[e] = s;
+    for (int w = threadIdx.x; w < nwg; w += kAWG) s += partial[(size_t)w * nent + e];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int h = kAWG / 2; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[e] = red[0];
 }
 
-int launch_hf_density(const double *psi, int nvec, const double *g, const double *Y, const double *wq, double *vg,
-                      int64_t P, int nk, int Q, double ceiling, hipStream_t s)
+int launch_unclamped_mask(const double *vg, int64_t n, double ceiling, uint64_t *mask, hipStream_t s)
+{
+    const int64_t nw = (n + 63) / 64;
+    const int64_t groups = (nw + kAWG / 64 - 1) / (kAWG / 64);
+    hipLaunchKernelGGL(k_unclamped_mask, dim3((unsigned)(groups < 65536 ? groups : 65536)), dim3(kAWG), 0, s, vg, n,
+                       ceiling, mask);
+    BSL_HIP(hipGetLastError());
+    return BSL_OK;
+}
+
+int launch_hf_density(const double *psi, int nvec, const double *g, const double *Y, const double *wq,
+                      const uint64_t *unclamped, double *om, int64_t P, int nk, int Q, hipStream_t s)
 {
     const size_t lds = sizeof(double) * (size_t)nvec * nk;
     if (lds > 65536) return fail(BSL_E_ARG, "too many eigenvectors for one density pass");
-    hipLaunchKernelGGL(k_hf_density, dim3((unsigned)P), dim3(kAWG), lds, s, psi, nvec, g, Y, wq, vg, P, nk, Q,
-                       ceiling);
+    hipLaunchKernelGGL(k_hf_density, dim3((unsigned)P), dim3(kAWG), lds, s, psi, nvec, g, Y, wq, unclamped, om, P,
+                       nk, Q);
     BSL_HIP(hipGetLastError());
     return BSL_OK;
 }
@@ -198,12 +316,14 @@ int launch_hf_grad(const double *om, int64_t P, int Q, const double *gx, const d
 {
     const int nwg = hf_grad_groups(P, npb);
     const size_t lds = sizeof(double) * (4 * (size_t)ns + npb) + sizeof(int) * ns;
-    if (npb > kAWG || lds > 65536) return fail(BSL_E_ARG, "adjoint: too many sites");
+    // dynamic LDS beside the static staging arrays of k_hf_grad, within 64 KiB
+    const size_t lds_static = sizeof(double) * (kAWG / 64 + 16 * kAWG + 4 * 4 * kHfSites) + sizeof(int) * (kAWG / 64);
+    if (npb > kAWG || lds + lds_static > 65536) return fail(BSL_E_ARG, "adjoint: too many sites");
     hipLaunchKernelGGL(k_hf_grad, dim3(nwg), dim3(kAWG), lds, s, om, P, Q, gx, gy, gz, nx, ny, off, site4, lj, ns,
                        q_h, q_cm, lj_a, lj_b, npb, partial);
     BSL_HIP(hipGetLastError());
     const int nent = 4 * ns + 4;
-    hipLaunchKernelGGL(k_hf_reduce, dim3((nent + 255) / 256), dim3(256), 0, s, partial, nwg, nent, out);
+    hipLaunchKernelGGL(k_hf_reduce, dim3(nent), dim3(kAWG), 0, s, partial, nwg, nent, out);
     BSL_HIP(hipGetLastError());
     return BSL_OK;
 }

@@ -419,6 +419,7 @@ extern "C" int bsl_destroy(bsl_ctx *c)
     dev_free(c->d_gy);
     dev_free(c->d_gz);
     dev_free(c->d_vgrid);
+    dev_free(c->d_unclamped);
     dev_free(c->d_X);
     dev_free(c->d_R);
     dev_free(c->d_T);
@@ -1092,11 +1093,22 @@ extern "C" int bsl_hf_gradient(bsl_ctx *c, int32_t nvec, const double *psi, cons
     for (int k = 0; k < nvec && !rc; k++)
         rc = upload_carried(c, psi + (size_t)k * c->n_full, d_psi + (size_t)k * c->n_kry);
     if (!rc) rc = dev_alloc(&d_vg, (size_t)c->P * Q);
-    if (!rc) rc = make_vgrid(c, d_vg);
+    // which nodes are below the ceiling clamp: fixed by the context's
+    // potential, found from its grid once and kept (one bit per node)
+    if (!rc && !c->d_unclamped) {
+        rc = make_vgrid(c, d_vg);
+        if (!rc) rc = dev_alloc(&c->d_unclamped, ((size_t)c->P * Q + 63) / 64);
+        if (!rc) rc = launch_unclamped_mask(d_vg, c->P * Q, p.ceiling, c->d_unclamped, c->stream);
+        if (rc) {
+            (void)hipStreamSynchronize(c->stream);
+            dev_free(c->d_unclamped);
+            c->d_unclamped = nullptr;
+        }
+    }
     if (!rc) rc = dev_upload(&d_Y, Y);
     if (!rc) rc = dev_upload(&d_wq, wq);
     if (!rc) rc = dev_upload(&d_g, gv);
-    if (!rc) rc = launch_hf_density(d_psi, nvec, d_g, d_Y, d_wq, d_vg, c->P, nk, Q, p.ceiling, c->stream);
+    if (!rc) rc = launch_hf_density(d_psi, nvec, d_g, d_Y, d_wq, c->d_unclamped, d_vg, c->P, nk, Q, c->stream);
     if (!rc) rc = dev_upload(&d_off, off);
     if (!rc) rc = dev_upload(&d_site, site4);
     if (!rc) rc = dev_upload(&d_lj, lj);

@@ -234,9 +234,10 @@ int ctx_start(bsl_ctx *c, const double *v0, int start, uint64_t seed);
 int ctx_step(bsl_ctx *c);
 
 // Hellmann-Feynman adjoint (bsl_adjoint.hip)
-int launch_hf_density(const double *psi, int nvec, const double *g, const double *Y, const double *wq, double *vg,
-                      int64_t P, int nk, int Q, double ceiling, hipStream_t s);
+int launch_hf_density(const double *psi, int nvec, const double *g, const double *Y, const double *wq,
+                      const uint64_t *unclamped, double *om, int64_t P, int nk, int Q, hipStream_t s);
 int hf_grad_groups(int64_t P, int npb);
+int launch_unclamped_mask(const double *vg, int64_t n, double ceiling, uint64_t *mask, hipStream_t s);
 int launch_hf_grad(const double *om, int64_t P, int Q, const double *gx, const double *gy, const double *gz, int nx,
                    int ny, const double *off, const double *site4, const int *lj, int ns, double q_h, double q_cm,
                    double lj_a, double lj_b, int npb, double *partial, double *out, hipStream_t s);
@@ -264,6 +265,7 @@ struct bsl_ctx {
     void *vpk64 = nullptr;             // FP64 blocks kept for refinement when op is mixed
     double *d_gx = nullptr, *d_gy = nullptr, *d_gz = nullptr;
     double *d_vgrid = nullptr;         // [P][Q] potential grid (kept for the quadrature calculator)
+    uint64_t *d_unclamped = nullptr;   // bit p*Q + q: V(p, q) < ceiling; built by the first gradient call
     int Q = 0;
     double *d_X = nullptr, *d_R = nullptr, *d_T = nullptr;
     bsl::Scal *d_scal = nullptr;

@@ -101,6 +101,19 @@ def test_gpu_gradient_matches_oracle():
     assert np.max(np.abs(flat[:4 * ns] - ref[:4 * ns])) / scale < 1e-10
     for e in range(4 * ns, 4 * ns + 4):
         assert abs(flat[e] - ref[e]) <= 1e-10 * abs(ref[e])
+    # the clamp mask is found by the first call and kept: a second call with
+    # other vectors and weights, on a grid with clamped nodes, still matches
+    g1, g2 = c.vgrid()
+    assert np.count_nonzero(g1 >= kw["ceiling"]) + np.count_nonzero(g2 >= kw["ceiling"]) > 0
+    psi2 = rng.uniform(-1, 1, (3, o.N))
+    psi2 /= np.linalg.norm(psi2, axis=1, keepdims=True)
+    g_2 = np.array([0.2, 1.1, -0.5])
+    ref2 = o.hf_gradient(psi2, g_2)
+    got2 = c.hf_gradient(psi2, g_2)
+    assert np.max(np.abs(got2["site_q"] - ref2[0:4 * ns:4])) / np.max(np.abs(ref2[:4 * ns])) < 1e-10
+    assert np.max(np.abs(got2["site_pos"] - ref2[:4 * ns].reshape(ns, 4)[:, 1:4])) / np.max(np.abs(ref2[:4 * ns])) < 1e-10
+    for e, name in zip(range(4 * ns, 4 * ns + 4), ("q_h2_h", "q_h2_cm", "lj_a", "lj_b")):
+        assert abs(got2[name] - ref2[e]) <= 1e-10 * abs(ref2[e])
 
 
 NITER = 800

@@ -249,8 +249,11 @@ int bsl_step_bytes(const bsl_ctx *ctx, double *out);
 // unit norm; g: nvec weights (e.g. the gradient of a loss with respect to
 // each eigenvalue).  grad (4*n_sites + 4 doubles): d/d site_q[s] at [4*s],
 // d/d site_pos[s][d] at [4*s + 1 + d]; then d/d q_h2_h, q_h2_cm, lj_a, lj_b
-// at [4*n_sites + 0..3].  No reference counterpart (the reference has no
-// derivatives).
+// at [4*n_sites + 0..3].  The first call on a context evaluates the
+// potential grid once to find the clamped nodes and keeps one bit per
+// (point, node) (P * Q / 8 bytes, Q = 2 * n_theta * n_phi) until
+// bsl_destroy; later calls skip that pass.  No reference counterpart (the
+// reference has no derivatives).
 int bsl_hf_gradient(bsl_ctx *ctx, int32_t nvec, const double *psi, const double *g, double *grad);
 
 // ---- batched recurrences (no reference counterpart: the reference runs one

@@ -21,13 +21,15 @@ namespace {
 constexpr double kEps2 = 2.0 * 1.1102230246251565e-16;
 
 // GNU SRAND/RAND as used by GENRAN (genran.f:4-32): the Park-Miller
-// "minimal standard" generator, a = 16807, m = 2^31 - 1, seed 0 -> 123459876.
+// "minimal standard" generator, a = 16807, m = 2^31 - 1, seed 0 -> 123459876;
+// RAND returns (s - 1) with its low 9 bits cleared, over 2^31, a REAL*4 value
+// (22 significant bits, so the quotient is exact in float and double).
 void park_miller(int seed, double *r, int n)
 {
     uint64_t s = seed ? (uint64_t)seed : 123459876ull;
     for (int i = 0; i < n; i++) {
         s = (16807ull * s) % 2147483647ull;
-        r[i] = (double)((float)(s - 1) / (float)2147483646.0);
+        r[i] = (double)((s - 1) & ~(uint64_t)511) * (1.0 / 2147483648.0);
     }
 }
 

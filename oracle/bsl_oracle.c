@@ -805,16 +805,16 @@ void bso_lanczos(const bso_state *s, int niter, const double *v0, double *alpha,
 #define CW_EPS (2.0 * 1.1102230246251565e-16) /* 2 * dlamch('e') */
 
 /* GENRAN genran.f:4-32 with GNU SRAND/RAND: Park-Miller minimal standard
- * (a = 16807, m = 2^31-1); SRAND(0) selects 123459876.  The REAL*4
- * conversion (s-1)/(m-1) is the published generator's; libgfortran's exact
- * float rounding is not in the reference (parity unpinned). */
+ * (a = 16807, m = 2^31-1); SRAND(0) selects 123459876.  RAND returns the
+ * REAL*4 value ((s - 1) with its low 9 bits cleared) / 2^31, which is exact.
+ * Pinned bitwise against the reference's GENRAN compiled with the GNU
+ * runtime (oracle/_ref, tests/test_oracle_pins.py). */
 void bso_genran(int seed, double *r, int n)
 {
     uint64_t st = seed ? (uint64_t)seed : 123459876ull;
     for (int i = 0; i < n; i++) {
         st = (16807ull * st) % 2147483647ull;
-        float f = (float)(st - 1) / (float)2147483646.0;
-        r[i] = (double)f;
+        r[i] = (double)((st - 1) & ~(uint64_t)511) * (1.0 / 2147483648.0);
     }
 }
 

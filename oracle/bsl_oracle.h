@@ -23,14 +23,16 @@
  *   Lanczos loop 1                  numpy Lanczos on the dense operator,
  *                                   reference run-dir eigenvalues
  *   BISEC / ISOEV                   brute force (eigvals of T and T-hat),
- *                                   reference run-dir eigenvalues
+ *                                   reference run-dir eigenvalues, and the
+ *                                   reference's own lancbis.cpp + Fortran
+ *                                   built from source (oracle/_ref)
  *   Hellmann-Feynman gradient       central finite differences of eigenvalues
  *                                   of this oracle (tests/test_adjoint.py)
  *   INVERR / TRIVEC / INVERM /
  *   STURMI / GENRAN                 residual checks; run-dir error column and
- *                                   lv / eigVecFile fixtures.  The GNU RAND
- *                                   float conversion is parity unpinned (it
- *                                   only seeds inverse iteration).
+ *                                   lv / eigVecFile fixtures; the reference's
+ *                                   own Fortran built from source with GNU
+ *                                   RAND and LAPACK DLAMCH (oracle/_ref)
  */
 #ifndef BSL_ORACLE_H
 #define BSL_ORACLE_H

@@ -7,6 +7,9 @@
 #include <string>
 
 #include "interface.h"
+#include "lanczos.h"
+
+extern "C" void genran_(int *, double *, int *);
 
 namespace {
 generalStor *g_general = nullptr;
@@ -134,4 +137,66 @@ extern "C" __attribute__((visibility("default"))) double ref_tip4p_coulomb(const
     pos.COOR(1) = y;
     pos.COOR(2) = z;
     return Q_TIP4P_Eng(pos, q, atomType, atomPos, nAtoms);
+}
+
+// The reference's Cullum-Willoughby driver lancbis (lancbis.cpp:5-105: BISEC,
+// ISOEV, INVERR from its Fortran sources) on plain arrays: alpha[niter],
+// beta[niter + 1], beta2[niter + 1] (entry 0 unused); the ngood good
+// eigenvalues and their error estimates go to eval / err (cap entries at
+// most).  Returns ngood.
+extern "C" __attribute__((visibility("default"))) int ref_lancbis(int niter, const double *alpha,
+                                                                  const double *beta, const double *beta2,
+                                                                  double emin, double emax, double *eval,
+                                                                  double *err, int cap)
+{
+    VECT a(niter), b(niter + 1), b2(niter + 1), ev(niter), ee(niter);
+    for (int i = 0; i < niter; i++) a(i) = alpha[i];
+    for (int i = 0; i <= niter; i++) {
+        b(i) = beta[i];
+        b2(i) = beta2[i];
+    }
+    int ngood = 0;
+    lancbis(niter, ev, ee, emin, emax, ngood, a, b, b2);
+    for (int i = 0; i < ngood && i < cap; i++) {
+        eval[i] = ev(i);
+        err[i] = ee(i);
+    }
+    return ngood;
+}
+
+// The reference's T_m eigenvectors (TRIVEC through lanczosvectors,
+// lanczosEigenValuesVectors.cpp:509-561): evtr[j * niter + i] = component i
+// of the vector of eval[j]; mamin[j] the size of the T_m used for it.
+extern "C" __attribute__((visibility("default"))) void ref_trivec(int niter, const double *alpha,
+                                                                  const double *beta, const double *beta2,
+                                                                  int ngood, const double *eval, double *evtr,
+                                                                  double *mamin)
+{
+    double *la = new double[niter], *lb = new double[niter + 1], *lb2 = new double[niter + 1];
+    double *le = new double[ngood], *w1 = new double[niter], *w2 = new double[niter];
+    for (int i = 0; i < niter; i++) {
+        la[i] = alpha[i];
+        w1[i] = w2[i] = 0.0;
+    }
+    lb[0] = lb2[0] = 0.0;
+    for (int j = 1; j <= niter; j++) {
+        lb[j] = beta[j];
+        lb2[j] = beta2[j];
+    }
+    for (int j = 0; j < ngood; j++) le[j] = eval[j];
+    for (int i = 0; i < niter * ngood; i++) evtr[i] = 0.0;
+    for (int i = 0; i < ngood; i++) mamin[i] = 0.0;
+    FORTRAN(trivec)(la, lb, lb2, w1, w2, &niter, le, &ngood, evtr, mamin);
+    delete[] la;
+    delete[] lb;
+    delete[] lb2;
+    delete[] le;
+    delete[] w1;
+    delete[] w2;
+}
+
+// GENRAN (genran.f:4-32) with the GNU runtime's SRAND / RAND
+extern "C" __attribute__((visibility("default"))) void ref_genran(int seed, double *r, int n)
+{
+    genran_(&seed, r, &n);
 }

@@ -308,6 +308,10 @@ def ref_lib():
     lib.ref_quad.argtypes = [_dp, _dp, _dp, _dp, _dp, _dp, _dp]
     lib.ref_spce_sites.restype = C.c_int
     lib.ref_spce_sites.argtypes = [C.c_char_p, _dp, C.c_char_p]
+    lib.ref_lancbis.restype = C.c_int
+    lib.ref_lancbis.argtypes = [C.c_int, _dp, _dp, _dp, C.c_double, C.c_double, _dp, _dp, C.c_int]
+    lib.ref_trivec.argtypes = [C.c_int, _dp, _dp, _dp, C.c_int, _dp, _dp, _dp]
+    lib.ref_genran.argtypes = [C.c_int, _dp, C.c_int]
     lib.ref_spce_energy.restype = C.c_double
     lib.ref_spce_energy.argtypes = [C.c_char_p, C.c_double, C.c_double, C.c_double, C.c_double, C.c_int]
     return lib

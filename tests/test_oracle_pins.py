@@ -325,11 +325,90 @@ def test_ritz_vectors_are_eigenvectors_of_h():
 
 
 def test_park_miller_stream():
-    """The minimal standard generator: seed 1 -> 16807, 282475249, ..."""
+    """GNU RAND: the minimal standard generator (seed 1 -> 16807, 282475249,
+    1622650073), returned as (s - 1) with its low 9 bits cleared, over 2^31."""
     r = O.genran(1, 3)
-    m = 2147483647
-    s = [16807, 282475249, 1622650073]
-    assert np.allclose(r, [np.float32((x - 1) / np.float32(m - 1)) for x in s], rtol=1e-6)
+    s = np.array([16807, 282475249, 1622650073])
+    assert np.array_equal(r, ((s - 1) & ~511) / 2.0 ** 31)
+    assert r[0] == 2.0 ** -17
+
+
+def _tridiagonals():
+    """Random tridiagonals (some with repeated diagonal halves) and the
+    coefficients of two oracle Lanczos runs (the longer one has ghosts)."""
+    out = []
+    for seed in range(60):
+        rng = np.random.default_rng(seed)
+        m = int(5 + seed % 70)
+        a = rng.standard_normal(m) * 3
+        if seed % 3 == 0:
+            a = np.concatenate([a[: m // 2], a[: m - m // 2]])
+        b = np.concatenate([[0.0], np.abs(rng.standard_normal(m))])
+        out.append((a, b, b * b, -5.0, 5.0))
+    spec, _ = cs.cage_case(5, 2, 5, 5, "All")
+    o = O.Oracle(spec)
+    for m in (120, 700):
+        a, b, b2 = o.lanczos(m)
+        out.append((a, b, b2, -10.0, 10.0))
+    return out
+
+
+@pytest.mark.skipif(not O.ref_available(), reason="reference build (oracle/_ref) not present")
+def test_genran_matches_reference_build():
+    """GENRAN against the reference's genran.f built with the GNU runtime's
+    SRAND / RAND: identical streams."""
+    R = O.ref_lib()
+    for seed in (0, 1, 7, 12345, 2 ** 30 + 3):
+        r = np.zeros(5000)
+        R.ref_genran(seed, r, 5000)
+        assert np.array_equal(O.genran(seed, 5000), r), seed
+
+
+@pytest.mark.skipif(not O.ref_available(), reason="reference build (oracle/_ref) not present")
+def test_lancbis_matches_reference_build():
+    """Good eigenvalues and INVERR error estimates against the reference's
+    own lancbis.cpp with BISEC, ISOEV and INVERR built from its Fortran (GNU
+    RAND, LAPACK DLAMCH): identical to the last bit."""
+    R = O.ref_lib()
+    n = 0
+    for a, b, b2, lo, hi in _tridiagonals():
+        try:
+            e1, r1 = O.lancbis(a, b, b2, lo, hi)
+        except RuntimeError:
+            continue  # the reference STOPs the process there
+        m = len(a)
+        ev, er = np.zeros(m), np.zeros(m)
+        ng = R.ref_lancbis(m, a, b, b2, lo, hi, ev, er, m)
+        assert ng == len(e1)
+        assert np.array_equal(ev[:ng], e1) and np.array_equal(er[:ng], r1)
+        n += ng
+    assert n > 300
+
+
+@pytest.mark.skipif(not O.ref_available(), reason="reference build (oracle/_ref) not present")
+def test_trivec_matches_reference_build():
+    """T_m eigenvectors and the sizes MAMIN of the T_m used for them against
+    the reference's trivec.f (with STURMI, INVERM, GENRAN) built from
+    source: identical to the last bit."""
+    R = O.ref_lib()
+    n = 0
+    for a, b, b2, lo, hi in _tridiagonals():
+        try:
+            ev, _ = O.lancbis(a, b, b2, lo, hi)
+            if len(ev) == 0:
+                continue
+            v1, m1 = O.trivec(a, b, b2, ev)
+        except RuntimeError:
+            continue
+        m, ng = len(a), len(ev)
+        evtr, mamin = np.zeros(m * ng), np.zeros(ng)
+        R.ref_trivec(m, a, b, b2, ng, ev, evtr, mamin)
+        # MAMIN is an INTEGER array in trivec.f; the reference passes a
+        # double buffer, so its first ng 4-byte words hold the values
+        assert np.array_equal(evtr, v1)
+        assert np.array_equal(mamin.view(np.int32)[:ng], m1)
+        n += 1
+    assert n > 30
 
 
 # --------------------------------------------------- reference's outputs ---

@@ -295,6 +295,34 @@ def ref_available():
     return os.path.exists(REF_SO)
 
 
+# the reference's own Lanczos program (lanczosEigenValuesVectors.cpp), built
+# from source into oracle/_ref by oracle/Makefile.ref
+REF_LANCZOS = os.path.join(REPO, "oracle", "_ref", "ref_lanczos")
+
+
+def run_ref_lanczos(lanczos_deck, hv_deck, cwd):
+    """Runs the reference program on two decks in cwd; returns its stdout
+    alpha, beta (Loop 1 coefficients as printed, 16 digits) and the output
+    directory it made."""
+    import subprocess
+    r = subprocess.run([REF_LANCZOS, lanczos_deck, hv_deck], cwd=cwd, capture_output=True, text=True, timeout=600,
+                       env=dict(os.environ, OMP_NUM_THREADS="4"))
+    if r.returncode != 0:
+        raise RuntimeError(r.stderr[-2000:])
+    lines = r.stdout.split("\n")
+    k = lines.index("Iteration j alpha_j beta_j") + 1
+    a, b = [], []
+    while k < len(lines):
+        t = lines[k].split()
+        if len(t) != 3 or not t[0].isdigit():
+            break
+        a.append(float(t[1]))
+        b.append(float(t[2]))
+        k += 1
+    out = [l.split(": ", 1)[1] for l in lines if l.startswith("The output directory is: ")]
+    return np.array(a), np.array(b), out[0]
+
+
 def ref_lib():
     lib = C.CDLL(REF_SO)
     lib.ref_setup.restype = C.c_longlong

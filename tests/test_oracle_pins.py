@@ -590,3 +590,27 @@ def test_reference_hv_with_injected_blocks(tmp_path):
     u = np.zeros(N)
     R.ref_hv(v, u)
     assert np.array_equal(u, ref)
+
+
+@pytest.mark.skipif(not os.path.exists(O.REF_LANCZOS), reason="reference program (oracle/_ref) not built")
+def test_reference_program_loop1_and_levels(tmp_path):
+    """The reference's own Lanczos program (Loop 1 from the uniform start,
+    lancbis, output files) on a 10^3 x 16 cage deck against the oracle:
+    every alpha and beta to its 16 printed digits, every good eigenvalue and
+    error estimate of boundstates.txt to its 16 printed digits."""
+    import test_driver as td
+    t = str(tmp_path)
+    ld = td.lanczos_deck(t, niter=300, calc="linRotCartSph_NoQuad_ContGrid_Alavi_SPCE")
+    hd = td.hv_deck(t)
+    a_ref, b_ref, d = O.run_ref_lanczos(ld, hd, t)
+    spec, _ = cs.cage_case(10, 3, 10, 10, "All")
+    o = O.Oracle(spec)
+    a, b, b2 = o.lanczos(300)
+    assert len(a_ref) == 300
+    assert np.max(np.abs(a_ref - a) / np.abs(a)) < 1e-15
+    assert np.max(np.abs(b_ref - b[1:]) / np.abs(b[1:])) < 1e-15
+    ev, er = O.lancbis(a, b, b2, -10.0, 10.0)
+    rows = np.loadtxt(os.path.join(t, d, "boundstates.txt"), skiprows=4)
+    assert rows.shape == (len(ev), 3)
+    assert np.max(np.abs(rows[:, 1] - ev) / np.abs(ev)) < 1e-15
+    assert np.all(np.abs(rows[:, 2] - er) <= 1e-15 * np.abs(er) + 1e-300)

@@ -300,12 +300,13 @@ def ref_available():
 REF_LANCZOS = os.path.join(REPO, "oracle", "_ref", "ref_lanczos")
 
 
-def run_ref_lanczos(lanczos_deck, hv_deck, cwd):
-    """Runs the reference program on two decks in cwd; returns its stdout
+def run_ref_lanczos(lanczos_deck, hv_deck, cwd, pot_deck=None):
+    """Runs the reference program on its decks in cwd; returns its stdout
     alpha, beta (Loop 1 coefficients as printed, 16 digits) and the output
     directory it made."""
     import subprocess
-    r = subprocess.run([REF_LANCZOS, lanczos_deck, hv_deck], cwd=cwd, capture_output=True, text=True, timeout=600,
+    args = [REF_LANCZOS, lanczos_deck, hv_deck] + ([pot_deck] if pot_deck else [])
+    r = subprocess.run(args, cwd=cwd, capture_output=True, text=True, timeout=600,
                        env=dict(os.environ, OMP_NUM_THREADS="4"))
     if r.returncode != 0:
         raise RuntimeError(r.stderr[-2000:])

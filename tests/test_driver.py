@@ -35,6 +35,21 @@ def hv_deck(tmp, nx=10, lmax=3, na=10, nb=10, sym="All", xmax=0.4, pxmax=0.48):
     return p
 
 
+def iho_decks(tmp, nx=21, lmax=3, na=20, nb=30, xmax=0.4, omega_cm1=100.0, mass=2.016, ceiling=1.0e6):
+    """Hv deck and potential deck (argv[3]: mass, angular frequency,
+    ceiling; IsoHarmOsc_Routines.cpp:38-62) of the isotropic oscillator
+    workload: H2 (B = 59.322 cm^-1) in a 100 cm^-1 well, 21 points per axis,
+    l <= 3, 20 x 30 quadrature."""
+    from bsl_amd import physics
+    shutil.copyfile(os.path.join(cs.GOLDEN, "sII_smallCage11_CM_EA_ZYZ.xyz"), os.path.join(tmp, "cage.xyz"))
+    hv = os.path.join(tmp, "hv.txt")
+    write_deck(hv, [nx, xmax, nx, xmax, nx, xmax, lmax, na, nb, 793, 0.48, 793, 0.48, 793, 0.48, mass,
+                    repr(physics.cm1_to_kjmol(59.322)), "cage.xyz", "DriverTest", repr(ceiling), "FALSE", "All"])
+    pot = os.path.join(tmp, "pot.txt")
+    write_deck(pot, [mass, repr(physics.omega_from_cm1(omega_cm1)), repr(ceiling)])
+    return hv, pot
+
+
 def run(args, cwd):
     return subprocess.run([DRIVER] + args, cwd=cwd, capture_output=True, text=True, timeout=600)
 
@@ -204,3 +219,46 @@ def test_driver_matches_reference_program(tmp_path):
     for c in (1, 2):
         s = np.sign(np.dot(vr[:, c], vh[:, c]))
         assert np.max(np.abs(vr[:, c] - s * vh[:, c])) < 1e-6
+
+
+@pytest.mark.gpu
+def test_driver_matches_reference_program_iho(tmp_path):
+    """The plain-FP64 isotropic oscillator workload (21^3, l <= 3, 20 x 30,
+    300 steps from the uniform start) through the drop-in driver and the
+    reference's own program on the same three decks: the good eigenvalues of
+    the two boundstates.txt files pair up one to one within 1e-9 relative,
+    and the device recurrence's alpha and beta (bsl_lanczos, uniform start)
+    are within 1e-9 relative of the ones the reference program prints over
+    the first 40 steps (about 3e-13 measured).  From about step 45 the
+    ground level has converged and the lost orthogonality amplifies any
+    rounding difference between two FP64 runs (the reference's own alpha
+    and beta against a device run differ by 1e-3 at step 60 and 0.4 by step 150),
+    so the later coefficients are compared through the levels."""
+    import _oracle as O
+    if not os.path.exists(O.REF_LANCZOS):
+        pytest.skip("reference program (oracle/_ref) not built")
+    rows = {}
+    for side in ("ref", "hip"):
+        t = str(tmp_path / side)
+        os.makedirs(t)
+        ld = lanczos_deck(t, niter=300, calc="linRotCartSph_NoQuad_IsoHarmOsc", word="none", nvec=0)
+        hv, pot = iho_decks(t)
+        if side == "ref":
+            a_ref, b_ref, d = O.run_ref_lanczos(ld, hv, t, pot)
+            d = os.path.join(t, d)
+        else:
+            r = run([ld, hv, pot], t)
+            assert r.returncode == 0, r.stderr[-2000:]
+            d = os.path.join(t, [x for x in os.listdir(t) if x.startswith("DriverTest_")][0])
+        rows[side] = _read_boundstates(d)
+    ref, hip = rows["ref"][:, 0], rows["hip"][:, 0]
+    assert len(ref) == len(hip) and len(ref) > 10
+    assert np.max(np.abs(ref - hip) / np.abs(ref)) < 1e-9
+    import bsl_amd as B
+    from bsl_amd import physics
+    _, kw = cs.iho_case(21, 3, 20, 30, x_max=0.4, rot=physics.cm1_to_kjmol(59.322))
+    with B.Calculator(**kw) as c:
+        a, b, _ = c.lanczos(300, start="uniform")
+    assert len(a_ref) == 300
+    assert np.max(np.abs(a[:40] - a_ref[:40]) / np.abs(a_ref[:40])) < 1e-9
+    assert np.max(np.abs(b[1:41] - b_ref[:40]) / np.abs(b_ref[:40])) < 1e-9

@@ -614,3 +614,25 @@ def test_reference_program_loop1_and_levels(tmp_path):
     assert rows.shape == (len(ev), 3)
     assert np.max(np.abs(rows[:, 1] - ev) / np.abs(ev)) < 1e-15
     assert np.all(np.abs(rows[:, 2] - er) <= 1e-15 * np.abs(er) + 1e-300)
+
+
+@pytest.mark.skipif(not os.path.exists(O.REF_LANCZOS), reason="reference program (oracle/_ref) not built")
+def test_reference_program_iho_workload(tmp_path):
+    """The plain-FP64 isotropic oscillator workload (21^3, l <= 3, 20 x 30,
+    300 steps) through the reference's own program against the oracle:
+    alpha, beta and the good levels to their 16 printed digits."""
+    import test_driver as td
+    t = str(tmp_path)
+    ld = td.lanczos_deck(t, niter=300, calc="linRotCartSph_NoQuad_IsoHarmOsc", word="none", nvec=0)
+    hv, pot = td.iho_decks(t)
+    a_ref, b_ref, d = O.run_ref_lanczos(ld, hv, t, pot)
+    from bsl_amd import physics
+    spec, _ = cs.iho_case(21, 3, 20, 30, x_max=0.4, rot=physics.cm1_to_kjmol(59.322))
+    o = O.Oracle(spec)
+    a, b, b2 = o.lanczos(300)
+    assert np.max(np.abs(a_ref - a) / np.abs(a)) < 1e-15
+    assert np.max(np.abs(b_ref - b[1:]) / np.abs(b[1:])) < 1e-15
+    ev, _ = O.lancbis(a, b, b2, -10.0, 10.0)
+    rows = np.loadtxt(os.path.join(t, d, "boundstates.txt"), skiprows=4)
+    assert rows.shape[0] == len(ev) > 10
+    assert np.max(np.abs(rows[:, 1] - ev) / np.abs(ev)) < 1e-15

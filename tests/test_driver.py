@@ -21,9 +21,10 @@ def write_deck(path, values):
             f.write("name%d= %s\n" % (i, v))
 
 
-def lanczos_deck(tmp, niter=300, calc="linRotCartSph_HIP_NoQuad_ContGrid_Alavi_SPCE", word="partial", nvec=2):
+def lanczos_deck(tmp, niter=300, calc="linRotCartSph_HIP_NoQuad_ContGrid_Alavi_SPCE", word="partial", nvec=2,
+                 window=(-10, 10)):
     p = os.path.join(tmp, "lanczos.txt")
-    write_deck(p, [niter, -10, 10, word, nvec, calc, tmp + "/"])
+    write_deck(p, [niter, window[0], window[1], word, nvec, calc, tmp + "/"])
     return p
 
 
@@ -47,6 +48,23 @@ def iho_decks(tmp, nx=21, lmax=3, na=20, nb=30, xmax=0.4, omega_cm1=100.0, mass=
                     repr(physics.cm1_to_kjmol(59.322)), "cage.xyz", "DriverTest", repr(ceiling), "FALSE", "All"])
     pot = os.path.join(tmp, "pot.txt")
     write_deck(pot, [mass, repr(physics.omega_from_cm1(omega_cm1)), repr(ceiling)])
+    return hv, pot
+
+
+def string_decks(tmp, calc, nx=8, lmax=3, na=8, nb=9):
+    """Hv deck (and potential deck where the calculator reads argv[3]) of a
+    small cage11 workload for one of the reference's calculator strings; the
+    grid-mapped ones get a 17 x 19 x 23 potential universe that holds the
+    box."""
+    shutil.copyfile(os.path.join(cs.GOLDEN, "sII_smallCage11_CM_EA_ZYZ.xyz"), os.path.join(tmp, "cage.xyz"))
+    univ = [17, 0.44, 19, 0.5, 23, 0.41] if "_ContGrid_" not in calc else [793, 0.48, 793, 0.48, 793, 0.48]
+    hv = os.path.join(tmp, "hv.txt")
+    write_deck(hv, [nx, 0.4, nx, 0.4, nx, 0.4, lmax, na, nb] + univ +
+               [2.01588, 0.7096487130193394, "cage.xyz", "DriverTest", 18, "FALSE", "All"])
+    pot = None
+    if calc.endswith("_Coulomb"):
+        pot = os.path.join(tmp, "pot.txt")
+        write_deck(pot, [1.0, -0.5, -200.0])
     return hv, pot
 
 
@@ -262,3 +280,47 @@ def test_driver_matches_reference_program_iho(tmp_path):
     assert len(a_ref) == 300
     assert np.max(np.abs(a[:40] - a_ref[:40]) / np.abs(a_ref[:40])) < 1e-9
     assert np.max(np.abs(b[1:41] - b_ref[:40]) / np.abs(b_ref[:40])) < 1e-9
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("calc", ["linRotCartSph_NoQuad_ContGrid_Alavi_TIP4P", "linRotCartSph_NoQuad_Alavi_SPCE",
+                                  "linRotCartSph_NoQuad_Alavi_TIP4P", "linRotCartSph_NoQuad_Coulomb"])
+def test_driver_matches_reference_program_strings(tmp_path, calc):
+    """The reference's other calculator strings (TIP4P sites, the
+    grid-mapped potential universe, the Coulomb probe) on an 8^3 x 16
+    workload, 300 steps: the drop-in driver's converged levels and the
+    reference program's (built from source) agree within 1e-9 relative.
+
+    The Coulomb probe clamps the whole box at its -200 kJ/mol cap, so V is a
+    constant and H separates into box x rotor states.  The uniform start has
+    even translational parity on every axis; the even-axis kinetic kernels
+    contract v(i) + v(n-1-i) and v(i) - v(n-1-i) separately and keep that
+    parity exactly, while the reference's plain products let rounding seed
+    the odd states, which then converge (levels spaced 3 E1 beside the even
+    ones' 8 E1).  There every level of the driver is checked against the
+    reference's list, not the other way round."""
+    import _oracle as O
+    if not os.path.exists(O.REF_LANCZOS):
+        pytest.skip("reference program (oracle/_ref) not built")
+    rows = {}
+    for side in ("ref", "hip"):
+        t = str(tmp_path / side)
+        os.makedirs(t)
+        ld = lanczos_deck(t, niter=300, calc=calc, word="none", nvec=0,
+                          window=(-400, 400) if calc.endswith("_Coulomb") else (-10, 10))
+        hv, pot = string_decks(t, calc)
+        if side == "ref":
+            d = os.path.join(t, O.run_ref_lanczos(ld, hv, t, pot)[2])
+        else:
+            r = run([ld, hv] + ([pot] if pot else []), t)
+            assert r.returncode == 0, r.stderr[-2000:]
+            d = os.path.join(t, [x for x in os.listdir(t) if x.startswith("DriverTest_")][0])
+        rows[side] = _read_boundstates(d)
+    pairs = [(rows["hip"], rows["ref"])]
+    if not calc.endswith("_Coulomb"):
+        pairs.append((rows["ref"], rows["hip"]))
+    for a, b in pairs:
+        conv = a[(a[:, 1] < 1e-10), 0]
+        assert len(conv) >= 5
+        d = np.min(np.abs(conv[:, None] - b[None, :, 0]), axis=1)
+        assert np.max(d / np.abs(conv)) < 1e-9

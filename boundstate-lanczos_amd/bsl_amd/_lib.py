@@ -26,7 +26,7 @@ EXPORTS = (
     "bsl_potential_kernel", "bsl_hf_gradient", "bsl_batch_create", "bsl_batch_destroy", "bsl_batch_size",
     "bsl_batch_member", "bsl_batch_lanczos", "bsl_batch_time_steps", "bsl_lancbis_device", "bsl_lancbis_tol",
     "bsl_lancbis_device_tol", "bsl_trivec_tol", "bsl_get_potential_form", "bsl_set_potential_form",
-    "bsl_gaunt_table",
+    "bsl_gaunt_table", "bsl_lanczos_noalloc",
 )
 
 
@@ -69,6 +69,7 @@ def load():
     lib.bsl_hv.argtypes = [C.c_void_p, _dp, _dp]
     lib.bsl_hv_device.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
     lib.bsl_lanczos.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_int32, C.c_uint64, _dp, _dp, _dp]
+    lib.bsl_lanczos_noalloc.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_int32, C.c_uint64, _dp, _dp, _dp]
     lib.bsl_lancbis.argtypes = [C.c_int32, _dp, _dp, _dp, C.c_double, C.c_double, _dp, _dp,
                                 C.POINTER(C.c_int32)]
     lib.bsl_lancbis_device.argtypes = [C.c_int32, _dp, _dp, _dp, C.c_double, C.c_double, _dp, _dp,

@@ -135,6 +135,22 @@ int bsl_hv_device(bsl_ctx *ctx, const double *d_vec, double *d_uec);
 int bsl_lanczos(bsl_ctx *ctx, int32_t niter, const double *v0, int32_t start, uint64_t seed,
                 double *alpha, double *beta, double *beta2);
 
+/* bsl_lanczos without any device allocation in the call.  No reference
+ * counterpart: for callers that cannot allocate device memory (code under
+ * HIP graph capture, serving loops, harnesses that forbid hipMalloc), on a
+ * context prepared beforehand by bsl_create.  Same arguments, same kernels
+ * and the same coefficients, bitwise.  The recurrence scalars, the reduction
+ * partials and the coefficient lists live in one static device array per
+ * GPU (about 1.2 MiB, held until the process ends) instead of buffers the
+ * library allocates; hence at most BSL_NOALLOC_MAX_STEPS steps and potential
+ * kernels of at most BSL_NOALLOC_MAX_PARTIALS workgroups (BSL_E_ARG
+ * otherwise).  All calls on one GPU share that array: calls that overlap on
+ * one GPU race. */
+#define BSL_NOALLOC_MAX_STEPS 32768
+#define BSL_NOALLOC_MAX_PARTIALS 16384
+int bsl_lanczos_noalloc(bsl_ctx *ctx, int32_t niter, const double *v0, int32_t start, uint64_t seed,
+                        double *alpha, double *beta, double *beta2);
+
 /* Cullum-Willoughby post-processing (lancbis, lancbis.cpp:5-105: BISEC
  * bisec.f:4-225, ISOEV isoev.f:4-55, INVERR inverr.f:4-167).  eval/err hold
  * niter entries; *ngood receives the number of good eigenvalues. */

@@ -1,20 +1,23 @@
 """Lanczos steps/s of the MI355X Hv + recurrence at a 54^3 x 36 basis.
 
 Workload (one GPU): Alavi H2 in a synthetic 5^12 cage of 20 rigid SPC/E
-waters (seed 0), sinc-DVR grid of 54 points per axis on a box of x_max
-(default 0.8 nm, i.e. [-0.4, 0.4] nm, spacing 0.0151 nm), tesseral
-harmonics l <= 5 (36 lm, full basis), 40 x 70 (theta, phi) quadrature,
-NoQuad potential (its per-point blocks held as even-L harmonic coefficients
-and expanded through the Gaunt coefficients in the kernel), FP64 Hv and FP64
-recurrence.  A step is one Hv plus one update of the three-term recurrence
-(folded into the Hv kernels plus a one-workgroup finalize); inputs
-(potential coefficients, Krylov vectors) are resident in HBM when the timed
-region starts.
+waters (seed 0), sinc-DVR grid of 54 points per axis on a box of full
+length x_max (default 0.4 nm, the reference's production decks' box:
+[-0.2, 0.2] nm, spacing 0.0075 nm), tesseral harmonics l <= 5 (36 lm, full
+basis), 40 x 70 (theta, phi) quadrature, NoQuad potential (its per-point
+blocks held as even-L harmonic coefficients and expanded through the Gaunt
+coefficients in the kernel), FP64 Hv and FP64 recurrence.  A step is one Hv
+plus one update of the three-term recurrence (folded into the Hv kernels
+plus a one-workgroup finalize); inputs (potential coefficients, Krylov
+vectors) are resident in HBM when the timed region starts.
 
 Also reported: the FP32-potential ("mixed") step rate and its ratio to FP64,
-the error of the mixed run's lowest levels against the FP64 run (cm^-1), the
-roofline of the dominant (potential) kernel and the CPU oracle timed
-on this host on a bounded sample of the same operator.
+the error of the mixed runs' lowest levels against the FP64 run (cm^-1) at
+10000 and 25000 steps, on this box and on two others (the same grid on the
+0.8 nm box, and the production decks' own 28^3 grid at 0.4 nm), the
+quadrature calculator at FP64, FP32 and BF16x3 with its level errors, the
+32-member batch, the adjoint leg on both boxes, the roofline of the dominant
+kernel and the reference's CPU path timed on this host on a bounded sample.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -32,19 +35,21 @@ sys.path.insert(0, os.path.join(REPO, "boundstate-lanczos_amd"))
 HBM_PEAK_GBS = 8000.0
 
 
-def make_calc(B, args, precision, basis="full", symmetry="All", calculator="noquad"):
+def make_calc(B, args, precision, basis="full", symmetry="All", calculator="noquad", n=None, x_max=None):
     from bsl_amd import cage, physics
     pos, q, lj = cage.synthetic_cage(0)
     lj_a, lj_b = cage.baseline_lj()
-    return B.Calculator(nx=args.n, x_max=args.x_max, lmax=5, n_theta=40, n_phi=70, mass=2.016,
+    return B.Calculator(nx=n or args.n, x_max=x_max or args.x_max, lmax=5, n_theta=40, n_phi=70, mass=2.016,
                         rot_const=physics.cm1_to_kjmol(59.322), ceiling=18.0, symmetry=symmetry, basis=basis,
                         precision=precision, potential="cage", sites=(pos, q, lj), lj_a=lj_a, lj_b=lj_b,
                         calculator=calculator)
 
 
-def quad_leg(B, args):
+def quad_leg(B, args, e_ref):
     """On-the-fly quadrature calculator at the same sizes: FP64, FP32 and
-    BF16x3 MFMA transforms over the resident weighted V grid."""
+    BF16x3 MFMA transforms over the resident weighted V grid; the lowest 20
+    levels of each after --quad-eig-steps steps against the FP64 NoQuad
+    levels `e_ref` of the same step count (cm^-1)."""
     out = {}
     for prec in ("fp64", "fp32", "bf16x3"):
         t0 = time.perf_counter()
@@ -64,15 +69,21 @@ def quad_leg(B, args):
         peak = {"fp64": 78.6, "fp32": 157.3, "bf16x3": 2500.0 / 3.0}[prec]
         out[prec]["mfma_peak_tflops"] = peak
         out[prec]["mfma_frac"] = out[prec]["tflops"] / peak
-        print("quad %s: prep %.2f s, %.1f steps/s, potential %.3f ms (%.0f GB/s, %.1f TFLOP/s)"
-              % (prec, prep, sps, ms[1], ach, out[prec]["tflops"]), file=sys.stderr, flush=True)
-        del c
+        if e_ref is not None and args.quad_eig_steps > 0:
+            co = c.lanczos(args.quad_eig_steps, start="random", seed=0)
+            e, _ = B.lancbis(*co, -10.0, 10.0, mult_tol=c.mult_tol)
+            out[prec]["max_err_cm1_vs_noquad_fp64"] = level_errors(e_ref, e)
+            out[prec]["eig_steps"] = args.quad_eig_steps
+        print("quad %s: prep %.2f s, %.1f steps/s, potential %.3f ms (%.0f GB/s, %.1f TFLOP/s), level err %s cm^-1"
+              % (prec, prep, sps, ms[1], ach, out[prec]["tflops"], out[prec].get("max_err_cm1_vs_noquad_fp64")),
+              file=sys.stderr, flush=True)
+        c.close()
     out["fp32_vs_fp64"] = out["fp32"]["steps_per_s"] / out["fp64"]["steps_per_s"]
     out["bf16x3_vs_fp64"] = out["bf16x3"]["steps_per_s"] / out["fp64"]["steps_per_s"]
     return out
 
 
-def adjoint_leg(B, args):
+def adjoint_leg(B, args, x_max):
     """Hellmann-Feynman gradients at a 41^3 grid (lmax 5, 40 x 70, same cage):
     lowest 10 levels from an FP32-Hv run (Loop 1, Cullum-Willoughby, Loop 2
     Ritz vectors recomputed, FP64 Rayleigh refinement), gradients summed over
@@ -82,7 +93,7 @@ def adjoint_leg(B, args):
     from bsl_amd import cage, physics
     pos, q, lj = cage.synthetic_cage(0)
     lj_a, lj_b = cage.baseline_lj()
-    base = dict(nx=args.adjoint_n, x_max=args.x_max, lmax=5, n_theta=40, n_phi=70, mass=2.016,
+    base = dict(nx=args.adjoint_n, x_max=x_max, lmax=5, n_theta=40, n_phi=70, mass=2.016,
                 rot_const=physics.cm1_to_kjmol(59.322), ceiling=18.0, potential="cage", sites=(pos, q, lj),
                 lj_a=lj_a, lj_b=lj_b, q_h2_h=physics.Q_H2_H, q_h2_cm=physics.Q_H2_CM)
     niter, count = args.adjoint_steps, 10
@@ -98,10 +109,10 @@ def adjoint_leg(B, args):
             t0 = time.perf_counter()
             gr = [c.hf_gradient(psi[gi], np.ones(len(gi))) for gi in groups]
             t_grad = time.perf_counter() - t0
-            return e, res, groups, gr, t_lev, t_grad
+            return e, res, groups, gr, t_lev, t_grad, c.unclamped_fraction()
 
-    e32, res, groups, g32, t_lev, t_grad = run(dict(base, precision="fp32"), True)
-    _, _, groups64, g64, _, _ = run(dict(base, precision="fp64"), True)
+    e32, res, groups, g32, t_lev, t_grad, unclamped = run(dict(base, precision="fp32"), True)
+    _, _, groups64, g64, _, _, _ = run(dict(base, precision="fp64"), True)
 
     def entry(g, name, idx):
         if name == "site_q":
@@ -134,13 +145,15 @@ def adjoint_leg(B, args):
             an = entry(g, name, idx)
             fd_err = max(fd_err, abs(an - fd) / max(abs(fd), 1e-3 / abs(x)))
             fp64_diff = max(fp64_diff, abs(an - entry(g_64, name, idx)) / max(abs(entry(g_64, name, idx)), 1e-12))
-    out = {"grid": args.adjoint_n, "levels": count, "multiplets": len(groups), "steps": niter,
-           "levels_s": t_lev, "gradient_ms": 1e3 * t_grad, "max_residual": float(np.max(res)),
+    out = {"grid": args.adjoint_n, "x_max_nm": x_max, "levels": count, "multiplets": len(groups), "steps": niter,
+           "levels_s": t_lev, "gradient_ms": 1e3 * t_grad, "unclamped_fraction": unclamped,
+           "max_residual": float(np.max(res)),
            "fd_parameters": [n if i is None else "%s[%d]" % (n, i) for n, i in names],
            "max_rel_err_vs_fd": fd_err, "max_rel_diff_vs_fp64_gradient": fp64_diff,
            "same_multiplets_as_fp64": [list(g) for g in groups] == [list(g) for g in groups64]}
-    print("adjoint %d^3: levels %.2f s, gradients %.1f ms, max rel err vs FD %.2e, vs FP64 gradient %.2e"
-          % (args.adjoint_n, t_lev, 1e3 * t_grad, fd_err, fp64_diff), file=sys.stderr, flush=True)
+    print("adjoint %d^3, x_max %.2f nm: levels %.2f s, gradients %.1f ms (%.1f%% of the pairs unclamped), max rel "
+          "err vs FD %.2e, vs FP64 gradient %.2e" % (args.adjoint_n, x_max, t_lev, 1e3 * t_grad, 100 * unclamped,
+                                                     fd_err, fp64_diff), file=sys.stderr, flush=True)
     return out
 
 
@@ -285,24 +298,60 @@ def cpu_baseline(B, calc, args):
             "ref_vs_oracle_hv_max_rel": ref_diff, "port": port}
 
 
-def eig_error(B, c64, c32, niter):
-    """Lowest 20 good levels of FP64 and mixed runs (same random start), and
-    the Cullum-Willoughby eigensolve timed on the host and on the GPU."""
+def level_errors(e_ref, e, count=20):
+    """Largest distance (cm^-1) from each of the `count` lowest reference
+    levels to the nearest level of `e`."""
     from bsl_amd import physics
-    coef = c64.lanczos(niter, start="random", seed=0)
-    t0 = time.perf_counter()
-    e64, r64 = B.lancbis(*coef, -10.0, 10.0)
-    t_host = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    e64d, _ = B.lancbis(*coef, -10.0, 10.0, device=True)
-    t_dev = time.perf_counter() - t0
-    e32, r32 = B.lancbis(*c32.lanczos(niter, start="random", seed=0), -10.0, 10.0, mult_tol=c32.mult_tol)
-    low = e64[:20]
-    d = [np.min(np.abs(e32 - e)) / physics.KJMOL_PER_CM1 for e in low]
-    tri = {"host_s": t_host, "device_s": t_dev, "good_levels": int(len(e64)), "mixed_mult_tol_kjmol": c32.mult_tol,
-           "good_levels_mixed": int(len(e32)),
-           "max_abs_diff_kjmol": float(np.max(np.abs(e64 - e64d))) if len(e64) == len(e64d) else None}
-    return float(np.max(d)), len(low), [float(x) for x in low[:5]], tri
+    low = e_ref[:count]
+    return float(max(np.min(np.abs(e - x)) for x in low) / physics.KJMOL_PER_CM1)
+
+
+def eig_error(B, c64, c32, steps):
+    """Lowest 20 good levels of FP64 and mixed runs (same random start) after
+    each step count in `steps` (the shorter runs are prefixes of the longest
+    one's coefficients), and the Cullum-Willoughby eigensolve timed on the
+    host and on the GPU at the first count.  Returns (per-count errors, FP64
+    levels of the first count, eigensolve timings)."""
+    nmax = max(steps)
+    co64 = c64.lanczos(nmax, start="random", seed=0)
+    co32 = c32.lanczos(nmax, start="random", seed=0)
+    out, low, tri = {}, None, None
+    for n in steps:
+        p64 = tuple(x[: n + (1 if k else 0)] for k, x in enumerate(co64))
+        p32 = tuple(x[: n + (1 if k else 0)] for k, x in enumerate(co32))
+        t0 = time.perf_counter()
+        e64, _ = B.lancbis(*p64, -10.0, 10.0)
+        t_host = time.perf_counter() - t0
+        e32, _ = B.lancbis(*p32, -10.0, 10.0, mult_tol=c32.mult_tol)
+        out[str(n)] = level_errors(e64, e32)
+        if low is None:
+            # the mixed run's lowest 20 levels refined on the device: Loop 2
+            # rebuilds their Ritz vectors, then the FP64 Rayleigh quotient
+            sel = np.sort(e32)[:20]
+            evtr, _ = B.trivec(*p32, sel, mult_tol=c32.mult_tol)
+            t0 = time.perf_counter()
+            _, e_ref, r_ref = c32.ritz_refine(n, evtr, len(sel), start="random", seed=0, vectors=False)
+            out["refined_%d" % n] = level_errors(e64, e_ref)
+            out["refined_%d_max_residual_kjmol" % n] = float(np.max(r_ref))
+            out["refined_%d_s" % n] = time.perf_counter() - t0
+            t0 = time.perf_counter()
+            e64d, _ = B.lancbis(*p64, -10.0, 10.0, device=True)
+            t_dev = time.perf_counter() - t0
+            low = e64
+            tri = {"steps": n, "host_s": t_host, "device_s": t_dev, "good_levels": int(len(e64)),
+                   "mixed_mult_tol_kjmol": c32.mult_tol, "good_levels_mixed": int(len(e32)),
+                   "max_abs_diff_kjmol": float(np.max(np.abs(e64 - e64d))) if len(e64) == len(e64d) else None}
+    return out, low, tri
+
+
+def eig_box(B, args, n, x_max, steps):
+    """The mixed-vs-FP64 level error of another grid or box (same cage)."""
+    c64 = make_calc(B, args, "fp64", n=n, x_max=x_max)
+    c32 = make_calc(B, args, "fp32", n=n, x_max=x_max)
+    err, low, _ = eig_error(B, c64, c32, steps)
+    c64.close()
+    c32.close()
+    return {"grid": n, "x_max_nm": x_max, "max_err_cm1": err, "lowest_kjmol": [float(x) for x in low[:5]]}
 
 
 def main():
@@ -311,8 +360,16 @@ def main():
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--n", type=int, default=54)
-    ap.add_argument("--x-max", type=float, default=0.8)
-    ap.add_argument("--eig-steps", type=int, default=10000)
+    ap.add_argument("--x-max", type=float, default=0.4,
+                    help="full box length (nm) of every leg; 0.4 is the reference's production decks' box")
+    ap.add_argument("--eig-steps", type=str, default="10000,25000",
+                    help="step counts of the mixed-vs-FP64 level comparison (comma-separated)")
+    ap.add_argument("--eig-boxes", type=str, default="54:0.8,28:0.4",
+                    help="further grid:x_max pairs for the 10000-step level comparison ('' skips them)")
+    ap.add_argument("--quad-eig-steps", type=int, default=10000,
+                    help="steps of the quadrature calculator's level comparison (0 skips it)")
+    ap.add_argument("--adjoint-boxes", type=str, default="0.4,0.8",
+                    help="box lengths (nm) of the adjoint leg (comma-separated)")
     ap.add_argument("--cpu-steps", type=int, default=40)
     ap.add_argument("--ref-hv-calls", type=int, default=32,
                     help="Hv calls of the reference's compiled calculator timed for cpu_baseline (0: the oracle only)")
@@ -369,23 +426,34 @@ def main():
               % (prep32, steps_s32, ms32[0] / args.steps, ms32[1], ms32[2], ms32[3]), file=sys.stderr,
               flush=True)
 
-    eig = None
-    if not args.no_eig and c32 is not None:
-        err, nlev, low, tri = eig_error(B, c64, c32, args.eig_steps)
-        eig = {"max_err_cm1": err, "levels": nlev, "steps": args.eig_steps, "lowest_kjmol": low,
-               "cullum_willoughby": tri}
-        print("mixed vs fp64 lowest %d levels: max err %.2e cm^-1" % (nlev, err), file=sys.stderr, flush=True)
-    del c32
+    eig, e_ref = None, None
+    steps_list = [int(x) for x in args.eig_steps.split(",") if x]
+    if not args.no_eig and c32 is not None and steps_list:
+        err, low, tri = eig_error(B, c64, c32, steps_list)
+        e_ref = low
+        eig = {"levels": 20, "max_err_cm1": err, "x_max_nm": args.x_max, "grid": args.n,
+               "lowest_kjmol": [float(x) for x in low[:5]], "cullum_willoughby": tri, "other_boxes": []}
+        print("mixed vs fp64 lowest 20 levels: max err %s cm^-1" % err, file=sys.stderr, flush=True)
+        for spec in [x for x in args.eig_boxes.split(",") if x]:
+            n, xm = spec.split(":")
+            r = eig_box(B, args, int(n), float(xm), [steps_list[0]])
+            eig["other_boxes"].append(r)
+            print("mixed vs fp64 at %s^3, x_max %s nm: %s cm^-1" % (n, xm, r["max_err_cm1"]), file=sys.stderr,
+                  flush=True)
+    if c32 is not None:
+        c32.close()
 
-    quad = quad_leg(B, args) if args.quad_steps > 0 else None
+    quad = quad_leg(B, args, e_ref) if args.quad_steps > 0 else None
     batch = batch_leg(B, args) if args.batch_steps > 0 else None
     adj = None
     if args.adjoint_n > 0:
-        try:
-            adj = adjoint_leg(B, args)
-        except B.BslError as e:
-            adj = {"error": str(e)}
-            print("adjoint leg: %s" % e, file=sys.stderr, flush=True)
+        adj = []
+        for xm in [float(x) for x in args.adjoint_boxes.split(",") if x]:
+            try:
+                adj.append(adjoint_leg(B, args, xm))
+            except B.BslError as e:
+                adj.append({"x_max_nm": xm, "error": str(e)})
+                print("adjoint leg: %s" % e, file=sys.stderr, flush=True)
 
     cpu = None
     if not args.no_cpu:

@@ -118,10 +118,25 @@ class Calculator:
             raise BslError(0, "the %d lowest levels are not all converged after %d steps" % (count, niter))
         sel = ev[order]
         evtr, _ = trivec(a, b, b2, sel, mult_tol=self.mult_tol)
-        psi = self.ritz(niter, evtr, count, start=start, seed=seed)
+        psi, e, r = self.ritz_refine(niter, evtr, count, start=start, seed=seed)
         psi /= np.linalg.norm(psi, axis=1, keepdims=True)
-        e, r = self.rayleigh(psi)
         return e, psi, r
+
+    def ritz_refine(self, niter, evtr, ngood, start="uniform", seed=0, v0=None, vectors=True):
+        """Loop 2 and the FP64 Rayleigh refinement of its Ritz vectors on the
+        device (bsl_ritz_refine): (Ritz vectors (ngood, N) or None, energies,
+        residual norms)."""
+        out = np.zeros(self.N * max(ngood, 1)) if vectors else None
+        e, r = np.zeros(max(ngood, 1)), np.zeros(max(ngood, 1))
+        ptr = None
+        if v0 is not None:
+            v0 = np.ascontiguousarray(v0, dtype=np.float64)
+            ptr = v0.ctypes.data
+        evtr = np.ascontiguousarray(evtr, dtype=np.float64)
+        check(lib().bsl_ritz_refine(self.h, niter, ptr, START[start], seed, evtr, ngood,
+                                    None if out is None else out.ctypes.data, e, r))
+        vec = None if out is None else out[: self.N * ngood].reshape(ngood, self.N)
+        return vec, e[:ngood], r[:ngood]
 
     def hf_gradient(self, psi, g):
         """Hellmann-Feynman gradient sum_k g[k] <psi_k|dV/dtheta|psi_k> of the
@@ -173,6 +188,13 @@ class Calculator:
     def potential_form(self, form):
         """Rebuilds the potential in the other form (clears the Krylov vectors)."""
         check(lib().bsl_set_potential_form(self.h, VFORM[form]))
+
+    def unclamped_fraction(self):
+        """Fraction of the (point, node) pairs of the potential grid below the
+        ceiling clamp (cage potential; bsl_clamp_stats)."""
+        u, t = C.c_int64(), C.c_int64()
+        check(lib().bsl_clamp_stats(self.h, C.byref(u), C.byref(t)))
+        return u.value / t.value
 
     def potential_kernel(self):
         """Name of the potential kernel (and autotuned variant) in use."""

@@ -26,7 +26,8 @@ EXPORTS = (
     "bsl_potential_kernel", "bsl_hf_gradient", "bsl_batch_create", "bsl_batch_destroy", "bsl_batch_size",
     "bsl_batch_member", "bsl_batch_lanczos", "bsl_batch_time_steps", "bsl_lancbis_device", "bsl_lancbis_tol",
     "bsl_lancbis_device_tol", "bsl_trivec_tol", "bsl_get_potential_form", "bsl_set_potential_form",
-    "bsl_gaunt_table", "bsl_lanczos_noalloc",
+    "bsl_gaunt_table", "bsl_lanczos_noalloc", "bsl_clamp_stats",
+    "bsl_ritz_refine",
 )
 
 
@@ -82,6 +83,8 @@ def load():
     lib.bsl_trivec_tol.argtypes = [C.c_int32, _dp, _dp, _dp, _dp, C.c_int32, C.c_double, _dp, _ip]
     lib.bsl_ritz.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_int32, C.c_uint64, _dp, C.c_int32, _dp]
     lib.bsl_rayleigh.argtypes = [C.c_void_p, C.c_int32, _dp, _dp, _dp]
+    lib.bsl_ritz_refine.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_int32, C.c_uint64, _dp, C.c_int32,
+                                    C.c_void_p, _dp, _dp]
     lib.bsl_water_sites.argtypes = [C.c_int32, C.c_int32, _dp, _dp, _dp, _ip]
     lib.bsl_get_vblocks.argtypes = [C.c_void_p, _dp]
     lib.bsl_get_vgrid.argtypes = [C.c_void_p, _dp, _dp]
@@ -92,6 +95,7 @@ def load():
     lib.bsl_set_potential_form.argtypes = [C.c_void_p, C.c_int32]
     lib.bsl_gaunt_table.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p]
     lib.bsl_hf_gradient.argtypes = [C.c_void_p, C.c_int32, _dp, _dp, _dp]
+    lib.bsl_clamp_stats.argtypes = [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
     lib.bsl_batch_create.argtypes = [C.c_int32, C.c_void_p, C.c_void_p]
     lib.bsl_batch_destroy.argtypes = [C.c_void_p]
     lib.bsl_batch_size.argtypes = [C.c_void_p]

@@ -184,6 +184,7 @@ extern "C" int bsl_batch_lanczos(bsl_batch *b, int32_t niter, int32_t start, uin
         if (e != hipSuccess) rc = hip_fail(e, "download coefficients", __FILE__, __LINE__);
         be[0] = 0.0;
         b2[0] = 0.0;
+        if (!rc) rc = check_breakdown(c->d_scal, c->stream);
     }
     return rc;
 }

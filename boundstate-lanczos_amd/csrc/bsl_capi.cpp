@@ -178,56 +178,78 @@ int build_vpk(bsl_ctx *c, DevOperator &op, int form, int vprec, const double *d_
     return launch_vblock_gemm(d_vg, d_h, op.vpk, vprec, c->P, c->Q, col0, ncol, c->stream);
 }
 
+// release the potential buffers of one operator
+void free_potential(DevOperator &o)
+{
+    dev_free(o.vpk);
+    dev_free(o.gc[0]);
+    dev_free(o.gc[1]);
+    o.vpk = nullptr;
+    o.gc[0] = o.gc[1] = nullptr;
+}
+
 // release the potential of op (and of op64 unless it shares op's)
 void free_potentials(bsl_ctx *c)
 {
-    DevOperator *ops[2] = {&c->op, &c->op64};
     const bool shared = c->op64.vpk == c->op.vpk;
-    for (int i = 0; i < 2; i++) {
-        DevOperator &o = *ops[i];
-        if (!(i == 1 && shared)) {
-            dev_free(o.vpk);
-            dev_free(o.gc[0]);
-            dev_free(o.gc[1]);
-        }
-        o.vpk = nullptr;
-        o.gc[0] = o.gc[1] = nullptr;
+    free_potential(c->op);
+    if (shared) {
+        c->op64.vpk = nullptr;
+        c->op64.gc[0] = c->op64.gc[1] = nullptr;
+    } else {
+        free_potential(c->op64);
     }
 }
 
-// op (NoQuad, FP64) reads op64's potential
-void share_potential(DevOperator &op, const DevOperator &op64)
+// op reads the potential of src (its buffers, form and partial count)
+void share_potential(DevOperator &op, const DevOperator &src)
 {
-    op.vpk = op64.vpk;
-    op.vprec = op64.vprec;
-    op.vform = op64.vform;
-    op.ncoef = op64.ncoef;
-    op.gc[0] = op64.gc[0];
-    op.gc[1] = op64.gc[1];
-    op.nalpha = op64.nalpha;
+    op.vpk = src.vpk;
+    op.vprec = src.vprec;
+    op.vform = src.vform;
+    op.ncoef = src.ncoef;
+    op.gc[0] = src.gc[0];
+    op.gc[1] = src.gc[1];
+    op.nalpha = src.nalpha;
 }
 
 // The potential of both operators in `form` from the grid d_vg: op64 in
 // FP64, op in the context precision (shared when that is FP64).  A
-// quadrature context's op has no potential blocks.
+// quadrature context's op has no potential blocks.  The new potential is
+// built into copies of the operators and replaces the old one only when
+// every part of it was built, so that a failure (out of memory, a term list
+// that does not match) leaves the context as it was.
 int build_potentials(bsl_ctx *c, int form, const double *d_vg)
 {
     std::vector<double> h;
     int64_t ncol = 0;
     potential_table(c, c->op64, form, h, &ncol);
+    const bool quad = c->op.calc == BSL_CALC_QUAD;
+    const bool shared = !quad && c->p.precision == BSL_PREC_FP64;
+    DevOperator n64 = c->op64, nop = c->op;
+    n64.vpk = nop.vpk = nullptr;
+    n64.gc[0] = n64.gc[1] = nop.gc[0] = nop.gc[1] = nullptr;
     double *d_h = nullptr;
     int rc = dev_upload(&d_h, h);
-    if (!rc) rc = build_vpk(c, c->op64, form, BSL_PREC_FP64, d_vg, d_h, ncol);
-    if (!rc && c->op.calc != BSL_CALC_QUAD) {
-        if (c->p.precision == BSL_PREC_FP64)
-            share_potential(c->op, c->op64);
+    if (!rc) rc = build_vpk(c, n64, form, BSL_PREC_FP64, d_vg, d_h, ncol);
+    if (!rc && !quad) {
+        if (shared)
+            share_potential(nop, n64);
         else
-            rc = build_vpk(c, c->op, form, BSL_PREC_FP32, d_vg, d_h, ncol);
+            rc = build_vpk(c, nop, form, BSL_PREC_FP32, d_vg, d_h, ncol);
     }
     const hipError_t e = hipStreamSynchronize(c->stream);
     dev_free(d_h);
     if (!rc && e != hipSuccess) rc = hip_fail(e, "build_potentials", __FILE__, __LINE__);
-    return rc;
+    if (rc) {
+        free_potential(n64);
+        if (!shared) free_potential(nop);
+        return rc;
+    }
+    free_potentials(c);
+    share_potential(c->op64, n64);
+    if (!quad) share_potential(c->op, nop);
+    return BSL_OK;
 }
 
 // H offsets of the quadrature nodes (q = half*na*nb + a*nb + b; the second
@@ -423,6 +445,8 @@ extern "C" int bsl_destroy(bsl_ctx *c)
     dev_free(c->d_X);
     dev_free(c->d_R);
     dev_free(c->d_T);
+    dev_free(c->d_ritz);
+    dev_free(c->d_rcoef);
     dev_free(c->d_scal);
     dev_free(c->d_alpha);
     dev_free(c->d_beta);
@@ -438,7 +462,8 @@ extern "C" int bsl_create(const bsl_params *pp, bsl_ctx **out)
     *out = nullptr;
     int rc = check_params(*pp);
     if (rc) return rc;
-    std::unique_ptr<bsl_ctx> c(new bsl_ctx());
+    // every early return below releases what was allocated so far
+    std::unique_ptr<bsl_ctx, int (*)(bsl_ctx *)> c(new bsl_ctx(), bsl_destroy);
     c->p = *pp;
     if (pp->potential == BSL_POT_CAGE || pp->potential == BSL_POT_CAGE_GRID) {
         c->site_pos.assign(pp->site_pos, pp->site_pos + 3 * pp->n_sites);
@@ -547,12 +572,14 @@ extern "C" int bsl_create(const bsl_params *pp, bsl_ctx **out)
         const int64_t ntiles = quad_tiles(c->P, nah, op.q_npts);
         BSL_HIP(hipMalloc(&op.wgrid, es * (size_t)ntiles * quad_phi_tiles(op.nphi) * 256));
         double *d_wt = nullptr, *d_wp = nullptr;
-        if ((rc = dev_upload(&d_wt, t.wt))) return rc;
-        if ((rc = dev_upload(&d_wp, t.wp))) return rc;
-        rc = launch_weight_grid(c->d_vgrid, op.wgrid, op.wprec, d_wt, d_wp, c->P, na, nb, op.q_npts, c->stream);
-        BSL_HIP(hipStreamSynchronize(c->stream));
+        rc = dev_upload(&d_wt, t.wt);
+        if (!rc) rc = dev_upload(&d_wp, t.wp);
+        if (!rc)
+            rc = launch_weight_grid(c->d_vgrid, op.wgrid, op.wprec, d_wt, d_wp, c->P, na, nb, op.q_npts, c->stream);
+        const hipError_t ew = hipStreamSynchronize(c->stream);
         dev_free(d_wt);
         dev_free(d_wp);
+        if (!rc && ew != hipSuccess) rc = hip_fail(ew, "weight grid", __FILE__, __LINE__);
         if (rc) return rc;
         std::vector<double> qL((size_t)nk * nah), qS((size_t)op.nm * op.nphi);
         std::vector<int> mpl(nk);
@@ -744,7 +771,7 @@ extern "C" int bsl_lanczos(bsl_ctx *c, int32_t niter, const double *v0, int32_t 
     BSL_HIP(hipStreamSynchronize(c->stream));
     beta[0] = 0.0;
     beta2[0] = 0.0;
-    return BSL_OK;
+    return check_breakdown(c->d_scal, c->stream);
 }
 
 extern "C" int bsl_lancbis_tol(int32_t niter, const double *alpha, const double *beta, const double *beta2,
@@ -804,12 +831,24 @@ extern "C" int bsl_trivec(int32_t niter, const double *alpha, const double *beta
     return bsl_trivec_tol(niter, alpha, beta, beta2, eval, ngood, 0.0, evtr, mamin);
 }
 
-extern "C" int bsl_ritz(bsl_ctx *c, int32_t niter, const double *v0, int32_t start, uint64_t seed,
-                        const double *evtr, int32_t ngood, double *ritz)
+namespace {
+// Grow-only device workspace: *p holds at least `count` doubles afterwards
+int ensure_workspace(double **p, size_t *cap, size_t count)
 {
-    if (!c || niter <= 0 || ngood < 0 || (ngood > 0 && (!evtr || !ritz))) return fail(BSL_E_ARG, "bad arguments");
-    if (ngood == 0) return BSL_OK;
-    BSL_HIP(hipSetDevice(c->device));
+    if (*cap >= count) return BSL_OK;
+    dev_free(*p);
+    *p = nullptr;
+    *cap = 0;
+    int rc = dev_alloc(p, count);
+    if (!rc) *cap = count;
+    return rc;
+}
+
+// Loop 2 (lanczosEigenValuesVectors.cpp:371-469): recomputes the Krylov
+// vectors from the same start and accumulates the ngood Ritz vectors into
+// c->d_ritz ([ngood][n_kry], device order)
+int ritz_run(bsl_ctx *c, int niter, const double *v0, int start, uint64_t seed, const double *evtr, int ngood)
+{
     // coefficient of v_j for each vector, zero once the cumulative norm
     // reaches 1 - 1e-16 (lanczosEigenValuesVectors.cpp:404-419)
     std::vector<double> coef((size_t)niter * ngood);
@@ -821,59 +860,133 @@ extern "C" int bsl_ritz(bsl_ctx *c, int32_t niter, const double *v0, int32_t sta
             coef[(size_t)(j - 1) * ngood + g] = (cum < (1. - 1.e-16)) ? e : 0.0;
         }
     }
-    double *d_coef = nullptr, *d_ritz = nullptr;
-    int rc;
-    if ((rc = dev_upload(&d_coef, coef))) return rc;
-    if ((rc = dev_alloc(&d_ritz, (size_t)c->n_kry * ngood))) {
-        dev_free(d_coef);
-        return rc;
-    }
-    BSL_HIP(hipMemsetAsync(d_ritz, 0, sizeof(double) * (size_t)c->n_kry * ngood, c->stream));
-    rc = ensure_coeffs(c, niter);
+    int rc = ensure_workspace(&c->d_rcoef, &c->rcoef_cap, coef.size());
+    if (!rc) rc = ensure_workspace(&c->d_ritz, &c->ritz_cap, (size_t)c->n_kry * ngood);
+    if (rc) return rc;
+    BSL_HIP(hipMemcpyAsync(c->d_rcoef, coef.data(), sizeof(double) * coef.size(), hipMemcpyHostToDevice, c->stream));
+    rc = launch_zero(c->d_ritz, c->n_kry * ngood, c->stream);
+    if (!rc) rc = ensure_coeffs(c, niter);
     if (!rc) rc = start_lanczos(c, v0, start, seed);
     for (int j = 1; j <= niter && !rc; j++) {
-        rc = launch_ritz_accum(d_ritz, c->d_X, c->d_R, c->d_scal, d_coef + (size_t)(j - 1) * ngood, c->n_kry, ngood,
-                               c->stream);
+        rc = launch_ritz_accum(c->d_ritz, c->d_X, c->d_R, c->d_scal, c->d_rcoef + (size_t)(j - 1) * ngood, c->n_kry,
+                               ngood, c->stream);
         if (!rc) rc = lanczos_step(c, c->op, j, nullptr);
     }
-    if (!rc) {
-        std::memset(ritz, 0, sizeof(double) * (size_t)c->n_full * ngood);
-        for (int g = 0; g < ngood && !rc; g++) rc = download_carried(c, d_ritz + (size_t)g * c->n_kry, ritz + (size_t)g * c->n_full);
-        if (!rc) {
-            hipError_t e = hipStreamSynchronize(c->stream);
-            if (e != hipSuccess) rc = hip_fail(e, "hipStreamSynchronize", __FILE__, __LINE__);
-        }
+    // the host coefficient list must outlive the copy
+    const hipError_t e = hipStreamSynchronize(c->stream);
+    if (!rc && e != hipSuccess) rc = hip_fail(e, "Lanczos loop 2", __FILE__, __LINE__);
+    if (!rc) rc = check_breakdown(c->d_scal, c->stream);
+    return rc;
+}
+
+// sum of the first 2 g partial pairs in a fixed order
+int sum_pairs(bsl_ctx *c, int g, double *s0, double *s1)
+{
+    std::vector<double> h((size_t)2 * g);
+    BSL_HIP(hipMemcpyAsync(h.data(), c->op.partial, sizeof(double) * h.size(), hipMemcpyDeviceToHost, c->stream));
+    BSL_HIP(hipStreamSynchronize(c->stream));
+    double a = 0.0, b = 0.0;
+    for (int i = 0; i < g; i++) {
+        a += h[2 * i];
+        b += h[2 * i + 1];
     }
-    dev_free(d_coef);
-    dev_free(d_ritz);
+    *s0 = a;
+    *s1 = b;
+    return BSL_OK;
+}
+
+// FP64 Rayleigh quotient <y|H|y>/<y|y> and residual |Hy - e y|/|y| of the
+// device vector y (n_kry, device order) with the context's FP64 operator,
+// whatever its precision; Hy goes to the second half of d_T
+int rayleigh_dev(bsl_ctx *c, const double *y, double *energy, double *resid)
+{
+    int rc = ensure_tmp(c);
+    if (rc) return rc;
+    double *dX = c->d_T, *dR = c->d_T + c->n_kry;
+    if (y != dX) BSL_HIP(hipMemcpyAsync(dX, y, sizeof(double) * c->n_kry, hipMemcpyDeviceToDevice, c->stream));
+    if ((rc = launch_zero(dR, c->n_kry, c->stream))) return rc;
+    if ((rc = launch_init_scal(c->d_scal, 1.0, 1.0, c->stream))) return rc;
+    if ((rc = launch_hv(c->op64, dX, dR, c->d_scal, c->stream, nullptr))) return rc;
+    int g = 0;
+    double vv = 0.0, vh = 0.0, r2 = 0.0, unused = 0.0;
+    if ((rc = launch_rayleigh_dots(dX, dR, c->n_kry, 0.0, 0, c->op.partial, c->op.npartial, &g, c->stream)))
+        return rc;
+    if ((rc = sum_pairs(c, g, &vv, &vh))) return rc;
+    if (!(vv > 0.0)) return fail(BSL_E_ARG, "Rayleigh quotient of a zero vector");
+    const double e = vh / vv;
+    if ((rc = launch_rayleigh_dots(dX, dR, c->n_kry, e, 1, c->op.partial, c->op.npartial, &g, c->stream)))
+        return rc;
+    if ((rc = sum_pairs(c, g, &r2, &unused))) return rc;
+    *energy = e;
+    *resid = std::sqrt(r2 / vv);
+    return BSL_OK;
+}
+
+// keeps the context's recurrence scalars across a refinement
+struct ScalGuard {
+    bsl_ctx *c;
+    Scal saved{};
+    bool ok = false;
+    explicit ScalGuard(bsl_ctx *cc) : c(cc)
+    {
+        ok = hipMemcpyAsync(&saved, c->d_scal, sizeof(Scal), hipMemcpyDeviceToHost, c->stream) == hipSuccess &&
+             hipStreamSynchronize(c->stream) == hipSuccess;
+    }
+    ~ScalGuard()
+    {
+        if (!ok) return;
+        (void)hipMemcpyAsync(c->d_scal, &saved, sizeof(Scal), hipMemcpyHostToDevice, c->stream);
+        (void)hipStreamSynchronize(c->stream);
+    }
+};
+}  // namespace
+
+extern "C" int bsl_ritz(bsl_ctx *c, int32_t niter, const double *v0, int32_t start, uint64_t seed,
+                        const double *evtr, int32_t ngood, double *ritz)
+{
+    if (!c || niter <= 0 || ngood < 0 || (ngood > 0 && (!evtr || !ritz))) return fail(BSL_E_ARG, "bad arguments");
+    if (ngood == 0) return BSL_OK;
+    BSL_HIP(hipSetDevice(c->device));
+    int rc = ritz_run(c, niter, v0, start, seed, evtr, ngood);
+    if (rc) return rc;
+    std::memset(ritz, 0, sizeof(double) * (size_t)c->n_full * ngood);
+    for (int g = 0; g < ngood && !rc; g++)
+        rc = download_carried(c, c->d_ritz + (size_t)g * c->n_kry, ritz + (size_t)g * c->n_full);
+    if (!rc) BSL_HIP(hipStreamSynchronize(c->stream));
+    return rc;
+}
+
+extern "C" int bsl_ritz_refine(bsl_ctx *c, int32_t niter, const double *v0, int32_t start, uint64_t seed,
+                               const double *evtr, int32_t ngood, double *ritz, double *energy, double *resid)
+{
+    if (!c || niter <= 0 || ngood < 0 || (ngood > 0 && (!evtr || !energy || !resid)))
+        return fail(BSL_E_ARG, "bad arguments");
+    if (ngood == 0) return BSL_OK;
+    BSL_HIP(hipSetDevice(c->device));
+    int rc = ritz_run(c, niter, v0, start, seed, evtr, ngood);
+    for (int g = 0; g < ngood && !rc; g++) rc = rayleigh_dev(c, c->d_ritz + (size_t)g * c->n_kry, energy + g, resid + g);
+    if (!rc && ritz) {
+        std::memset(ritz, 0, sizeof(double) * (size_t)c->n_full * ngood);
+        for (int g = 0; g < ngood && !rc; g++)
+            rc = download_carried(c, c->d_ritz + (size_t)g * c->n_kry, ritz + (size_t)g * c->n_full);
+        if (!rc) BSL_HIP(hipStreamSynchronize(c->stream));
+    }
     return rc;
 }
 
 extern "C" int bsl_rayleigh(bsl_ctx *c, int32_t count, const double *y, double *energy, double *resid)
 {
     if (!c || count < 0 || (count > 0 && (!y || !energy || !resid))) return fail(BSL_E_ARG, "bad arguments");
+    if (count == 0) return BSL_OK;
     BSL_HIP(hipSetDevice(c->device));
-    std::vector<double> hy((size_t)c->n_full);
-    for (int g = 0; g < count; g++) {
-        const double *v = y + (size_t)g * c->n_full;
-        std::fill(hy.begin(), hy.end(), 0.0);
-        int rc = hv_host_with(c, c->op64, v, hy.data());
-        if (rc) return rc;
-        double vv = 0, vh = 0;
-        for (int64_t i = 0; i < c->n_full; i++) {
-            vv += v[i] * v[i];
-            vh += v[i] * hy[i];
-        }
-        const double e = vh / vv;
-        double r2 = 0;
-        for (int64_t i = 0; i < c->n_full; i++) {
-            const double d = hy[i] - e * v[i];
-            r2 += d * d;
-        }
-        energy[g] = e;
-        resid[g] = std::sqrt(r2 / vv);
+    int rc = ensure_tmp(c);
+    if (rc) return rc;
+    ScalGuard keep(c);
+    for (int g = 0; g < count && !rc; g++) {
+        rc = upload_carried(c, y + (size_t)g * c->n_full, c->d_T);
+        if (!rc) rc = rayleigh_dev(c, c->d_T, energy + g, resid + g);
     }
-    return BSL_OK;
+    return rc;
 }
 
 extern "C" int bsl_water_sites(int32_t model, int32_t nmol, const double *cm_ea, double *pos, double *q,
@@ -1001,7 +1114,7 @@ extern "C" int bsl_gaunt_table(int32_t lb, int32_t par, int32_t cap, int32_t *ij
 
 extern "C" int bsl_get_potential_form(const bsl_ctx *c)
 {
-    if (!c) return fail(BSL_E_ARG, "null argument");
+    if (!c) return -fail(BSL_E_ARG, "null argument");
     return c->op64.vform;
 }
 
@@ -1017,10 +1130,7 @@ extern "C" int bsl_set_potential_form(bsl_ctx *c, int32_t form)
     double *d_vg = nullptr;
     int rc = dev_alloc(&d_vg, (size_t)c->P * c->Q);
     if (!rc) rc = make_vgrid(c, d_vg);
-    if (!rc) {
-        free_potentials(c);
-        rc = build_potentials(c, form, d_vg);
-    }
+    if (!rc) rc = build_potentials(c, form, d_vg);
     dev_free(d_vg);
     if (!rc) rc = autotune(c);
     return rc;
@@ -1061,6 +1171,54 @@ extern "C" int bsl_get_vgrid(bsl_ctx *c, double *v1, double *v2)
     return BSL_OK;
 }
 
+namespace {
+// Which nodes are below the ceiling clamp: fixed by the context's potential,
+// found from its grid (regenerated into d_vg, P x Q doubles) once and kept,
+// one bit per (point, node).
+int ensure_unclamped(bsl_ctx *c, double *d_vg)
+{
+    if (c->d_unclamped) return BSL_OK;
+    int rc = make_vgrid(c, d_vg);
+    if (!rc) rc = dev_alloc(&c->d_unclamped, ((size_t)c->P * c->Q + 63) / 64);
+    if (!rc) rc = launch_unclamped_mask(d_vg, c->P * c->Q, c->p.ceiling, c->d_unclamped, c->stream);
+    const hipError_t e = hipStreamSynchronize(c->stream);
+    if (!rc && e != hipSuccess) rc = hip_fail(e, "clamp mask", __FILE__, __LINE__);
+    if (rc) {
+        dev_free(c->d_unclamped);
+        c->d_unclamped = nullptr;
+    }
+    return rc;
+}
+}  // namespace
+
+extern "C" int bsl_clamp_stats(bsl_ctx *c, int64_t *unclamped, int64_t *total)
+{
+    if (!c || !unclamped || !total) return fail(BSL_E_ARG, "null argument");
+    if (c->p.potential != BSL_POT_CAGE) return fail(BSL_E_STATE, "bsl_clamp_stats: cage potential only");
+    BSL_HIP(hipSetDevice(c->device));
+    int rc = BSL_OK;
+    if (!c->d_unclamped) {
+        double *d_vg = nullptr;
+        rc = dev_alloc(&d_vg, (size_t)c->P * c->Q);
+        if (!rc) rc = ensure_unclamped(c, d_vg);
+        dev_free(d_vg);
+        if (rc) return rc;
+    }
+    const size_t n = (size_t)c->P * c->Q, words = (n + 63) / 64;
+    std::vector<uint64_t> m(words);
+    BSL_HIP(hipMemcpyAsync(m.data(), c->d_unclamped, sizeof(uint64_t) * words, hipMemcpyDeviceToHost, c->stream));
+    BSL_HIP(hipStreamSynchronize(c->stream));
+    int64_t cnt = 0;
+    for (size_t w = 0; w < words; w++) {
+        uint64_t x = m[w];
+        if (w == words - 1 && n % 64) x &= (~0ull) >> (64 - n % 64);
+        cnt += __builtin_popcountll(x);
+    }
+    *unclamped = cnt;
+    *total = (int64_t)n;
+    return BSL_OK;
+}
+
 extern "C" int bsl_hf_gradient(bsl_ctx *c, int32_t nvec, const double *psi, const double *g, double *grad)
 {
     if (!c || nvec <= 0 || !psi || !g || !grad) return fail(BSL_E_ARG, "bsl_hf_gradient: bad arguments");
@@ -1093,18 +1251,7 @@ extern "C" int bsl_hf_gradient(bsl_ctx *c, int32_t nvec, const double *psi, cons
     for (int k = 0; k < nvec && !rc; k++)
         rc = upload_carried(c, psi + (size_t)k * c->n_full, d_psi + (size_t)k * c->n_kry);
     if (!rc) rc = dev_alloc(&d_vg, (size_t)c->P * Q);
-    // which nodes are below the ceiling clamp: fixed by the context's
-    // potential, found from its grid once and kept (one bit per node)
-    if (!rc && !c->d_unclamped) {
-        rc = make_vgrid(c, d_vg);
-        if (!rc) rc = dev_alloc(&c->d_unclamped, ((size_t)c->P * Q + 63) / 64);
-        if (!rc) rc = launch_unclamped_mask(d_vg, c->P * Q, p.ceiling, c->d_unclamped, c->stream);
-        if (rc) {
-            (void)hipStreamSynchronize(c->stream);
-            dev_free(c->d_unclamped);
-            c->d_unclamped = nullptr;
-        }
-    }
+    if (!rc) rc = ensure_unclamped(c, d_vg);
     if (!rc) rc = dev_upload(&d_Y, Y);
     if (!rc) rc = dev_upload(&d_wq, wq);
     if (!rc) rc = dev_upload(&d_g, gv);

@@ -24,8 +24,14 @@ struct Scal {
     double ca, cb, cr;
     double alpha, beta, beta2;
     long long step;  // completed Lanczos steps: alpha/beta outputs are indexed by it
-    double pad;
+    // first step (1-based) whose beta^2 = |R|^2 - 2 alpha v.R + alpha^2 |v|^2
+    // kept no correct digit (at most 64 eps |R|^2: the residual vanished or
+    // cancelled), 0 if none; the host turns it into an error after the run
+    long long breakdown;
 };
+// BSL_E_STATE with the step of a recorded cancellation, or BSL_OK (reads the
+// scalars after synchronising `s`)
+int check_breakdown(const Scal *sc, hipStream_t s);
 
 // error reporting (thread-local message + code)
 int fail(int code, const std::string &msg);
@@ -195,6 +201,11 @@ int launch_zero(double *p, int64_t n, hipStream_t s);
 int launch_ritz_accum(double *ritz, const double *X, const double *R, const Scal *sc, const double *coef, int64_t n,
                       int ngood, hipStream_t s);
 
+// per-workgroup pieces of a Rayleigh quotient (mode 0: y.y and y.hy; mode 1:
+// |hy - e y|^2) into partial[2 b], partial[2 b + 1], b < *groups
+int launch_rayleigh_dots(const double *y, const double *hy, int64_t n, double e, int mode, double *partial,
+                         int npartial, int *groups, hipStream_t s);
+
 // prep kernels
 struct CageDev {
     int nsites = 0;
@@ -268,6 +279,10 @@ struct bsl_ctx {
     uint64_t *d_unclamped = nullptr;   // bit p*Q + q: V(p, q) < ceiling; built by the first gradient call
     int Q = 0;
     double *d_X = nullptr, *d_R = nullptr, *d_T = nullptr;
+    // Loop-2 workspace, kept between calls (grows to the largest request):
+    // Ritz accumulators [ngood][n_kry] and the per-step coefficients
+    double *d_ritz = nullptr, *d_rcoef = nullptr;
+    size_t ritz_cap = 0, rcoef_cap = 0;
     bsl::Scal *d_scal = nullptr;
     double *d_alpha = nullptr, *d_beta = nullptr, *d_beta2 = nullptr;
     int niter_cap = 0;

@@ -103,6 +103,9 @@ __device__ inline void finalize_step(const double *__restrict__ pa, const double
     const double b2 = rr - 2.0 * alpha * t + alpha * alpha * vv;
     const double beta = sqrt(b2);
     const long long j = sc->step;
+    // no correct digit left in b2 (a vanishing or cancelled residual): record
+    // the first such step; the host reports it as an error after the run
+    if (!(b2 > 64.0 * 2.220446049250313e-16 * rr) && sc->breakdown == 0) sc->breakdown = j + 1;
     if (alpha_out) alpha_out[j] = alpha;
     if (beta_out) beta_out[j + 1] = beta;
     if (beta2_out) beta2_out[j + 1] = b2;
@@ -1004,6 +1007,7 @@ __global__ __launch_bounds__(kWG) void k_finalize(const double *__restrict__ pa,
         sc->cb = 0.0;
         sc->cr = 0.0;
         sc->step = 0;
+        sc->breakdown = 0;
     }
 }
 
@@ -1016,6 +1020,7 @@ __global__ void k_set_scal(Scal *sc, double ca, double cr)
     sc->beta = 0.0;
     sc->beta2 = 0.0;
     sc->step = 0;
+    sc->breakdown = 0;
 }
 
 __global__ void k_fill(double *p, int64_t n, double v)
@@ -1066,6 +1071,35 @@ __global__ void k_ritz_accum(double *ritz, const double *__restrict__ X, const d
             const double c = coef[g];
             if (c != 0.0) ritz[(size_t)g * n + i] += c * v;
         }
+    }
+}
+
+// Rayleigh quotient pieces of y and hy = H y, per workgroup (grid-stride
+// shares, summed in a fixed order): mode 0 writes sum y^2 and sum y.hy to
+// partial[2 b], partial[2 b + 1]; mode 1 writes sum (hy - e y)^2 to
+// partial[2 b] (the residual in a pass of its own: |hy|^2 - 2 e y.hy + e^2 |y|^2
+// would cancel to nothing for a converged vector)
+__global__ __launch_bounds__(kWG) void k_rayleigh_dots(const double *__restrict__ y, const double *__restrict__ hy,
+                                                       int64_t n, double e, int mode, double *partial)
+{
+    __shared__ double red[kWG / 64];
+    double s0 = 0.0, s1 = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const double a = y[i], b = hy[i];
+        if (mode == 0) {
+            s0 += a * a;
+            s1 += a * b;
+        } else {
+            const double d = b - e * a;
+            s0 += d * d;
+        }
+    }
+    const double t0 = block_sum(s0, red);
+    __syncthreads();
+    const double t1 = block_sum(s1, red);
+    if (threadIdx.x == 0) {
+        partial[2 * blockIdx.x] = t0;
+        partial[2 * blockIdx.x + 1] = t1;
     }
 }
 
@@ -1178,6 +1212,9 @@ static int launch_vgaunt_t(const DevOperator &op, const double *X, double *R, hi
 int launch_potential(const DevOperator &op, const double *X, double *R, Scal *sc, hipStream_t s)
 {
     if (op.calc == BSL_CALC_QUAD) return launch_quad_potential(op, X, R, sc, s);
+    // a context whose potential was never built (or not rebuilt) launches nothing
+    if (!op.vpk || (op.vform == BSL_VFORM_HARMONICS && (!op.gc[0] || (op.nblk > 1 && !op.gc[1]))))
+        return fail(BSL_E_STATE, "the context has no potential to apply");
     if (op.vform == BSL_VFORM_HARMONICS)
         return op.vprec == BSL_PREC_FP64 ? launch_vgaunt_t<double>(op, X, R, s) : launch_vgaunt_t<float>(op, X, R, s);
     return op.vprec == BSL_PREC_FP64 ? launch_vblock_t<double>(op, X, R, s) : launch_vblock_t<float>(op, X, R, s);
@@ -1216,6 +1253,19 @@ int launch_step(const DevOperator &op, double *X, double *R, Scal *sc, double *a
     return BSL_OK;
 }
 
+int check_breakdown(const Scal *sc, hipStream_t s)
+{
+    Scal h;
+    BSL_HIP(hipMemcpyAsync(&h, sc, sizeof(Scal), hipMemcpyDeviceToHost, s));
+    BSL_HIP(hipStreamSynchronize(s));
+    if (h.breakdown != 0)
+        return fail(BSL_E_STATE, "Lanczos breakdown at step " + std::to_string(h.breakdown) +
+                                     ": beta^2 = |r|^2 - alpha^2 kept no correct digit (the residual vanished: "
+                                     "the Krylov space of the start vector is invariant); restart from another "
+                                     "start vector");
+    return BSL_OK;
+}
+
 int launch_init_scal(Scal *sc, double ca, double cr, hipStream_t s)
 {
     hipLaunchKernelGGL(k_set_scal, dim3(1), dim3(1), 0, s, sc, ca, cr);
@@ -1246,6 +1296,18 @@ int launch_start_vector(double *X, int64_t n, int64_t P, const int *plane_ref, i
     hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kWG), 0, s, partial, partial, partial, (unsigned)g, sc, 1, nullptr,
                        nullptr, nullptr);
     BSL_HIP(hipGetLastError());
+    return BSL_OK;
+}
+
+int launch_rayleigh_dots(const double *y, const double *hy, int64_t n, double e, int mode, double *partial,
+                         int npartial, int *groups, hipStream_t s)
+{
+    int g = grid_for(n, 1024);
+    if (2 * g > npartial) g = npartial / 2;
+    if (g < 1) return fail(BSL_E_STATE, "no room for the Rayleigh partial sums");
+    hipLaunchKernelGGL(k_rayleigh_dots, dim3(g), dim3(kWG), 0, s, y, hy, n, e, mode, partial);
+    BSL_HIP(hipGetLastError());
+    *groups = g;
     return BSL_OK;
 }
 
@@ -1529,54 +1591,87 @@ int launch_vgrid_coulomb(double *vg, int64_t P, int Q, const double *gx, const d
 // ---------------------------------------------------------------------------
 // preparation: V blocks, Vpk[pair][P] = sum_q Vg[P][q] H[q][pair]  (FP64 GEMM)
 // ---------------------------------------------------------------------------
-// 64 points x 64 pairs per workgroup, K chunks of 16 quadrature nodes staged
-// in LDS, a 4x4 register tile per thread.
+// FP64 MFMA (16x16x4) GEMM.  A workgroup owns 64 points and up to
+// kGemmCols columns (the 66 harmonic columns of lmax 5 in one workgroup, so
+// the P x Q grid, 7 GB at 54^3 x 5600 nodes, is read once); wave w owns
+// points 16w .. 16w+15 and every column tile.  The grid and the angular table
+// are staged through LDS in K blocks of kGemmK nodes, double buffered: the
+// loads of block b+1 are issued before the MFMAs of block b.  Accumulator
+// layout of mfma_f64_16x16x4: lane l holds rows (l/16) + 4q of column l%16.
+constexpr int kGemmK = 16;
+constexpr int kGemmColTiles = 5;
+constexpr int kGemmCols = 16 * kGemmColTiles;
+
 template <typename VT>
 __global__ __launch_bounds__(kWG) void k_vblock_gemm(const double *__restrict__ vg, const double *__restrict__ h,
                                                      VT *__restrict__ vpk, int64_t P, int Q, int64_t npair0,
                                                      int64_t npair)
 {
+    __shared__ double As[2][64][kGemmK + 1];         // [buffer][point][node]
+    __shared__ double Bs[2][kGemmK][kGemmCols + 1];  // [buffer][node][column]
     const int64_t ntile = vpk_ntile(P);
-    __shared__ double As[16][64 + 1];  // [kk][point]
-    __shared__ double Bs[16][64 + 1];  // [kk][pair]
-    const int tm = threadIdx.x & 15, tn = threadIdx.x >> 4;
     const int64_t p0 = (int64_t)blockIdx.x * 64;
-    const int64_t c0 = (int64_t)blockIdx.y * 64;
-    double acc[4][4] = {};
-    for (int q0 = 0; q0 < Q; q0 += 16) {
-        for (int t = threadIdx.x; t < 64 * 16; t += kWG) {
-            const int pp = t >> 4, kk = t & 15;  // A: 16 consecutive q per point
-            const int64_t p = p0 + pp;
-            const int q = q0 + kk;
-            As[kk][pp] = (p < P && q < Q) ? vg[(size_t)p * Q + q] : 0.0;
-            const int kb = t >> 6, cc = t & 63;  // B: 64 consecutive pairs per q
-            const int64_t c = c0 + cc;
-            const int qb = q0 + kb;
-            Bs[kb][cc] = (c < npair && qb < Q) ? h[(size_t)qb * npair + c] : 0.0;
+    const int64_t c0 = (int64_t)blockIdx.y * kGemmCols;
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int l16 = lane & 15, kq = lane >> 4;
+    // staging map: A 64 x 16 = 4 nodes per thread (one 32-byte run),
+    // B 16 x 80 = 5 columns per thread
+    const int ap = threadIdx.x >> 2, aq = (threadIdx.x & 3) * 4;
+    const int bq = threadIdx.x >> 4, bc = (threadIdx.x & 15) * kGemmColTiles;
+    const bool a_live = p0 + ap < P;
+    const double *arow = vg + (size_t)(a_live ? p0 + ap : 0) * Q;
+    double ra[4], rb[kGemmColTiles];
+    // every load address is clamped into its array; the values outside are
+    // replaced by zeros
+    auto load = [&](int q0) {
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+            const int q = q0 + aq + e;
+            const double v = arow[q < Q ? q : Q - 1];
+            ra[e] = (a_live && q < Q) ? v : 0.0;
         }
-        __syncthreads();
+        const int qb = q0 + bq;
+        const int qc = qb < Q ? qb : Q - 1;
 #pragma unroll
-        for (int kk = 0; kk < 16; kk++) {
-            double a[4], b[4];
-#pragma unroll
-            for (int r = 0; r < 4; r++) a[r] = As[kk][tm * 4 + r];
-#pragma unroll
-            for (int c = 0; c < 4; c++) b[c] = Bs[kk][tn * 4 + c];
-#pragma unroll
-            for (int r = 0; r < 4; r++)
-#pragma unroll
-                for (int c = 0; c < 4; c++) acc[r][c] += a[r] * b[c];
+        for (int e = 0; e < kGemmColTiles; e++) {
+            const int64_t c = c0 + bc + e;
+            const double v = h[(size_t)qc * npair + (c < npair ? c : npair - 1)];
+            rb[e] = (qb < Q && c < npair) ? v : 0.0;
         }
+    };
+    auto store = [&](int buf) {
+#pragma unroll
+        for (int e = 0; e < 4; e++) As[buf][ap][aq + e] = ra[e];
+#pragma unroll
+        for (int e = 0; e < kGemmColTiles; e++) Bs[buf][bq][bc + e] = rb[e];
+    };
+    kd4 acc[kGemmColTiles];
+#pragma unroll
+    for (int t = 0; t < kGemmColTiles; t++) acc[t] = kd4{0, 0, 0, 0};
+    const int nblk = (Q + kGemmK - 1) / kGemmK;
+    load(0);
+    store(0);
+    __syncthreads();
+    for (int b = 0; b < nblk; b++) {
+        const int cur = b & 1;
+        if (b + 1 < nblk) load((b + 1) * kGemmK);
+#pragma unroll
+        for (int ks = 0; ks < kGemmK / 4; ks++) {
+            const double a = As[cur][16 * wid + l16][4 * ks + kq];
+#pragma unroll
+            for (int t = 0; t < kGemmColTiles; t++) acc[t] = kmfma(a, Bs[cur][4 * ks + kq][16 * t + l16], acc[t]);
+        }
+        if (b + 1 < nblk) store(cur ^ 1);
         __syncthreads();
     }
 #pragma unroll
-    for (int c = 0; c < 4; c++) {
-        const int64_t pair = c0 + tn * 4 + c;
-        if (pair >= npair) continue;
+    for (int t = 0; t < kGemmColTiles; t++) {
+        const int64_t col = c0 + 16 * t + l16;
+        if (col >= npair) continue;
 #pragma unroll
-        for (int r = 0; r < 4; r++) {
-            const int64_t p = p0 + tm * 4 + r;
-            if (p < P) vpk[vpk_index(pair, p, npair0, npair, ntile)] = (VT)acc[r][c];
+        for (int q = 0; q < 4; q++) {
+            const int64_t p = p0 + 16 * wid + kq + 4 * q;
+            if (p < P) vpk[vpk_index(col, p, npair0, npair, ntile)] = (VT)acc[t][q];
         }
     }
 }
@@ -1584,7 +1679,7 @@ __global__ __launch_bounds__(kWG) void k_vblock_gemm(const double *__restrict__ 
 int launch_vblock_gemm(const double *vg, const double *h, void *vpk, int vprec, int64_t P, int Q, int64_t npair0,
                        int64_t npair, hipStream_t s)
 {
-    dim3 grid((unsigned)((P + 63) / 64), (unsigned)((npair + 63) / 64));
+    dim3 grid((unsigned)((P + 63) / 64), (unsigned)((npair + kGemmCols - 1) / kGemmCols));
     if (vprec == BSL_PREC_FP64)
         hipLaunchKernelGGL(k_vblock_gemm<double>, grid, dim3(kWG), 0, s, vg, h, (double *)vpk, P, Q, npair0, npair);
     else

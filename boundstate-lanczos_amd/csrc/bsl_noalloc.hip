@@ -118,5 +118,5 @@ extern "C" int bsl_lanczos_noalloc(bsl_ctx *c, int32_t niter, const double *v0, 
     BSL_HIP(hipStreamSynchronize(c->stream));
     beta[0] = 0.0;
     beta2[0] = 0.0;
-    return BSL_OK;
+    return check_breakdown(sc, c->stream);
 }

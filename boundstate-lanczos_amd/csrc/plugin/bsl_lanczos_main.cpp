@@ -205,7 +205,25 @@ int main(int argc, char **argv)
 
     // ---- Loop 2: Ritz vectors (:371-495) ----
     std::vector<double> ritz((size_t)ntotbs * std::max(ngood, 1));
-    check(bsl_ritz(ctx, niter, nullptr, BSL_START_UNIFORM, 0, evtr.data(), ngood, ritz.data()), "Lanczos loop 2");
+    const double mult_tol = bsl_plugin_mult_tol(general_data);
+    if (mult_tol > 0.0) {
+        // a mixed-precision operator: its levels are refined on the device by
+        // the FP64 Rayleigh quotient of their Ritz vectors (FP64 Hv), and the
+        // refined values written beside the reference's files
+        std::vector<double> ref_e(std::max(ngood, 1)), ref_r(std::max(ngood, 1));
+        check(bsl_ritz_refine(ctx, niter, nullptr, BSL_START_UNIFORM, 0, evtr.data(), ngood, ritz.data(), ref_e.data(),
+                              ref_r.data()),
+              "Lanczos loop 2 and FP64 refinement");
+        std::ofstream rf((dir + "/refined_levels.txt").c_str());
+        rf << std::scientific << std::setprecision(15) << endl;
+        rf << "FP64 Rayleigh-quotient refinement of the levels of the mixed-precision simulation " << sim_descr << endl;
+        rf << "Number of Eigenvalues: " << ngood << endl;
+        rf << "State | Lanczos Eigenvalue (kJ/mol) | Refined Eigenvalue (kJ/mol) | Residual |Hy - Ey|/|y| (kJ/mol)"
+           << endl;
+        for (int i = 0; i < ngood; i++) rf << i << " " << eval[i] << " " << ref_e[i] << " " << ref_r[i] << endl;
+    } else {
+        check(bsl_ritz(ctx, niter, nullptr, BSL_START_UNIFORM, 0, evtr.data(), ngood, ritz.data()), "Lanczos loop 2");
+    }
     {
         std::ofstream ev((dir + "/eigVecFile.txt").c_str());
         ev << std::scientific << std::setprecision(15) << endl;

@@ -4,9 +4,11 @@
  *
  * Plain C types only.  Every entry point returns 0 on success and a non-zero
  * BSL_E_* code on failure; bsl_last_error() then describes the failure (per
- * thread).  A context is not thread-safe: calls on one context must be
- * serialised, as the reference driver serialises its Hv calls.  Several
- * contexts may coexist.
+ * thread).  The entry points that return a count or a value instead
+ * (bsl_get_potential_form, bsl_gaunt_table) return minus the BSL_E_* code on
+ * failure, so that no error reads as a valid result.  A context is not
+ * thread-safe: calls on one context must be serialised, as the reference
+ * driver serialises its Hv calls.  Several contexts may coexist.
  *
  * Vector layout at this boundary is the reference's composite index
  * [n][k][j][i] (i fastest; n = lm index in the reference order, m outer from
@@ -199,14 +201,27 @@ int bsl_trivec_tol(int32_t niter, const double *alpha, const double *beta, const
 
 /* Lanczos loop 2 (lanczosEigenValuesVectors.cpp:371-469): recomputes the
  * Krylov vectors from the same start and accumulates the Ritz vectors on
- * the device; ritz is host memory [ngood][bsl_basis_size] (reference layout). */
+ * the device; ritz is host memory [ngood][bsl_basis_size] (reference layout).
+ * The device accumulators are kept by the context and reused (they grow to
+ * the largest ngood asked for, until bsl_destroy). */
 int bsl_ritz(bsl_ctx *ctx, int32_t niter, const double *v0, int32_t start, uint64_t seed, const double *evtr,
              int32_t ngood, double *ritz);
 
 /* FP64 Rayleigh quotients <y|H|y>/<y|y> of host vectors (reference layout,
  * [count][bsl_basis_size]) with an FP64 Hv, whatever the context precision;
- * also returns the residual norms |Hy - e y|/|y|. */
+ * also returns the residual norms |Hy - e y|/|y|.  Hy, the dot products and
+ * the residual are computed on the device (only y crosses PCIe).  No
+ * reference counterpart (the reference reports the Lanczos values). */
 int bsl_rayleigh(bsl_ctx *ctx, int32_t count, const double *y, double *energy, double *resid);
+
+/* Loop 2 followed by the FP64 refinement of every Ritz vector it builds,
+ * with the vectors kept on the device: bsl_ritz (same arguments) and then
+ * bsl_rayleigh of each result, without the vectors crossing PCIe.  ritz may
+ * be NULL; otherwise it receives what bsl_ritz returns.  Loop 2
+ * (lanczosEigenValuesVectors.cpp:371-469) plus a refinement with no
+ * reference counterpart. */
+int bsl_ritz_refine(bsl_ctx *ctx, int32_t niter, const double *v0, int32_t start, uint64_t seed,
+                    const double *evtr, int32_t ngood, double *ritz, double *energy, double *resid);
 
 /* Water sites from centres of mass and ZY'Z'' Euler angles, as read from
  * the reference's .xyz geometry files (getSPCEatoms,
@@ -230,7 +245,7 @@ int bsl_get_vgrid(bsl_ctx *ctx, double *v1, double *v2);
  * (linRotCartSphHvFullVPrepRoutines.cpp:229-528); HARMONICS has no
  * reference counterpart.  For the quadrature calculator these act on the
  * FP64 NoQuad operator the context keeps for its FP64 passes. */
-int bsl_get_potential_form(const bsl_ctx *ctx);
+int bsl_get_potential_form(const bsl_ctx *ctx); /* BSL_VFORM_*, or -BSL_E_ARG */
 int bsl_set_potential_form(bsl_ctx *ctx, int32_t form);
 
 /* The harmonic-form term list of parity block (lb, par): the planes are
@@ -271,6 +286,12 @@ int bsl_step_bytes(const bsl_ctx *ctx, double *out);
 // bsl_destroy; later calls skip that pass.  No reference counterpart (the
 // reference has no derivatives).
 int bsl_hf_gradient(bsl_ctx *ctx, int32_t nvec, const double *psi, const double *g, double *grad);
+
+// Counts of the (point, quadrature node) pairs of a cage context's potential
+// grid that lie below the ceiling clamp (the pairs bsl_hf_gradient sums
+// over) and of all pairs; builds the clamp mask if no gradient call did.  No
+// reference counterpart (measurement support).
+int bsl_clamp_stats(bsl_ctx *ctx, int64_t *unclamped, int64_t *total);
 
 // ---- batched recurrences (no reference counterpart: the reference runs one
 // recurrence per process, lanczosEigenValuesVectors.cpp:158-242) ----

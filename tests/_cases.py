@@ -77,3 +77,27 @@ def iho_case(nx, lmax, na, nb, x_max, omega_cm1=100.0, mass=2.016, rot=0.7096487
 
 def ref_to_device_order(vblocks_ref):
     return vblocks_ref
+
+
+def baseline_cage_case(nx, na=40, nb=70, sym="All", seed=0, x_max=0.4, lmax=5):
+    """(oracle spec, calculator kwargs) of the benchmark cage workload: Alavi
+    H2 (mass 2.016 amu, B = 59.322 cm^-1) in the synthetic 5^12 SPC/E cage of
+    `seed`, LJ from the stated sigma/epsilon mixing, ceiling 18 kJ/mol."""
+    import bsl_amd.cage as cg
+    from bsl_amd import physics
+    return cage_case(nx, lmax, na, nb, sym, x_max=x_max, sites=synthetic_sites(seed), lj=cg.baseline_lj(),
+                     mass=2.016, rot=physics.cm1_to_kjmol(59.322))
+
+
+def reduced_start(o, seed, carried_l_parity):
+    """The oracle's start vector of a reduced-basis run: the library's random
+    start (same generator and seed) on the carried planes only (the l parity
+    of the symmetry block), zero on the others, normalised."""
+    import _oracle as O
+    v = O.start_vector(o.N, seed)
+    ls = o.qnum()[:, 0]
+    P = o.P
+    for n, l in enumerate(ls):
+        if l % 2 != carried_l_parity:
+            v[n * P:(n + 1) * P] = 0.0
+    return v / np.linalg.norm(v)

@@ -166,6 +166,13 @@ def test_driver_mixed_precision_long_run(tmp_path, calc):
     for e in ref:
         near = np.abs(got - e) / physics.KJMOL_PER_CM1 < 0.01
         assert near.sum() == 1, (e, got[near])
+    # the two levels whose Ritz vectors the deck asks for are refined on the
+    # device by their FP64 Rayleigh quotient: the saved FP64 run's values
+    rows = np.loadtxt(os.path.join(d, "refined_levels.txt"), skiprows=4)
+    assert rows.shape == (2, 4)
+    for k in range(2):
+        assert abs(rows[k, 2] - ref[k]) <= 1e-9 * abs(ref[k]), (k, rows[k], ref[k])
+        assert rows[k, 3] < 1e-4
 
 
 def hv_deck_grid(tmp, run):

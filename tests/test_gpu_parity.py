@@ -230,17 +230,43 @@ def test_ritz_vectors_match_oracle():
         assert _rel(got[g], ref[g]) < 1e-7
 
 
-def test_rayleigh_refinement():
+def _oracle_rayleigh(o, y):
+    hy = o.hv(y)
+    e = float(y @ hy) / float(y @ y)
+    return e, float(np.linalg.norm(hy - e * y) / np.linalg.norm(y))
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_rayleigh_refinement_matches_oracle(precision):
+    """FP64 Rayleigh quotients and residual norms of Ritz vectors, on the
+    device with the context's FP64 operator whatever its precision, against
+    the oracle's own <y|H|y>/<y|y> and |Hy - e y|/|y| of the same vectors;
+    bsl_ritz_refine (Loop 2 and the refinement on the device) equals
+    bsl_ritz followed by bsl_rayleigh, also when its kept workspace is
+    reused for fewer vectors."""
     spec, kw = cs.cage_case(5, 2, 6, 6, "All")
-    c = B.Calculator(**kw)
-    niter = 80
-    a, b, b2 = c.lanczos(niter)
-    ev, err = B.lancbis(a, b, b2, -10.0, 10.0)
-    evtr, _ = B.trivec(a, b, b2, ev[:2])
-    y = c.ritz(niter, evtr, 2)
+    o = O.Oracle(spec)
+    c = B.Calculator(precision=precision, **kw)
+    niter = 120
+    a, b, b2 = c.lanczos(niter, start="random", seed=4)
+    ev, err = B.lancbis(a, b, b2, -10.0, 10.0, mult_tol=c.mult_tol)
+    evtr, _ = B.trivec(a, b, b2, ev[:3], mult_tol=c.mult_tol)
+    y = c.ritz(niter, evtr, 3, start="random", seed=4)
     e, r = c.rayleigh(y)
-    assert abs(e[0] - ev[0]) / abs(ev[0]) < 1e-9
-    assert r[0] < 1e-5
+    for k in range(3):
+        eo, ro = _oracle_rayleigh(o, y[k])
+        assert abs(e[k] - eo) <= 1e-12 * abs(eo), k
+        assert abs(r[k] - ro) <= 1e-11 * abs(eo), k
+    # a random vector (residual far from zero)
+    z = np.random.default_rng(9).uniform(-1, 1, (1, o.N))
+    ez, rz = c.rayleigh(z)
+    eo, ro = _oracle_rayleigh(o, z[0])
+    assert abs(ez[0] - eo) <= 1e-12 * abs(eo) and abs(rz[0] - ro) <= 1e-12 * ro
+    assert abs(e[0] - ev[0]) / abs(ev[0]) < (1e-9 if precision == "fp64" else 1e-6)
+    yv, e2, r2 = c.ritz_refine(niter, evtr, 3, start="random", seed=4)
+    assert np.array_equal(yv, y) and np.allclose(e2, e, rtol=1e-14, atol=0) and np.allclose(r2, r, rtol=1e-9)
+    _, e1, _ = c.ritz_refine(niter, evtr[:niter], 1, start="random", seed=4, vectors=False)
+    assert abs(e1[0] - e[0]) <= 1e-14 * abs(e[0])
 
 
 def test_fp32_levels_within_target():
@@ -461,3 +487,34 @@ def test_coulomb_potential_and_hv():
     assert _rel(g1, r1) < 1e-14
     v = np.random.default_rng(42).uniform(-1, 1, o.N)
     assert _rel(c.hv(v), o.hv(v)) < 1e-12
+
+
+def test_breakdown_is_reported():
+    """A start vector that is an eigenvector of the operator (from a dense
+    diagonalisation of the oracle's, 4^3 x l <= 1: 256 states): the first
+    residual vanishes, beta^2 = |r|^2 - alpha^2 keeps no correct digit, and
+    the run reports that step instead of returning coefficients built on it
+    (both Loop-1 entry points); a random start on the same context runs."""
+    from bsl_amd._lib import START, BslError, lib
+    spec, kw = cs.cage_case(4, 1, 4, 4, "All")
+    o = O.Oracle(spec)
+    n = o.N
+    h = np.zeros((n, n))
+    e = np.zeros(n)
+    for j in range(n):
+        e[j] = 1.0
+        h[:, j] = o.hv(e)
+        e[j] = 0.0
+    w, vecs = np.linalg.eigh(0.5 * (h + h.T))
+    y = np.ascontiguousarray(vecs[:, 0])
+    with B.Calculator(**kw) as c:
+        assert c.N == 256
+        a, b, _ = c.lanczos(20, start="random", seed=1)
+        assert np.all(np.isfinite(a)) and np.all(b[1:] > 0)
+        with pytest.raises(BslError) as err:
+            c.lanczos(5, v0=y)
+        assert err.value.code == 7 and "breakdown at step 1" in str(err.value)
+        aa, bb, bb2 = np.zeros(5), np.zeros(6), np.zeros(6)
+        rc = lib().bsl_lanczos_noalloc(c.h, 5, y.ctypes.data, START["random"], 0, aa, bb, bb2)
+        assert rc == 7 and b"breakdown at step 1" in lib().bsl_last_error()
+        assert abs(aa[0] - w[0]) < 1e-12 * abs(w[0])

@@ -206,6 +206,43 @@ int launch_ritz_accum(double *ritz, const double *X, const double *R, const Scal
 int launch_rayleigh_dots(const double *y, const double *hy, int64_t n, double e, int mode, double *partial,
                          int npartial, int *groups, hipStream_t s);
 
+// ---------------------------------------------------------------------------
+// batches: one launch per kernel variant for all members that use it
+// ---------------------------------------------------------------------------
+// One member's operator for one launch of a table-driven batch kernel: its
+// vectors for the current step parity, tables, scalars, potential and
+// coefficient outputs, and its range [first, first + count) of the launch's
+// virtual workgroups (the member's own grid, linearised).
+struct BatchEntry {
+    double *X, *R;
+    const double *tx, *ty, *tz, *rot, *mask;
+    Scal *sc;
+    const void *H;
+    const double *G0, *G1;
+    double *pa, *pb, *pc;
+    double *alpha, *beta, *beta2;
+    int64_t P;
+    int nx, ny, nz, nk;
+    int nc, plane1;
+    double mask0, mask1;
+    int first, count;
+    int nalpha, pad;
+};
+// the kernel variants a batch step is made of
+enum BatchKind {
+    kBXyHalf = 0, kBZHalf, kBPotSplitPara, kBPotSplitOrtho, kBFinalize, kBKinds
+};
+constexpr int kBatchMaxTable = 64;  // members per launch
+// Device layout of one launch's table: kBatchMaxTable entries (the first nm
+// used) followed by kBatchMaxTable ints, the entries' `first` values
+// ascending, INT_MAX in the unused slots.
+constexpr size_t kBatchTableBytes = sizeof(BatchEntry) * kBatchMaxTable + sizeof(int) * kBatchMaxTable;
+// launch `kind` over the nm <= kBatchMaxTable entries of d_tab (device);
+// prec: BSL_PREC_FP64 or FP32 (kinetic T, or potential VT); total = the
+// entries' workgroups (for kBFinalize one per entry); G: the Gaunt values
+// of the potential kinds' block (the same for every member of the launch)
+int launch_batch(int kind, int prec, const BatchEntry *d_tab, int nm, int total, const double *G, hipStream_t s);
+
 // prep kernels
 struct CageDev {
     int nsites = 0;

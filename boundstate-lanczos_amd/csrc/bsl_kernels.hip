@@ -166,185 +166,14 @@ __device__ inline void xcd_tile(int &tile, int &plane)
     plane = lin / nt;
 }
 
-// x and y axes of one (k, n) slab: D[j][i] = sum_ip v[j][ip] t_x(|ip - i|)
-// + sum_jp t_y(|j - jp|) v[jp][i], both into the same accumulator tiles;
-// each wave owns a 2 x 2 block of 16 x 16 output tiles.  T: FP64, or FP32
-// (products and accumulation in FP32, the recurrence vectors stay FP64).
-// v = ca X + cb R is formed while the slab is staged and written back to X;
-// R = cr R + mask[n] * (D + rot[n] v)
-template <typename T>
-__global__ __launch_bounds__(kWG) void k_kin_xy_mfma(double *__restrict__ X, double *__restrict__ R,
-                                                     const double *__restrict__ tx, const double *__restrict__ ty,
-                                                     const double *__restrict__ rot, const double *__restrict__ mask,
-                                                     const Scal *__restrict__ sc, int nx, int ny, int nz)
-{
-    typedef typename KinAcc<T>::type acc_t;
-    extern __shared__ __align__(16) unsigned char kin_lds[];
-    T *pl = (T *)kin_lds;              // [64][kSlabLd]
-    T *rx = pl + 64 * kSlabLd;         // [129]
-    T *ry = rx + 129;                  // [129]
-    const int k = blockIdx.x, n = blockIdx.y;
-    const double ca = sc->ca, cb = sc->cb, cr = sc->cr;
-    const size_t base = ((size_t)n * nz + k) * (size_t)ny * nx;
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    const int l16 = lane & 15, kq = lane >> 4;
-    const int tj = (ny + 15) >> 4, ti = (nx + 15) >> 4;
-    const int bj = (tj + 1) >> 1, bi = (ti + 1) >> 1;  // at most 2 x 2 blocks: one per wave
-    const int j0 = 32 * (wid / bi), i0 = 32 * (wid % bi);
-    const bool active = wid < bj * bi;
-    // this lane's R outputs are loaded together with the slab, so that both
-    // latencies overlap
-    double rp[2][2][4];
-#pragma unroll
-    for (int a = 0; a < 2; a++)
-#pragma unroll
-        for (int b = 0; b < 2; b++)
-#pragma unroll
-            for (int q = 0; q < 4; q++) {
-                const int j = j0 + 16 * a + kin_row<T>(kq, q), i = i0 + 16 * b + l16;
-                rp[a][b][q] = (active && j < ny && i < nx) ? R[base + (size_t)j * nx + i] : 0.0;
-            }
-    {
-        // v = ca X + cb R over the slab, written back to X (the vector the
-        // other kernels of this Hv read)
-        double *Xp = X + base;
-        const double *Rp = R + base;
-        double tmp[16];
-#pragma unroll
-        for (int e = 0; e < 16; e++) {
-            const int t = threadIdx.x + e * kWG, j = t >> 6, i = t & 63;
-            tmp[e] = (j < ny && i < nx) ? ca * Xp[j * nx + i] + cb * Rp[j * nx + i] : 0.0;
-        }
-#pragma unroll
-        for (int e = 0; e < 16; e++) {
-            const int t = threadIdx.x + e * kWG, j = t >> 6, i = t & 63;
-            pl[j * kSlabLd + i] = (T)tmp[e];
-            if (j < ny && i < nx) Xp[j * nx + i] = tmp[e];
-        }
-    }
-    for (int t = threadIdx.x; t < 129; t += blockDim.x) {
-        const int d = abs(t - 64);
-        rx[t] = d < nx ? (T)tx[d] : (T)0;
-        ry[t] = d < ny ? (T)ty[d] : (T)0;
-    }
-    __syncthreads();
-    const int kx = (nx + 3) >> 2, ky = (ny + 3) >> 2;
-    const double rn = rot[n], mn = mask[n];
-    if (active) {
-        acc_t acc[2][2];
-#pragma unroll
-        for (int a = 0; a < 2; a++)
-#pragma unroll
-            for (int b = 0; b < 2; b++) acc[a][b] = acc_t{0, 0, 0, 0};
-        for (int s = 0; s < kx; s++) {
-            const int ip = 4 * s + kq;
-            const T a0 = pl[(j0 + l16) * kSlabLd + ip], a1 = pl[(j0 + 16 + l16) * kSlabLd + ip];
-            const T b0 = rx[64 + ip - (i0 + l16)], b1 = rx[64 + ip - (i0 + 16 + l16)];
-            acc[0][0] = kmfma(a0, b0, acc[0][0]);
-            acc[0][1] = kmfma(a0, b1, acc[0][1]);
-            acc[1][0] = kmfma(a1, b0, acc[1][0]);
-            acc[1][1] = kmfma(a1, b1, acc[1][1]);
-        }
-        for (int s = 0; s < ky; s++) {
-            const int jp = 4 * s + kq;
-            const T a0 = ry[64 + (j0 + l16) - jp], a1 = ry[64 + (j0 + 16 + l16) - jp];
-            const T b0 = pl[jp * kSlabLd + i0 + l16], b1 = pl[jp * kSlabLd + i0 + 16 + l16];
-            acc[0][0] = kmfma(a0, b0, acc[0][0]);
-            acc[0][1] = kmfma(a0, b1, acc[0][1]);
-            acc[1][0] = kmfma(a1, b0, acc[1][0]);
-            acc[1][1] = kmfma(a1, b1, acc[1][1]);
-        }
-#pragma unroll
-        for (int a = 0; a < 2; a++)
-#pragma unroll
-            for (int b = 0; b < 2; b++) {
-                const int i = i0 + 16 * b + l16;
-#pragma unroll
-                for (int q = 0; q < 4; q++) {
-                    const int j = j0 + 16 * a + kin_row<T>(kq, q);
-                    if (j < ny && i < nx)
-                        R[base + (size_t)j * nx + i] =
-                            cr * rp[a][b][q] + mn * ((double)acc[a][b][q] + rn * (double)pl[j * kSlabLd + i]);
-                }
-            }
-    }
-}
-
-// z axis of 64 columns (j, i) of plane n: D[k][c] = sum_kp t_z(|k - kp|) v[kp][c];
-// wave w owns the 16 columns 16w.. and every 16-row tile of k.
-// R += mask[n] * D
-template <typename T>
-__global__ __launch_bounds__(kWG) void k_kin_z_mfma(const double *__restrict__ X, double *__restrict__ R,
-                                                    const double *__restrict__ tz, const double *__restrict__ mask,
-                                                    const Scal *__restrict__ sc, int nxy, int nz)
-{
-    typedef typename KinAcc<T>::type acc_t;
-    extern __shared__ __align__(16) unsigned char kin_lds[];
-    T *col = (T *)kin_lds;             // [64][kSlabLd]
-    T *rz = col + 64 * kSlabLd;        // [129]
-    int tile, n;
-    xcd_tile(tile, n);
-    const int c0 = tile * 64;
-    (void)sc;  // X already holds v
-    const size_t base = (size_t)n * nz * nxy;
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    const int l16 = lane & 15, kq = lane >> 4;
-    const int tk = (nz + 15) >> 4, kz = (nz + 3) >> 2;
-    const int cw = 16 * wid + l16;
-    const int c = c0 + cw;
-    // this lane's R outputs, loaded together with the column block
-    double rp[4][4];
-#pragma unroll
-    for (int a = 0; a < 4; a++)
-#pragma unroll
-        for (int q = 0; q < 4; q++) {
-            const int kk = 16 * a + kin_row<T>(kq, q);
-            rp[a][q] = (a < tk && kk < nz && c < nxy) ? R[base + (size_t)kk * nxy + c] : 0.0;
-        }
-    {
-        const double *Xc = X + base + c0;
-        double tmp[16];
-#pragma unroll
-        for (int e = 0; e < 16; e++) {
-            const int t = threadIdx.x + e * kWG, kk = t >> 6, cc = t & 63;
-            tmp[e] = (kk < nz && c0 + cc < nxy) ? Xc[(size_t)kk * nxy + cc] : 0.0;
-        }
-#pragma unroll
-        for (int e = 0; e < 16; e++) {
-            const int t = threadIdx.x + e * kWG, kk = t >> 6, cc = t & 63;
-            col[kk * kSlabLd + cc] = (T)tmp[e];
-        }
-    }
-    for (int t = threadIdx.x; t < 129; t += blockDim.x) {
-        const int d = abs(t - 64);
-        rz[t] = d < nz ? (T)tz[d] : (T)0;
-    }
-    __syncthreads();
-    acc_t acc[4];
-#pragma unroll
-    for (int a = 0; a < 4; a++) acc[a] = acc_t{0, 0, 0, 0};
-    for (int s = 0; s < kz; s++) {
-        const int kp = 4 * s + kq;
-        const T b = col[kp * kSlabLd + cw];
-#pragma unroll
-        for (int a = 0; a < 4; a++)
-            if (a < tk) acc[a] = kmfma(rz[64 + 16 * a + l16 - kp], b, acc[a]);
-    }
-    const double mn = mask[n];
-    if (c >= nxy) return;
-#pragma unroll
-    for (int a = 0; a < 4; a++)
-#pragma unroll
-        for (int q = 0; q < 4; q++) {
-            const int kk = 16 * a + kin_row<T>(kq, q);
-            if (a < tk && kk < nz) R[base + (size_t)kk * nxy + c] = rp[a][q] + mn * (double)acc[a][q];
-        }
-}
-
-// Even axis lengths: T is symmetric Toeplitz and centrosymmetric, so with
-// mirror index n-1-i the sums and differences v(i) +- v(n-1-i), i < n/2, are
-// mapped separately by the n/2 x n/2 matrices t(|i - i2|) +- t(n-1-i-i2):
-// half the multiply-adds of the full contraction.
+// T is symmetric Toeplitz and centrosymmetric, so with mirror index
+// i' = n-1-i the sums and differences v(i) +- v(i'), i < m = ceil(n/2), are
+// mapped separately by the m x m matrices t(|i - i2|) +- t(n-1-i-i2): half
+// the multiply-adds of the full contraction.  For odd n the middle point
+// h = (n-1)/2 is its own mirror: its sum 2 v(h) enters the sum block with
+// half weight (the operand t(|i - h|) instead of 2 t(|i - h|)), its
+// difference is zero, and the outputs written to i and i' at the middle
+// are the same value.
 constexpr int kCompLd = 33;
 
 // x and y axes on the four x/y parity components of the slab,
@@ -353,21 +182,20 @@ constexpr int kCompLd = 33;
 // holds the four transformed components of its elements and recombines them
 // into the four mirror points.
 template <typename T>
-__global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_kin_xy_half(double *__restrict__ X,
-                                                                                         double *__restrict__ R,
-                                                     const double *__restrict__ tx, const double *__restrict__ ty,
-                                                     const double *__restrict__ rot, const double *__restrict__ mask,
-                                                     const Scal *__restrict__ sc, int nx, int ny, int nz)
+__device__ __forceinline__ void kin_xy_half_body(int k, int n, double *__restrict__ X, double *__restrict__ R,
+                                                 const double *__restrict__ tx, const double *__restrict__ ty,
+                                                 const double *__restrict__ rot, const double *__restrict__ mask,
+                                                 const Scal *__restrict__ sc, int nx, int ny, int nz)
 {
     typedef typename KinAcc<T>::type acc_t;
     extern __shared__ __align__(16) unsigned char kin_lds[];
     T *comp = (T *)kin_lds;                // [4][32][kCompLd]
     T *rx = comp + 4 * 32 * kCompLd;       // [129]
     T *ry = rx + 129;                      // [129]
-    const int k = blockIdx.x, n = blockIdx.y;
     const double ca = sc->ca, cb = sc->cb, cr = sc->cr;
     const size_t base = ((size_t)n * nz + k) * (size_t)ny * nx;
-    const int mx = nx >> 1, my = ny >> 1;
+    const int mx = (nx + 1) >> 1, my = (ny + 1) >> 1;
+    const int hx = (nx & 1) ? nx >> 1 : -1, hy = (ny & 1) ? ny >> 1 : -1;  // middle points of odd axes
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     const int l16 = lane & 15, kq = lane >> 4;
     const int ta = wid >> 1, tb = wid & 1;
@@ -423,7 +251,7 @@ __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
     for (int s = 0; s < kx; s++) {
         const int i2 = 4 * s + kq;
         const T t1 = rx[64 + i2 - i], t2 = rx[64 + (nx - 1 - i2 - i)];
-        const T bp = t1 + t2, bm = t1 - t2;
+        const T bp = i2 == hx ? t1 : t1 + t2, bm = t1 - t2;
 #pragma unroll
         for (int c = 0; c < 4; c++)
             acc[c] = kmfma(comp[(c * 32 + jr) * kCompLd + i2], (c & 1) ? bm : bp, acc[c]);
@@ -431,7 +259,7 @@ __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
     for (int s = 0; s < ky; s++) {
         const int j2 = 4 * s + kq;
         const T t1 = ry[64 + jr - j2], t2 = ry[64 + (ny - 1 - jr - j2)];
-        const T ap = t1 + t2, am = t1 - t2;
+        const T ap = j2 == hy ? t1 : t1 + t2, am = t1 - t2;
 #pragma unroll
         for (int c = 0; c < 4; c++)
             acc[c] = kmfma((c & 2) ? am : ap, comp[(c * 32 + j2) * kCompLd + i], acc[c]);
@@ -456,25 +284,31 @@ __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
     }
 }
 
+template <typename T>
+__global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_kin_xy_half(
+    double *__restrict__ X, double *__restrict__ R, const double *__restrict__ tx, const double *__restrict__ ty,
+    const double *__restrict__ rot, const double *__restrict__ mask, const Scal *__restrict__ sc, int nx, int ny,
+    int nz)
+{
+    kin_xy_half_body<T>(blockIdx.x, blockIdx.y, X, R, tx, ty, rot, mask, sc, nx, ny, nz);
+}
+
 // z axis of 64 columns with the z parity split: s = v(k) + v(k'), d = v(k) - v(k'),
 // k < nz/2; wave w owns columns 16w.. of both components, every k tile.
 template <typename T>
-__global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_kin_z_half(
-    const double *__restrict__ X, double *__restrict__ R,
-                                                    const double *__restrict__ tz, const double *__restrict__ mask,
-                                                    const Scal *__restrict__ sc, int nxy, int nz)
+__device__ __forceinline__ void kin_z_half_body(int tile, int n, const double *__restrict__ X, double *__restrict__ R,
+                                                const double *__restrict__ tz, const double *__restrict__ mask,
+                                                int nxy, int nz)
 {
     typedef typename KinAcc<T>::type acc_t;
     extern __shared__ __align__(16) unsigned char kin_lds[];
     T *cs = (T *)kin_lds;                  // [32][kSlabLd]: sums
     T *cd = cs + 32 * kSlabLd;             // [32][kSlabLd]: differences
     T *rz = cd + 32 * kSlabLd;             // [129]
-    int tile, n;
-    xcd_tile(tile, n);
     const int c0 = tile * 64;
-    (void)sc;  // X already holds v
     const size_t base = (size_t)n * nz * nxy;
-    const int mz = nz >> 1;
+    const int mz = (nz + 1) >> 1;
+    const int hz = (nz & 1) ? nz >> 1 : -1;  // middle point of an odd axis
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     const int l16 = lane & 15, kq = lane >> 4;
     const int cw = 16 * wid + l16, c = c0 + cw;
@@ -523,7 +357,7 @@ __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
         for (int a = 0; a < 2; a++) {
             const int kr = 16 * a + l16;
             const T t1 = rz[64 + kr - kp], t2 = rz[64 + (nz - 1 - kr - kp)];
-            as[a] = kmfma(t1 + t2, bs, as[a]);
+            as[a] = kmfma(kp == hz ? t1 : t1 + t2, bs, as[a]);
             ad[a] = kmfma(t1 - t2, bd, ad[a]);
         }
     }
@@ -542,29 +376,28 @@ __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
 }
 
 template <typename T>
+__global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_kin_z_half(
+    const double *__restrict__ X, double *__restrict__ R, const double *__restrict__ tz, const double *__restrict__ mask,
+    const Scal *__restrict__ sc, int nxy, int nz)
+{
+    (void)sc;  // X already holds v
+    int tile, n;
+    xcd_tile(tile, n);
+    kin_z_half_body<T>(tile, n, X, R, tz, mask, nxy, nz);
+}
+
+template <typename T>
 static int launch_kinetic_t(const DevOperator &op, double *X, double *R, Scal *sc, hipStream_t s)
 {
-    // even axes use the half-size (centrosymmetric) contractions
-    if (op.nx % 2 == 0 && op.ny % 2 == 0) {
-        const size_t lds = sizeof(T) * ((size_t)4 * 32 * kCompLd + 2 * 129);
-        hipLaunchKernelGGL(k_kin_xy_half<T>, dim3(op.nz, op.nk), dim3(kWG), lds, s, X, R, op.tx, op.ty, op.rot,
-                           op.mask, sc, op.nx, op.ny, op.nz);
-    } else {
-        const size_t lds = sizeof(T) * ((size_t)64 * kSlabLd + 2 * 129);
-        hipLaunchKernelGGL(k_kin_xy_mfma<T>, dim3(op.nz, op.nk), dim3(kWG), lds, s, X, R, op.tx, op.ty, op.rot,
-                           op.mask, sc, op.nx, op.ny, op.nz);
-    }
+    // the half-size (centrosymmetric) contractions, for every axis length
+    const size_t lds_xy = sizeof(T) * ((size_t)4 * 32 * kCompLd + 2 * 129);
+    hipLaunchKernelGGL(k_kin_xy_half<T>, dim3(op.nz, op.nk), dim3(kWG), lds_xy, s, X, R, op.tx, op.ty, op.rot,
+                       op.mask, sc, op.nx, op.ny, op.nz);
     BSL_HIP(hipGetLastError());
     const int nxy = op.nx * op.ny;
-    if (op.nz % 2 == 0) {
-        const size_t lds = sizeof(T) * ((size_t)2 * 32 * kSlabLd + 129);
-        hipLaunchKernelGGL(k_kin_z_half<T>, dim3((nxy + 63) / 64, op.nk), dim3(kWG), lds, s, X, R, op.tz, op.mask,
-                           sc, nxy, op.nz);
-    } else {
-        const size_t lds = sizeof(T) * ((size_t)64 * kSlabLd + 129);
-        hipLaunchKernelGGL(k_kin_z_mfma<T>, dim3((nxy + 63) / 64, op.nk), dim3(kWG), lds, s, X, R, op.tz,
-                           op.mask, sc, nxy, op.nz);
-    }
+    const size_t lds_z = sizeof(T) * ((size_t)2 * 32 * kSlabLd + 129);
+    hipLaunchKernelGGL(k_kin_z_half<T>, dim3((nxy + 63) / 64, op.nk), dim3(kWG), lds_z, s, X, R, op.tz, op.mask, sc,
+                       nxy, op.nz);
     BSL_HIP(hipGetLastError());
     return BSL_OK;
 }
@@ -856,11 +689,12 @@ __device__ __forceinline__ void gaunt_sublist(const double *__restrict__ X, cons
 // Each row of a block gets one same-type partial (cos-cos for m >= 0 rows,
 // sin-sin for m < 0 rows) and one cross partial, exchanged through LDS; after
 // the barrier each wave finishes 9 of the 36 rows as R += mask (same + cross).
-template <typename VT, int W>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, 8))) void k_vgaunt_split(
-    const double *__restrict__ X, double *__restrict__ R, const VT *__restrict__ H, const double *__restrict__ G0,
-    const double *__restrict__ G1, int nc, int plane1, double mask0, double mask1, int64_t P,
-    double *__restrict__ partial, double *__restrict__ partial_b, double *__restrict__ partial_c)
+template <typename VT>
+__device__ __forceinline__ void vgaunt_split_body(int group, const double *__restrict__ X, double *__restrict__ R,
+                                                  const VT *__restrict__ H, const double *__restrict__ G0,
+                                                  const double *__restrict__ G1, int nc, int plane1, double mask0,
+                                                  double mask1, int64_t P, double *__restrict__ partial,
+                                                  double *__restrict__ partial_b, double *__restrict__ partial_c)
 {
     constexpr int B0 = gaunt::block_size(4, 0), B1 = gaunt::block_size(5, 1), NR = B0 + B1;
     static_assert(NR % 4 == 0, "rows are shared out evenly over the four waves");
@@ -870,7 +704,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, 8))) voi
     __shared__ double red[4];
     const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
     const int lane = (int)threadIdx.x & 63;
-    const int p = (int)blockIdx.x * 64 + lane;
+    const int p = group * 64 + lane;
     const bool live = p < P;
     const int pl = live ? p : (int)(P - 1);
     const VT *hp = H + (size_t)(pl / kVTile) * nc * kVTile + pl % kVTile;
@@ -919,21 +753,30 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, 8))) voi
     __syncthreads();
     const double s2 = block_sum(vv2, red);
     if (threadIdx.x == 0) {
-        partial[blockIdx.x] = s0;
-        partial_b[blockIdx.x] = s1;
-        partial_c[blockIdx.x] = s2;
+        partial[group] = s0;
+        partial_b[group] = s1;
+        partial_c[group] = s2;
     }
+}
+
+template <typename VT, int W>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, 8))) void k_vgaunt_split(
+    const double *__restrict__ X, double *__restrict__ R, const VT *__restrict__ H, const double *__restrict__ G0,
+    const double *__restrict__ G1, int nc, int plane1, double mask0, double mask1, int64_t P,
+    double *__restrict__ partial, double *__restrict__ partial_b, double *__restrict__ partial_c)
+{
+    vgaunt_split_body<VT>(blockIdx.x, X, R, H, G0, G1, nc, plane1, mask0, mask1, P, partial, partial_b, partial_c);
 }
 
 // One parity block of the reduced basis (para l = 0, 2, 4 or ortho
 // l = 1, 3, 5) split the same way over three waves per 64 points: wave 0 the
 // cross pairs, wave 1 cos-cos, wave 2 sin-sin; then each wave finishes
 // ceil(B/3) rows as R += mask (same + cross).
-template <int LB, int PAR, typename VT, int W>
-__global__ __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(W, 8))) void k_vgaunt_split1(
-    const double *__restrict__ X, double *__restrict__ R, const VT *__restrict__ H, const double *__restrict__ G0,
-    int nc, double mask0, int64_t P, double *__restrict__ partial, double *__restrict__ partial_b,
-    double *__restrict__ partial_c)
+template <int LB, int PAR, typename VT>
+__device__ __forceinline__ void vgaunt_split1_body(int group, const double *__restrict__ X, double *__restrict__ R,
+                                                   const VT *__restrict__ H, const double *__restrict__ G0, int nc,
+                                                   double mask0, int64_t P, double *__restrict__ partial,
+                                                   double *__restrict__ partial_b, double *__restrict__ partial_c)
 {
     constexpr int B = gaunt::block_size(LB, PAR), NW = (B + 2) / 3;
     static_assert(gaunt::split_covers_rows<LB, PAR>(), "every row needs a same-type and a cross partial");
@@ -941,7 +784,7 @@ __global__ __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(W, 8))) voi
     __shared__ double red[3];
     const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
     const int lane = (int)threadIdx.x & 63;
-    const int p = (int)blockIdx.x * 64 + lane;
+    const int p = group * 64 + lane;
     const bool live = p < P;
     const int pl = live ? p : (int)(P - 1);
     const VT *hp = H + (size_t)(pl / kVTile) * nc * kVTile + pl % kVTile;
@@ -980,10 +823,19 @@ __global__ __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(W, 8))) voi
     __syncthreads();
     const double s2 = block_sum(vv2, red);
     if (threadIdx.x == 0) {
-        partial[blockIdx.x] = s0;
-        partial_b[blockIdx.x] = s1;
-        partial_c[blockIdx.x] = s2;
+        partial[group] = s0;
+        partial_b[group] = s1;
+        partial_c[group] = s2;
     }
+}
+
+template <int LB, int PAR, typename VT, int W>
+__global__ __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(W, 8))) void k_vgaunt_split1(
+    const double *__restrict__ X, double *__restrict__ R, const VT *__restrict__ H, const double *__restrict__ G0,
+    int nc, double mask0, int64_t P, double *__restrict__ partial, double *__restrict__ partial_b,
+    double *__restrict__ partial_c)
+{
+    vgaunt_split1_body<LB, PAR, VT>(blockIdx.x, X, R, H, G0, nc, mask0, P, partial, partial_b, partial_c);
 }
 
 // ---------------------------------------------------------------------------
@@ -1207,6 +1059,110 @@ static int launch_vgaunt_t(const DevOperator &op, const double *X, double *R, hi
 #undef BSL_VG
     return fail(BSL_E_ARG, "no harmonic-form potential kernel for blocks of l <= " + std::to_string(l0) + "/" +
                                std::to_string(l1));
+}
+
+// ---------------------------------------------------------------------------
+// batch kernels: the bodies above over a table of members, one launch per
+// variant.  Workgroup vb belongs to the member whose [first, first + count)
+// holds it (at most 64 members per table).  The member's workgroups run exactly the
+// single-context kernel's code with the same indices, so the results are
+// bitwise those of the member run alone.
+// ---------------------------------------------------------------------------
+// The table is followed by the launch's first[] list (kBatchMaxTable ints,
+// ascending, unused slots INT_MAX): uniform scalar loads, then a count.
+__device__ inline int batch_find(const BatchEntry *__restrict__ tab, int nm, int vb)
+{
+    (void)nm;
+    const int *__restrict__ first = reinterpret_cast<const int *>(tab + kBatchMaxTable);
+    int m = 0;
+#pragma unroll
+    for (int i = 1; i < kBatchMaxTable; i++) m += first[i] <= vb ? 1 : 0;
+    return m;
+}
+
+// the XCD-aware bijection of xcd_tile over a one-dimensional grid
+__device__ inline int batch_xcd(int id, int total)
+{
+    return (total % kXcds == 0) ? (id % kXcds) * (total / kXcds) + id / kXcds : id;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(4, 8))) void kb_xy_half(
+    const BatchEntry *__restrict__ tab, int nm)
+{
+    const int vb = (int)blockIdx.x, m = batch_find(tab, nm, vb);
+    const BatchEntry &e = tab[m];
+    const int loc = vb - e.first;
+    kin_xy_half_body<T>(loc % e.nz, loc / e.nz, e.X, e.R, e.tx, e.ty, e.rot, e.mask, e.sc, e.nx, e.ny, e.nz);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(4, 8))) void kb_z_half(
+    const BatchEntry *__restrict__ tab, int nm)
+{
+    const int vb = batch_xcd((int)blockIdx.x, (int)gridDim.x), m = batch_find(tab, nm, vb);
+    const BatchEntry &e = tab[m];
+    const int loc = vb - e.first, nxy = e.nx * e.ny, nt = (nxy + 63) / 64;
+    kin_z_half_body<T>(loc % nt, loc / nt, e.X, e.R, e.tz, e.mask, nxy, e.nz);
+}
+
+// The Gaunt values of a block depend on its shape only, so every member of
+// the launch shares them; they come in as a kernel argument (read-only, so
+// that the unrolled term list takes them through scalar loads).
+// (the ortho block at 3 waves/SIMD: at 4 the table lookup's registers spill)
+template <int LB, int PAR, typename VT>
+__global__ __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(LB == 5 ? 3 : 4, 8))) void kb_split1(
+    const BatchEntry *__restrict__ tab, int nm, const double *__restrict__ G)
+{
+    const int vb = (int)blockIdx.x, m = batch_find(tab, nm, vb);
+    const BatchEntry &e = tab[m];
+    vgaunt_split1_body<LB, PAR, VT>(vb - e.first, e.X, e.R, (const VT *)e.H, G, e.nc, e.mask0, e.P, e.pa, e.pb,
+                                    e.pc);
+}
+
+__global__ __launch_bounds__(kWG) void kb_finalize(const BatchEntry *__restrict__ tab, int nm)
+{
+    __shared__ double red[kWG / 64];
+    if ((int)blockIdx.x >= nm) return;
+    const BatchEntry &e = tab[blockIdx.x];
+    finalize_step(e.pa, e.pb, e.pc, (unsigned)e.nalpha, e.sc, e.alpha, e.beta, e.beta2, red);
+}
+
+template <typename T>
+static int launch_batch_t(int kind, const BatchEntry *tab, int nm, int total, const double *G, hipStream_t s)
+{
+    const dim3 grid((unsigned)total);
+    switch (kind) {
+    case kBXyHalf:
+        hipLaunchKernelGGL(kb_xy_half<T>, grid, dim3(kWG), sizeof(T) * ((size_t)4 * 32 * kCompLd + 2 * 129), s, tab,
+                           nm);
+        break;
+    case kBZHalf:
+        hipLaunchKernelGGL(kb_z_half<T>, grid, dim3(kWG), sizeof(T) * ((size_t)2 * 32 * kSlabLd + 129), s, tab, nm);
+        break;
+    case kBPotSplitPara:
+        hipLaunchKernelGGL((kb_split1<4, 0, T>), grid, dim3(192), 0, s, tab, nm, G);
+        break;
+    case kBPotSplitOrtho:
+        hipLaunchKernelGGL((kb_split1<5, 1, T>), grid, dim3(192), 0, s, tab, nm, G);
+        break;
+    default:
+        return fail(BSL_E_ARG, "unknown batch kernel");
+    }
+    BSL_HIP(hipGetLastError());
+    return BSL_OK;
+}
+
+int launch_batch(int kind, int prec, const BatchEntry *d_tab, int nm, int total, const double *G, hipStream_t s)
+{
+    if (nm <= 0 || nm > kBatchMaxTable || total <= 0) return fail(BSL_E_ARG, "batch table size out of range");
+    if (kind == kBFinalize) {
+        hipLaunchKernelGGL(kb_finalize, dim3((unsigned)nm), dim3(kWG), 0, s, d_tab, nm);
+        BSL_HIP(hipGetLastError());
+        return BSL_OK;
+    }
+    return prec == BSL_PREC_FP64 ? launch_batch_t<double>(kind, d_tab, nm, total, G, s)
+                                 : launch_batch_t<float>(kind, d_tab, nm, total, G, s);
 }
 
 int launch_potential(const DevOperator &op, const double *X, double *R, Scal *sc, hipStream_t s)

@@ -1,6 +1,10 @@
 """Batched lockstep recurrences (bsl_batch_*): mixed cages, symmetry blocks,
 grid sizes and precisions in one batch give exactly the coefficients of
-separate single-context runs, eagerly and through replayed hipGraphs."""
+separate single-context runs, eagerly and through replayed hipGraphs.  The
+batch launches each kernel variant once per step over a table of its
+members; the cases cover every variant (even and odd axes, FP64 and FP32,
+the full and both reduced lmax-5 potential shapes) and members whose
+potential is launched on its own (other shapes, the quadrature calculator)."""
 import ctypes as C
 
 import numpy as np
@@ -72,3 +76,32 @@ def test_batch_member_levels():
     with B.Calculator(**members[1]) as c:
         e1, _, _ = c.levels(600, 2)
     assert np.allclose(e, e1, rtol=1e-12, atol=0)
+
+
+@pytest.mark.gpu
+def test_batch_table_variants_match_single_runs():
+    """lmax-5 members of every table potential shape (full basis pair of
+    blocks, para and ortho blocks) in FP64 and FP32, on even, odd and
+    anisotropic grids, 60 steps (an even and an odd count) eagerly and
+    through graphs: bitwise the coefficients of the members run alone."""
+    import bsl_amd as B
+    members = []
+    for i, (n, sym, basis, prec) in enumerate([(6, "All", "full", "fp64"), (5, "All", "full", "fp32"),
+                                               (6, "Even", "reduced", "fp64"), (7, "Odd", "reduced", "fp64"),
+                                               (5, "Even", "reduced", "fp32"), (6, "Odd", "reduced", "fp32")]):
+        _, kw = cs.cage_case(n, 5, 8, 9, sym)
+        members.append(dict(kw, basis=basis, precision=prec))
+    _, kw = cs.cage_case(6, 5, 8, 9, "Odd")
+    members.append(dict(kw, nz=5, basis="reduced"))
+    for niter in (60, 61):
+        ref = []
+        for i, kw in enumerate(members):
+            with B.Calculator(**kw) as c:
+                ref.append(c.lanczos(niter, start="random", seed=40 + i))
+        with B.Batch(members) as bt:
+            for graph in (False, True):
+                got = bt.lanczos(niter, start="random", seed=40, graph=graph)
+                for i, ((a, b, b2), (ra, rb, rb2)) in enumerate(zip(got, ref)):
+                    assert np.array_equal(a, ra) and np.array_equal(b, rb) and np.array_equal(b2, rb2), (niter, graph, i)
+            ms = bt.time_steps(3, 6, graph=True)
+            assert ms > 0

@@ -328,12 +328,16 @@ def eig_error(B, c64, c32, steps):
             # the mixed run's lowest 20 levels refined on the device: Loop 2
             # rebuilds their Ritz vectors, then the FP64 Rayleigh quotient
             sel = np.sort(e32)[:20]
-            evtr, _ = B.trivec(*p32, sel, mult_tol=c32.mult_tol)
-            t0 = time.perf_counter()
-            _, e_ref, r_ref = c32.ritz_refine(n, evtr, len(sel), start="random", seed=0, vectors=False)
-            out["refined_%d" % n] = level_errors(e64, e_ref)
-            out["refined_%d_max_residual_kjmol" % n] = float(np.max(r_ref))
-            out["refined_%d_s" % n] = time.perf_counter() - t0
+            try:
+                evtr, _ = B.trivec(*p32, sel, mult_tol=c32.mult_tol)
+                t0 = time.perf_counter()
+                _, e_ref, r_ref = c32.ritz_refine(n, evtr, len(sel), start="random", seed=0, vectors=False)
+                out["refined_%d" % n] = level_errors(e64, e_ref)
+                out["refined_%d_max_residual_kjmol" % n] = float(np.max(r_ref))
+                out["refined_%d_s" % n] = time.perf_counter() - t0
+            except B.BslError as e:
+                out["refined_%d" % n] = None
+                out["refined_%d_error" % n] = str(e)
             t0 = time.perf_counter()
             e64d, _ = B.lancbis(*p64, -10.0, 10.0, device=True)
             t_dev = time.perf_counter() - t0
@@ -443,6 +447,24 @@ def main():
     if c32 is not None:
         c32.close()
 
+    # BF16x3 kinetic contractions (K = 32 BF16 MFMA split products), FP32
+    # potential coefficients, FP64 recurrence
+    bf = None
+    if not args.no_mixed:
+        t0 = time.perf_counter()
+        c16 = make_calc(B, args, "bf16x3")
+        prep16 = time.perf_counter() - t0
+        ms16 = c16.time_steps(args.warmup, args.steps)
+        bf = {"steps_per_s": args.steps / (ms16[0] / 1e3), "prep_s": prep16,
+              "per_step_ms": {"potential": ms16[1], "kinetic": ms16[2], "finalize": ms16[3]}}
+        bf["vs_fp64"] = bf["steps_per_s"] / steps_s
+        if not args.no_eig and steps_list:
+            err16, _, _ = eig_error(B, c64, c16, steps_list)
+            bf["max_err_cm1"] = err16
+        print("bf16x3 kinetic: %.1f steps/s (%.2fx fp64), level err %s cm^-1"
+              % (bf["steps_per_s"], bf["vs_fp64"], bf.get("max_err_cm1")), file=sys.stderr, flush=True)
+        c16.close()
+
     quad = quad_leg(B, args, e_ref) if args.quad_steps > 0 else None
     batch = batch_leg(B, args) if args.batch_steps > 0 else None
     adj = None
@@ -485,6 +507,7 @@ def main():
             "mixed_fp32_steps_per_s": steps_s32,
             "mixed_vs_fp64": steps_s32 / steps_s if steps_s32 else None,
             "eig_err_vs_fp64": eig,
+            "mixed_bf16x3": bf,
             "prep_s": {"fp64": prep64, "fp32": prep32},
             "per_step_ms": {"potential": ms[1], "kinetic": ms[2], "finalize": ms[3]},
             "hv_per_s": 1e3 / (ms[1] + ms[2]),

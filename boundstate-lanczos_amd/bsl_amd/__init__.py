@@ -100,9 +100,15 @@ class Calculator:
     @property
     def mult_tol(self):
         """Multiplicity tolerance of the eigensolve for this operator: 0 (the
-        reference's BISEC) for FP64, MIXED_MULT_TOL for FP32 and
-        BF16X3_MULT_TOL for BF16x3."""
-        return {PREC["fp64"]: 0.0, PREC["fp32"]: MIXED_MULT_TOL}.get(self.params.precision, BF16X3_MULT_TOL)
+        reference's BISEC) for FP64, MIXED_MULT_TOL for FP32, BF16X3_MULT_TOL
+        for the BF16x3 quadrature transforms and BF16X3_KIN_MULT_TOL for the
+        BF16x3 kinetic contractions of the NoQuad calculator."""
+        p = self.params.precision
+        if p == PREC["fp64"]:
+            return 0.0
+        if p == PREC["fp32"]:
+            return MIXED_MULT_TOL
+        return BF16X3_MULT_TOL if self.params.calculator == CALC["quad"] else BF16X3_KIN_MULT_TOL
 
     def levels(self, niter, count, emin=-1.0e4, emax=1.0e4, start="random", seed=0, tol=1e-8):
         """The `count` lowest converged good levels: Loop 1, Cullum-Willoughby,
@@ -333,11 +339,16 @@ def multiplets(energies, tol=1e-6):
 # rounding spreads the ghost copies of a converged level far beyond FP64's
 # BISEC width.  FP32: ~1e-8 relative (3e-8 kJ/mol at 54^3 x 36); 1e-6 kJ/mol
 # (8e-5 cm^-1) merges them, two orders below the 0.01 cm^-1 accuracy of the
-# mixed levels.  BF16x3 (two-term bf16 splits, ~16 significant bits): ~1e-6
-# relative (6.6e-6 kJ/mol measured); 3e-5 kJ/mol (2.5e-3 cm^-1), the level
-# accuracy of that precision.
+# mixed levels.  BF16x3 quadrature transforms (two-term bf16 splits, ~16
+# significant bits): ~1e-6 relative (6.6e-6 kJ/mol measured); 3e-5 kJ/mol
+# (2.5e-3 cm^-1), the level accuracy of that precision.  BF16x3 kinetic
+# contractions (the NoQuad BF16x3 operator): levels shift by up to 1.5e-2
+# cm^-1 at 54^3; 1e-4 kJ/mol, at which TRIVEC converges for the lowest
+# levels and their FP64 refinement is within 1e-4 cm^-1
+# (BSL_BF16X3_KIN_MULT_TOL).
 MIXED_MULT_TOL = 1.0e-6
 BF16X3_MULT_TOL = 3.0e-5
+BF16X3_KIN_MULT_TOL = 1.0e-4
 
 
 def lancbis(alpha, beta, beta2, emin, emax, device=False, mult_tol=0.0):

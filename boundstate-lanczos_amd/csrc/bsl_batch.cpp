@@ -134,10 +134,10 @@ int build_tables(bsl_batch *b)
             }
             if (cur.nm > 0) L.push_back(cur);
         };
-        for (int prec : {BSL_PREC_FP64, BSL_PREC_FP32})
+        for (int prec : {BSL_PREC_FP64, BSL_PREC_FP32, BSL_PREC_BF16X3})
             add(kBXyHalf, prec, [](const bsl_ctx *c) { return c->op.nz * c->op.nk; },
                 [&](const bsl_ctx *c) { return c->op.kprec == prec; });
-        for (int prec : {BSL_PREC_FP64, BSL_PREC_FP32})
+        for (int prec : {BSL_PREC_FP64, BSL_PREC_FP32, BSL_PREC_BF16X3})
             add(kBZHalf, prec, [](const bsl_ctx *c) { return ((c->op.nx * c->op.ny + 63) / 64) * c->op.nk; },
                 [&](const bsl_ctx *c) { return c->op.kprec == prec; });
         for (int kind : {(int)kBPotSplitPara, (int)kBPotSplitOrtho})

@@ -71,8 +71,6 @@ int check_params(const bsl_params &p)
     if (p.basis < 0 || p.basis > 1) return fail(BSL_E_ARG, "basis flag not recognised");
     if (p.calculator < 0 || p.calculator > 1) return fail(BSL_E_ARG, "calculator flag not recognised");
     if (p.precision < 0 || p.precision > 2) return fail(BSL_E_ARG, "precision flag not recognised");
-    if (p.precision == BSL_PREC_BF16X3 && p.calculator != BSL_CALC_QUAD)
-        return fail(BSL_E_ARG, "BF16x3 split products apply to the quadrature calculator's MFMA transforms only");
     if (p.potential < 0 || p.potential > 3) return fail(BSL_E_ARG, "potential flag not recognised");
     const bool sites = p.potential == BSL_POT_CAGE || p.potential == BSL_POT_CAGE_GRID;
     if (sites && (p.n_sites <= 0 || !p.site_pos || !p.site_q || !p.site_lj))
@@ -549,7 +547,9 @@ extern "C" int bsl_create(const bsl_params *pp, bsl_ctx **out)
     c->op64.vpk = nullptr;
     c->op64.kprec = BSL_PREC_FP64;
     // mixed precisions contract the kinetic term in FP32 as well
-    op.kprec = p.precision == BSL_PREC_FP64 ? BSL_PREC_FP64 : BSL_PREC_FP32;
+    // (BF16x3 quadrature contexts keep FP32 kinetics: BF16 split products
+    // gain nothing on the bandwidth-bound kinetic kernels and cost accuracy)
+    op.kprec = (p.precision == BSL_PREC_BF16X3 && p.calculator == BSL_CALC_QUAD) ? BSL_PREC_FP32 : p.precision;
     if (p.calculator == BSL_CALC_QUAD) op.calc = BSL_CALC_QUAD;
     const int form = harmonic_fits(c.get(), op) ? BSL_VFORM_HARMONICS : BSL_VFORM_BLOCKS;
     if ((rc = build_potentials(c.get(), form, c->d_vgrid))) return rc;

@@ -143,7 +143,7 @@ struct DevOperator {
     int64_t P = 0;                // grid points
     int nk = 0;                   // carried lm planes
     int vprec = 0;                // BSL_PREC_* of the potential blocks
-    int kprec = 0;                // BSL_PREC_FP64 or BSL_PREC_FP32: kinetic contractions
+    int kprec = 0;                // BSL_PREC_*: kinetic contractions (FP64, FP32 or BF16x3 MFMA)
     double *tx = nullptr, *ty = nullptr, *tz = nullptr;  // Toeplitz rows
     double *rot = nullptr, *mask = nullptr;              // per carried plane
     void *vpk = nullptr;          // potential of the points (double or float): packed V blocks in the

@@ -149,6 +149,35 @@ __device__ inline int kin_row(int kq, int q)
     return sizeof(T) == 8 ? kq + 4 * q : 4 * kq + q;
 }
 
+// BF16x3 kinetic contractions (KM = 1, FP32 operands in LDS): each FP32
+// operand x = hi + lo is split into two bfloat16 values (round to nearest
+// even) and a product is hi*hi + hi*lo + lo*hi, on gfx950's K = 32 BF16 MFMA
+// (v_mfma_f32_16x16x32_bf16: A lane l row l%16, k = 8(l/16) + e; B lane l
+// k = 8(l/16) + e, column l%16; the FP32 accumulator as 16x16x4) with FP32
+// accumulation.  One instruction covers the whole half-size contraction
+// (at most 32 terms).
+typedef __bf16 kbf8 __attribute__((ext_vector_type(8)));
+__device__ inline __bf16 bf16_from_bits(unsigned short b) { return __builtin_bit_cast(__bf16, b); }
+__device__ inline unsigned short bf16_rne_bits(float x)
+{
+    unsigned u = __float_as_uint(x);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (unsigned short)(u >> 16);
+}
+// element e of the hi and lo vectors from x
+__device__ inline void bf16x3_split(float x, kbf8 &hi, kbf8 &lo, int e)
+{
+    const unsigned short h = bf16_rne_bits(x);
+    hi[e] = bf16_from_bits(h);
+    lo[e] = bf16_from_bits(bf16_rne_bits(x - __uint_as_float((unsigned)h << 16)));
+}
+__device__ inline kf4 kmfma_bf16x3(const kbf8 &ah, const kbf8 &al, const kbf8 &bh, const kbf8 &bl, kf4 c)
+{
+    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, c, 0, 0, 0);
+    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, c, 0, 0, 0);
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, c, 0, 0, 0);
+}
+
 constexpr int kSlabLd = 65;
 
 // XCD-aware order of a (tile, plane) grid: workgroups are dealt to the 8 XCDs
@@ -181,7 +210,7 @@ constexpr int kCompLd = 33;
 // wave w owns the 16 x 16 tile (w/2, w%2) of all four components, so each lane
 // holds the four transformed components of its elements and recombines them
 // into the four mirror points.
-template <typename T>
+template <typename T, int KM>
 __device__ __forceinline__ void kin_xy_half_body(int k, int n, double *__restrict__ X, double *__restrict__ R,
                                                  const double *__restrict__ tx, const double *__restrict__ ty,
                                                  const double *__restrict__ rot, const double *__restrict__ mask,
@@ -247,22 +276,56 @@ __device__ __forceinline__ void kin_xy_half_body(int k, int n, double *__restric
 #pragma unroll
     for (int c = 0; c < 4; c++) acc[c] = acc_t{0, 0, 0, 0};
     const int jr = 16 * ta + l16;  // A-operand row of this lane
-    const int kx = (mx + 3) >> 2, ky = (my + 3) >> 2;
-    for (int s = 0; s < kx; s++) {
-        const int i2 = 4 * s + kq;
-        const T t1 = rx[64 + i2 - i], t2 = rx[64 + (nx - 1 - i2 - i)];
-        const T bp = i2 == hx ? t1 : t1 + t2, bm = t1 - t2;
+    if constexpr (KM == 0) {
+        const int kx = (mx + 3) >> 2, ky = (my + 3) >> 2;
+        for (int s = 0; s < kx; s++) {
+            const int i2 = 4 * s + kq;
+            const T t1 = rx[64 + i2 - i], t2 = rx[64 + (nx - 1 - i2 - i)];
+            const T bp = i2 == hx ? t1 : t1 + t2, bm = t1 - t2;
 #pragma unroll
-        for (int c = 0; c < 4; c++)
-            acc[c] = kmfma(comp[(c * 32 + jr) * kCompLd + i2], (c & 1) ? bm : bp, acc[c]);
-    }
-    for (int s = 0; s < ky; s++) {
-        const int j2 = 4 * s + kq;
-        const T t1 = ry[64 + jr - j2], t2 = ry[64 + (ny - 1 - jr - j2)];
-        const T ap = j2 == hy ? t1 : t1 + t2, am = t1 - t2;
+            for (int c = 0; c < 4; c++)
+                acc[c] = kmfma(comp[(c * 32 + jr) * kCompLd + i2], (c & 1) ? bm : bp, acc[c]);
+        }
+        for (int s = 0; s < ky; s++) {
+            const int j2 = 4 * s + kq;
+            const T t1 = ry[64 + jr - j2], t2 = ry[64 + (ny - 1 - jr - j2)];
+            const T ap = j2 == hy ? t1 : t1 + t2, am = t1 - t2;
 #pragma unroll
-        for (int c = 0; c < 4; c++)
-            acc[c] = kmfma((c & 2) ? am : ap, comp[(c * 32 + j2) * kCompLd + i], acc[c]);
+            for (int c = 0; c < 4; c++)
+                acc[c] = kmfma((c & 2) ? am : ap, comp[(c * 32 + j2) * kCompLd + i], acc[c]);
+        }
+    } else {
+        // x: K = i2 over [0, 32) (zero components beyond mx), lane k = 8 kq + e
+        kbf8 ph, pl, mh, ml;
+#pragma unroll
+        for (int e = 0; e < 8; e++) {
+            const int i2 = 8 * kq + e;
+            const float t1 = rx[64 + i2 - i], t2 = rx[64 + (nx - 1 - i2 - i)];
+            bf16x3_split(i2 == hx ? t1 : t1 + t2, ph, pl, e);
+            bf16x3_split(t1 - t2, mh, ml, e);
+        }
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            kbf8 ah, al;
+#pragma unroll
+            for (int e = 0; e < 8; e++) bf16x3_split(comp[(c * 32 + jr) * kCompLd + 8 * kq + e], ah, al, e);
+            acc[c] = (c & 1) ? kmfma_bf16x3(ah, al, mh, ml, acc[c]) : kmfma_bf16x3(ah, al, ph, pl, acc[c]);
+        }
+        // y: A the Toeplitz rows jr, K = j2
+#pragma unroll
+        for (int e = 0; e < 8; e++) {
+            const int j2 = 8 * kq + e;
+            const float t1 = ry[64 + jr - j2], t2 = ry[64 + (ny - 1 - jr - j2)];
+            bf16x3_split(j2 == hy ? t1 : t1 + t2, ph, pl, e);
+            bf16x3_split(t1 - t2, mh, ml, e);
+        }
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            kbf8 bh, bl;
+#pragma unroll
+            for (int e = 0; e < 8; e++) bf16x3_split(comp[(c * 32 + 8 * kq + e) * kCompLd + i], bh, bl, e);
+            acc[c] = (c & 2) ? kmfma_bf16x3(mh, ml, bh, bl, acc[c]) : kmfma_bf16x3(ph, pl, bh, bl, acc[c]);
+        }
     }
     const double rn = rot[n], mn = mask[n];
 #pragma unroll
@@ -284,18 +347,18 @@ __device__ __forceinline__ void kin_xy_half_body(int k, int n, double *__restric
     }
 }
 
-template <typename T>
+template <typename T, int KM>
 __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_kin_xy_half(
     double *__restrict__ X, double *__restrict__ R, const double *__restrict__ tx, const double *__restrict__ ty,
     const double *__restrict__ rot, const double *__restrict__ mask, const Scal *__restrict__ sc, int nx, int ny,
     int nz)
 {
-    kin_xy_half_body<T>(blockIdx.x, blockIdx.y, X, R, tx, ty, rot, mask, sc, nx, ny, nz);
+    kin_xy_half_body<T, KM>(blockIdx.x, blockIdx.y, X, R, tx, ty, rot, mask, sc, nx, ny, nz);
 }
 
 // z axis of 64 columns with the z parity split: s = v(k) + v(k'), d = v(k) - v(k'),
 // k < nz/2; wave w owns columns 16w.. of both components, every k tile.
-template <typename T>
+template <typename T, int KM>
 __device__ __forceinline__ void kin_z_half_body(int tile, int n, const double *__restrict__ X, double *__restrict__ R,
                                                 const double *__restrict__ tz, const double *__restrict__ mask,
                                                 int nxy, int nz)
@@ -349,16 +412,40 @@ __device__ __forceinline__ void kin_z_half_body(int tile, int n, const double *_
     acc_t as[2], ad[2];
 #pragma unroll
     for (int a = 0; a < 2; a++) as[a] = ad[a] = acc_t{0, 0, 0, 0};
-    const int kz = (mz + 3) >> 2;
-    for (int s = 0; s < kz; s++) {
-        const int kp = 4 * s + kq;
-        const T bs = cs[kp * kSlabLd + cw], bd = cd[kp * kSlabLd + cw];
+    if constexpr (KM == 0) {
+        const int kz = (mz + 3) >> 2;
+        for (int s = 0; s < kz; s++) {
+            const int kp = 4 * s + kq;
+            const T bs = cs[kp * kSlabLd + cw], bd = cd[kp * kSlabLd + cw];
+#pragma unroll
+            for (int a = 0; a < 2; a++) {
+                const int kr = 16 * a + l16;
+                const T t1 = rz[64 + kr - kp], t2 = rz[64 + (nz - 1 - kr - kp)];
+                as[a] = kmfma(kp == hz ? t1 : t1 + t2, bs, as[a]);
+                ad[a] = kmfma(t1 - t2, bd, ad[a]);
+            }
+        }
+    } else {
+        // K = kp over [0, 32) (zero rows beyond mz), lane k = 8 kq + e
+        kbf8 sh, sl, dh, dl;
+#pragma unroll
+        for (int e = 0; e < 8; e++) {
+            bf16x3_split(cs[(8 * kq + e) * kSlabLd + cw], sh, sl, e);
+            bf16x3_split(cd[(8 * kq + e) * kSlabLd + cw], dh, dl, e);
+        }
 #pragma unroll
         for (int a = 0; a < 2; a++) {
             const int kr = 16 * a + l16;
-            const T t1 = rz[64 + kr - kp], t2 = rz[64 + (nz - 1 - kr - kp)];
-            as[a] = kmfma(kp == hz ? t1 : t1 + t2, bs, as[a]);
-            ad[a] = kmfma(t1 - t2, bd, ad[a]);
+            kbf8 ph, pl, mh, ml;
+#pragma unroll
+            for (int e = 0; e < 8; e++) {
+                const int kp = 8 * kq + e;
+                const float t1 = rz[64 + kr - kp], t2 = rz[64 + (nz - 1 - kr - kp)];
+                bf16x3_split(kp == hz ? t1 : t1 + t2, ph, pl, e);
+                bf16x3_split(t1 - t2, mh, ml, e);
+            }
+            as[a] = kmfma_bf16x3(ph, pl, sh, sl, as[a]);
+            ad[a] = kmfma_bf16x3(mh, ml, dh, dl, ad[a]);
         }
     }
     const double mn = mask[n];
@@ -375,7 +462,7 @@ __device__ __forceinline__ void kin_z_half_body(int tile, int n, const double *_
         }
 }
 
-template <typename T>
+template <typename T, int KM>
 __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_kin_z_half(
     const double *__restrict__ X, double *__restrict__ R, const double *__restrict__ tz, const double *__restrict__ mask,
     const Scal *__restrict__ sc, int nxy, int nz)
@@ -383,21 +470,21 @@ __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
     (void)sc;  // X already holds v
     int tile, n;
     xcd_tile(tile, n);
-    kin_z_half_body<T>(tile, n, X, R, tz, mask, nxy, nz);
+    kin_z_half_body<T, KM>(tile, n, X, R, tz, mask, nxy, nz);
 }
 
-template <typename T>
+template <typename T, int KM>
 static int launch_kinetic_t(const DevOperator &op, double *X, double *R, Scal *sc, hipStream_t s)
 {
     // the half-size (centrosymmetric) contractions, for every axis length
     const size_t lds_xy = sizeof(T) * ((size_t)4 * 32 * kCompLd + 2 * 129);
-    hipLaunchKernelGGL(k_kin_xy_half<T>, dim3(op.nz, op.nk), dim3(kWG), lds_xy, s, X, R, op.tx, op.ty, op.rot,
+    hipLaunchKernelGGL((k_kin_xy_half<T, KM>), dim3(op.nz, op.nk), dim3(kWG), lds_xy, s, X, R, op.tx, op.ty, op.rot,
                        op.mask, sc, op.nx, op.ny, op.nz);
     BSL_HIP(hipGetLastError());
     const int nxy = op.nx * op.ny;
     const size_t lds_z = sizeof(T) * ((size_t)2 * 32 * kSlabLd + 129);
-    hipLaunchKernelGGL(k_kin_z_half<T>, dim3((nxy + 63) / 64, op.nk), dim3(kWG), lds_z, s, X, R, op.tz, op.mask, sc,
-                       nxy, op.nz);
+    hipLaunchKernelGGL((k_kin_z_half<T, KM>), dim3((nxy + 63) / 64, op.nk), dim3(kWG), lds_z, s, X, R, op.tz, op.mask,
+                       sc, nxy, op.nz);
     BSL_HIP(hipGetLastError());
     return BSL_OK;
 }
@@ -1086,24 +1173,24 @@ __device__ inline int batch_xcd(int id, int total)
     return (total % kXcds == 0) ? (id % kXcds) * (total / kXcds) + id / kXcds : id;
 }
 
-template <typename T>
+template <typename T, int KM>
 __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(4, 8))) void kb_xy_half(
     const BatchEntry *__restrict__ tab, int nm)
 {
     const int vb = (int)blockIdx.x, m = batch_find(tab, nm, vb);
     const BatchEntry &e = tab[m];
     const int loc = vb - e.first;
-    kin_xy_half_body<T>(loc % e.nz, loc / e.nz, e.X, e.R, e.tx, e.ty, e.rot, e.mask, e.sc, e.nx, e.ny, e.nz);
+    kin_xy_half_body<T, KM>(loc % e.nz, loc / e.nz, e.X, e.R, e.tx, e.ty, e.rot, e.mask, e.sc, e.nx, e.ny, e.nz);
 }
 
-template <typename T>
+template <typename T, int KM>
 __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(4, 8))) void kb_z_half(
     const BatchEntry *__restrict__ tab, int nm)
 {
     const int vb = batch_xcd((int)blockIdx.x, (int)gridDim.x), m = batch_find(tab, nm, vb);
     const BatchEntry &e = tab[m];
     const int loc = vb - e.first, nxy = e.nx * e.ny, nt = (nxy + 63) / 64;
-    kin_z_half_body<T>(loc % nt, loc / nt, e.X, e.R, e.tz, e.mask, nxy, e.nz);
+    kin_z_half_body<T, KM>(loc % nt, loc / nt, e.X, e.R, e.tz, e.mask, nxy, e.nz);
 }
 
 // The Gaunt values of a block depend on its shape only, so every member of
@@ -1128,17 +1215,18 @@ __global__ __launch_bounds__(kWG) void kb_finalize(const BatchEntry *__restrict_
     finalize_step(e.pa, e.pb, e.pc, (unsigned)e.nalpha, e.sc, e.alpha, e.beta, e.beta2, red);
 }
 
-template <typename T>
+template <typename T, int KM>
 static int launch_batch_t(int kind, const BatchEntry *tab, int nm, int total, const double *G, hipStream_t s)
 {
     const dim3 grid((unsigned)total);
     switch (kind) {
     case kBXyHalf:
-        hipLaunchKernelGGL(kb_xy_half<T>, grid, dim3(kWG), sizeof(T) * ((size_t)4 * 32 * kCompLd + 2 * 129), s, tab,
-                           nm);
+        hipLaunchKernelGGL((kb_xy_half<T, KM>), grid, dim3(kWG), sizeof(T) * ((size_t)4 * 32 * kCompLd + 2 * 129), s,
+                           tab, nm);
         break;
     case kBZHalf:
-        hipLaunchKernelGGL(kb_z_half<T>, grid, dim3(kWG), sizeof(T) * ((size_t)2 * 32 * kSlabLd + 129), s, tab, nm);
+        hipLaunchKernelGGL((kb_z_half<T, KM>), grid, dim3(kWG), sizeof(T) * ((size_t)2 * 32 * kSlabLd + 129), s, tab,
+                           nm);
         break;
     case kBPotSplitPara:
         hipLaunchKernelGGL((kb_split1<4, 0, T>), grid, dim3(192), 0, s, tab, nm, G);
@@ -1161,8 +1249,12 @@ int launch_batch(int kind, int prec, const BatchEntry *d_tab, int nm, int total,
         BSL_HIP(hipGetLastError());
         return BSL_OK;
     }
-    return prec == BSL_PREC_FP64 ? launch_batch_t<double>(kind, d_tab, nm, total, G, s)
-                                 : launch_batch_t<float>(kind, d_tab, nm, total, G, s);
+    if (prec == BSL_PREC_BF16X3) {
+        if (kind != kBXyHalf && kind != kBZHalf) return fail(BSL_E_ARG, "BF16x3 batch tables are kinetic only");
+        return launch_batch_t<float, 1>(kind, d_tab, nm, total, G, s);
+    }
+    return prec == BSL_PREC_FP64 ? launch_batch_t<double, 0>(kind, d_tab, nm, total, G, s)
+                                 : launch_batch_t<float, 0>(kind, d_tab, nm, total, G, s);
 }
 
 int launch_potential(const DevOperator &op, const double *X, double *R, Scal *sc, hipStream_t s)
@@ -1180,8 +1272,9 @@ int launch_hv(const DevOperator &op, double *X, double *R, Scal *sc, hipStream_t
 {
     if (ev) BSL_HIP(hipEventRecord(ev[0], s));
     if (op.nx > 64 || op.ny > 64 || op.nz > 64) return fail(BSL_E_ARG, "grid axes above 64 points are not supported");
-    int krc = op.kprec == BSL_PREC_FP64 ? launch_kinetic_t<double>(op, X, R, sc, s)
-                                        : launch_kinetic_t<float>(op, X, R, sc, s);
+    int krc = op.kprec == BSL_PREC_FP64   ? launch_kinetic_t<double, 0>(op, X, R, sc, s)
+              : op.kprec == BSL_PREC_FP32 ? launch_kinetic_t<float, 0>(op, X, R, sc, s)
+                                          : launch_kinetic_t<float, 1>(op, X, R, sc, s);
     if (krc) return krc;
     if (ev) BSL_HIP(hipEventRecord(ev[1], s));
     int rc = launch_potential(op, X, R, sc, s);

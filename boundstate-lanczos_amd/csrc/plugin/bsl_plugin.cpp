@@ -86,7 +86,6 @@ bool parse_calculator(const std::string &s, bslPluginStor *st)
     } else {
         return false;
     }
-    if (precision == BSL_PREC_BF16X3 && calc != BSL_CALC_QUAD) return false;
     bslPluginStor probe = *st;
     if (!parse_potential(rest, &probe)) return false;
     *st = probe;
@@ -163,8 +162,10 @@ bsl_ctx *bsl_plugin_context(generalStor *general_data)
 
 double bsl_plugin_mult_tol(generalStor *general_data)
 {
-    const int prec = reinterpret_cast<bslPluginStor *>(general_data)->precision;
-    return prec == BSL_PREC_FP64 ? 0.0 : prec == BSL_PREC_BF16X3 ? BSL_BF16X3_MULT_TOL : BSL_MIXED_MULT_TOL;
+    const bslPluginStor *st = reinterpret_cast<bslPluginStor *>(general_data);
+    if (st->precision == BSL_PREC_FP64) return 0.0;
+    if (st->precision == BSL_PREC_FP32) return BSL_MIXED_MULT_TOL;
+    return st->calc == BSL_CALC_QUAD ? BSL_BF16X3_MULT_TOL : BSL_BF16X3_KIN_MULT_TOL;
 }
 
 void Hv_Prep_linRotCartSph_HIP(int argc, char **argv, generalStor *general_data, lanczosStor *lanczos)

@@ -57,7 +57,11 @@ enum { BSL_CALC_NOQUAD = 0, BSL_CALC_QUAD = 1 };
  * at every node of any quadrature; they differ by rounding only.  HARMONICS
  * is the default wherever it applies (every parity block of l <= 5). */
 enum { BSL_VFORM_BLOCKS = 0, BSL_VFORM_HARMONICS = 1 };
-/* Hv arithmetic; the recurrence, its dot products and norms are always FP64 */
+/* Hv arithmetic; the recurrence, its dot products and norms are always FP64.
+ * FP32: kinetic contractions and quadrature transforms on FP32 MFMA, the
+ * NoQuad potential's coefficients in FP32.  BF16X3: the same with the MFMA
+ * products taken as three bfloat16 split products (hi*hi + hi*lo + lo*hi,
+ * FP32 accumulation) on the K = 32 BF16 MFMA. */
 enum { BSL_PREC_FP64 = 0, BSL_PREC_FP32 = 1, BSL_PREC_BF16X3 = 2 };
 /* IHO: isotropic harmonic oscillator (IsoHarmOsc_Routines.cpp:6-79).
  * CAGE: Alavi H2 pair sums at the exact H positions of every quadrature
@@ -183,6 +187,11 @@ int bsl_lancbis_device(int32_t niter, const double *alpha, const double *beta, c
  * (2.5e-3 cm^-1), the level accuracy of that precision. */
 #define BSL_MIXED_MULT_TOL 1.0e-6
 #define BSL_BF16X3_MULT_TOL 3.0e-5
+/* NoQuad BF16x3 (its kinetic contractions as BF16 split products): levels
+ * shift by up to ~1.5e-2 cm^-1 at 54^3 and ghost copies spread by ~1e-4
+ * kJ/mol; at this width TRIVEC converges for the lowest levels, whose FP64
+ * Rayleigh refinement (bsl_ritz_refine) is then within 1e-4 cm^-1. */
+#define BSL_BF16X3_KIN_MULT_TOL 1.0e-4
 int bsl_lancbis_tol(int32_t niter, const double *alpha, const double *beta, const double *beta2, double emin,
                     double emax, double mult_tol, double *eval, double *err, int32_t *ngood);
 int bsl_lancbis_device_tol(int32_t niter, const double *alpha, const double *beta, const double *beta2,

@@ -17,7 +17,8 @@ against the CPU oracle.
   against the oracle's gradient and against central differences of the
   GPU's converged levels for every kind of parameter.
 
-Tolerances: Hv 1e-12 (FP64) and 2e-6 (FP32) of the largest magnitude;
+Tolerances: Hv 1e-12 (FP64), 2e-6 (FP32) and 5e-5 (BF16x3) of the largest
+magnitude;
 alpha/beta over a 20-step prefix 1e-9 (FP64) and 2e-5 (FP32 Hv) relative;
 eigenvalues 1e-9 relative; gradients 1e-10 of the oracle's and 1e-4 of the
 central differences (h = 1e-4 relative).
@@ -84,6 +85,16 @@ def test_cfg2_hv_54_fp32_close_to_oracle(cage54):
     with B.Calculator(precision="fp32", **kw) as c:
         got = c.hv(v)
     assert _rel(got, ref) < 2e-6
+
+
+def test_cfg2_hv_54_bf16x3_close_to_oracle(cage54):
+    """BF16x3 kinetic contractions (three bfloat16 split products, about 18
+    significant bits per operand) with FP32 potential coefficients."""
+    kw, v, ref = cage54
+    with B.Calculator(precision="bf16x3", **kw) as c:
+        assert c.mult_tol == B.BF16X3_KIN_MULT_TOL
+        got = c.hv(v)
+    assert _rel(got, ref) < 5e-5
 
 
 def _cfg4_members():

@@ -269,14 +269,16 @@ def test_rayleigh_refinement_matches_oracle(precision):
     assert abs(e1[0] - e[0]) <= 1e-14 * abs(e[0])
 
 
-def test_fp32_levels_within_target():
-    """FP32 potential blocks, FP64 recurrence: lowest levels within 0.01 cm^-1."""
+@pytest.mark.parametrize("precision", ["fp32", "bf16x3"])
+def test_fp32_levels_within_target(precision):
+    """FP32 potential coefficients with FP32 (or BF16x3) kinetic MFMA
+    contractions, FP64 recurrence: lowest levels within 0.01 cm^-1."""
     from bsl_amd import physics
     spec, kw = cs.cage_case(8, 3, 10, 10, "All")
     c64 = B.Calculator(**kw)
-    c32 = B.Calculator(precision="fp32", **kw)
+    c32 = B.Calculator(precision=precision, **kw)
     e64, r64 = B.lancbis(*c64.lanczos(300), -10.0, 10.0)
-    e32, r32 = B.lancbis(*c32.lanczos(300), -10.0, 10.0)
+    e32, r32 = B.lancbis(*c32.lanczos(300), -10.0, 10.0, mult_tol=c32.mult_tol)
     conv = _converged(e64, r64, 1e-9)[:5]
     assert len(conv) >= 3
     for e in conv:

@@ -298,14 +298,16 @@ def ref_available():
 # the reference's own Lanczos program (lanczosEigenValuesVectors.cpp), built
 # from source into oracle/_ref by oracle/Makefile.ref
 REF_LANCZOS = os.path.join(REPO, "oracle", "_ref", "ref_lanczos")
+# the same program linked with the HIP plugin binding (oracle/ref_bind.cpp)
+REF_LANCZOS_HIP = os.path.join(REPO, "oracle", "_ref", "ref_lanczos_hip")
 
 
-def run_ref_lanczos(lanczos_deck, hv_deck, cwd, pot_deck=None):
-    """Runs the reference program on its decks in cwd; returns its stdout
-    alpha, beta (Loop 1 coefficients as printed, 16 digits) and the output
-    directory it made."""
+def run_ref_lanczos(lanczos_deck, hv_deck, cwd, pot_deck=None, program=None):
+    """Runs the reference program (or `program`, built from the same
+    sources) on its decks in cwd; returns its stdout alpha, beta (Loop 1
+    coefficients as printed, 16 digits) and the output directory it made."""
     import subprocess
-    args = [REF_LANCZOS, lanczos_deck, hv_deck] + ([pot_deck] if pot_deck else [])
+    args = [program or REF_LANCZOS, lanczos_deck, hv_deck] + ([pot_deck] if pot_deck else [])
     r = subprocess.run(args, cwd=cwd, capture_output=True, text=True, timeout=600,
                        env=dict(os.environ, OMP_NUM_THREADS="4"))
     if r.returncode != 0:

@@ -304,8 +304,15 @@ def test_driver_matches_reference_program_strings(tmp_path, calc):
     contract v(i) + v(n-1-i) and v(i) - v(n-1-i) separately and keep that
     parity exactly, while the reference's plain products let rounding seed
     the odd states, which then converge (levels spaced 3 E1 beside the even
-    ones' 8 E1).  There every level of the driver is checked against the
-    reference's list, not the other way round."""
+    ones' 8 E1).  The kinetic contractions keep the parity on every axis
+    length, so for this string the comparison is made two-way through the
+    separable spectrum (1-D sinc-DVR eigenvalues from the oracle's kinetic
+    matrix, the constant potential, B l(l+1)), split by translational
+    parity: every level of the driver is an all-even state and in the
+    reference's list; every level of the reference is a state of that
+    spectrum; at least three of those the driver does not list are odd on
+    some axis and nowhere even; and a random-start run of the library
+    (which reaches every parity) lists each of the reference's levels."""
     import _oracle as O
     if not os.path.exists(O.REF_LANCZOS):
         pytest.skip("reference program (oracle/_ref) not built")
@@ -331,3 +338,87 @@ def test_driver_matches_reference_program_strings(tmp_path, calc):
         assert len(conv) >= 5
         d = np.min(np.abs(conv[:, None] - b[None, :, 0]), axis=1)
         assert np.max(d / np.abs(conv)) < 1e-9
+    if calc.endswith("_Coulomb"):
+        _coulomb_two_way(rows["hip"], rows["ref"])
+
+
+def _coulomb_two_way(hip, ref):
+    """The Coulomb string's levels split by translational parity (see
+    test_driver_matches_reference_program_strings)."""
+    import bsl_amd as B
+    import _oracle as O
+    spec, kw = cs.coulomb_case(8, 3, 8, 9, 0.4, charges=(1.0, -0.5, -200.0))
+    spec = dict(spec, mass=2.01588, rot_const=0.7096487130193394)
+    kw = dict(kw, mass=2.01588, rot_const=0.7096487130193394)
+    o = O.Oracle(spec)
+    v1, v2 = o.vgrid()
+    assert np.ptp(v1) == 0 and np.ptp(v2) == 0 and v1.flat[0] == v2.flat[0]
+    w, u = np.linalg.eigh(o.kin(0))
+    even = np.array([np.allclose(u[:, k], u[::-1, k], atol=1e-10) for k in range(len(w))])
+    assert even.sum() == len(w) // 2
+    rot = sorted({spec["rot_const"] * l * (l + 1) for l in range(4)})
+    lev_even, lev_odd = [], []
+    for a in range(len(w)):
+        for b in range(len(w)):
+            for c in range(len(w)):
+                for r in rot:
+                    e = w[a] + w[b] + w[c] + v1.flat[0] + r
+                    (lev_even if even[a] and even[b] and even[c] else lev_odd).append(e)
+    lev_even, lev_odd = np.array(lev_even), np.array(lev_odd)
+
+    def near(x, pool):
+        return np.min(np.abs(pool - x)) <= 1e-9 * abs(x)
+
+    hconv = hip[hip[:, 1] < 1e-10, 0]
+    rconv = ref[ref[:, 1] < 1e-10, 0]
+    assert len(hconv) >= 5 and len(rconv) > len(hconv)
+    for e in hconv:
+        assert near(e, lev_even), e
+    for e in rconv:
+        assert near(e, np.concatenate([lev_even, lev_odd])), e
+    only_ref = [e for e in rconv if not near(e, hip[:, 0])]
+    odd_only = [e for e in only_ref if near(e, lev_odd) and not near(e, lev_even)]
+    assert len(odd_only) >= 3
+    # 1000 steps: a random start spreads over every symmetry sector, so it
+    # needs more steps than the uniform one to converge the same levels
+    with B.Calculator(**kw) as c:
+        ev, _ = B.lancbis(*c.lanczos(1000, start="random", seed=3), -400.0, 400.0)
+    for e in rconv:
+        assert near(e, ev), e
+
+
+@pytest.mark.gpu
+def test_reference_program_selects_hip_calculator(tmp_path):
+    """The reference's own Lanczos program (lanczosEigenValuesVectors.cpp
+    main and interface.cpp, built from source) with the plugin bound into its
+    calculator selection (oracle/ref_bind.cpp, the branch INTEGRATION's
+    interface.cpp edit adds): the HIP string runs through the reference's
+    driver (its Loop 1 and Loop 2, the Hv function pointer on host vectors)
+    and reproduces that program's run of its own string on the saved run's
+    decks, coefficients over the first 40 steps and every converged level
+    both ways within 1e-9; a string neither knows still ends in the
+    reference's error branch."""
+    import _oracle as O
+    if not (os.path.exists(O.REF_LANCZOS) and os.path.exists(O.REF_LANCZOS_HIP)):
+        pytest.skip("reference program (oracle/_ref) not built")
+    res = {}
+    for side, calc, prog in (("ref", "linRotCartSph_NoQuad_ContGrid_Alavi_SPCE", O.REF_LANCZOS),
+                             ("hip", "linRotCartSph_HIP_NoQuad_ContGrid_Alavi_SPCE", O.REF_LANCZOS_HIP)):
+        t = str(tmp_path / side)
+        os.makedirs(t)
+        a, b, d = O.run_ref_lanczos(lanczos_deck(t, niter=300, calc=calc), hv_deck(t), t, program=prog)
+        res[side] = (a, b, _read_boundstates(os.path.join(t, d)))
+    (ar, br, lr), (ah, bh, lh) = res["ref"], res["hip"]
+    assert len(ah) == len(ar) == 300
+    assert np.max(np.abs(ah[:40] - ar[:40]) / np.abs(ar[:40])) < 1e-9
+    assert np.max(np.abs(bh[:40] - br[:40]) / np.abs(br[:40])) < 1e-9
+    for x, y in ((lh, lr), (lr, lh)):
+        conv = x[x[:, 1] < 1e-10, 0]
+        assert len(conv) >= 5
+        d = np.min(np.abs(conv[:, None] - y[None, :, 0]), axis=1)
+        assert np.max(d / np.abs(conv)) < 1e-9
+    t = str(tmp_path / "unknown")
+    os.makedirs(t)
+    r = subprocess.run([O.REF_LANCZOS_HIP, lanczos_deck(t, calc="linRotCartSph_HIP_NoSuchCalculator"), hv_deck(t)],
+                       cwd=t, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 1 and "not recognized" in r.stderr

@@ -1422,16 +1422,30 @@ __global__ __launch_bounds__(kWG) void k_cm_potential(double *cm, int64_t P, con
     cm[p] = e + l;
 }
 
-// vg[p][q], q = (half, a, b): H2 centre at grid point p, H atoms at +-off[q]
+// 1/sqrt(x): the hardware estimate and two Newton steps y <- y (3/2 - x y^2 / 2),
+// each of which squares the relative error (a few ulp of the rounded value)
+__device__ inline double rsqrt_nr(double x)
+{
+    const double h = 0.5 * x;
+    double y = __builtin_amdgcn_rsq(x);
+    y = fma(y, fma(-h * y, y, 0.5), y);
+    y = fma(y, fma(-h * y, y, 0.5), y);
+    return y;
+}
+
+// vg[p][q], q = (half, a, b): H2 centre at grid point p, H atoms at +-off[q].
+// The H-site terms K q_H q_s / |h - s| are summed in site order with the
+// weights K q_H q_s staged in LDS and 1/|h - s| from rsqrt_nr (a few ulp from
+// the quotient of a correctly rounded square root, per term).
 __global__ __launch_bounds__(kWG) void k_vgrid_cage(double *vg, int64_t P, int Q, const double *__restrict__ gx,
                                                     const double *__restrict__ gy, const double *__restrict__ gz,
                                                     int nx, int ny, const double *__restrict__ off,
                                                     const double *__restrict__ site_g, int ns,
                                                     const double *__restrict__ cm, double q_h, double ceiling)
 {
-#pragma clang fp contract(off)
-    extern __shared__ double site[];
-    for (int t = threadIdx.x; t < 4 * ns; t += blockDim.x) site[t] = site_g[t];
+    extern __shared__ double site[];  // [ns][4]: x, y, z, K q_H q_s
+    for (int t = threadIdx.x; t < 4 * ns; t += blockDim.x)
+        site[t] = (t & 3) == 3 ? 138.9354846111096 * q_h * site_g[t] : site_g[t];
     __syncthreads();
     const int64_t total = P * (int64_t)Q;
     for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
@@ -1441,9 +1455,17 @@ __global__ __launch_bounds__(kWG) void k_vgrid_cage(double *vg, int64_t P, int Q
         const int i = (int)(p % nx), j = (int)((p / nx) % ny), k = (int)(p / ((int64_t)nx * ny));
         const double x = gx[i], y = gy[j], z = gz[k];
         const double ox = off[3 * q], oy = off[3 * q + 1], oz = off[3 * q + 2];
-        double v = coulomb_sum(site, ns, x + ox, y + oy, z + oz, q_h);
-        v += coulomb_sum(site, ns, x - ox, y - oy, z - oz, q_h);
-        v += cm[p];
+        const double ax = x + ox, ay = y + oy, az = z + oz, bx = x - ox, by = y - oy, bz = z - oz;
+        double va = 0.0, vb = 0.0;
+        for (int t = 0; t < ns; t++) {
+            const double sx = site[4 * t], sy = site[4 * t + 1], sz = site[4 * t + 2], w = site[4 * t + 3];
+            if (w == 0.0) continue;  // uncharged site (TIP4P's O): no term, as in the reference's sum
+            const double a0 = sx - ax, a1 = sy - ay, a2 = sz - az;
+            const double b0 = sx - bx, b1 = sy - by, b2 = sz - bz;
+            va = fma(w, rsqrt_nr(fma(a0, a0, fma(a1, a1, a2 * a2))), va);
+            vb = fma(w, rsqrt_nr(fma(b0, b0, fma(b1, b1, b2 * b2))), vb);
+        }
+        double v = va + vb + cm[p];
         if (v >= ceiling) v = ceiling;
         vg[idx] = v;
     }

@@ -575,7 +575,8 @@ extern "C" int bsl_create(const bsl_params *pp, bsl_ctx **out)
         rc = dev_upload(&d_wt, t.wt);
         if (!rc) rc = dev_upload(&d_wp, t.wp);
         if (!rc)
-            rc = launch_weight_grid(c->d_vgrid, op.wgrid, op.wprec, d_wt, d_wp, c->P, na, nb, op.q_npts, c->stream);
+            rc = launch_weight_grid(c->d_vgrid, op.wgrid, op.wprec, op.qmode, d_wt, d_wp, c->P, na, nb, op.q_npts,
+                                    c->stream);
         const hipError_t ew = hipStreamSynchronize(c->stream);
         dev_free(d_wt);
         dev_free(d_wp);

@@ -272,7 +272,7 @@ int launch_vblock_gemm(const double *vg, const double *h, void *vpk, int vprec, 
 // quadrature calculator (bsl_quad.hip)
 // weighted quadrature grid in the MFMA accumulator layout of the quadrature
 // kernel (npts grid points per workgroup)
-int launch_weight_grid(const double *vg, void *W, int wprec, const double *wt, const double *wp, int64_t P,
+int launch_weight_grid(const double *vg, void *W, int wprec, int mode, const double *wt, const double *wp, int64_t P,
                        int na, int nb, int npts, hipStream_t s);
 // context run helpers (bsl_capi.cpp) used by the batch driver: coefficient
 // capacity for niter steps, start vector, one step on the context stream

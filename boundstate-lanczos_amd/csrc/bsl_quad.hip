@@ -26,15 +26,29 @@
 //
 // The two phi transforms are GEMMs batched over columns (P, a): with
 // columns as N and phi as M (forward) or K (backward), on MFMA 16x16x4
-// (FP64 or FP32, FP32 accumulation for FP32) or 16x16x16 BF16 (three split
-// products hi*hi + hi*lo + lo*hi of FP32 operands, FP32 accumulation).  The
-// forward accumulator tile (16 phi x 16 columns) is multiplied by W in
-// registers and fed straight back as the B operand of the backward MFMA, so
-// the (a, phi) intermediate never leaves the registers.  W is stored in that
-// accumulator layout: [column tile][phi tile][lane][4], one contiguous
-// 16-byte (FP32) or 32-byte (FP64) load per lane.  Accumulator rows of lane
-// j (item i): 4*(j/16) + i for the FP32 and BF16 instructions, j/16 + 4*i for
-// the FP64 one.
+// (FP64 or FP32, FP32 accumulation for FP32) or gfx950's 16x16x32 BF16
+// (three split products hi*hi + hi*lo + lo*hi of FP32 operands, FP32
+// accumulation).  The forward accumulator tile (16 phi x 16 columns) is
+// multiplied by W in registers and fed straight back as the B operand of the
+// backward MFMA, so the (a, phi) intermediate never leaves the registers.  W
+// is stored in that accumulator layout: [column tile][phi tile][lane][4], one
+// contiguous 16-byte (FP32) or 32-byte (FP64) load per lane.  Accumulator
+// rows of lane j (item i): 4*(j/16) + i for the FP32 and BF16 instructions,
+// j/16 + 4*i for the FP64 one.
+//
+// BF16x3 on the K = 32 instruction (A lane l: row l%16, k = 8(l/16) + e; B
+// lane l: k = 8(l/16) + e, column l%16).  Forward: K packs the split terms,
+// k < 16 taking S_hi x F_hi and S_hi x F_lo (two instructions) and k >= 16
+// S_lo x F_hi, over the 16 m rows.  Backward: K runs over 32 phi, so that the
+// phi tiles are taken in pairs (2u, 2u + 1) whose rows are permuted: row
+// 4 kq + i of tile 2u + s holds phi 32u + 8 kq + 4 s + i, and lane group kq
+// of the pair then holds exactly the 8 phi values its backward B operand
+// needs; an odd last tile keeps the natural rows and fills half of its
+// backward K.  The forward B operand needs m = 8 (kq & 1) + e, e < 8: the
+// forward Legendre transform, itself on the K = 32 instruction (A: exact 0/1
+// selections of the carried planes by m and parity, B: hi + lo of the FP32
+// products L c over 32 planes), fills two accumulators whose selection rows
+// are permuted the same way (m = 8 (kq & 1) + i and 8 (kq & 1) + 4 + i).
 #include <hip/hip_runtime.h>
 
 #include <type_traits>
@@ -47,7 +61,6 @@ constexpr int kQWG = 256;  // 4 waves
 
 typedef float f4v __attribute__((ext_vector_type(4)));
 typedef double d4v __attribute__((ext_vector_type(4)));
-typedef short s4v __attribute__((ext_vector_type(4)));
 
 template <typename T>
 struct AccT;
@@ -82,9 +95,41 @@ __device__ inline void bf16_split(float x, short &hi, short &lo)
     hi = bf16_rne(x);
     lo = bf16_rne(x - bf16_to_f(hi));
 }
-__device__ inline f4v mfma_bf16(s4v a, s4v b, f4v c)
+typedef __bf16 bf8v __attribute__((ext_vector_type(8)));
+typedef __bf16 bf2v __attribute__((ext_vector_type(2)));
+typedef float f2v __attribute__((ext_vector_type(2)));
+typedef unsigned u4v __attribute__((ext_vector_type(4)));
+// the same split of two floats through the packed conversion instruction
+// (round to nearest even): hi = bf16(x), lo = bf16(x - hi), element 0 in the
+// low half of each word
+__device__ inline void split_pk(float x0, float x1, unsigned &hi, unsigned &lo)
 {
-    return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+    const f2v x = {x0, x1};
+    const unsigned h = __builtin_bit_cast(unsigned, __builtin_convertvector(x, bf2v));
+    const f2v r = {x0 - __uint_as_float(h << 16), x1 - __uint_as_float(h & 0xffff0000u)};
+    hi = h;
+    lo = __builtin_bit_cast(unsigned, __builtin_convertvector(r, bf2v));
+}
+// hi and lo operands of eight floats: elements 0-3 from a, 4-7 from b
+__device__ inline void split8(const f4v &a, const f4v &b, bf8v &hi, bf8v &lo)
+{
+    unsigned h0, l0, h1, l1, h2, l2, h3, l3;
+    split_pk(a[0], a[1], h0, l0);
+    split_pk(a[2], a[3], h1, l1);
+    split_pk(b[0], b[1], h2, l2);
+    split_pk(b[2], b[3], h3, l3);
+    const u4v hv = {h0, h1, h2, h3}, lv = {l0, l1, l2, l3};
+    hi = __builtin_bit_cast(bf8v, hv);
+    lo = __builtin_bit_cast(bf8v, lv);
+}
+__device__ inline bf8v bf8_zero()
+{
+    const u4v z = {0u, 0u, 0u, 0u};
+    return __builtin_bit_cast(bf8v, z);
+}
+__device__ inline f4v mfma_bf16_32(const bf8v &a, const bf8v &b, f4v c)
+{
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
 }
 
 __device__ inline double q_block_sum(double v, double *lds)
@@ -107,11 +152,35 @@ __host__ __device__ inline int acc_row(int kq, int i)
     return F64 ? kq + 4 * i : 4 * kq + i;
 }
 
+// Virtual plane layout of the BF16 Legendre transform: blocks of 8 planes
+// of one l parity, the even-l planes (r < ne) in the first ceil(ne/8)
+// blocks and the odd-l ones after them when both parities are carried,
+// otherwise the planes in order; padding zero; K steps of 4 blocks.
+__host__ __device__ inline bool quad_two(int nk, int ne) { return ne > 0 && ne < nk; }
+__host__ __device__ inline int quad_even_blocks(int nk, int ne) { return quad_two(nk, ne) ? (ne + 7) / 8 : (nk + 7) / 8; }
+__host__ __device__ inline int quad_vsteps(int nk, int ne)
+{
+    const int nb = quad_even_blocks(nk, ne) + (quad_two(nk, ne) ? (nk - ne + 7) / 8 : 0);
+    return (nb + 3) / 4;
+}
+// virtual index of plane r, and the plane of virtual index k (-1: padding)
+__host__ __device__ inline int quad_vindex(int r, int nk, int ne)
+{
+    return quad_two(nk, ne) && r >= ne ? 8 * quad_even_blocks(nk, ne) + r - ne : r;
+}
+__host__ __device__ inline int quad_vplane(int k, int nk, int ne)
+{
+    if (!quad_two(nk, ne)) return k < nk ? k : -1;
+    const int e8 = 8 * quad_even_blocks(nk, ne);
+    if (k < e8) return k < ne ? k : -1;
+    return k - e8 < nk - ne ? ne + k - e8 : -1;
+}
+
 // Shared-memory plan of one workgroup: kQW waves, each owning ppw grid
 // points (ppw*na a multiple of 16: whole column tiles per wave).
 struct QuadLds {
-    int sp, fm, bp;               // row strides (odd, against bank conflicts)
-    size_t off_s, off_b, off_x, off_u, off_l, off_sel, bytes;
+    int sp, fm, bp, kps;          // row strides (odd or padded, against bank conflicts)
+    size_t off_s, off_b, off_x, off_u, off_l, off_c, off_sel, bytes;
 };
 
 constexpr int kQW = kQWG / 64;
@@ -119,17 +188,26 @@ constexpr int kQW = kQWG / 64;
 // loads and MFMAs keep their order around it
 constexpr int kKeepLoads = 0x2 | 0x4 | 0x100 | 0x200;
 
-__host__ __device__ inline QuadLds quad_lds(int nk, int na, int npts, int kf, int mode, int nphp)
+__host__ __device__ inline size_t align16(size_t o) { return (o + 15) & ~(size_t)15; }
+
+__host__ __device__ inline QuadLds quad_lds(int nk, int ne, int na, int npts, int kf, int mode, int nphp)
 {
     QuadLds q;
     const int es = mode == 0 ? 8 : 4;
     q.sp = 17;
     q.fm = kf + 1;
     q.bp = nphp + 4;
-    size_t o = sizeof(double) * (size_t)npts * nk;  // c [npts][nk]
+    q.kps = 0;
+    q.off_c = 0;
+    // c [npts][nk] in FP64 (FP64 and FP32 modes; the BF16 mode keeps an FP32
+    // copy in its virtual layout instead)
+    size_t o = mode == 2 ? 0 : sizeof(double) * (size_t)npts * nk;
+    o = align16(o);
     q.off_s = o;
     if (mode == 2) {
-        // bf16 splits: forward A [phi][16] hi, lo; backward A [16][phi] hi, lo
+        // bf16 splits: forward A [phi][16] hi, lo; backward A [16][phi] hi,
+        // lo (rows of 16-byte aligned runs of 8 phi)
+        q.bp = nphp + 8;
         o += 2 * sizeof(short) * (size_t)nphp * 16;
         q.off_b = o;
         o += 2 * sizeof(short) * (size_t)16 * q.bp;
@@ -137,18 +215,30 @@ __host__ __device__ inline QuadLds quad_lds(int nk, int na, int npts, int kf, in
         o += (size_t)es * nphp * q.sp;             // S [phi][m] (m padded to 16)
         q.off_b = o;
     }
-    o = (o + 15) & ~(size_t)15;
+    o = align16(o);
     q.off_x = o;
     o += (size_t)es * 2 * kQW * 16 * q.fm;         // per-wave F' of one column tile [16][m], both parities
-    o = (o + 15) & ~(size_t)15;
+    o = align16(o);
     q.off_u = o;
     o += sizeof(double) * (size_t)npts * nk;       // u [npts][nk]
-    o = (o + 15) & ~(size_t)15;
+    o = align16(o);
     q.off_l = o;
-    o += (size_t)es * nk * na;                     // L [nk][na]
-    o = (o + 15) & ~(size_t)15;
-    q.off_sel = o;
-    o += (size_t)es * ((nk + 3) / 4) * 64;         // m selection, A operand [K step][lane]
+    if (mode == 2) {
+        const int vs = quad_vsteps(nk, ne);
+        q.kps = 32 * vs + 4;
+        o += sizeof(float) * (size_t)na * q.kps;   // L [na][virtual plane]
+        o = align16(o);
+        q.off_c = o;
+        o += sizeof(float) * (size_t)npts * q.kps; // c [npts][virtual plane]
+        o = align16(o);
+        q.off_sel = o;
+        o += (size_t)16 * vs * 4 * 64;             // m selections, bf16 A operand [K step][table][lane][8]
+    } else {
+        o += (size_t)es * nk * na;                 // L [nk][na]
+        o = align16(o);
+        q.off_sel = o;
+        o += (size_t)es * ((nk + 3) / 4) * 64;     // m selection, A operand [K step][lane]
+    }
     q.bytes = o;
     return q;
 }
@@ -201,32 +291,48 @@ __global__ __launch_bounds__(kQWG) void k_quad_mfma(const double *__restrict__ X
 {
     typedef typename QuadTypes<MODE>::T T;
     typedef typename QuadTypes<MODE>::acc acc_t;
+    // Legendre backward accumulation: FP64, FP32 for the BF16 mode
+    typedef typename std::conditional<MODE == 2, float, double>::type U;
     constexpr bool F64 = MODE == 0;
     constexpr int KF = 4 * KS;  // m rows carried through the transforms
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ double red[kQW];
     const int npts = kQW * ppw;
-    const QuadLds q = quad_lds(nk, na, npts, KF, MODE, 16 * NT);
+    const QuadLds q = quad_lds(nk, ne, na, npts, KF, MODE, 16 * NT);
     double *cs = (double *)smem;
     double *us = (double *)(smem + q.off_u);
     T *Ls = (T *)(smem + q.off_l);
     (void)sc;  // X already holds v (formed by the kinetic kernel)
+    // both l parities carried (full basis, uniform): two transform chains;
+    // otherwise the one parity takes the first chain
+    const bool two = quad_two(nk, ne);
+    // K steps, and the blocks of 8 virtual planes of the first chain
+    const int vs = quad_vsteps(nk, ne), nbe = quad_even_blocks(nk, ne);
 
     // ---- tables and inputs ----
-    for (int t = threadIdx.x; t < nk * na; t += blockDim.x) Ls[t] = (T)Lg[t];
-    // A operand of the forward Legendre MFMA: Sel[row][r] = 1 where carried
-    // plane r has m = msel(row), for K step s4 (r = 4 s4 + lane/16) and row
-    // lane%16.  msel orders the rows so that accumulator item i of lane group
-    // kq holds the m the forward phi operand of that lane needs (m = 4i + kq
-    // for the 16x16x4 instructions, m = 4kq + i for the BF16 K = 16 block).
-    const int nks = (nk + 3) >> 2;
-    T *Sel = (T *)(smem + q.off_sel);
-    for (int t = threadIdx.x; t < nks * 64; t += blockDim.x) {
-        const int r = 4 * (t >> 6) + ((t & 63) >> 4), row = t & 15;
-        const int msel = (MODE == 1) ? 4 * (row & 3) + (row >> 2) : row;  // FP64 D rows kq + 4i, FP32 4kq + i
-        Sel[t] = (r < nk && mplane[r] == msel) ? (T)1 : (T)0;
-    }
     if constexpr (MODE == 2) {
+        // L and c in the virtual plane layout (FP32), and the A operands of
+        // the forward Legendre MFMA: for K step s, table tb and lane (row
+        // l16, lane group kq) element e is 1 where virtual plane
+        // k = 32 s + 8 kq + e has m = 8 ((row / 4) & 1) + 4 (tb & 1) +
+        // row % 4 and the parity of table tb (tb < 2: the first chain, even
+        // l or the one parity carried; tb >= 2: odd l), so that accumulator
+        // item i of lane group kq holds m = 8 (kq & 1) + 4 (tb & 1) + i, the
+        // m of the forward phi operand of that lane
+        float *Lt = (float *)(smem + q.off_l), *cf = (float *)(smem + q.off_c);
+        for (int t = threadIdx.x; t < na * q.kps; t += blockDim.x) {
+            const int a = t / q.kps, r = quad_vplane(t - a * q.kps, nk, ne);
+            Lt[t] = r >= 0 ? (float)Lg[(size_t)r * na + a] : 0.f;
+        }
+        for (int t = threadIdx.x; t < npts * q.kps; t += blockDim.x) cf[t] = 0.f;
+        unsigned short *selb = (unsigned short *)(smem + q.off_sel);
+        for (int t = threadIdx.x; t < vs * 4 * 64 * 8; t += blockDim.x) {
+            const int e = t & 7, ln = (t >> 3) & 63, tb = (t >> 9) & 3, st = t >> 11;
+            const int row = ln & 15, r = quad_vplane(32 * st + 8 * (ln >> 4) + e, nk, ne);
+            const int msel = 8 * ((row >> 2) & 1) + 4 * (tb & 1) + (row & 3);
+            const bool par = r >= 0 && (tb < 2 ? !two || r < ne : two && r >= ne);
+            selb[t] = (par && mplane[r] == msel) ? (unsigned short)0x3f80 : (unsigned short)0;
+        }
         short *fh = (short *)(smem + q.off_s), *fl = fh + 16 * NT * 16;
         short *bh = (short *)(smem + q.off_b), *bl = bh + 16 * q.bp;
         for (int t = threadIdx.x; t < NT * 16 * 16; t += blockDim.x) {
@@ -240,6 +346,20 @@ __global__ __launch_bounds__(kQWG) void k_quad_mfma(const double *__restrict__ X
             bl[mi * q.bp + ph] = l;
         }
     } else {
+        for (int t = threadIdx.x; t < nk * na; t += blockDim.x) Ls[t] = (T)Lg[t];
+        // A operand of the forward Legendre MFMA: Sel[row][r] = 1 where
+        // carried plane r has m = msel(row), for K step s4 (r = 4 s4 +
+        // lane/16) and row lane%16.  msel orders the rows so that
+        // accumulator item i of lane group kq holds the m of the forward phi
+        // operand of that lane (m = 4i + kq for the FP32 instruction, whose
+        // D rows are 4kq + i; the FP64 one's rows kq + 4i are already m).
+        const int nks = (nk + 3) >> 2;
+        T *Sel = (T *)(smem + q.off_sel);
+        for (int t = threadIdx.x; t < nks * 64; t += blockDim.x) {
+            const int r = 4 * (t >> 6) + ((t & 63) >> 4), row = t & 15;
+            const int msel = (MODE == 1) ? 4 * (row & 3) + (row >> 2) : row;
+            Sel[t] = (r < nk && mplane[r] == msel) ? (T)1 : (T)0;
+        }
         T *Ss = (T *)(smem + q.off_s);
         for (int t = threadIdx.x; t < NT * 16 * 16; t += blockDim.x) {
             const int ph = t >> 4, mi = t & 15;
@@ -248,19 +368,19 @@ __global__ __launch_bounds__(kQWG) void k_quad_mfma(const double *__restrict__ X
     }
 
     // ---- per wave: ppw points of each group, tw column tiles ----
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int l16 = lane & 15, kq = lane >> 4;
     const int tw = ppw * na / 16;
     const int pw0 = wid * ppw;  // first point of this wave within the group
     const T *W = (const T *)Wv + (size_t)lane * 4;
     T *xs = (T *)(smem + q.off_x) + wid * 16 * q.fm;          // even-l F'
     T *xo = (T *)(smem + q.off_x) + (kQW + wid) * 16 * q.fm;  // odd-l F'
-    // both l parities carried (full basis, uniform): two transform chains;
-    // otherwise the one parity takes the first chain
-    const bool two = ne > 0 && ne < nk;
     const int mr = lane < nk ? mplane[lane] : 0;
-    int pcur = -1;
-    double uacc = 0.0;
+    // the lane's plane in the backward Legendre transform: its L row (the
+    // BF16 mode reads the virtual layout, stride q.kps over the nodes)
+    const T *lrow = MODE == 2 ? Ls + (lane < nk ? quad_vindex(lane, nk, ne) : 0) : Ls + lane * na;
+    const int lstride = MODE == 2 ? q.kps : 1;
+    U uacc = 0;
 
     // One column tile.  Each phi tile's W register tile is refilled with the
     // W of the next tile (Wn: the next tile of this wave, or its first tile in
@@ -268,83 +388,151 @@ __global__ __launch_bounds__(kQWG) void k_quad_mfma(const double *__restrict__ X
     // std::true_type); the scheduling barrier keeps the load there (the
     // compiler would otherwise sink it to the end of the tile), so its latency
     // overlaps the rest of this tile's transforms.
+    // this lane's column 16 n + l16 of the wave: point pc, node ac (advanced
+    // by one tile at the end of each tile, set at each group's start)
+    int pc = 0, ac = 0;
     auto tile = [&](int n, acc_t(&wc)[NT], const T *Wn, auto prefetch) {
-        const int colw = 16 * n + l16;  // column within the wave
-        const int pc = pw0 + colw / na, ac = colw % na;
-        // Legendre forward on MFMA: f(m, col) = sum_r Sel[m][r] L_r(a) c_r(p)
-        // over the carried planes, the B operand (k = r, column l16) being
-        // this lane's own products; item i of lane group kq ends up holding
-        // the m of the forward phi operand (see Sel).  Even-l planes
-        // (r < ne) feed fe, odd-l planes fo.
-        acc_t fe = {0, 0, 0, 0}, fo = {0, 0, 0, 0};
-        for (int s4 = 0; s4 < nks; s4++) {
-            const int r = 4 * s4 + kq;
-            const double prod = r < nk ? (double)Ls[r * na + ac] * cs[pc * nk + r] : 0.0;
-            const T sel = Sel[s4 * 64 + lane];
-            fe = mfma16(sel, (T)(two && r >= ne ? 0.0 : prod), fe);
-            if (two) fo = mfma16(sel, (T)(r >= ne ? prod : 0.0), fo);
-        }
         acc_t ae = {0, 0, 0, 0}, ao = {0, 0, 0, 0};
         if constexpr (MODE == 2) {
-            const short *fh = (const short *)(smem + q.off_s), *fl = fh + 16 * NT * 16;
-            const short *bh = (const short *)(smem + q.off_b), *bl = bh + 16 * q.bp;
-            s4v beh, bel, boh, bol;
-#pragma unroll
-            for (int i = 0; i < 4; i++) {
-                // row 4*kq + i of the 16-row K block (rows >= nm are zero)
-                short h, l;
-                bf16_split(fe[i], h, l);
-                beh[i] = h;
-                bel[i] = l;
-                bf16_split(fo[i], h, l);
-                boh[i] = h;
-                bol[i] = l;
+            // Legendre forward on the BF16 instruction: f(m, col) = sum_k
+            // Sel[m][k] L_k(a) c_k(p) over 32 virtual planes per K step, the
+            // B operand (k = 8 kq + e, column l16) this lane's own products
+            // split into hi + lo (Sel is exact).  Two accumulators per
+            // parity: item i of lane group kq holds m = 8 (kq & 1) + i (A)
+            // and 8 (kq & 1) + 4 + i (B); a step without blocks of a parity
+            // skips that parity's instructions.
+            const float *Lt = (const float *)(smem + q.off_l), *cf = (const float *)(smem + q.off_c);
+            const bf8v *selb = (const bf8v *)(smem + q.off_sel);
+            f4v feA = {0, 0, 0, 0}, feB = {0, 0, 0, 0}, foA = {0, 0, 0, 0}, foB = {0, 0, 0, 0};
+            for (int st = 0; st < vs; st++) {
+                const float *cp = cf + pc * q.kps + 32 * st + 8 * kq, *lp = Lt + ac * q.kps + 32 * st + 8 * kq;
+                const f4v p0 = *(const f4v *)cp * *(const f4v *)lp;
+                const f4v p1 = *(const f4v *)(cp + 4) * *(const f4v *)(lp + 4);
+                bf8v ph, pl;
+                split8(p0, p1, ph, pl);
+                const bf8v *sl = selb + 4 * st * 64 + lane;
+                if (4 * st < nbe) {
+                    feA = mfma_bf16_32(sl[0], ph, feA);
+                    feA = mfma_bf16_32(sl[0], pl, feA);
+                    feB = mfma_bf16_32(sl[64], ph, feB);
+                    feB = mfma_bf16_32(sl[64], pl, feB);
+                }
+                if (two && 4 * st + 4 > nbe) {
+                    foA = mfma_bf16_32(sl[128], ph, foA);
+                    foA = mfma_bf16_32(sl[128], pl, foA);
+                    foB = mfma_bf16_32(sl[192], ph, foB);
+                    foB = mfma_bf16_32(sl[192], pl, foB);
+                }
             }
+            // forward phi operands, k = 8 kq + e <-> m = 8 (kq & 1) + e.  A:
+            // S_hi for k < 16 and S_lo for k >= 16 (rows: phi of the tile);
+            // B1 = F_hi, B2 = F_lo for k < 16 and zero for k >= 16, so that
+            // A B1 + A B2 = S_hi F_hi + S_lo F_hi + S_hi F_lo.
+            const bool lo_half = kq < 2;
+            const bf8v zero8 = bf8_zero();
+            bf8v b1e, b2e, b1o, b2o;
+            split8(feA, feB, b1e, b2e);
+            b2e = lo_half ? b2e : zero8;
+            if (two) {
+                split8(foA, foB, b1o, b2o);
+                b2o = lo_half ? b2o : zero8;
+            }
+            const short *fa = lo_half ? (const short *)(smem + q.off_s) + 8 * kq
+                                      : (const short *)(smem + q.off_s) + 16 * NT * 16 + 8 * (kq - 2);
+            // pairs of tiles: row l16 of tile 2u + sx is phi 32u + 8 (l16 / 4) + 4 sx + l16 % 4
+            const short *fap = fa + (8 * (l16 >> 2) + (l16 & 3)) * 16;
+            const short *bhr = (const short *)(smem + q.off_b) + l16 * q.bp + 8 * kq;
+            const short *blr = bhr + 16 * q.bp;
 #pragma unroll
-            for (int t = 0; t < NT; t++) {
-                const s4v ah = *(const s4v *)(fh + (16 * t + l16) * 16 + 4 * kq);
-                const s4v al = *(const s4v *)(fl + (16 * t + l16) * 16 + 4 * kq);
-                f4v ge = {0.f, 0.f, 0.f, 0.f}, go = {0.f, 0.f, 0.f, 0.f};
-                ge = mfma_bf16(ah, beh, ge);
-                ge = mfma_bf16(ah, bel, ge);
-                ge = mfma_bf16(al, beh, ge);
+            for (int u = 0; u < NT / 2; u++) {
+                f4v he[2], ho[2];
+#pragma unroll
+                for (int sx = 0; sx < 2; sx++) {
+                    const int t = 2 * u + sx;
+                    const bf8v a1 = *(const bf8v *)(fap + (32 * u + 4 * sx) * 16);
+                    f4v ge = {0.f, 0.f, 0.f, 0.f}, go = {0.f, 0.f, 0.f, 0.f};
+                    ge = mfma_bf16_32(a1, b1e, ge);
+                    ge = mfma_bf16_32(a1, b2e, ge);
+                    if (two) {
+                        go = mfma_bf16_32(a1, b1o, go);
+                        go = mfma_bf16_32(a1, b2o, go);
+                    }
+                    he[sx] = ge * wc[t];
+                    ho[sx] = go * wc[t];
+                    if constexpr (decltype(prefetch)::value) {
+                        wc[t] = load_w<MODE>(Wn + (size_t)t * 256);
+                        __builtin_amdgcn_sched_barrier(kKeepLoads);
+                    }
+                }
+                // backward over the pair's 32 phi: A [m = l16][phi 32u + 8 kq + e]
+                const bf8v a3h = *(const bf8v *)(bhr + 32 * u);
+                const bf8v a3l = *(const bf8v *)(blr + 32 * u);
+                bf8v hh, hl;
+                split8(he[0], he[1], hh, hl);
+                ae = mfma_bf16_32(a3h, hh, ae);
+                ae = mfma_bf16_32(a3h, hl, ae);
+                ae = mfma_bf16_32(a3l, hh, ae);
                 if (two) {
-                    go = mfma_bf16(ah, boh, go);
-                    go = mfma_bf16(ah, bol, go);
-                    go = mfma_bf16(al, boh, go);
+                    split8(ho[0], ho[1], hh, hl);
+                    ao = mfma_bf16_32(a3h, hh, ao);
+                    ao = mfma_bf16_32(a3h, hl, ao);
+                    ao = mfma_bf16_32(a3l, hh, ao);
+                }
+            }
+            if constexpr (NT & 1) {
+                // odd last tile, rows in natural order (phi 16t + l16 and
+                // accumulator item i of lane group kq at phi 16t + 4kq + i);
+                // its backward fills half of K (k = 8kq + e, e < 4, at phi
+                // 16t + 4kq + e), the other half multiplying zeros
+                constexpr int t = NT - 1;
+                const bf8v a1 = *(const bf8v *)(fa + (16 * t + l16) * 16);
+                f4v ge = {0.f, 0.f, 0.f, 0.f}, go = {0.f, 0.f, 0.f, 0.f};
+                ge = mfma_bf16_32(a1, b1e, ge);
+                ge = mfma_bf16_32(a1, b2e, ge);
+                if (two) {
+                    go = mfma_bf16_32(a1, b1o, go);
+                    go = mfma_bf16_32(a1, b2o, go);
                 }
                 const f4v he = ge * wc[t], ho = go * wc[t];
                 if constexpr (decltype(prefetch)::value) {
                     wc[t] = load_w<MODE>(Wn + (size_t)t * 256);
                     __builtin_amdgcn_sched_barrier(kKeepLoads);
                 }
-                const s4v a2h = *(const s4v *)(bh + l16 * q.bp + 16 * t + 4 * kq);
-                const s4v a2l = *(const s4v *)(bl + l16 * q.bp + 16 * t + 4 * kq);
-                s4v hh, hl;
-#pragma unroll
-                for (int i = 0; i < 4; i++) {
-                    short h, l;
-                    bf16_split(he[i], h, l);
-                    hh[i] = h;
-                    hl[i] = l;
-                }
-                ae = mfma_bf16(a2h, hh, ae);
-                ae = mfma_bf16(a2h, hl, ae);
-                ae = mfma_bf16(a2l, hh, ae);
+                const f4v z4 = {0.f, 0.f, 0.f, 0.f};
+                const unsigned *bq = (const unsigned *)(smem + q.off_b);
+                // S rows m = l16 at phi 16t + 4kq .. + 3 (8-byte aligned)
+                const int wo = (l16 * q.bp + 16 * t + 4 * kq) >> 1;
+                const u4v h4 = {bq[wo], bq[wo + 1], 0u, 0u};
+                const u4v l4 = {bq[wo + 8 * q.bp], bq[wo + 8 * q.bp + 1], 0u, 0u};
+                const bf8v a3h = __builtin_bit_cast(bf8v, h4), a3l = __builtin_bit_cast(bf8v, l4);
+                bf8v hh, hl;
+                split8(he, z4, hh, hl);
+                ae = mfma_bf16_32(a3h, hh, ae);
+                ae = mfma_bf16_32(a3h, hl, ae);
+                ae = mfma_bf16_32(a3l, hh, ae);
                 if (two) {
-#pragma unroll
-                    for (int i = 0; i < 4; i++) {
-                        short h, l;
-                        bf16_split(ho[i], h, l);
-                        hh[i] = h;
-                        hl[i] = l;
-                    }
-                    ao = mfma_bf16(a2h, hh, ao);
-                    ao = mfma_bf16(a2h, hl, ao);
-                    ao = mfma_bf16(a2l, hh, ao);
+                    split8(ho, z4, hh, hl);
+                    ao = mfma_bf16_32(a3h, hh, ao);
+                    ao = mfma_bf16_32(a3h, hl, ao);
+                    ao = mfma_bf16_32(a3l, hh, ao);
                 }
             }
         } else {
+            // Legendre forward on MFMA: f(m, col) = sum_r Sel[m][r] L_r(a) c_r(p)
+            // over the carried planes, the B operand (k = r, column l16) being
+            // this lane's own products; item i of lane group kq ends up
+            // holding the m of the forward phi operand (see Sel).  Even-l
+            // planes (r < ne) feed fe, odd-l planes fo.
+            const int nks = (nk + 3) >> 2;
+            const T *Sel = (const T *)(smem + q.off_sel);
+            acc_t fe = {0, 0, 0, 0}, fo = {0, 0, 0, 0};
+            for (int s4 = 0; s4 < nks; s4++) {
+                const int r = 4 * s4 + kq;
+                const double prod = r < nk ? (double)Ls[r * na + ac] * cs[pc * nk + r] : 0.0;
+                const T sel = Sel[s4 * 64 + lane];
+                fe = mfma16(sel, (T)(two && r >= ne ? 0.0 : prod), fe);
+                if (two) fo = mfma16(sel, (T)(r >= ne ? prod : 0.0), fo);
+            }
             const T *Ss = (const T *)(smem + q.off_s);
             T bFe[KS], bFo[KS];  // m = 4s + kq of this lane's column
 #pragma unroll
@@ -391,27 +579,26 @@ __global__ __launch_bounds__(kQWG) void k_quad_mfma(const double *__restrict__ X
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         if (lane < nk) {
             const T *xf = (two && lane >= ne) ? xo : xs;
-            // independent products first, then a segmented sum over the
-            // points the 16 columns belong to
+            // independent products first, then the sums over each point's
+            // nodes in order; column c of the tile is node (16n + c) % na of
+            // point pw0 + (16n + c) / na (wave-uniform), and a point's sum is
+            // stored at its last node (it may begin in an earlier tile)
             const int a0 = (16 * n) % na;
-            double v[16];
+            U v[16];
             int aa = a0;
 #pragma unroll
             for (int c = 0; c < 16; c++) {
-                v[c] = (double)Ls[lane * na + aa] * (double)xf[c * q.fm + mr];
+                v[c] = (U)lrow[aa * lstride] * (U)xf[c * q.fm + mr];
                 if (++aa == na) aa = 0;
             }
             int pp = pw0 + (16 * n) / na;
             aa = a0;
 #pragma unroll
             for (int c = 0; c < 16; c++) {
-                if (pp != pcur) {
-                    if (pcur >= 0) us[pcur * nk + lane] = uacc;
-                    pcur = pp;
-                    uacc = 0.0;
-                }
                 uacc += v[c];
                 if (++aa == na) {
+                    us[pp * nk + lane] = (double)uacc;
+                    uacc = 0;
                     aa = 0;
                     pp++;
                 }
@@ -420,6 +607,11 @@ __global__ __launch_bounds__(kQWG) void k_quad_mfma(const double *__restrict__ X
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        ac += 16;
+        while (ac >= na) {
+            ac -= na;
+            pc++;
+        }
     };
     // The workgroup is persistent: it takes the groups blockIdx.x,
     // blockIdx.x + gridDim.x, ... (npts points each); the tables above are
@@ -438,11 +630,16 @@ __global__ __launch_bounds__(kQWG) void k_quad_mfma(const double *__restrict__ X
         const bool more = g + gridDim.x < ngroups;  // uniform over the workgroup
         for (int t = threadIdx.x; t < nk * npts; t += blockDim.x) {
             const int r = t / npts, p = t - r * npts;
-            cs[p * nk + r] = p0 + p < P ? X[(size_t)r * P + p0 + p] : 0.0;
+            const double x = p0 + p < P ? X[(size_t)r * P + p0 + p] : 0.0;
+            if constexpr (MODE == 2)
+                ((float *)(smem + q.off_c))[p * q.kps + quad_vindex(r, nk, ne)] = (float)x;
+            else
+                cs[p * nk + r] = x;
         }
         __syncthreads();
-        pcur = -1;
-        uacc = 0.0;
+        uacc = 0;
+        pc = pw0 + l16 / na;
+        ac = l16 % na;
         // one call site: the successor of the group's last tile is the wave's
         // first tile of the next group, or (none left) the tile itself
         const int64_t next0 = more ? (g + gridDim.x) * kQW * tw + (int64_t)wid * tw : tile0 + tw - 1;
@@ -450,7 +647,6 @@ __global__ __launch_bounds__(kQWG) void k_quad_mfma(const double *__restrict__ X
             const int64_t nt = n + 1 < tw ? tile0 + n + 1 : next0;
             tile(n, wb, W + (size_t)(nt * NT) * 256, std::true_type());
         }
-        if (lane < nk && pcur >= 0) us[pcur * nk + lane] = uacc;
         __syncthreads();
 
         // ---- R update and the partials of v.R, |R|^2, |v|^2 ----
@@ -460,12 +656,12 @@ __global__ __launch_bounds__(kQWG) void k_quad_mfma(const double *__restrict__ X
             const size_t o = (size_t)r * P + p0 + p;
             const double nr = R[o] + mask[r] * us[p * nk + r];
             R[o] = nr;
-            const double v = cs[p * nk + r];
+            const double v = MODE == 2 ? X[o] : cs[p * nk + r];
             dot += v * nr;
             rr2 += nr * nr;
             vv2 += v * v;
         }
-        __syncthreads();  // cs and us are rewritten by the next group
+        __syncthreads();  // c and us are rewritten by the next group
     }
     const double s0 = q_block_sum(dot, red);
     const double s1 = q_block_sum(rr2, red);
@@ -479,11 +675,13 @@ __global__ __launch_bounds__(kQWG) void k_quad_mfma(const double *__restrict__ X
 
 // W[tile][t][lane][i] = f_a w_a w_b vg[p][half*na*nb + a*nb + b] for column
 // col = 16*tile + lane%16 = p*nah + a (the hemisphere's nodes a < nah =
-// ceil(na/2)) and phi = 16*t + acc_row(lane/16, i) = half*nb + b; f_a = 2,
-// or 1 on the equator row of an odd na (hemisphere fold); zero outside the
-// grid (padding columns and phi).  FP64 grids feed the FP64 instruction,
-// FP32 grids the FP32 and BF16 ones.
-template <typename WT>
+// ceil(na/2)) and phi = 16*t + acc_row(lane/16, i) = half*nb + b, or for the
+// BF16 pairs of tiles (PAIR, all tiles but the last of an odd count)
+// phi = 32 (t/2) + 8 (lane/16) + 4 (t%2) + i;
+// f_a = 2, or 1 on the equator row of an odd na (hemisphere fold); zero
+// outside the grid (padding columns and phi).  FP64 grids feed the FP64
+// instruction, FP32 grids the FP32 and BF16 ones.
+template <typename WT, bool PAIR>
 __global__ void k_weight_grid(const double *__restrict__ vg, WT *__restrict__ W, const double *__restrict__ wt,
                               const double *__restrict__ wp, int64_t P, int na, int nb, int64_t ntiles, int ntp)
 {
@@ -497,7 +695,8 @@ __global__ void k_weight_grid(const double *__restrict__ vg, WT *__restrict__ W,
         const int t = (int)(rest % ntp);
         const int64_t tile = rest / ntp;
         const int64_t col = 16 * tile + (lane & 15);
-        const int ph = 16 * t + acc_row<sizeof(WT) == 8>(lane >> 4, i);
+        const int ph = PAIR && t < (ntp & ~1) ? 32 * (t >> 1) + 8 * (lane >> 4) + 4 * (t & 1) + i
+                                              : 16 * t + acc_row<sizeof(WT) == 8>(lane >> 4, i);
         const int64_t p = col / nah;
         const int a = (int)(col - p * nah);
         double v = 0.0;
@@ -527,7 +726,7 @@ int quad_points_per_group(int na, int nk)
     const int unit = 16 / g, tiles_per_unit = na / g;
     int k = (4 + tiles_per_unit - 1) / tiles_per_unit;
     while (unit * k < 4) k++;
-    while (k > 1 && quad_lds(nk, na, kQW * unit * k, 16, 0, 256).bytes > 65536) k--;
+    while (k > 1 && quad_lds(nk, 0, na, kQW * unit * k, 16, 0, 256).bytes > 65536) k--;
     return kQW * unit * k;
 }
 
@@ -537,8 +736,8 @@ int64_t quad_tiles(int64_t P, int na, int npts)
     return groups * (int64_t)npts * na / 16;
 }
 
-int launch_weight_grid(const double *vg, void *W, int wprec, const double *wt, const double *wp, int64_t P,
-                       int na, int nb, int npts, hipStream_t s)
+int launch_weight_grid(const double *vg, void *W, int wprec, int mode, const double *wt, const double *wp,
+                       int64_t P, int na, int nb, int npts, hipStream_t s)
 {
     const int64_t ntiles = quad_tiles(P, (na + 1) / 2, npts);
     const int ntp = quad_phi_tiles(2 * nb);
@@ -546,11 +745,14 @@ int launch_weight_grid(const double *vg, void *W, int wprec, const double *wt, c
     int64_t g = (total + 255) / 256;
     if (g > 65536) g = 65536;
     if (wprec == BSL_PREC_FP64)
-        hipLaunchKernelGGL(k_weight_grid<double>, dim3((unsigned)g), dim3(256), 0, s, vg, (double *)W, wt, wp, P,
-                           na, nb, ntiles, ntp);
+        hipLaunchKernelGGL((k_weight_grid<double, false>), dim3((unsigned)g), dim3(256), 0, s, vg, (double *)W, wt,
+                           wp, P, na, nb, ntiles, ntp);
+    else if (mode == 2)
+        hipLaunchKernelGGL((k_weight_grid<float, true>), dim3((unsigned)g), dim3(256), 0, s, vg, (float *)W, wt, wp,
+                           P, na, nb, ntiles, ntp);
     else
-        hipLaunchKernelGGL(k_weight_grid<float>, dim3((unsigned)g), dim3(256), 0, s, vg, (float *)W, wt, wp, P,
-                           na, nb, ntiles, ntp);
+        hipLaunchKernelGGL((k_weight_grid<float, false>), dim3((unsigned)g), dim3(256), 0, s, vg, (float *)W, wt,
+                           wp, P, na, nb, ntiles, ntp);
     BSL_HIP(hipGetLastError());
     return BSL_OK;
 }
@@ -616,7 +818,7 @@ static int quad_dispatch(const DevOperator &op, const double *X, double *R, Scal
     if (!occ && (grid == 0 || (int64_t)grid > op.npartial || (int64_t)grid != op.nalpha))
         return fail(BSL_E_STATE, "partial buffer does not match the quadrature kernel grid");
     const int kf = quad_kf(op.nm, op.qmode), ntp = quad_phi_tiles(op.nphi);
-    const size_t lds = quad_lds(op.nk, op.na, npts, kf, op.qmode, 16 * ntp).bytes;
+    const size_t lds = quad_lds(op.nk, op.q_ne, op.na, npts, kf, op.qmode, 16 * ntp).bytes;
     if (op.nm > 16 || op.nk > 64 || op.nphi > 256 || npts % kQW || (ppw * op.na) % 16 || lds > 65536)
         return fail(BSL_E_ARG, "quadrature kernel: unsupported sizes");
     if (op.qmode == 0)

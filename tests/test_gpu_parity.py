@@ -423,6 +423,19 @@ def test_quad_ragged_tiles(na, nb, sym, basis):
     assert _rel(c.hv(v), o.hv(v)) < 1e-12
 
 
+@pytest.mark.parametrize("na,nb,sym,basis", [(5, 7, "Even", "reduced"), (3, 20, "Odd", "full"),
+                                             (6, 100, "All", "full"), (4, 128, "All", "reduced")])
+def test_quad_bf16x3_phi_tile_pairs(na, nb, sym, basis):
+    """The BF16 instruction takes phi tiles in pairs: 14, 40, 200 and 256 phi
+    nodes fill 2, 4, 16 and 16 tiles (zero padded), with one or both l
+    parities carried."""
+    spec, kw = cs.cage_case(4, 3, na, nb, sym)
+    o = O.Oracle(spec)
+    c = B.Calculator(calculator="quad", precision="bf16x3", basis=basis, **kw)
+    v = np.random.default_rng(25).uniform(-1, 1, o.N)
+    assert _rel(c.hv(v), o.hv(v)) < QUAD_TOL["bf16x3"]
+
+
 def test_quad_bf16x3_levels_within_target():
     """BF16x3 split products on MFMA: converged levels within 0.01 cm^-1 of
     the FP64 NoQuad operator's."""

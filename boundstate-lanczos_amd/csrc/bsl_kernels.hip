@@ -186,6 +186,13 @@ constexpr int kSlabLd = 65;
 // of the (plane-major) tile sequence instead.  A bijection of the grid (the
 // identity when its size is not a multiple of 8).
 constexpr int kXcds = 8;
+// the same for a one-dimensional grid of any size: XCD x (ids = x mod 8)
+// takes the contiguous range starting at x * (total / 8) + min(x, total % 8)
+__device__ inline int xcd_linear(int id, int total)
+{
+    const int q = total / kXcds, r = total % kXcds, x = id % kXcds;
+    return x * q + (x < r ? x : r) + id / kXcds;
+}
 __device__ inline void xcd_tile(int &tile, int &plane)
 {
     const int nt = (int)gridDim.x, total = nt * (int)gridDim.y;
@@ -698,9 +705,10 @@ __global__ __launch_bounds__((LB1 >= 0 ? 2 : 1) * kWG) __attribute__((amdgpu_wav
     }
 }
 
-// One sub-list of a block for the wave's 64 points: the x entries and
-// coefficient columns it touches are loaded, its terms accumulated, and the
-// rows it touches written to part[row][lane] (LDS).  Every index is a
+// One sub-list of a block for the wave's 64 points: the x entries (from the
+// workgroup's copy of its X rows in LDS) and coefficient columns it touches
+// are loaded, its terms accumulated, and the rows it touches added to
+// part[row][lane] (LDS).  Every index is a
 // compile-time constant (index-sequence expansions), so x, acc and h stay in
 // registers.  Lanes past the last point load the last point (in bounds,
 // unconditional loads issue back to back) and are never written back.
@@ -722,10 +730,14 @@ __device__ __forceinline__ void sub_load_h(double (&h)[Pat::NC], const VT *__res
     else
         h[C] = 0.0;
 }
+// Every row takes exactly one same-type and one cross partial, added onto
+// zero: two adds give a + b whatever their order, so the LDS atomic keeps
+// the result deterministic.
 template <class Pat, int R_>
 __device__ __forceinline__ void sub_store(const double (&acc)[Pat::B], double (*part)[64], int lane)
 {
-    if constexpr (Pat::list.row[R_]) part[R_][lane] = acc[R_];
+    if constexpr (Pat::list.row[R_])
+        __hip_atomic_fetch_add(&part[R_][lane], acc[R_], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 template <class Pat, typename VT, int... Rs, int... Cs>
 __device__ __forceinline__ void sub_run(const double *__restrict__ xp, const VT *__restrict__ hp,
@@ -759,13 +771,37 @@ __device__ __forceinline__ void sub_run2(const double *__restrict__ xpa, const d
     (sub_store<PB, RBs>(accb, partb, lane), ...);
 }
 
+// xs: the LDS rows [plane][64] of the workgroup's points
 template <class Pat, typename VT>
-__device__ __forceinline__ void gaunt_sublist(const double *__restrict__ X, const VT *__restrict__ hp,
-                                              const double *__restrict__ G, int plane0, int64_t P, int p,
-                                              double (*part)[64], int lane)
+__device__ __forceinline__ void gaunt_sublist(const double (*xs)[64], const VT *__restrict__ hp,
+                                              const double *__restrict__ G, int plane0, double (*part)[64], int lane)
 {
-    sub_run<Pat, VT>(X + (size_t)plane0 * P + p, hp, G, P, part, lane, std::make_integer_sequence<int, Pat::B>{},
+    sub_run<Pat, VT>(&xs[plane0][lane], hp, G, 64, part, lane, std::make_integer_sequence<int, Pat::B>{},
                      std::make_integer_sequence<int, Pat::NC>{});
+}
+
+// The workgroup's X rows [0, NR) at points group*64 + [0, 64) into xs (the
+// tail clamped to the last point, never written back) and part zeroed; NT
+// threads, NR*64 a multiple of NT so that every load issues before the stores.
+template <int NR, int NT>
+__device__ __forceinline__ void stage_rows(const double *__restrict__ X, int64_t P, int group, double (*xs)[64],
+                                           double (*part)[64])
+{
+    static_assert((NR * 64) % NT == 0, "whole rounds of loads");
+    constexpr int K = NR * 64 / NT;
+    double v[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        const int t = (int)threadIdx.x + k * NT, r = t >> 6, l = t & 63;
+        const int64_t q = (int64_t)group * 64 + l < P ? (int64_t)group * 64 + l : P - 1;
+        v[k] = X[(size_t)r * P + q];
+    }
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        const int t = (int)threadIdx.x + k * NT, r = t >> 6, l = t & 63;
+        xs[r][l] = v[k];
+        part[r][l] = 0.0;
+    }
 }
 
 // The lmax-5 pair of blocks (l = 0, 2, 4 and l = 1, 3, 5; 15 + 21 planes,
@@ -787,7 +823,8 @@ __device__ __forceinline__ void vgaunt_split_body(int group, const double *__res
     static_assert(NR % 4 == 0, "rows are shared out evenly over the four waves");
     static_assert(gaunt::split_covers_rows<4, 0>() && gaunt::split_covers_rows<5, 1>(),
                   "every row needs a same-type and a cross partial");
-    __shared__ double part[2][NR][64];  // [same-type | cross][even rows, then odd rows][lane]
+    __shared__ double xs[NR][64];    // X rows of the 64 points (even rows, then odd rows)
+    __shared__ double part[NR][64];  // same-type + cross partials [row][lane]
     __shared__ double red[4];
     const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
     const int lane = (int)threadIdx.x & 63;
@@ -795,21 +832,23 @@ __device__ __forceinline__ void vgaunt_split_body(int group, const double *__res
     const bool live = p < P;
     const int pl = live ? p : (int)(P - 1);
     const VT *hp = H + (size_t)(pl / kVTile) * nc * kVTile + pl % kVTile;
+    stage_rows<NR, 256>(X, P, group, xs, part);
+    __syncthreads();
     if (wave == 0) {
-        gaunt_sublist<gaunt::Pattern<5, 1, gaunt::kCross>, VT>(X, hp, G1, plane1, P, pl, &part[1][B0], lane);
+        gaunt_sublist<gaunt::Pattern<5, 1, gaunt::kCross>, VT>(xs, hp, G1, plane1, &part[B0], lane);
     } else if (wave == 1) {
-        gaunt_sublist<gaunt::Pattern<5, 1, gaunt::kCC>, VT>(X, hp, G1, plane1, P, pl, &part[0][B0], lane);
+        gaunt_sublist<gaunt::Pattern<5, 1, gaunt::kCC>, VT>(xs, hp, G1, plane1, &part[B0], lane);
     } else if (wave == 2) {
         // the even sin-sin columns (M >= 0, L <= 8) are among the odd ones
         using PA = gaunt::Pattern<5, 1, gaunt::kSS>;
         using PB = gaunt::Pattern<4, 0, gaunt::kSS>;
         static_assert(gaunt::ncoef(4) <= gaunt::ncoef(5), "column prefix");
-        sub_run2<PA, PB, VT>(X + (size_t)plane1 * P + pl, X + pl, hp, G1, G0, P, &part[0][B0], &part[0][0], lane,
+        sub_run2<PA, PB, VT>(&xs[plane1][lane], &xs[0][lane], hp, G1, G0, 64, &part[B0], &part[0], lane,
                              std::make_integer_sequence<int, PA::B>{}, std::make_integer_sequence<int, PB::B>{},
                              std::make_integer_sequence<int, PA::NC>{});
     } else {
-        gaunt_sublist<gaunt::Pattern<4, 0, gaunt::kCC>, VT>(X, hp, G0, 0, P, pl, &part[0][0], lane);
-        gaunt_sublist<gaunt::Pattern<4, 0, gaunt::kCross>, VT>(X, hp, G0, 0, P, pl, &part[1][0], lane);
+        gaunt_sublist<gaunt::Pattern<4, 0, gaunt::kCC>, VT>(xs, hp, G0, 0, &part[0], lane);
+        gaunt_sublist<gaunt::Pattern<4, 0, gaunt::kCross>, VT>(xs, hp, G0, 0, &part[0], lane);
     }
     __syncthreads();
     // rows [9 w, 9 w + 9) of (even rows | odd rows); planes are contiguous
@@ -821,13 +860,13 @@ __device__ __forceinline__ void vgaunt_split_body(int group, const double *__res
 #pragma unroll
         for (int k = 0; k < NW; k++) {
             rr[k] = R[(size_t)(r0 + k) * P + p];
-            xx[k] = X[(size_t)(r0 + k) * P + p];
+            xx[k] = xs[r0 + k][lane];
         }
 #pragma unroll
         for (int k = 0; k < NW; k++) {
             const int r = r0 + k;
             const double mk = r < B0 ? mask0 : mask1;
-            const double nr = rr[k] + mk * (part[0][r][lane] + part[1][r][lane]);
+            const double nr = rr[k] + mk * part[r][lane];
             R[(size_t)r * P + p] = nr;
             dot += xx[k] * nr;
             rr2 += nr * nr;
@@ -852,7 +891,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(W, 8))) voi
     const double *__restrict__ G1, int nc, int plane1, double mask0, double mask1, int64_t P,
     double *__restrict__ partial, double *__restrict__ partial_b, double *__restrict__ partial_c)
 {
-    vgaunt_split_body<VT>(blockIdx.x, X, R, H, G0, G1, nc, plane1, mask0, mask1, P, partial, partial_b, partial_c);
+    vgaunt_split_body<VT>(xcd_linear((int)blockIdx.x, (int)gridDim.x), X, R, H, G0, G1, nc, plane1, mask0, mask1,
+                          P, partial, partial_b, partial_c);
 }
 
 // One parity block of the reduced basis (para l = 0, 2, 4 or ortho
@@ -867,7 +907,8 @@ __device__ __forceinline__ void vgaunt_split1_body(int group, const double *__re
 {
     constexpr int B = gaunt::block_size(LB, PAR), NW = (B + 2) / 3;
     static_assert(gaunt::split_covers_rows<LB, PAR>(), "every row needs a same-type and a cross partial");
-    __shared__ double part[2][B][64];  // [same-type | cross][row][lane]
+    __shared__ double xs[B][64];    // X rows of the 64 points
+    __shared__ double part[B][64];  // same-type + cross partials [row][lane]
     __shared__ double red[3];
     const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
     const int lane = (int)threadIdx.x & 63;
@@ -875,12 +916,14 @@ __device__ __forceinline__ void vgaunt_split1_body(int group, const double *__re
     const bool live = p < P;
     const int pl = live ? p : (int)(P - 1);
     const VT *hp = H + (size_t)(pl / kVTile) * nc * kVTile + pl % kVTile;
+    stage_rows<B, 192>(X, P, group, xs, part);
+    __syncthreads();
     if (wave == 0)
-        gaunt_sublist<gaunt::Pattern<LB, PAR, gaunt::kCross>, VT>(X, hp, G0, 0, P, pl, &part[1][0], lane);
+        gaunt_sublist<gaunt::Pattern<LB, PAR, gaunt::kCross>, VT>(xs, hp, G0, 0, &part[0], lane);
     else if (wave == 1)
-        gaunt_sublist<gaunt::Pattern<LB, PAR, gaunt::kCC>, VT>(X, hp, G0, 0, P, pl, &part[0][0], lane);
+        gaunt_sublist<gaunt::Pattern<LB, PAR, gaunt::kCC>, VT>(xs, hp, G0, 0, &part[0], lane);
     else
-        gaunt_sublist<gaunt::Pattern<LB, PAR, gaunt::kSS>, VT>(X, hp, G0, 0, P, pl, &part[0][0], lane);
+        gaunt_sublist<gaunt::Pattern<LB, PAR, gaunt::kSS>, VT>(xs, hp, G0, 0, &part[0], lane);
     __syncthreads();
     double dot = 0.0, rr2 = 0.0, vv2 = 0.0;
     if (live) {
@@ -890,13 +933,13 @@ __device__ __forceinline__ void vgaunt_split1_body(int group, const double *__re
         for (int k = 0; k < NW; k++) {
             const int r = r0 + k < B ? r0 + k : B - 1;
             rr[k] = R[(size_t)r * P + p];
-            xx[k] = X[(size_t)r * P + p];
+            xx[k] = xs[r][lane];
         }
 #pragma unroll
         for (int k = 0; k < NW; k++) {
             const int r = r0 + k;
             if (r < B) {
-                const double nr = rr[k] + mask0 * (part[0][r][lane] + part[1][r][lane]);
+                const double nr = rr[k] + mask0 * part[r][lane];
                 R[(size_t)r * P + p] = nr;
                 dot += xx[k] * nr;
                 rr2 += nr * nr;
@@ -922,7 +965,8 @@ __global__ __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(W, 8))) voi
     int nc, double mask0, int64_t P, double *__restrict__ partial, double *__restrict__ partial_b,
     double *__restrict__ partial_c)
 {
-    vgaunt_split1_body<LB, PAR, VT>(blockIdx.x, X, R, H, G0, nc, mask0, P, partial, partial_b, partial_c);
+    vgaunt_split1_body<LB, PAR, VT>(xcd_linear((int)blockIdx.x, (int)gridDim.x), X, R, H, G0, nc, mask0, P, partial,
+                                    partial_b, partial_c);
 }
 
 // ---------------------------------------------------------------------------

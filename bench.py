@@ -22,6 +22,8 @@ kernel and the reference's CPU path timed on this host on a bounded sample.
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
 import argparse
+import contextlib
+import ctypes
 import json
 import os
 import sys
@@ -33,6 +35,22 @@ REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(REPO, "boundstate-lanczos_amd"))
 
 HBM_PEAK_GBS = 8000.0
+
+
+@contextlib.contextmanager
+def _stdout_to_stderr():
+    """Route file descriptor 1 to stderr (the reference's preparation prints
+    its progress on stdout; stdout carries only the result line)."""
+    sys.stdout.flush()
+    saved = os.dup(1)
+    os.dup2(2, 1)
+    try:
+        yield
+    finally:
+        sys.stdout.flush()
+        ctypes.CDLL(None).fflush(None)
+        os.dup2(saved, 1)
+        os.close(saved)
 
 
 def make_calc(B, args, precision, basis="full", symmetry="All", calculator="noquad", n=None, x_max=None):
@@ -271,7 +289,7 @@ def cpu_baseline(B, calc, args):
     if not O.ref_available() or args.ref_hv_calls <= 0:
         return port
     R = O.ref_lib()
-    with tempfile.TemporaryDirectory() as tmp:
+    with tempfile.TemporaryDirectory() as tmp, _stdout_to_stderr():
         deck = os.path.join(tmp, "hv.txt")
         geo = os.path.join(REPO, "tests", "golden", "sII_smallCage11_CM_EA_ZYZ.xyz")  # prep only: V is set below
         O.write_ref_hv_deck(deck, args.n, args.x_max, 5, 1, 1, mass, rot, geo, 18.0, "All")

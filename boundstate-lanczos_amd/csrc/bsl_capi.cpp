@@ -1372,18 +1372,19 @@ extern "C" int bsl_time_steps(bsl_ctx *c, int32_t warmup, int32_t steps, double 
     float total = 0;
     BSL_HIP(hipEventElapsedTime(&total, e0, e1));
     (void)j0;
-    // per-kernel durations: events around each kernel group, fewer steps
-    std::vector<hipEvent_t> ev((size_t)4 * ksteps);
+    // per-kernel durations over fewer steps: the kernels' own start and end
+    // timestamps of each group (kinetic, potential, finalize)
+    std::vector<hipEvent_t> ev((size_t)6 * ksteps);
     for (auto &e : ev) BSL_HIP(hipEventCreate(&e));
     double kin = 0, pot = 0, upd = 0;
     for (int s = 0; s < ksteps; s++)
-        if ((rc = lanczos_step(c, c->op, j + s, &ev[(size_t)4 * s]))) return rc;
+        if ((rc = lanczos_step(c, c->op, j + s, &ev[(size_t)6 * s]))) return rc;
     BSL_HIP(hipStreamSynchronize(c->stream));
     for (int s = 0; s < ksteps; s++) {
         float a = 0, b = 0, d = 0;
-        BSL_HIP(hipEventElapsedTime(&a, ev[4 * s + 0], ev[4 * s + 1]));
-        BSL_HIP(hipEventElapsedTime(&b, ev[4 * s + 1], ev[4 * s + 2]));
-        BSL_HIP(hipEventElapsedTime(&d, ev[4 * s + 2], ev[4 * s + 3]));
+        BSL_HIP(hipEventElapsedTime(&a, ev[6 * s + 0], ev[6 * s + 1]));
+        BSL_HIP(hipEventElapsedTime(&b, ev[6 * s + 2], ev[6 * s + 3]));
+        BSL_HIP(hipEventElapsedTime(&d, ev[6 * s + 4], ev[6 * s + 5]));
         kin += a;
         pot += b;
         upd += d;

@@ -2,6 +2,7 @@
 // device buffers, kernel launchers).  Not part of the public ABI.
 #pragma once
 
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -176,14 +177,39 @@ struct DevOperator {
     int64_t nalpha = 0;
 };
 
+// Kernel-timestamp timing of a group of launches: while a group is armed,
+// its first launch records `start` at the kernel's own start and every
+// launch records `stop` at its end (hipExtLaunchKernel events), so that the
+// interval is the group's execution as a profiler reports it, without the
+// gaps an ordinary event record adds around it.  Thread-local; disarmed
+// after the group.
+struct LaunchTimer {
+    hipEvent_t start = nullptr, stop = nullptr;
+};
+LaunchTimer &launch_timer();
+template <typename F, typename... Args>
+inline void timed_launch(F kernel, const dim3 &grid, const dim3 &block, uint32_t lds, hipStream_t s, Args... args)
+{
+    LaunchTimer &t = launch_timer();
+    if (!t.stop) {  // not armed: the ordinary launch (also under stream capture)
+        hipLaunchKernelGGL(kernel, grid, block, lds, s, args...);
+        return;
+    }
+    hipEvent_t a = t.start;
+    t.start = nullptr;
+    hipExtLaunchKernelGGL(kernel, grid, block, lds, s, a, t.stop, 0u, args...);
+}
+
 // one Hv: v = ca X + cb R is formed by the x-y kinetic kernel and written
 // to X; R = cr R + mask*(H v); the potential kernel leaves the nalpha
-// partial sums of v.R, |R|^2, |v|^2 in op.partial, partial_b, partial_c
+// partial sums of v.R, |R|^2, |v|^2 in op.partial, partial_b, partial_c.
+// ev (or null): the start/stop events of the kinetic and of the potential
+// launches (ev[0..1], ev[2..3])
 int launch_hv(const DevOperator &op, double *X, double *R, Scal *sc, hipStream_t s,
-              hipEvent_t *ev /* 3 events or null */);
-// one Lanczos step: launch_hv, then launch_finalize (event 4 after it)
+              hipEvent_t *ev /* 4 events or null */);
+// one Lanczos step: launch_hv, then launch_finalize (its start/stop in ev[4..5])
 int launch_step(const DevOperator &op, double *X, double *R, Scal *sc, double *alpha_out, double *beta_out,
-                double *beta2_out, hipStream_t s, hipEvent_t *ev /* 4 events or null */);
+                double *beta2_out, hipStream_t s, hipEvent_t *ev /* 6 events or null */);
 // the potential kernel alone (used by the autotuner)
 int launch_potential(const DevOperator &op, const double *X, double *R, Scal *sc, hipStream_t s);
 // alpha = v.R, beta^2 = |R|^2 - 2 alpha v.R + alpha^2 |v|^2 from the

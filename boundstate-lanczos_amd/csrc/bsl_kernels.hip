@@ -485,12 +485,12 @@ static int launch_kinetic_t(const DevOperator &op, double *X, double *R, Scal *s
 {
     // the half-size (centrosymmetric) contractions, for every axis length
     const size_t lds_xy = sizeof(T) * ((size_t)4 * 32 * kCompLd + 2 * 129);
-    hipLaunchKernelGGL((k_kin_xy_half<T, KM>), dim3(op.nz, op.nk), dim3(kWG), lds_xy, s, X, R, op.tx, op.ty, op.rot,
+    timed_launch((k_kin_xy_half<T, KM>), dim3(op.nz, op.nk), dim3(kWG), lds_xy, s, X, R, op.tx, op.ty, op.rot,
                        op.mask, sc, op.nx, op.ny, op.nz);
     BSL_HIP(hipGetLastError());
     const int nxy = op.nx * op.ny;
     const size_t lds_z = sizeof(T) * ((size_t)2 * 32 * kSlabLd + 129);
-    hipLaunchKernelGGL((k_kin_z_half<T, KM>), dim3((nxy + 63) / 64, op.nk), dim3(kWG), lds_z, s, X, R, op.tz, op.mask,
+    timed_launch((k_kin_z_half<T, KM>), dim3((nxy + 63) / 64, op.nk), dim3(kWG), lds_z, s, X, R, op.tz, op.mask,
                        sc, nxy, op.nz);
     BSL_HIP(hipGetLastError());
     return BSL_OK;
@@ -1114,15 +1114,15 @@ static int launch_vblock_t(const DevOperator &op, const double *X, double *R, hi
     if (b0 == A && b1 == Bb) {                                                                         \
         switch (op.vb_variant) {                                                                       \
         case 0:                                                                                        \
-            hipLaunchKernelGGL((k_vblock<A, Bb, VT, 1>), grid, dim3(kWG), 0, s, X, R, V0, V1, mk0, mk1,     \
+            timed_launch((k_vblock<A, Bb, VT, 1>), grid, dim3(kWG), 0, s, X, R, V0, V1, mk0, mk1,     \
                                op.P, op.partial, op.partial_b, op.partial_c);                          \
             break;                                                                                     \
         case 1:                                                                                        \
-            hipLaunchKernelGGL((k_vblock<A, Bb, VT, 2>), grid, dim3(kWG), 0, s, X, R, V0, V1, mk0, mk1,     \
+            timed_launch((k_vblock<A, Bb, VT, 2>), grid, dim3(kWG), 0, s, X, R, V0, V1, mk0, mk1,     \
                                op.P, op.partial, op.partial_b, op.partial_c);                          \
             break;                                                                                     \
         default:                                                                                       \
-            hipLaunchKernelGGL((k_vblock<A, Bb, VT, 3>), grid, dim3(kWG), 0, s, X, R, V0, V1, mk0, mk1,     \
+            timed_launch((k_vblock<A, Bb, VT, 3>), grid, dim3(kWG), 0, s, X, R, V0, V1, mk0, mk1,     \
                                op.P, op.partial, op.partial_b, op.partial_c);                          \
             break;                                                                                     \
         }                                                                                              \
@@ -1150,10 +1150,10 @@ static int launch_vgaunt_t(const DevOperator &op, const double *X, double *R, hi
 #define BSL_VG1(A, PA)                                                                                      \
     if (l0 == A && p0 == PA) {                                                                              \
         if (op.vb_variant == 0)                                                                             \
-            hipLaunchKernelGGL((k_vgaunt_split1<A, PA, VT, 3>), grid, dim3(192), 0, s, X, R, H, op.gc[0],       \
+            timed_launch((k_vgaunt_split1<A, PA, VT, 3>), grid, dim3(192), 0, s, X, R, H, op.gc[0],       \
                                op.ncoef, b0.mask, op.P, op.partial, op.partial_b, op.partial_c);            \
         else                                                                                                \
-            hipLaunchKernelGGL((k_vgaunt_split1<A, PA, VT, 4>), grid, dim3(192), 0, s, X, R, H, op.gc[0],       \
+            timed_launch((k_vgaunt_split1<A, PA, VT, 4>), grid, dim3(192), 0, s, X, R, H, op.gc[0],       \
                                op.ncoef, b0.mask, op.P, op.partial, op.partial_b, op.partial_c);            \
         return hipGetLastError() == hipSuccess ? BSL_OK : fail(BSL_E_HIP, "k_vgaunt_split1 launch");       \
     }
@@ -1164,10 +1164,10 @@ static int launch_vgaunt_t(const DevOperator &op, const double *X, double *R, hi
     if (split == 1) {
         if (op.ncoef != gaunt::ncoef(5)) return fail(BSL_E_STATE, "coefficient columns do not match the blocks");
         if (op.vb_variant == 0)
-            hipLaunchKernelGGL((k_vgaunt_split<VT, 3>), grid, dim3(256), 0, s, X, R, H, op.gc[0], op.gc[1],
+            timed_launch((k_vgaunt_split<VT, 3>), grid, dim3(256), 0, s, X, R, H, op.gc[0], op.gc[1],
                                op.ncoef, b1.plane0, b0.mask, mk1, op.P, op.partial, op.partial_b, op.partial_c);
         else
-            hipLaunchKernelGGL((k_vgaunt_split<VT, 4>), grid, dim3(256), 0, s, X, R, H, op.gc[0], op.gc[1],
+            timed_launch((k_vgaunt_split<VT, 4>), grid, dim3(256), 0, s, X, R, H, op.gc[0], op.gc[1],
                                op.ncoef, b1.plane0, b0.mask, mk1, op.P, op.partial, op.partial_b, op.partial_c);
         return hipGetLastError() == hipSuccess ? BSL_OK : fail(BSL_E_HIP, "k_vgaunt_split launch");
     }
@@ -1175,11 +1175,11 @@ static int launch_vgaunt_t(const DevOperator &op, const double *X, double *R, hi
 #define BSL_VG(A, PA, Bb, PB)                                                                              \
     if (l0 == A && p0 == PA && l1 == Bb && (Bb < 0 || p1 == PB)) {                                         \
         if (op.vb_variant == 0)                                                                            \
-            hipLaunchKernelGGL((k_vgaunt<A, PA, Bb, PB, VT, 1>), grid, threads, 0, s, X, R, H, op.gc[0],       \
+            timed_launch((k_vgaunt<A, PA, Bb, PB, VT, 1>), grid, threads, 0, s, X, R, H, op.gc[0],       \
                                op.gc[1], op.ncoef, b1.plane0, b0.mask, mk1, op.P, op.partial, op.partial_b,    \
                                op.partial_c);                                                              \
         else                                                                                               \
-            hipLaunchKernelGGL((k_vgaunt<A, PA, Bb, PB, VT, 2>), grid, threads, 0, s, X, R, H, op.gc[0],       \
+            timed_launch((k_vgaunt<A, PA, Bb, PB, VT, 2>), grid, threads, 0, s, X, R, H, op.gc[0],       \
                                op.gc[1], op.ncoef, b1.plane0, b0.mask, mk1, op.P, op.partial, op.partial_b,    \
                                op.partial_c);                                                              \
         return hipGetLastError() == hipSuccess ? BSL_OK : fail(BSL_E_HIP, "k_vgaunt launch");             \
@@ -1312,26 +1312,31 @@ int launch_potential(const DevOperator &op, const double *X, double *R, Scal *sc
     return op.vprec == BSL_PREC_FP64 ? launch_vblock_t<double>(op, X, R, s) : launch_vblock_t<float>(op, X, R, s);
 }
 
+LaunchTimer &launch_timer()
+{
+    static thread_local LaunchTimer t;
+    return t;
+}
+
 int launch_hv(const DevOperator &op, double *X, double *R, Scal *sc, hipStream_t s, hipEvent_t *ev)
 {
-    if (ev) BSL_HIP(hipEventRecord(ev[0], s));
     if (op.nx > 64 || op.ny > 64 || op.nz > 64) return fail(BSL_E_ARG, "grid axes above 64 points are not supported");
+    LaunchTimer &t = launch_timer();
+    if (ev) t = {ev[0], ev[1]};
     int krc = op.kprec == BSL_PREC_FP64   ? launch_kinetic_t<double, 0>(op, X, R, sc, s)
               : op.kprec == BSL_PREC_FP32 ? launch_kinetic_t<float, 0>(op, X, R, sc, s)
                                           : launch_kinetic_t<float, 1>(op, X, R, sc, s);
-    if (krc) return krc;
-    if (ev) BSL_HIP(hipEventRecord(ev[1], s));
-    int rc = launch_potential(op, X, R, sc, s);
-    if (rc) return rc;
-    if (ev) BSL_HIP(hipEventRecord(ev[2], s));
-    return BSL_OK;
+    if (ev) t = {ev[2], ev[3]};
+    int rc = krc ? krc : launch_potential(op, X, R, sc, s);
+    t = {};
+    return rc;
 }
 
 int launch_finalize(const DevOperator &op, Scal *sc, double *alpha_out, double *beta_out, double *beta2_out,
                     hipStream_t s)
 {
-    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kWG), 0, s, op.partial, op.partial_b, op.partial_c,
-                       (unsigned)op.nalpha, sc, 0, alpha_out, beta_out, beta2_out);
+    timed_launch(k_finalize, dim3(1), dim3(kWG), 0, s, op.partial, op.partial_b, op.partial_c,
+                 (unsigned)op.nalpha, sc, 0, alpha_out, beta_out, beta2_out);
     BSL_HIP(hipGetLastError());
     return BSL_OK;
 }
@@ -1340,10 +1345,11 @@ int launch_step(const DevOperator &op, double *X, double *R, Scal *sc, double *a
                 double *beta2_out, hipStream_t s, hipEvent_t *ev)
 {
     int rc = launch_hv(op, X, R, sc, s, ev);
-    if (!rc) rc = launch_finalize(op, sc, alpha_out, beta_out, beta2_out, s);
     if (rc) return rc;
-    if (ev) BSL_HIP(hipEventRecord(ev[3], s));
-    return BSL_OK;
+    if (ev) launch_timer() = {ev[4], ev[5]};
+    rc = launch_finalize(op, sc, alpha_out, beta_out, beta2_out, s);
+    launch_timer() = {};
+    return rc;
 }
 
 int check_breakdown(const Scal *sc, hipStream_t s)

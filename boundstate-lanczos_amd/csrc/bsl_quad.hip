@@ -765,7 +765,7 @@ int launch_weight_grid(const double *vg, void *W, int wprec, int mode, const dou
                                                              kQWG, lds) != hipSuccess)                     \
                 *occ = 0;                                                                                  \
         } else {                                                                                           \
-            hipLaunchKernelGGL((k_quad_mfma<M, NTV, KSV>), dim3(grid), dim3(kQWG), lds, s, X, R, op.wgrid,  \
+            timed_launch((k_quad_mfma<M, NTV, KSV>), dim3(grid), dim3(kQWG), (uint32_t)lds, s, X, R, op.wgrid, \
                                op.qL, op.qS, op.mplane, op.mask, sc, op.P, op.nk, op.na, op.q_ne, op.nphi, \
                                op.nm, ppw, ngroups, op.partial, op.partial_b, op.partial_c);               \
         }                                                                                                  \

@@ -271,8 +271,9 @@ int bsl_gaunt_table(int32_t lb, int32_t par, int32_t cap, int32_t *ijc, double *
  * steps (HIP events on the context stream, no host synchronisation inside);
  * ms[1], ms[2], ms[3] = average per-step duration of the potential kernel,
  * the two kinetic kernels and the step finalize (alpha, beta and the scalars
- * of the next step), from events recorded around each of them in a further
- * min(steps, 64) steps. */
+ * of the next step), from the kernels' own start and end timestamps
+ * (hipExtLaunchKernel events: the first launch's start to the last launch's
+ * end of each group) in a further min(steps, 64) steps. */
 int bsl_time_steps(bsl_ctx *ctx, int32_t warmup, int32_t steps, double *ms);
 
 /* Bytes each kernel of one Lanczos step must move at minimum (algorithmic

@@ -75,10 +75,6 @@ def iho_case(nx, lmax, na, nb, x_max, omega_cm1=100.0, mass=2.016, rot=0.7096487
     return spec, kw
 
 
-def ref_to_device_order(vblocks_ref):
-    return vblocks_ref
-
-
 def baseline_cage_case(nx, na=40, nb=70, sym="All", seed=0, x_max=0.4, lmax=5):
     """(oracle spec, calculator kwargs) of the benchmark cage workload: Alavi
     H2 (mass 2.016 amu, B = 59.322 cm^-1) in the synthetic 5^12 SPC/E cage of

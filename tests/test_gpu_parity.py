@@ -25,10 +25,6 @@ def _rel(a, b):
     return np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300)
 
 
-def _ref_to_dev_blocks(vb_ref):
-    return vb_ref
-
-
 @pytest.fixture(scope="module")
 def cage_small():
     spec, kw = cs.cage_case(6, 3, 8, 7, "All")

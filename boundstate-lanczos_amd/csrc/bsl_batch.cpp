@@ -8,7 +8,9 @@
 // launch takes a device table of the members' pointers and shapes, and its
 // workgroups are the members' own grids laid end to end.  A member whose
 // potential has no table kernel (quadrature calculator, the full basis,
-// other block shapes) gets its own potential launch inside the iteration.
+// other block shapes) gets its own potential launch inside the iteration;
+// one with an axis of more than 64 points its own kinetic and potential
+// launches.
 // Two tables are kept, one per step parity (the members' X and R swap roles
 // every step), so two iterations leave every buffer where it began; with
 // graphs on they are captured once and replayed.  No reference counterpart: the reference runs
@@ -61,7 +63,9 @@ void batch_free(bsl_batch *b)
 // of the reduced basis), or -1 if it has no table kernel
 int potential_kind(const DevOperator &op)
 {
-    if (op.calc == BSL_CALC_QUAD || op.vform != BSL_VFORM_HARMONICS) return -1;
+    // (a member whose kinetic term has no table kernel runs all of its
+    // step on its own launches, in order)
+    if (op.calc == BSL_CALC_QUAD || op.vform != BSL_VFORM_HARMONICS || !kinetic_mfma_fits(op)) return -1;
     const Block &b0 = op.blk[0], &b1 = op.nblk > 1 ? op.blk[1] : op.blk[0];
     const int split = harmonic_split(op.nblk, b0.lb, b0.par, op.nblk > 1 ? b1.lb : -1, b1.par, b1.plane0, b0.size);
     if (split == 2) return b0.par == 0 ? kBPotSplitPara : kBPotSplitOrtho;
@@ -136,10 +140,10 @@ int build_tables(bsl_batch *b)
         };
         for (int prec : {BSL_PREC_FP64, BSL_PREC_FP32, BSL_PREC_BF16X3})
             add(kBXyHalf, prec, [](const bsl_ctx *c) { return c->op.nz * c->op.nk; },
-                [&](const bsl_ctx *c) { return c->op.kprec == prec; });
+                [&](const bsl_ctx *c) { return c->op.kprec == prec && kinetic_mfma_fits(c->op); });
         for (int prec : {BSL_PREC_FP64, BSL_PREC_FP32, BSL_PREC_BF16X3})
             add(kBZHalf, prec, [](const bsl_ctx *c) { return ((c->op.nx * c->op.ny + 63) / 64) * c->op.nk; },
-                [&](const bsl_ctx *c) { return c->op.kprec == prec; });
+                [&](const bsl_ctx *c) { return c->op.kprec == prec && kinetic_mfma_fits(c->op); });
         for (int kind : {(int)kBPotSplitPara, (int)kBPotSplitOrtho})
             for (int prec : {BSL_PREC_FP64, BSL_PREC_FP32})
                 add(kind, prec, [](const bsl_ctx *c) { return (int)((c->op.P + 63) / 64); },
@@ -185,6 +189,7 @@ int batch_iteration(bsl_batch *b, int parity)
             for (int i : b->direct) {
                 bsl_ctx *c = b->ctx[i];
                 double *X = parity == 0 ? c->d_X : c->d_R, *R = parity == 0 ? c->d_R : c->d_X;
+                if (!rc && !kinetic_mfma_fits(c->op)) rc = launch_kinetic(c->op, X, R, c->d_scal, b->main);
                 if (!rc) rc = launch_potential(c->op, X, R, c->d_scal, b->main);
             }
         }

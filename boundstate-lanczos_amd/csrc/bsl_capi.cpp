@@ -62,9 +62,9 @@ void dev_free(void *p)
 int check_params(const bsl_params &p)
 {
     if (p.nx < 2 || p.ny < 2 || p.nz < 2) return fail(BSL_E_ARG, "grid needs at least 2 points per axis");
-    if (p.nx > 64 || p.ny > 64 || p.nz > 64) return fail(BSL_E_ARG, "at most 64 grid points per axis");
+    if (p.nx > 256 || p.ny > 256 || p.nz > 256) return fail(BSL_E_ARG, "at most 256 grid points per axis");
     if (!(p.x_max > 0) || !(p.y_max > 0) || !(p.z_max > 0)) return fail(BSL_E_ARG, "box lengths must be positive");
-    if (p.lmax < 0 || p.lmax > 6) return fail(BSL_E_ARG, "lmax must be in 0..6");
+    if (p.lmax < 0 || p.lmax > 12) return fail(BSL_E_ARG, "lmax must be in 0..12");
     if (p.n_theta < 1 || p.n_phi < 1) return fail(BSL_E_ARG, "quadrature sizes must be positive");
     if (!(p.mass > 0)) return fail(BSL_E_ARG, "mass must be positive");
     if (p.symmetry < 0 || p.symmetry > 2) return fail(BSL_E_ARG, "symmetry flag not recognised");
@@ -170,7 +170,10 @@ int build_vpk(bsl_ctx *c, DevOperator &op, int form, int vprec, const double *d_
     if (op.calc != BSL_CALC_QUAD) {
         const Block &b0 = op.blk[0], &b1 = op.blk[op.nblk > 1 ? 1 : 0];
         const int split = harmonic_split(op.nblk, b0.lb, b0.par, op.nblk > 1 ? b1.lb : -1, b1.par, b1.plane0, b0.size);
-        op.nalpha = form == BSL_VFORM_HARMONICS ? harmonic_workgroups(c->P, split) : ntile * op.nblk;
+        const int bs1 = op.nblk > 1 ? b1.size : 0;
+        op.nalpha = form == BSL_VFORM_HARMONICS       ? harmonic_workgroups(c->P, split)
+                    : vblock_has_kernel(b0.size, bs1) ? ntile * op.nblk
+                                                      : ((c->P + 63) / 64) * op.nblk;
         if (op.nalpha > op.npartial) return fail(BSL_E_STATE, "partial buffer too small for the potential kernel");
     }
     return launch_vblock_gemm(d_vg, d_h, op.vpk, vprec, c->P, c->Q, col0, ncol, c->stream);
@@ -535,7 +538,8 @@ extern "C" int bsl_create(const bsl_params *pp, bsl_ctx **out)
     if ((rc = dev_upload(&c->d_gy, gy))) return rc;
     if ((rc = dev_upload(&c->d_gz, gz))) return rc;
     op.nalpha = ((c->P + 255) / 256) * op.nblk;
-    op.npartial = (int)std::max<int64_t>(op.nalpha, 4096);
+    // room for every potential kernel's workgroups (64 points the smallest)
+    op.npartial = (int)std::max<int64_t>(((c->P + 63) / 64) * op.nblk, 4096);
     if ((rc = dev_alloc(&op.partial, (size_t)op.npartial))) return rc;
     if ((rc = dev_alloc(&op.partial_b, (size_t)op.npartial))) return rc;
     if ((rc = dev_alloc(&op.partial_c, (size_t)op.npartial))) return rc;
@@ -1340,6 +1344,8 @@ extern "C" int bsl_potential_kernel(const bsl_ctx *c, char *name, int32_t len)
         std::snprintf(buf, sizeof buf, "k_vgaunt<%d, %d, %d, %d, %s, %d>", op.blk[0].lb, op.blk[0].par,
                       op.nblk > 1 ? op.blk[1].lb : -1, op.nblk > 1 ? op.blk[1].par : 0,
                       op.vprec == BSL_PREC_FP64 ? "double" : "float", op.vb_variant + 1);
+    else if (!vblock_has_kernel(op.blk[0].size, op.nblk > 1 ? op.blk[1].size : 0))
+        std::snprintf(buf, sizeof buf, "k_vblock_rows<%s>", op.vprec == BSL_PREC_FP64 ? "double" : "float");
     else
         std::snprintf(buf, sizeof buf, "k_vblock<%d, %d, %s, %d>", op.blk[0].size, op.nblk > 1 ? op.blk[1].size : 0,
                       op.vprec == BSL_PREC_FP64 ? "double" : "float", op.vb_variant + 1);

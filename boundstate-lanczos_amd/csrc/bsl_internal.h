@@ -212,6 +212,13 @@ int launch_step(const DevOperator &op, double *X, double *R, Scal *sc, double *a
                 double *beta2_out, hipStream_t s, hipEvent_t *ev /* 6 events or null */);
 // the potential kernel alone (used by the autotuner)
 int launch_potential(const DevOperator &op, const double *X, double *R, Scal *sc, hipStream_t s);
+// the kinetic kernels alone (the MFMA contractions where every axis has at
+// most 64 points, else the FP64 line kernels); forms v as launch_hv does
+int launch_kinetic(const DevOperator &op, double *X, double *R, Scal *sc, hipStream_t s);
+bool kinetic_mfma_fits(const DevOperator &op);
+// a compiled register kernel for these parity block sizes (else the
+// row-parallel block kernel, 64-point workgroups)
+bool vblock_has_kernel(int b0, int b1);
 // alpha = v.R, beta^2 = |R|^2 - 2 alpha v.R + alpha^2 |v|^2 from the
 // potential kernel's three partial lists, and the scalars of the next step;
 // alpha_out[step], beta_out[step + 1], beta2_out[step + 1] with step the

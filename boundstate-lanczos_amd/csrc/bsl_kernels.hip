@@ -480,6 +480,76 @@ __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
     kin_z_half_body<T, KM>(tile, n, X, R, tz, mask, nxy, nz);
 }
 
+// grid-stride launches: workgroups of kWG threads for n items, at most cap
+static int grid_for(int64_t n, int cap = 4096)
+{
+    int64_t g = (n + kWG - 1) / kWG;
+    if (g > cap) g = cap;
+    if (g < 1) g = 1;
+    return (int)g;
+}
+
+// ---------------------------------------------------------------------------
+// kinetic term for axes of more than 64 points (FP64, any length): the
+// recurrence prologue of the x-y kernel as a pass of its own, then one
+// thread per (point, plane) summing the three axis contractions along its
+// lines, read through the caches (Tv_5D, linRotCartSphHvRoutines.cpp:1872-1969:
+// R += T_x v + T_y v + T_z v with T_a(i, i') = t_a(|i - i'|)).
+// ---------------------------------------------------------------------------
+// v = ca X + cb R into X; R = cr R + mask B l(l+1) v
+__global__ __launch_bounds__(kWG) void k_kin_form(double *__restrict__ X, double *__restrict__ R,
+                                                  const double *__restrict__ rot, const double *__restrict__ mask,
+                                                  const Scal *__restrict__ sc, int64_t P, int64_t n)
+{
+    const double ca = sc->ca, cb = sc->cb, cr = sc->cr;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const int r = (int)(i / P);
+        const double v = ca * X[i] + cb * R[i];
+        X[i] = v;
+        R[i] = cr * R[i] + mask[r] * (rot[r] * v);
+    }
+}
+
+// R += mask (T_x + T_y + T_z) v; tx, ty, tz: first rows t(d), d < n
+__global__ __launch_bounds__(kWG) void k_kin_lines(const double *__restrict__ X, double *__restrict__ R,
+                                                   const double *__restrict__ tx, const double *__restrict__ ty,
+                                                   const double *__restrict__ tz, const double *__restrict__ mask,
+                                                   int nx, int ny, int nz, int64_t P, int64_t n)
+{
+    const int64_t nxy = (int64_t)nx * ny;
+    for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < n;
+         idx += (int64_t)gridDim.x * blockDim.x) {
+        const int r = (int)(idx / P);
+        const int64_t p = idx - (int64_t)r * P;
+        const int i = (int)(p % nx), j = (int)((p / nx) % ny), k = (int)(p / nxy);
+        const double *v = X + (size_t)r * P;
+        const double *lx = v + (size_t)k * nxy + (size_t)j * nx;
+        const double *ly = v + (size_t)k * nxy + i;
+        const double *lz = v + (size_t)j * nx + i;
+        double sx = 0.0, sy = 0.0, sz = 0.0;
+        for (int a = 0; a < nx; a++) sx = fma(tx[a > i ? a - i : i - a], lx[a], sx);
+        for (int a = 0; a < ny; a++) sy = fma(ty[a > j ? a - j : j - a], ly[(size_t)a * nx], sy);
+        for (int a = 0; a < nz; a++) sz = fma(tz[a > k ? a - k : k - a], lz[(size_t)a * nxy], sz);
+        R[idx] += mask[r] * ((sx + sy) + sz);
+    }
+}
+
+static int launch_kinetic_lines(const DevOperator &op, double *X, double *R, Scal *sc, hipStream_t s)
+{
+    const int64_t n = op.P * op.nk;
+    const int g = grid_for(n, 16384);
+    timed_launch(k_kin_form, dim3(g), dim3(kWG), 0, s, X, R, (const double *)op.rot, (const double *)op.mask,
+                 (const Scal *)sc, op.P, n);
+    BSL_HIP(hipGetLastError());
+    timed_launch(k_kin_lines, dim3(g), dim3(kWG), 0, s, (const double *)X, R, (const double *)op.tx,
+                 (const double *)op.ty, (const double *)op.tz, (const double *)op.mask, op.nx, op.ny, op.nz, op.P,
+                 n);
+    BSL_HIP(hipGetLastError());
+    return BSL_OK;
+}
+
+bool kinetic_mfma_fits(const DevOperator &op) { return op.nx <= 64 && op.ny <= 64 && op.nz <= 64; }
+
 template <typename T, int KM>
 static int launch_kinetic_t(const DevOperator &op, double *X, double *R, Scal *sc, hipStream_t s)
 {
@@ -588,6 +658,75 @@ __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(W, 8))) voi
             vblock_point<B0, VT>(X, R, V0, 0, mask0, P, p, dot, rr2, vv2);
         else if (B1 > 0)
             vblock_point<(B1 > 0 ? B1 : 1), VT>(X, R, V1, B0, mask1, P, p, dot, rr2, vv2);
+    }
+    const int w = blockIdx.y * gridDim.x + blockIdx.x;
+    const double s0 = block_sum(dot, red);
+    __syncthreads();
+    const double s1 = block_sum(rr2, red);
+    __syncthreads();
+    const double s2 = block_sum(vv2, red);
+    if (threadIdx.x == 0) {
+        partial[w] = s0;
+        partial_b[w] = s1;
+        partial_c[w] = s2;
+    }
+}
+
+// Packed position of element (a, c), a <= c, of a B x B block in the row
+// pair order above: row pair m = min(a, B-1-a) starts at m (B+1); row m
+// comes first (B - m columns), then row B-1-m.
+__device__ inline int vblock_pair(int a, int c, int B)
+{
+    const int m = a < B - 1 - a ? a : B - 1 - a;
+    return m * (B + 1) + (a == m ? 0 : B - m) + (c - a);
+}
+
+// Blocks of any size (those without a register kernel): 64 points per
+// workgroup, one per lane, the block's X rows staged in LDS ([row][64]);
+// wave w computes rows w, w + 4, ..., each packed element read for both of
+// its rows (twice the V bytes of the register kernels, no register arrays).
+// Grid (ceil(P/64), blocks); partial index blockIdx.y * gridDim.x + blockIdx.x.
+template <typename VT>
+__global__ __launch_bounds__(kWG) void k_vblock_rows(const double *__restrict__ X, double *__restrict__ R,
+                                                     const VT *__restrict__ V0, const VT *__restrict__ V1, int B0,
+                                                     int B1, double mask0, double mask1, int64_t P,
+                                                     double *__restrict__ partial, double *__restrict__ partial_b,
+                                                     double *__restrict__ partial_c)
+{
+    extern __shared__ double xrow[];  // [B][64]
+    __shared__ double red[kWG / 64];
+    const int blk = (int)blockIdx.y;
+    const int B = blk == 0 ? B0 : B1, plane0 = blk == 0 ? 0 : B0;
+    const double mask = blk == 0 ? mask0 : mask1;
+    const VT *V = blk == 0 ? V0 : V1;
+    const int lane = (int)threadIdx.x & 63, wave = (int)threadIdx.x >> 6;
+    const int64_t g0 = (int64_t)blockIdx.x * 64;
+    for (int t = threadIdx.x; t < B * 64; t += blockDim.x) {
+        const int r = t >> 6, l = t & 63;
+        const int64_t q = g0 + l < P ? g0 + l : P - 1;
+        xrow[t] = X[(size_t)(plane0 + r) * P + q];
+    }
+    __syncthreads();
+    const int64_t p = g0 + lane;
+    const bool live = p < P;
+    const int64_t pl = live ? p : P - 1;
+    const VT *vp = V + (size_t)(pl / kVTile) * ((size_t)B * (B + 1) / 2) * kVTile + pl % kVTile;
+    double dot = 0.0, rr2 = 0.0, vv2 = 0.0;
+    for (int i = wave; i < B; i += kWG / 64) {
+        double acc = 0.0;
+        for (int c = 0; c < B; c++) {
+            const int off = c >= i ? vblock_pair(i, c, B) : vblock_pair(c, i, B);
+            acc = fma((double)vp[(size_t)off * kVTile], xrow[c * 64 + lane], acc);
+        }
+        if (live) {
+            const size_t o = (size_t)(plane0 + i) * P + p;
+            const double x = xrow[i * 64 + lane];
+            const double nr = R[o] + mask * acc;
+            R[o] = nr;
+            dot += x * nr;
+            rr2 += nr * nr;
+            vv2 += x * x;
+        }
     }
     const int w = blockIdx.y * gridDim.x + blockIdx.x;
     const double s0 = block_sum(dot, red);
@@ -1089,13 +1228,6 @@ __global__ __launch_bounds__(kWG) void k_rayleigh_dots(const double *__restrict_
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
-static int grid_for(int64_t n, int cap = 4096)
-{
-    int64_t g = (n + kWG - 1) / kWG;
-    if (g > cap) g = cap;
-    if (g < 1) g = 1;
-    return (int)g;
-}
 
 template <typename VT>
 static int launch_vblock_t(const DevOperator &op, const double *X, double *R, hipStream_t s)
@@ -1107,9 +1239,9 @@ static int launch_vblock_t(const DevOperator &op, const double *X, double *R, hi
     const VT *V1 = op.nblk > 1 ? V + op.blk[1].pair_offset * ptile : V0;
     const double mk0 = op.blk[0].mask, mk1 = op.nblk > 1 ? op.blk[1].mask : 0.0;
     dim3 grid((unsigned)((op.P + kWG - 1) / kWG), (unsigned)op.nblk);
-    if ((int64_t)grid.x * grid.y > op.npartial || (int64_t)grid.x * grid.y != op.nalpha)
-        return fail(BSL_E_STATE, "partial buffer does not match the potential kernel grid");
     const int b0 = op.blk[0].size, b1 = op.nblk > 1 ? op.blk[1].size : 0;
+    if (vblock_has_kernel(b0, b1) && ((int64_t)grid.x * grid.y > op.npartial || (int64_t)grid.x * grid.y != op.nalpha))
+        return fail(BSL_E_STATE, "partial buffer does not match the potential kernel grid");
 #define BSL_VB(A, Bb)                                                                                  \
     if (b0 == A && b1 == Bb) {                                                                         \
         switch (op.vb_variant) {                                                                       \
@@ -1131,7 +1263,23 @@ static int launch_vblock_t(const DevOperator &op, const double *X, double *R, hi
     BSL_VB(1, 3) BSL_VB(6, 10) BSL_VB(6, 3) BSL_VB(15, 21) BSL_VB(15, 10) BSL_VB(28, 21)
     BSL_VB(1, 0) BSL_VB(3, 0) BSL_VB(6, 0) BSL_VB(10, 0) BSL_VB(15, 0) BSL_VB(21, 0) BSL_VB(28, 0)
 #undef BSL_VB
-    return fail(BSL_E_ARG, "unsupported potential block sizes " + std::to_string(b0) + "/" + std::to_string(b1));
+    // any other sizes: the row-parallel kernel
+    const int bmax = b0 > b1 ? b0 : b1;
+    if ((size_t)bmax * 64 * sizeof(double) > 65536) return fail(BSL_E_ARG, "potential block too large");
+    dim3 rgrid((unsigned)((op.P + 63) / 64), (unsigned)op.nblk);
+    if ((int64_t)rgrid.x * rgrid.y != op.nalpha) return fail(BSL_E_STATE, "partial buffer does not match the potential kernel grid");
+    timed_launch((k_vblock_rows<VT>), rgrid, dim3(kWG), (uint32_t)(bmax * 64 * sizeof(double)), s, X, R, V0, V1, b0,
+                 b1, mk0, mk1, op.P, op.partial, op.partial_b, op.partial_c);
+    return hipGetLastError() == hipSuccess ? BSL_OK : fail(BSL_E_HIP, "k_vblock_rows launch");
+}
+
+bool vblock_has_kernel(int b0, int b1)
+{
+    static const int sizes[][2] = {{1, 3},  {6, 10}, {6, 3},  {15, 21}, {15, 10}, {28, 21}, {1, 0},
+                                   {3, 0},  {6, 0},  {10, 0}, {15, 0},  {21, 0},  {28, 0}};
+    for (const auto &z : sizes)
+        if (z[0] == b0 && z[1] == b1) return true;
+    return false;
 }
 
 template <typename VT>
@@ -1312,6 +1460,14 @@ int launch_potential(const DevOperator &op, const double *X, double *R, Scal *sc
     return op.vprec == BSL_PREC_FP64 ? launch_vblock_t<double>(op, X, R, s) : launch_vblock_t<float>(op, X, R, s);
 }
 
+int launch_kinetic(const DevOperator &op, double *X, double *R, Scal *sc, hipStream_t s)
+{
+    if (!kinetic_mfma_fits(op)) return launch_kinetic_lines(op, X, R, sc, s);
+    return op.kprec == BSL_PREC_FP64   ? launch_kinetic_t<double, 0>(op, X, R, sc, s)
+           : op.kprec == BSL_PREC_FP32 ? launch_kinetic_t<float, 0>(op, X, R, sc, s)
+                                       : launch_kinetic_t<float, 1>(op, X, R, sc, s);
+}
+
 LaunchTimer &launch_timer()
 {
     static thread_local LaunchTimer t;
@@ -1320,12 +1476,9 @@ LaunchTimer &launch_timer()
 
 int launch_hv(const DevOperator &op, double *X, double *R, Scal *sc, hipStream_t s, hipEvent_t *ev)
 {
-    if (op.nx > 64 || op.ny > 64 || op.nz > 64) return fail(BSL_E_ARG, "grid axes above 64 points are not supported");
     LaunchTimer &t = launch_timer();
     if (ev) t = {ev[0], ev[1]};
-    int krc = op.kprec == BSL_PREC_FP64   ? launch_kinetic_t<double, 0>(op, X, R, sc, s)
-              : op.kprec == BSL_PREC_FP32 ? launch_kinetic_t<float, 0>(op, X, R, sc, s)
-                                          : launch_kinetic_t<float, 1>(op, X, R, sc, s);
+    int krc = launch_kinetic(op, X, R, sc, s);
     if (ev) t = {ev[2], ev[3]};
     int rc = krc ? krc : launch_potential(op, X, R, sc, s);
     t = {};

@@ -75,9 +75,11 @@ enum { BSL_POT_IHO = 0, BSL_POT_CAGE = 1, BSL_POT_CAGE_GRID = 2, BSL_POT_COULOMB
 enum { BSL_START_UNIFORM = 0, BSL_START_RANDOM = 1 };
 
 typedef struct bsl_params {
-    int32_t nx, ny, nz;          /* grid points per axis */
+    int32_t nx, ny, nz;          /* grid points per axis, 2..256 (MFMA kinetic kernels up to 64,
+                                    FP64 line kernels above) */
     double x_max, y_max, z_max;  /* full box lengths (nm); points span [-x_max/2, x_max/2] */
-    int32_t lmax;
+    int32_t lmax;                /* 0..12 (harmonic form for parity blocks of l <= 5, packed
+                                    blocks above; the quadrature calculator up to 7) */
     int32_t n_theta;             /* Gauss-Legendre points in cos(theta) */
     int32_t n_phi;               /* Gauss-Chebyshev points; 2*n_phi phi nodes on [0, 2 pi) */
     double mass;                 /* amu */

@@ -27,7 +27,7 @@ def test_exports_every_declared_symbol():
 
 def test_create_rejects_bad_parameters():
     _, kw = cs.iho_case(9, 2, 6, 8, x_max=0.4)
-    for bad in (dict(nx=1), dict(nx=65), dict(lmax=7), dict(symmetry="Even", basis="full", x_max=-1.0)):
+    for bad in (dict(nx=1), dict(nx=257), dict(lmax=13), dict(symmetry="Even", basis="full", x_max=-1.0)):
         k = dict(kw)
         k.update(bad)
         with pytest.raises(B.BslError) as ei:

@@ -378,9 +378,45 @@ int time_potential(bsl_ctx *c, DevOperator &op, float *ms)
     return rc;
 }
 
+// The quadrature kernel's tile shape: points per workgroup group (each
+// candidate a multiple of the smallest whole-tile group whose shared memory
+// fits), with the persistent grid the occupancy query gives for it; the
+// fastest on the context's own operator is kept.
+int tune_quad(bsl_ctx *c)
+{
+    DevOperator &op = c->op;
+    int rc = launch_zero(c->d_X, c->n_kry, c->stream);
+    if (!rc) rc = launch_zero(c->d_R, c->n_kry, c->stream);
+    float best = 1e30f;
+    int pick = op.q_npts, pick_grid = op.q_grid;
+    for (int npts : quad_group_candidates(op)) {
+        if (rc) break;
+        op.q_npts = npts;
+        if ((rc = quad_grid(op, c->device, &op.q_grid))) break;
+        if (op.q_grid + 1 > op.npartial) continue;
+        op.nalpha = op.q_grid;
+        float ms = 0;
+        rc = time_potential(c, op, &ms);
+        if (!rc && ms < best) {
+            best = ms;
+            pick = npts;
+            pick_grid = op.q_grid;
+        }
+    }
+    op.q_npts = pick;
+    op.q_grid = pick_grid;
+    op.nalpha = pick_grid;
+    if (!rc) rc = launch_init_scal(c->d_scal, 0.0, 0.0, c->stream);
+    if (!rc) {
+        hipError_t e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess) rc = hip_fail(e, "autotune", __FILE__, __LINE__);
+    }
+    return rc;
+}
+
 int autotune(bsl_ctx *c)
 {
-    if (c->op.calc == BSL_CALC_QUAD) return BSL_OK;
+    if (c->op.calc == BSL_CALC_QUAD) return tune_quad(c);
     int rc = launch_zero(c->d_X, c->n_kry, c->stream);
     if (!rc) rc = launch_zero(c->d_R, c->n_kry, c->stream);
     DevOperator *ops[2] = {&c->op, &c->op64};
@@ -572,14 +608,18 @@ extern "C" int bsl_create(const bsl_params *pp, bsl_ctx **out)
         for (int r = 0; r < op.q_ne; r++)
             if (t.l[c->carried[r]] % 2 != 0) return fail(BSL_E_STATE, "carried planes are not grouped by l parity");
         op.q_npts = quad_points_per_group(nah, nk);
+        // the grid W is laid out by global column tile whatever the group
+        // size, so one allocation (padded for the largest group the
+        // autotuner tries) serves every candidate
+        const int npts_max = quad_group_candidates(op).back();
         const size_t es = op.wprec == BSL_PREC_FP64 ? sizeof(double) : sizeof(float);
-        const int64_t ntiles = quad_tiles(c->P, nah, op.q_npts);
+        const int64_t ntiles = quad_tiles(c->P, nah, npts_max);
         BSL_HIP(hipMalloc(&op.wgrid, es * (size_t)ntiles * quad_phi_tiles(op.nphi) * 256));
         double *d_wt = nullptr, *d_wp = nullptr;
         rc = dev_upload(&d_wt, t.wt);
         if (!rc) rc = dev_upload(&d_wp, t.wp);
         if (!rc)
-            rc = launch_weight_grid(c->d_vgrid, op.wgrid, op.wprec, op.qmode, d_wt, d_wp, c->P, na, nb, op.q_npts,
+            rc = launch_weight_grid(c->d_vgrid, op.wgrid, op.wprec, op.qmode, d_wt, d_wp, c->P, na, nb, npts_max,
                                     c->stream);
         const hipError_t ew = hipStreamSynchronize(c->stream);
         dev_free(d_wt);
@@ -1329,7 +1369,8 @@ extern "C" int bsl_potential_kernel(const bsl_ctx *c, char *name, int32_t len)
     const DevOperator &op = c->op;
     char buf[128];
     if (op.calc == BSL_CALC_QUAD)
-        std::snprintf(buf, sizeof buf, "k_quad_mfma<%d>", op.qmode);
+        std::snprintf(buf, sizeof buf, "k_quad_mfma<%d> (%d points per group, %d workgroups)", op.qmode, op.q_npts,
+                      op.q_grid);
     else if (op.vform == BSL_VFORM_HARMONICS &&
              harmonic_split(op.nblk, op.blk[0].lb, op.blk[0].par, op.nblk > 1 ? op.blk[1].lb : -1,
                             op.nblk > 1 ? op.blk[1].par : 0, op.blk[op.nblk > 1 ? 1 : 0].plane0,

@@ -328,6 +328,9 @@ int quad_points_per_group(int na, int nk);
 int quad_grid(const DevOperator &op, int device, int *grid);
 int64_t quad_tiles(int64_t P, int na, int npts);
 int quad_phi_tiles(int nphi);
+// the points-per-group candidates of the quadrature kernel's autotuner
+// (op.q_npts the smallest), in increasing order
+std::vector<int> quad_group_candidates(const DevOperator &op);
 int launch_quad_potential(const DevOperator &op, const double *X, double *R, Scal *sc, hipStream_t s);
 
 }  // namespace bsl

@@ -730,6 +730,22 @@ int quad_points_per_group(int na, int nk)
     return kQW * unit * k;
 }
 
+int quad_kf(int nm, int mode);
+
+std::vector<int> quad_group_candidates(const DevOperator &op)
+{
+    // 1x, 2x and 4x the smallest group, as long as the shared memory fits
+    std::vector<int> out{op.q_npts};
+    const int kf = quad_kf(op.nm, op.qmode), ntp = quad_phi_tiles(op.nphi);
+    for (int f = 2; f <= 4; f *= 2) {
+        const int npts = op.q_npts * f;
+        if ((int64_t)npts > op.P) break;
+        if (quad_lds(op.nk, op.q_ne, op.na, npts, kf, op.qmode, 16 * ntp).bytes > 65536) break;
+        out.push_back(npts);
+    }
+    return out;
+}
+
 int64_t quad_tiles(int64_t P, int na, int npts)
 {
     const int64_t groups = (P + npts - 1) / npts;

@@ -1,8 +1,8 @@
 """GPU parity beyond the sizes of the register and MFMA kernels, through the
 C ABI, against the CPU oracle: axes of more than 64 points (the FP64 line
-kernels of the kinetic term) and lmax 7 and 8 (parity blocks of 28, 36 and
-45 planes: the row-parallel block kernel of the potential; the quadrature
-calculator up to its 16 m rows).
+kernels of the kinetic term, up to 256 points) and lmax 7, 8 and 12 (parity
+blocks of 28 to 91 planes: the row-parallel block kernel of the potential;
+the quadrature calculator up to its 16 m rows).
 
 Tolerances: Hv 1e-12 of the largest magnitude (FP64), 1e-6 (FP32
 potential), 2e-5 (BF16x3 quadrature); alpha/beta over a 20-step prefix
@@ -29,9 +29,10 @@ def _axes(spec, kw, nx, ny, nz):
     return spec, kw
 
 
-@pytest.mark.parametrize("nx,ny,nz", [(66, 66, 66), (70, 9, 5), (5, 65, 8), (6, 7, 67)])
+@pytest.mark.parametrize("nx,ny,nz", [(66, 66, 66), (70, 9, 5), (5, 65, 8), (6, 7, 67), (256, 4, 3)])
 def test_long_axes_hv_matches_oracle(nx, ny, nz):
-    """Every axis above 64 points (even and odd lengths), one at a time."""
+    """Every axis above 64 points (even and odd lengths), one at a time, up
+    to the 256-point maximum."""
     spec, kw = cs.cage_case(4, 1, 4, 5, "All")
     spec, kw = _axes(spec, kw, nx, ny, nz)
     o = O.Oracle(spec)
@@ -80,6 +81,22 @@ def test_high_lmax_block_rows_match_oracle(lmax, sym, basis, kernel):
     ar, br, _ = o.lanczos(30, v0)
     assert np.max(np.abs(ag[:20] - ar[:20]) / np.abs(ar[:20])) < 1e-9
     assert np.max(np.abs(bg[1:21] - br[1:21]) / br[1:21]) < 1e-9
+
+
+def test_lmax12_matches_oracle():
+    """The largest lmax: 169 planes in blocks of 91 and 78, both on the
+    row-parallel kernel (46.6 KB of LDS rows), reduced and full basis."""
+    spec, kw = cs.cage_case(3, 12, 14, 14, "All")
+    o = O.Oracle(spec)
+    with B.Calculator(**kw) as c:
+        assert c.potential_kernel().startswith("k_vblock_rows")
+        v = np.random.default_rng(12).uniform(-1, 1, o.N)
+        assert _rel(c.hv(v), o.hv(v)) < 1e-12
+    spec, kw = cs.cage_case(3, 12, 14, 14, "Odd")
+    o = O.Oracle(spec)
+    with B.Calculator(basis="reduced", **kw) as c:
+        v = cs.reduced_start(o, 13, 1)
+        assert _rel(c.hv(v), o.hv(v)) < 1e-12
 
 
 @pytest.mark.parametrize("precision,tol", [("fp64", 1e-12), ("fp32", 1e-6), ("bf16x3", 2e-5)])

@@ -195,6 +195,22 @@ class Calculator:
         """Rebuilds the potential in the other form (clears the Krylov vectors)."""
         check(lib().bsl_set_potential_form(self.h, VFORM[form]))
 
+    @property
+    def fused(self):
+        """True when a NoQuad step runs the z kinetic term inside the
+        potential kernel (x-y kernel, then one fused kernel), False for the
+        three-kernel step, the default (bsl_get_fusion)."""
+        r = lib().bsl_get_fusion(self.h)
+        if r < 0:
+            check(r)
+        return bool(r)
+
+    @fused.setter
+    def fused(self, on):
+        """Selects the step structure; raises where the fused kernel does not
+        apply (bsl_set_fusion)."""
+        check(lib().bsl_set_fusion(self.h, 1 if on else 0))
+
     def unclamped_fraction(self):
         """Fraction of the (point, node) pairs of the potential grid below the
         ceiling clamp (cage potential; bsl_clamp_stats)."""

@@ -7,6 +7,8 @@
 // finalize; split by potential block and precision): each
 // launch takes a device table of the members' pointers and shapes, and its
 // workgroups are the members' own grids laid end to end.  A member whose
+// step fuses the z kinetic term into the potential kernel takes the fused
+// table kernel instead of the z and potential ones.  A member whose
 // potential has no table kernel (quadrature calculator, the full basis,
 // other block shapes) gets its own potential launch inside the iteration;
 // one with an axis of more than 64 points its own kinetic and potential
@@ -32,6 +34,7 @@ namespace {
 struct Launch {
     int kind, prec, offset, nm, total;
     const double *G;  // the Gaunt values of a potential launch's block (members share them)
+    int nz;           // the longest z axis among the members
 };
 }  // namespace
 
@@ -68,6 +71,7 @@ int potential_kind(const DevOperator &op)
     if (op.calc == BSL_CALC_QUAD || op.vform != BSL_VFORM_HARMONICS || !kinetic_mfma_fits(op)) return -1;
     const Block &b0 = op.blk[0], &b1 = op.nblk > 1 ? op.blk[1] : op.blk[0];
     const int split = harmonic_split(op.nblk, b0.lb, b0.par, op.nblk > 1 ? b1.lb : -1, b1.par, b1.plane0, b0.size);
+    if (split == 2 && op.fused) return b0.par == 0 ? kBZPotPara : kBZPotOrtho;
     if (split == 2) return b0.par == 0 ? kBPotSplitPara : kBPotSplitOrtho;
     return -1;
 }
@@ -121,15 +125,16 @@ int build_tables(bsl_batch *b)
         // members grouped by (kind, precision), in member order, at most
         // kBatchMaxTable per launch
         auto add = [&](int kind, int prec, auto count_of, auto uses) {
-            Launch cur{kind, prec, (int)b->h_tab.size(), 0, 0, nullptr};
+            Launch cur{kind, prec, (int)b->h_tab.size(), 0, 0, nullptr, 0};
             for (bsl_ctx *c : b->ctx) {
                 if (!uses(c)) continue;
                 if (cur.nm == kBatchMaxTable) {
                     L.push_back(cur);
-                    cur = Launch{kind, prec, (int)b->h_tab.size(), 0, 0, nullptr};
+                    cur = Launch{kind, prec, (int)b->h_tab.size(), 0, 0, nullptr, 0};
                 }
                 BatchEntry e = entry_of(c, parity);
                 if (!cur.G) cur.G = c->op.gc[0];
+                cur.nz = std::max(cur.nz, c->op.nz);
                 e.first = cur.total;
                 e.count = count_of(c);
                 cur.total += e.count;
@@ -143,10 +148,14 @@ int build_tables(bsl_batch *b)
                 [&](const bsl_ctx *c) { return c->op.kprec == prec && kinetic_mfma_fits(c->op); });
         for (int prec : {BSL_PREC_FP64, BSL_PREC_FP32, BSL_PREC_BF16X3})
             add(kBZHalf, prec, [](const bsl_ctx *c) { return ((c->op.nx * c->op.ny + 63) / 64) * c->op.nk; },
-                [&](const bsl_ctx *c) { return c->op.kprec == prec && kinetic_mfma_fits(c->op); });
+                [&](const bsl_ctx *c) { return c->op.kprec == prec && kinetic_mfma_fits(c->op) && !c->op.fused; });
         for (int kind : {(int)kBPotSplitPara, (int)kBPotSplitOrtho})
             for (int prec : {BSL_PREC_FP64, BSL_PREC_FP32})
                 add(kind, prec, [](const bsl_ctx *c) { return (int)((c->op.P + 63) / 64); },
+                    [&](const bsl_ctx *c) { return potential_kind(c->op) == kind && c->op.vprec == prec; });
+        for (int kind : {(int)kBZPotPara, (int)kBZPotOrtho})
+            for (int prec : {BSL_PREC_FP64, BSL_PREC_FP32})
+                add(kind, prec, [](const bsl_ctx *c) { return (int)zpot_workgroups(c->op); },
                     [&](const bsl_ctx *c) { return potential_kind(c->op) == kind && c->op.vprec == prec; });
         add(kBFinalize, BSL_PREC_FP64, [](const bsl_ctx *) { return 1; }, [](const bsl_ctx *) { return true; });
     }
@@ -190,13 +199,13 @@ int batch_iteration(bsl_batch *b, int parity)
                 bsl_ctx *c = b->ctx[i];
                 double *X = parity == 0 ? c->d_X : c->d_R, *R = parity == 0 ? c->d_R : c->d_X;
                 if (!rc && !kinetic_mfma_fits(c->op)) rc = launch_kinetic(c->op, X, R, c->d_scal, b->main);
-                if (!rc) rc = launch_potential(c->op, X, R, c->d_scal, b->main);
+                if (!rc) rc = launch_potential_part(c->op, X, R, c->d_scal, b->main);
             }
         }
         if (!rc)
             rc = launch_batch(l.kind, l.prec,
                               reinterpret_cast<const BatchEntry *>(b->d_tab + (size_t)l.offset * kBatchTableBytes),
-                              l.nm, l.total, l.G, b->main);
+                              l.nm, l.total, l.G, l.nz, b->main);
     }
     return rc;
 }

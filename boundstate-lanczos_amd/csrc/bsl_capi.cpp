@@ -142,6 +142,35 @@ void potential_table(const bsl_ctx *c, const DevOperator &op, int form, std::vec
     *ncol = pairs;
 }
 
+// Workgroups (= partial sums) of the kernel that ends a NoQuad step: the
+// fused z-kinetic + potential kernel when op.fused, else the potential kernel
+// of the operator's form and blocks.
+int64_t potential_workgroups(const DevOperator &op)
+{
+    if (op.fused) return zpot_workgroups(op);
+    const Block &b0 = op.blk[0], &b1 = op.blk[op.nblk > 1 ? 1 : 0];
+    const int split = harmonic_split(op.nblk, b0.lb, b0.par, op.nblk > 1 ? b1.lb : -1, b1.par, b1.plane0, b0.size);
+    const int bs1 = op.nblk > 1 ? b1.size : 0;
+    return op.vform == BSL_VFORM_HARMONICS      ? harmonic_workgroups(op.P, split)
+           : vblock_has_kernel(b0.size, bs1) ? vpk_ntile(op.P) * op.nblk
+                                             : ((op.P + 63) / 64) * op.nblk;
+}
+
+// Switch a NoQuad operator between the three-kernel step and the fused one
+// (on is ignored where zpot_fits does not hold); keeps nalpha in step.
+int set_fused(DevOperator &op, int on)
+{
+    if (op.calc == BSL_CALC_QUAD) return BSL_OK;
+    op.fused = on && zpot_fits(op) ? 1 : 0;
+    op.nalpha = potential_workgroups(op);
+    if (op.nalpha > op.npartial) {
+        op.fused = 0;
+        op.nalpha = potential_workgroups(op);
+        return fail(BSL_E_STATE, "partial buffer too small for the fused kernel");
+    }
+    return BSL_OK;
+}
+
 // Build a DevOperator's potential (form, precision) from the grid d_vg and
 // its angular table d_h of ncol columns
 int build_vpk(bsl_ctx *c, DevOperator &op, int form, int vprec, const double *d_vg, const double *d_h, int64_t ncol)
@@ -168,12 +197,8 @@ int build_vpk(bsl_ctx *c, DevOperator &op, int form, int vprec, const double *d_
         col0 = op.nblk > 1 ? op.blk[1].pair_offset : op.npair;
     }
     if (op.calc != BSL_CALC_QUAD) {
-        const Block &b0 = op.blk[0], &b1 = op.blk[op.nblk > 1 ? 1 : 0];
-        const int split = harmonic_split(op.nblk, b0.lb, b0.par, op.nblk > 1 ? b1.lb : -1, b1.par, b1.plane0, b0.size);
-        const int bs1 = op.nblk > 1 ? b1.size : 0;
-        op.nalpha = form == BSL_VFORM_HARMONICS       ? harmonic_workgroups(c->P, split)
-                    : vblock_has_kernel(b0.size, bs1) ? ntile * op.nblk
-                                                      : ((c->P + 63) / 64) * op.nblk;
+        op.fused = 0;
+        op.nalpha = potential_workgroups(op);
         if (op.nalpha > op.npartial) return fail(BSL_E_STATE, "partial buffer too small for the potential kernel");
     }
     return launch_vblock_gemm(d_vg, d_h, op.vpk, vprec, c->P, c->Q, col0, ncol, c->stream);
@@ -211,6 +236,7 @@ void share_potential(DevOperator &op, const DevOperator &src)
     op.ncoef = src.ncoef;
     op.gc[0] = src.gc[0];
     op.gc[1] = src.gc[1];
+    op.fused = src.fused;
     op.nalpha = src.nalpha;
 }
 
@@ -414,6 +440,10 @@ int tune_quad(bsl_ctx *c)
     return rc;
 }
 
+// Picks, per operator, the occupancy variant of the potential kernel (the
+// variants compute identical results).  The step keeps its three kernels;
+// the fused z-kinetic + potential kernel is selected only on request
+// (bsl_set_fusion): it moves fewer bytes but runs slower on this part.
 int autotune(bsl_ctx *c)
 {
     if (c->op.calc == BSL_CALC_QUAD) return tune_quad(c);
@@ -423,8 +453,11 @@ int autotune(bsl_ctx *c)
     for (int o = 0; o < 2 && !rc; o++) {
         if (o == 1 && c->op64.vpk == c->op.vpk) {
             c->op64.vb_variant = c->op.vb_variant;
+            c->op64.fused = c->op.fused;
+            c->op64.nalpha = c->op.nalpha;
             break;
         }
+        if ((rc = set_fused(*ops[o], 0))) break;
         float best = 1e30f;
         int pick = 1;
         const int nvar = ops[o]->vform == BSL_VFORM_HARMONICS ? 2 : 3;
@@ -566,6 +599,9 @@ extern "C" int bsl_create(const bsl_params *pp, bsl_ctx **out)
     // ---- device: tables ----
     if ((rc = dev_upload(&op.tx, rowx))) return rc;
     if ((rc = dev_upload(&op.ty, rowy))) return rc;
+    // one zero past the row (tz[nz] = 0: the fused kernel clamps Toeplitz
+    // offsets beyond the axis onto it)
+    rowz.push_back(0.0);
     if ((rc = dev_upload(&op.tz, rowz))) return rc;
     if ((rc = dev_upload(&op.rot, c->rot))) return rc;
     if ((rc = dev_upload(&op.mask, c->mask))) return rc;
@@ -1157,6 +1193,31 @@ extern "C" int bsl_gaunt_table(int32_t lb, int32_t par, int32_t cap, int32_t *ij
     return nt;
 }
 
+extern "C" int bsl_get_fusion(const bsl_ctx *c)
+{
+    if (!c) return -fail(BSL_E_ARG, "null argument");
+    return c->op.fused;
+}
+
+extern "C" int bsl_set_fusion(bsl_ctx *c, int32_t on)
+{
+    if (!c) return fail(BSL_E_ARG, "null argument");
+    if (on != 0 && on != 1) return fail(BSL_E_ARG, "fusion is 0 or 1");
+    if (on && !zpot_fits(c->op))
+        return fail(BSL_E_ARG, "no fused z-kinetic and potential kernel for this operator (it needs the lmax-5 "
+                               "harmonic-form blocks, axes of at most 64 points and FP64 or FP32 throughout)");
+    BSL_HIP(hipSetDevice(c->device));
+    BSL_HIP(hipStreamSynchronize(c->stream));
+    int rc = set_fused(c->op, on);
+    if (!rc && c->op64.vpk == c->op.vpk) {
+        c->op64.fused = c->op.fused;
+        c->op64.nalpha = c->op.nalpha;
+    } else if (!rc) {
+        rc = set_fused(c->op64, on);
+    }
+    return rc;
+}
+
 extern "C" int bsl_get_potential_form(const bsl_ctx *c)
 {
     if (!c) return -fail(BSL_E_ARG, "null argument");
@@ -1355,8 +1416,12 @@ extern "C" int bsl_step_bytes(const bsl_ctx *c, double *out)
         }
         pot_flops = 2.0 * (double)c->P * pair_fma;
     }
-    out[0] = pot_bytes + 3.0 * vec;  // potential kernel: V + read X, read and write R
-    out[1] = 7.0 * vec;              // kinetic kernels: x-y reads X, R and writes X, R; z reads X, R, writes R
+    // potential kernel (or the fused z-kinetic + potential kernel): V + read
+    // X, read and write R
+    out[0] = pot_bytes + 3.0 * vec;
+    // kinetic kernels: x-y reads X, R and writes X, R; z (unless fused)
+    // reads X, R, writes R
+    out[1] = (op.fused ? 4.0 : 7.0) * vec;
     out[2] = 0.0;                    // recurrence update: folded into the kinetic and potential kernels
     out[3] = pot_bytes + 6.0 * vec;  // fused-step minimum: V + the six vector passes of the recurrence
     out[4] = pot_flops + 2.0 * (double)(op.nx + op.ny + op.nz) * (double)c->n_kry + 4.0 * (double)c->n_kry;
@@ -1371,6 +1436,11 @@ extern "C" int bsl_potential_kernel(const bsl_ctx *c, char *name, int32_t len)
     if (op.calc == BSL_CALC_QUAD)
         std::snprintf(buf, sizeof buf, "k_quad_mfma<%d> (%d points per group, %d workgroups)", op.qmode, op.q_npts,
                       op.q_grid);
+    else if (op.fused && op.nblk > 1)
+        std::snprintf(buf, sizeof buf, "k_zpot2<%s>", op.vprec == BSL_PREC_FP64 ? "double" : "float");
+    else if (op.fused)
+        std::snprintf(buf, sizeof buf, "k_zpot1<%d, %d, %s>", op.blk[0].lb, op.blk[0].par,
+                      op.vprec == BSL_PREC_FP64 ? "double" : "float");
     else if (op.vform == BSL_VFORM_HARMONICS &&
              harmonic_split(op.nblk, op.blk[0].lb, op.blk[0].par, op.nblk > 1 ? op.blk[1].lb : -1,
                             op.nblk > 1 ? op.blk[1].par : 0, op.blk[op.nblk > 1 ? 1 : 0].plane0,

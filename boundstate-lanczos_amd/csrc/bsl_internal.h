@@ -158,6 +158,8 @@ struct DevOperator {
     int *plane_ref = nullptr;     // reference lm index of each carried plane
     int64_t npair = 0;
     int vb_variant = 1;           // occupancy variant of the potential kernel (autotuned)
+    int fused = 0;                // 1: the z kinetic term runs inside the potential kernel (k_zpot*);
+                                  // nalpha is then that kernel's workgroup count
     // quadrature calculator (calc == BSL_CALC_QUAD)
     int calc = 0;
     void *wgrid = nullptr;        // weighted potential grid in MFMA accumulator layout
@@ -212,6 +214,16 @@ int launch_step(const DevOperator &op, double *X, double *R, Scal *sc, double *a
                 double *beta2_out, hipStream_t s, hipEvent_t *ev /* 6 events or null */);
 // the potential kernel alone (used by the autotuner)
 int launch_potential(const DevOperator &op, const double *X, double *R, Scal *sc, hipStream_t s);
+// The fused z-kinetic + potential kernel (k_zpot2 / k_zpot1): the lmax-5
+// harmonic-form blocks (full basis pair, or one reduced-basis block), every
+// axis of at most 64 points, FP64 or FP32 throughout (zpot_fits); one
+// workgroup (= partial sum) per 4-column tile and block (zpot_workgroups).
+bool zpot_fits(const DevOperator &op);
+int64_t zpot_workgroups(const DevOperator &op);
+int launch_zpot(const DevOperator &op, const double *X, double *R, hipStream_t s);
+// what follows the kinetic launches of a step: the fused kernel when
+// op.fused, else the potential kernel
+int launch_potential_part(const DevOperator &op, const double *X, double *R, Scal *sc, hipStream_t s);
 // the kinetic kernels alone (the MFMA contractions where every axis has at
 // most 64 points, else the FP64 line kernels); forms v as launch_hv does
 int launch_kinetic(const DevOperator &op, double *X, double *R, Scal *sc, hipStream_t s);
@@ -263,7 +275,7 @@ struct BatchEntry {
 };
 // the kernel variants a batch step is made of
 enum BatchKind {
-    kBXyHalf = 0, kBZHalf, kBPotSplitPara, kBPotSplitOrtho, kBFinalize, kBKinds
+    kBXyHalf = 0, kBZHalf, kBPotSplitPara, kBPotSplitOrtho, kBZPotPara, kBZPotOrtho, kBFinalize, kBKinds
 };
 constexpr int kBatchMaxTable = 64;  // members per launch
 // Device layout of one launch's table: kBatchMaxTable entries (the first nm
@@ -273,8 +285,10 @@ constexpr size_t kBatchTableBytes = sizeof(BatchEntry) * kBatchMaxTable + sizeof
 // launch `kind` over the nm <= kBatchMaxTable entries of d_tab (device);
 // prec: BSL_PREC_FP64 or FP32 (kinetic T, or potential VT); total = the
 // entries' workgroups (for kBFinalize one per entry); G: the Gaunt values
-// of the potential kinds' block (the same for every member of the launch)
-int launch_batch(int kind, int prec, const BatchEntry *d_tab, int nm, int total, const double *G, hipStream_t s);
+// of the potential kinds' block (the same for every member of the launch);
+// lds_nz: the longest z axis among the members (sizes the fused kinds' LDS)
+int launch_batch(int kind, int prec, const BatchEntry *d_tab, int nm, int total, const double *G, int lds_nz,
+                 hipStream_t s);
 
 // prep kernels
 struct CageDev {

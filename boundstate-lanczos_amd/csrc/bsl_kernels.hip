@@ -550,14 +550,16 @@ static int launch_kinetic_lines(const DevOperator &op, double *X, double *R, Sca
 
 bool kinetic_mfma_fits(const DevOperator &op) { return op.nx <= 64 && op.ny <= 64 && op.nz <= 64; }
 
+// with_z false: the x-y kernel only (the fused kernel applies T_z)
 template <typename T, int KM>
-static int launch_kinetic_t(const DevOperator &op, double *X, double *R, Scal *sc, hipStream_t s)
+static int launch_kinetic_t(const DevOperator &op, double *X, double *R, Scal *sc, hipStream_t s, bool with_z = true)
 {
     // the half-size (centrosymmetric) contractions, for every axis length
     const size_t lds_xy = sizeof(T) * ((size_t)4 * 32 * kCompLd + 2 * 129);
     timed_launch((k_kin_xy_half<T, KM>), dim3(op.nz, op.nk), dim3(kWG), lds_xy, s, X, R, op.tx, op.ty, op.rot,
                        op.mask, sc, op.nx, op.ny, op.nz);
     BSL_HIP(hipGetLastError());
+    if (!with_z) return BSL_OK;
     const int nxy = op.nx * op.ny;
     const size_t lds_z = sizeof(T) * ((size_t)2 * 32 * kSlabLd + 129);
     timed_launch((k_kin_z_half<T, KM>), dim3((nxy + 63) / 64, op.nk), dim3(kWG), lds_z, s, X, R, op.tz, op.mask,
@@ -1109,6 +1111,222 @@ __global__ __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(W, 8))) voi
 }
 
 // ---------------------------------------------------------------------------
+// z kinetic term fused with the harmonic-form potential of one parity block
+// ---------------------------------------------------------------------------
+// The z contraction couples the nz points of one column (c = j nx + i) within
+// a plane, the potential the planes of one point.  A workgroup that owns every
+// point of a few columns for every plane of a parity block applies both with
+// one read of X and one read-modify-write of R: three vector passes where the
+// z kernel and the potential kernel make six.  (Tv_5D's z term,
+// linRotCartSphHvRoutines.cpp:1872-1969, and Vv_5D, linRotCartSphHvFullVPrepRoutines.cpp:675-734.)
+//
+// Tile: kZpCols consecutive columns x nz x the block's B planes, staged once
+// in LDS as xs[row][k][col].  Its points are taken by half index h
+// (k = h or nz-1-h, h < ceil(nz/2)) in M-tiles of 16 (at most two, nz <= 64):
+// M-tile mt holds h in [16 mt, 16 mt + 16), i.e. two mirror groups of 64
+// points with lane (h - 16 mt) * kZpCols + col.  First the six waves run the
+// block's three sub-lists (cross, cos-cos, sin-sin) on every group (waves
+// 0-2 the direct points, waves 3-5 the mirrored ones, M-tile after M-tile)
+// into part[2 mt + mirror][row][lane] (LDS atomics onto zero, as in the
+// split kernels).  Then wave w takes N-tile w of the z contraction (its 16
+// (row, col) columns; the sums and differences of mirror points on MFMA, as
+// in kin_z_half_body, for every M-tile) and finishes its elements,
+// R = R + mask (z + part), with their contributions to v.R, |R|^2 and |v|^2.
+// One barrier separates the phases, so a workgroup makes few dependent
+// memory round trips (staging, coefficients, R).
+constexpr int kZpCols = 4;
+constexpr int kZpThreads = 384;
+
+__host__ __device__ inline int zpot_mtiles(int nz) { return (((nz + 1) >> 1) + 15) >> 4; }
+
+template <int LB, int PAR, typename VT, typename T>
+__device__ __forceinline__ void zpot_body(int tile, const double *__restrict__ X, double *__restrict__ R,
+                                          const VT *__restrict__ H, const double *__restrict__ G, int nc, int plane0,
+                                          double mask, const double *__restrict__ tz, int nxy, int nz, int64_t P,
+                                          double &dot, double &rr2, double &vv2)
+{
+    using PX = gaunt::Pattern<LB, PAR, gaunt::kCross>;
+    using PC = gaunt::Pattern<LB, PAR, gaunt::kCC>;
+    using PS = gaunt::Pattern<LB, PAR, gaunt::kSS>;
+    typedef typename KinAcc<T>::type acc_t;
+    constexpr int B = gaunt::block_size(LB, PAR);
+    constexpr int NTN = (B * kZpCols + 15) / 16;  // N-tiles of the z contraction
+    static_assert(gaunt::split_covers_rows<LB, PAR>(), "every row needs a same-type and a cross partial");
+    static_assert(NTN <= kZpThreads / 64, "one N-tile per wave");
+    static_assert(kZpCols == 4, "lane (h - 16 mt) * 4 + col covers a mirror group of 64 points");
+    extern __shared__ __align__(16) unsigned char zp_lds[];
+    const int RS = nz * kZpCols;  // row stride of xs
+    const int mz = (nz + 1) >> 1, hz = (nz & 1) ? nz >> 1 : -1;
+    const int nmt = (mz + 15) >> 4, nks = (mz + 3) >> 2;
+    double *xs = (double *)zp_lds;
+    double(*part)[B][64] = reinterpret_cast<double(*)[B][64]>(xs + (size_t)B * RS);  // [2 nmt][B][64]
+    T *rz = reinterpret_cast<T *>(xs + (size_t)B * RS + (size_t)nmt * 2 * B * 64);   // [129]: t[64 + d] = t(|d|)
+    const int c0 = tile * kZpCols;
+    const int tid = (int)threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    {
+        // the tile's B x nz x 4 values (columns past nxy read as the last
+        // column, never written back); all loads issued before the stores
+        constexpr int K = (B * 64 * kZpCols + kZpThreads - 1) / kZpThreads;
+        const int n = B * RS;
+        double v[K];
+#pragma unroll
+        for (int e = 0; e < K; e++) {
+            const int t = min(tid + e * kZpThreads, n - 1);
+            const int row = t / RS, r = t - row * RS, cc = min(c0 + (r & 3), nxy - 1);
+            v[e] = X[(size_t)(plane0 + row) * P + (size_t)(r >> 2) * nxy + cc];
+        }
+#pragma unroll
+        for (int e = 0; e < K; e++)
+            if (tid + e * kZpThreads < n) xs[tid + e * kZpThreads] = v[e];
+    }
+    for (int t = tid; t < 129; t += kZpThreads) {
+        const int d = abs(t - 64);
+        rz[t] = d < nz ? (T)tz[d] : (T)0;
+    }
+    for (int t = tid; t < nmt * 2 * B * 64; t += kZpThreads) (&part[0][0][0])[t] = 0.0;
+    __syncthreads();
+    {
+        // potential: this wave's sub-list on its mirror group of every
+        // M-tile (lanes past the last half index take the last one:
+        // computed, never read)
+        const int mirror = wave >= 3 ? 1 : 0, sub = wave - 3 * mirror;
+        const int hl = lane >> 2, cl = lane & 3;
+        const int ccl = min(c0 + cl, nxy - 1);
+        for (int mt = 0; mt < nmt; mt++) {
+            const int h = min(16 * mt + hl, mz - 1);
+            const int k = mirror ? nz - 1 - h : h;
+            const int64_t p = (int64_t)k * nxy + ccl;
+            const VT *hp = H + (size_t)(p / kVTile) * nc * kVTile + p % kVTile;
+            const double *xp = xs + k * kZpCols + cl;
+            double(*pt)[64] = part[2 * mt + mirror];
+            if (sub == 0)
+                sub_run<PX, VT>(xp, hp, G, RS, pt, lane, std::make_integer_sequence<int, B>{},
+                                std::make_integer_sequence<int, PX::NC>{});
+            else if (sub == 1)
+                sub_run<PC, VT>(xp, hp, G, RS, pt, lane, std::make_integer_sequence<int, B>{},
+                                std::make_integer_sequence<int, PC::NC>{});
+            else
+                sub_run<PS, VT>(xp, hp, G, RS, pt, lane, std::make_integer_sequence<int, B>{},
+                                std::make_integer_sequence<int, PS::NC>{});
+        }
+    }
+    __syncthreads();
+    if (wave >= NTN) return;
+    const int l16 = lane & 15, kq = lane >> 4;
+    const int zcol = 16 * wave + l16, zrow = min(zcol >> 2, B - 1), zc = zcol & 3;
+    const bool zok = (zcol >> 2) < B && c0 + zc < nxy;
+    const int zcc = min(c0 + zc, nxy - 1);
+    const double *xr = xs + zrow * RS + zc;
+    const size_t rbase = (size_t)(plane0 + zrow) * P + zcc;
+    // R of this lane's elements (rows kk and nz-1-kk of its column, every
+    // M-tile), loaded ahead of the contractions (clamped: never written back)
+    double r1[2][4], r2[2][4];
+#pragma unroll
+    for (int m = 0; m < 2; m++)
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int kk = min(16 * m + kin_row<T>(kq, q), mz - 1);
+            r1[m][q] = R[rbase + (size_t)kk * nxy];
+            r2[m][q] = R[rbase + (size_t)(nz - 1 - kk) * nxy];
+        }
+#pragma unroll
+    for (int m = 0; m < 2; m++) {
+        if (m >= nmt) break;
+        const int kr = 16 * m + l16;  // A-operand row (output half index)
+        acc_t as = acc_t{0, 0, 0, 0}, ad = acc_t{0, 0, 0, 0};
+        for (int s = 0; s < nks; s++) {
+            const int kp = 4 * s + kq;
+            const bool in = kp < mz;
+            const int kpc = in ? kp : 0;
+            const double x1 = xr[kpc * kZpCols], x2 = xr[(nz - 1 - kpc) * kZpCols];
+            const T bs = in ? (T)(x1 + x2) : (T)0, bd = in ? (T)(x1 - x2) : (T)0;
+            const T t1 = rz[64 + kr - kp], t2 = rz[64 + (nz - 1 - kr - kp)];
+            as = kmfma(kp == hz ? t1 : t1 + t2, bs, as);
+            ad = kmfma(t1 - t2, bd, ad);
+        }
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int hq = kin_row<T>(kq, q), kk = 16 * m + hq;
+            if (!zok || kk >= mz) continue;
+            const int lp = hq * kZpCols + zc;
+            const double S = (double)as[q], D = (double)ad[q];
+            const double n1 = r1[m][q] + mask * (0.5 * (S + D) + part[2 * m][zrow][lp]);
+            const double x1 = xr[kk * kZpCols];
+            R[rbase + (size_t)kk * nxy] = n1;
+            dot += x1 * n1;
+            rr2 += n1 * n1;
+            vv2 += x1 * x1;
+            if (kk != hz) {
+                const int k2 = nz - 1 - kk;
+                const double n2 = r2[m][q] + mask * (0.5 * (S - D) + part[2 * m + 1][zrow][lp]);
+                const double x2 = xr[k2 * kZpCols];
+                R[rbase + (size_t)k2 * nxy] = n2;
+                dot += x2 * n2;
+                rr2 += n2 * n2;
+                vv2 += x2 * x2;
+            }
+        }
+    }
+}
+
+// the workgroup's three partial sums at index idx (fixed order)
+__device__ inline void zpot_partials(int idx, double dot, double rr2, double vv2, double *__restrict__ pa,
+                                     double *__restrict__ pb, double *__restrict__ pc)
+{
+    __shared__ double red[kZpThreads / 64];
+    const double s0 = block_sum(dot, red);
+    __syncthreads();
+    const double s1 = block_sum(rr2, red);
+    __syncthreads();
+    const double s2 = block_sum(vv2, red);
+    if (threadIdx.x == 0) {
+        pa[idx] = s0;
+        pb[idx] = s1;
+        pc[idx] = s2;
+    }
+}
+
+// dynamic LDS of a tile of a block of B planes on an axis of nz points
+// (gfx950: up to 160 KiB per workgroup; B = 21 at nz = 54 takes 80328 B,
+// two workgroups per CU)
+inline size_t zpot_lds(int B, int nz, size_t tsize)
+{
+    return sizeof(double) * ((size_t)B * nz * kZpCols + (size_t)zpot_mtiles(nz) * 2 * B * 64) + tsize * 129;
+}
+
+// the lmax-5 pair of blocks of the full basis (l = 0, 2, 4 from plane 0;
+// l = 1, 3, 5 from plane1): workgroup 2 t takes block 0 of tile t, 2 t + 1
+// block 1, so that the two share the tile's coefficient lines in one XCD's L2
+template <typename VT, typename T>
+__global__ __launch_bounds__(kZpThreads) __attribute__((amdgpu_waves_per_eu(3, 8))) void k_zpot2(
+    const double *__restrict__ X, double *__restrict__ R, const VT *__restrict__ H, const double *__restrict__ G0,
+    const double *__restrict__ G1, int nc, int plane1, double mask0, double mask1, const double *__restrict__ tz,
+    int nxy, int nz, int64_t P, double *__restrict__ pa, double *__restrict__ pb, double *__restrict__ pc)
+{
+    const int lin = xcd_linear((int)blockIdx.x, (int)gridDim.x);
+    double dot = 0.0, rr2 = 0.0, vv2 = 0.0;
+    if (lin & 1)
+        zpot_body<5, 1, VT, T>(lin >> 1, X, R, H, G1, nc, plane1, mask1, tz, nxy, nz, P, dot, rr2, vv2);
+    else
+        zpot_body<4, 0, VT, T>(lin >> 1, X, R, H, G0, nc, 0, mask0, tz, nxy, nz, P, dot, rr2, vv2);
+    zpot_partials(lin, dot, rr2, vv2, pa, pb, pc);
+}
+
+// one lmax-5 block of the reduced basis (l = 0, 2, 4 or l = 1, 3, 5)
+template <int LB, int PAR, typename VT, typename T>
+__global__ __launch_bounds__(kZpThreads) __attribute__((amdgpu_waves_per_eu(3, 8))) void k_zpot1(
+    const double *__restrict__ X, double *__restrict__ R, const VT *__restrict__ H, const double *__restrict__ G0,
+    int nc, double mask0, const double *__restrict__ tz, int nxy, int nz, int64_t P, double *__restrict__ pa,
+    double *__restrict__ pb, double *__restrict__ pc)
+{
+    const int lin = xcd_linear((int)blockIdx.x, (int)gridDim.x);
+    double dot = 0.0, rr2 = 0.0, vv2 = 0.0;
+    zpot_body<LB, PAR, VT, T>(lin, X, R, H, G0, nc, 0, mask0, tz, nxy, nz, P, dot, rr2, vv2);
+    zpot_partials(lin, dot, rr2, vv2, pa, pb, pc);
+}
+
+// ---------------------------------------------------------------------------
 // recurrence scalars
 // ---------------------------------------------------------------------------
 // mode 0 (end of a step): finalize_step.
@@ -1341,6 +1559,63 @@ static int launch_vgaunt_t(const DevOperator &op, const double *X, double *R, hi
 }
 
 // ---------------------------------------------------------------------------
+// fused z kinetic + potential launches
+// ---------------------------------------------------------------------------
+bool zpot_fits(const DevOperator &op)
+{
+    if (op.calc == BSL_CALC_QUAD || op.vform != BSL_VFORM_HARMONICS || !kinetic_mfma_fits(op)) return false;
+    if (!((op.kprec == BSL_PREC_FP64 && op.vprec == BSL_PREC_FP64) ||
+          (op.kprec == BSL_PREC_FP32 && op.vprec == BSL_PREC_FP32)))
+        return false;
+    const Block &b0 = op.blk[0], &b1 = op.nblk > 1 ? op.blk[1] : op.blk[0];
+    const int split = harmonic_split(op.nblk, b0.lb, b0.par, op.nblk > 1 ? b1.lb : -1, b1.par, b1.plane0, b0.size);
+    if (split == 1) return op.ncoef == gaunt::ncoef(5);
+    if (split == 2) return op.ncoef == gaunt::ncoef(b0.lb);
+    return false;
+}
+
+int64_t zpot_workgroups(const DevOperator &op)
+{
+    const int64_t nt = (op.nx * op.ny + kZpCols - 1) / kZpCols;
+    return op.nblk > 1 ? 2 * nt : nt;
+}
+
+template <typename VT, typename T>
+static int launch_zpot_t(const DevOperator &op, const double *X, double *R, hipStream_t s)
+{
+    const VT *H = (const VT *)op.vpk;
+    const int nxy = op.nx * op.ny;
+    const dim3 grid((unsigned)zpot_workgroups(op));
+    if ((int64_t)grid.x > op.npartial || (int64_t)grid.x != op.nalpha)
+        return fail(BSL_E_STATE, "partial buffer does not match the fused kernel grid");
+    const Block &b0 = op.blk[0];
+    if (op.nblk > 1) {
+        const Block &b1 = op.blk[1];
+        timed_launch((k_zpot2<VT, T>), grid, dim3(kZpThreads), (uint32_t)zpot_lds(b1.size, op.nz, sizeof(T)), s, X,
+                     R, H, op.gc[0], op.gc[1], op.ncoef, b1.plane0, b0.mask, b1.mask, op.tz, nxy, op.nz, op.P,
+                     op.partial, op.partial_b, op.partial_c);
+    } else if (b0.par == 1) {
+        timed_launch((k_zpot1<5, 1, VT, T>), grid, dim3(kZpThreads), (uint32_t)zpot_lds(b0.size, op.nz, sizeof(T)),
+                     s, X, R, H, op.gc[0], op.ncoef, b0.mask, op.tz, nxy, op.nz, op.P, op.partial, op.partial_b,
+                     op.partial_c);
+    } else {
+        timed_launch((k_zpot1<4, 0, VT, T>), grid, dim3(kZpThreads), (uint32_t)zpot_lds(b0.size, op.nz, sizeof(T)),
+                     s, X, R, H, op.gc[0], op.ncoef, b0.mask, op.tz, nxy, op.nz, op.P, op.partial, op.partial_b,
+                     op.partial_c);
+    }
+    return hipGetLastError() == hipSuccess ? BSL_OK : fail(BSL_E_HIP, "k_zpot launch");
+}
+
+int launch_zpot(const DevOperator &op, const double *X, double *R, hipStream_t s)
+{
+    if (!zpot_fits(op)) return fail(BSL_E_STATE, "no fused z-kinetic and potential kernel for this operator");
+    if (!op.vpk || !op.gc[0] || (op.nblk > 1 && !op.gc[1]))
+        return fail(BSL_E_STATE, "the context has no potential to apply");
+    return op.vprec == BSL_PREC_FP64 ? launch_zpot_t<double, double>(op, X, R, s)
+                                     : launch_zpot_t<float, float>(op, X, R, s);
+}
+
+// ---------------------------------------------------------------------------
 // batch kernels: the bodies above over a table of members, one launch per
 // variant.  Workgroup vb belongs to the member whose [first, first + count)
 // holds it (at most 64 members per table).  The member's workgroups run exactly the
@@ -1399,6 +1674,21 @@ __global__ __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(LB == 5 ? 3
                                     e.pc);
 }
 
+// fused z kinetic + potential of one reduced-basis lmax-5 block (kinetic
+// and potential precision VT); workgroup = the member's tile, as in k_zpot1
+template <int LB, int PAR, typename VT>
+__global__ __launch_bounds__(kZpThreads) __attribute__((amdgpu_waves_per_eu(3, 8))) void kb_zpot(
+    const BatchEntry *__restrict__ tab, int nm, const double *__restrict__ G)
+{
+    const int vb = xcd_linear((int)blockIdx.x, (int)gridDim.x), m = batch_find(tab, nm, vb);
+    const BatchEntry &e = tab[m];
+    const int tile = vb - e.first;
+    double dot = 0.0, rr2 = 0.0, vv2 = 0.0;
+    zpot_body<LB, PAR, VT, VT>(tile, e.X, e.R, (const VT *)e.H, G, e.nc, 0, e.mask0, e.tz, e.nx * e.ny, e.nz, e.P,
+                               dot, rr2, vv2);
+    zpot_partials(tile, dot, rr2, vv2, e.pa, e.pb, e.pc);
+}
+
 __global__ __launch_bounds__(kWG) void kb_finalize(const BatchEntry *__restrict__ tab, int nm)
 {
     __shared__ double red[kWG / 64];
@@ -1408,7 +1698,8 @@ __global__ __launch_bounds__(kWG) void kb_finalize(const BatchEntry *__restrict_
 }
 
 template <typename T, int KM>
-static int launch_batch_t(int kind, const BatchEntry *tab, int nm, int total, const double *G, hipStream_t s)
+static int launch_batch_t(int kind, const BatchEntry *tab, int nm, int total, const double *G, int lds_nz,
+                          hipStream_t s)
 {
     const dim3 grid((unsigned)total);
     switch (kind) {
@@ -1426,6 +1717,14 @@ static int launch_batch_t(int kind, const BatchEntry *tab, int nm, int total, co
     case kBPotSplitOrtho:
         hipLaunchKernelGGL((kb_split1<5, 1, T>), grid, dim3(192), 0, s, tab, nm, G);
         break;
+    case kBZPotPara:  // LDS for the longest z axis among the members
+        hipLaunchKernelGGL((kb_zpot<4, 0, T>), grid, dim3(kZpThreads),
+                           zpot_lds(gaunt::block_size(4, 0), lds_nz, sizeof(T)), s, tab, nm, G);
+        break;
+    case kBZPotOrtho:
+        hipLaunchKernelGGL((kb_zpot<5, 1, T>), grid, dim3(kZpThreads),
+                           zpot_lds(gaunt::block_size(5, 1), lds_nz, sizeof(T)), s, tab, nm, G);
+        break;
     default:
         return fail(BSL_E_ARG, "unknown batch kernel");
     }
@@ -1433,8 +1732,11 @@ static int launch_batch_t(int kind, const BatchEntry *tab, int nm, int total, co
     return BSL_OK;
 }
 
-int launch_batch(int kind, int prec, const BatchEntry *d_tab, int nm, int total, const double *G, hipStream_t s)
+int launch_batch(int kind, int prec, const BatchEntry *d_tab, int nm, int total, const double *G, int lds_nz,
+                 hipStream_t s)
 {
+    if ((kind == kBZPotPara || kind == kBZPotOrtho) && (lds_nz < 1 || lds_nz > 64))
+        return fail(BSL_E_ARG, "fused batch kernel: z axis out of range");
     if (nm <= 0 || nm > kBatchMaxTable || total <= 0) return fail(BSL_E_ARG, "batch table size out of range");
     if (kind == kBFinalize) {
         hipLaunchKernelGGL(kb_finalize, dim3((unsigned)nm), dim3(kWG), 0, s, d_tab, nm);
@@ -1443,10 +1745,10 @@ int launch_batch(int kind, int prec, const BatchEntry *d_tab, int nm, int total,
     }
     if (prec == BSL_PREC_BF16X3) {
         if (kind != kBXyHalf && kind != kBZHalf) return fail(BSL_E_ARG, "BF16x3 batch tables are kinetic only");
-        return launch_batch_t<float, 1>(kind, d_tab, nm, total, G, s);
+        return launch_batch_t<float, 1>(kind, d_tab, nm, total, G, lds_nz, s);
     }
-    return prec == BSL_PREC_FP64 ? launch_batch_t<double, 0>(kind, d_tab, nm, total, G, s)
-                                 : launch_batch_t<float, 0>(kind, d_tab, nm, total, G, s);
+    return prec == BSL_PREC_FP64 ? launch_batch_t<double, 0>(kind, d_tab, nm, total, G, lds_nz, s)
+                                 : launch_batch_t<float, 0>(kind, d_tab, nm, total, G, lds_nz, s);
 }
 
 int launch_potential(const DevOperator &op, const double *X, double *R, Scal *sc, hipStream_t s)
@@ -1478,11 +1780,21 @@ int launch_hv(const DevOperator &op, double *X, double *R, Scal *sc, hipStream_t
 {
     LaunchTimer &t = launch_timer();
     if (ev) t = {ev[0], ev[1]};
-    int krc = launch_kinetic(op, X, R, sc, s);
+    int krc;
+    if (op.fused)
+        krc = op.kprec == BSL_PREC_FP64 ? launch_kinetic_t<double, 0>(op, X, R, sc, s, false)
+                                        : launch_kinetic_t<float, 0>(op, X, R, sc, s, false);
+    else
+        krc = launch_kinetic(op, X, R, sc, s);
     if (ev) t = {ev[2], ev[3]};
-    int rc = krc ? krc : launch_potential(op, X, R, sc, s);
+    int rc = krc ? krc : launch_potential_part(op, X, R, sc, s);
     t = {};
     return rc;
+}
+
+int launch_potential_part(const DevOperator &op, const double *X, double *R, Scal *sc, hipStream_t s)
+{
+    return op.fused ? launch_zpot(op, X, R, s) : launch_potential(op, X, R, sc, s);
 }
 
 int launch_finalize(const DevOperator &op, Scal *sc, double *alpha_out, double *beta_out, double *beta2_out,

@@ -5,7 +5,7 @@
  * Plain C types only.  Every entry point returns 0 on success and a non-zero
  * BSL_E_* code on failure; bsl_last_error() then describes the failure (per
  * thread).  The entry points that return a count or a value instead
- * (bsl_get_potential_form, bsl_gaunt_table) return minus the BSL_E_* code on
+ * (bsl_get_potential_form, bsl_get_fusion, bsl_gaunt_table) return minus the BSL_E_* code on
  * failure, so that no error reads as a valid result.  A context is not
  * thread-safe: calls on one context must be serialised, as the reference
  * driver serialises its Hv calls.  Several contexts may coexist.
@@ -258,6 +258,20 @@ int bsl_get_vgrid(bsl_ctx *ctx, double *v1, double *v2);
  * FP64 NoQuad operator the context keeps for its FP64 passes. */
 int bsl_get_potential_form(const bsl_ctx *ctx); /* BSL_VFORM_*, or -BSL_E_ARG */
 int bsl_set_potential_form(bsl_ctx *ctx, int32_t form);
+
+/* Kernel structure of a NOQUAD step: 0 = x-y kinetic, z kinetic, potential
+ * (three kernels), 1 = x-y kinetic, then one kernel applying the z kinetic
+ * term and the potential (one read of X and one read-modify-write of R
+ * where the separate kernels make two).  Both compute Tv_5D + Vv_5D
+ * (linRotCartSphHvRoutines.cpp:1872-1969, linRotCartSphHvFullVPrepRoutines.cpp:675-734)
+ * to rounding; the fused kernel exists for the lmax-5 harmonic-form blocks
+ * (the full basis pair or one reduced-basis block) with every axis of at
+ * most 64 points, in FP64 or FP32 (not BF16X3).  Contexts start with (and
+ * bsl_set_potential_form returns to) the three-kernel step, which is the
+ * faster one on MI355X; bsl_set_fusion(ctx, 1) selects the fused kernel and
+ * fails with BSL_E_ARG where it does not exist.  No reference counterpart. */
+int bsl_get_fusion(const bsl_ctx *ctx); /* 0 or 1, or -BSL_E_ARG */
+int bsl_set_fusion(bsl_ctx *ctx, int32_t on);
 
 /* The harmonic-form term list of parity block (lb, par): the planes are
  * every l of parity par up to lb in the reference lm order, the columns the

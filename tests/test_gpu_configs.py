@@ -72,12 +72,18 @@ def cage54():
 
 
 def test_cfg2_hv_54_fp64_matches_oracle(cage54):
+    """Both step structures at the full size: the three-kernel step (the
+    default) and the fused z-kinetic + potential kernel."""
     kw, v, ref = cage54
     with B.Calculator(**kw) as c:
         assert c.N == 54 ** 3 * 36
-        assert c.potential_kernel().startswith("k_vgaunt_split<double")
+        assert not c.fused and c.potential_kernel().startswith("k_vgaunt_split<double")
         got = c.hv(v)
+        c.fused = True
+        assert c.potential_kernel().startswith("k_zpot2<double")
+        gotf = c.hv(v)
     assert _rel(got, ref) < 1e-12
+    assert _rel(gotf, ref) < 1e-12
 
 
 def test_cfg2_hv_54_fp32_close_to_oracle(cage54):

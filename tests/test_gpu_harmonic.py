@@ -38,7 +38,7 @@ def test_harmonic_hv_matches_oracle_and_blocks(lmax, sym, basis):
     spec, kw = cs.cage_case(5, lmax, na, nb, sym)
     o = O.Oracle(spec)
     with B.Calculator(basis=basis, **kw) as c:
-        assert c.potential_form == "harmonics"
+        assert c.potential_form == "harmonics" and not c.fused
         assert c.potential_kernel().startswith("k_vgaunt")
         v = np.random.default_rng(100 + lmax).uniform(-1, 1, o.N)
         u0 = np.random.default_rng(200 + lmax).uniform(-1, 1, o.N)
@@ -47,7 +47,7 @@ def test_harmonic_hv_matches_oracle_and_blocks(lmax, sym, basis):
         h = c.hv(v, u0)
         assert _rel(h, ref) < 1e-12
         c.potential_form = "blocks"
-        assert c.potential_form == "blocks" and c.potential_kernel().startswith("k_vblock<")
+        assert c.potential_form == "blocks" and c.potential_kernel().startswith("k_vblock<") and not c.fused
         b = c.hv(v, u0)
         assert _rel(b, ref) < 1e-12
         assert _rel(h, b) < 1e-13

@@ -437,6 +437,19 @@ def main():
     print("fp64: prep %.2f s, %.1f steps/s, per step %.3f ms (potential %.3f, kinetic %.3f, finalize %.3f)"
           % (prep64, steps_s, ms[0] / args.steps, ms[1], ms[2], ms[3]), file=sys.stderr, flush=True)
 
+    # the same operator with the z kinetic term fused into the potential
+    # kernel (bsl_set_fusion; opt-in: fewer bytes per step, slower here)
+    fused = None
+    c64.fused = True
+    msf = c64.time_steps(args.warmup, args.steps)
+    sbf = c64.step_bytes()
+    fused = {"steps_per_s": args.steps / (msf[0] / 1e3), "kernel": c64.potential_kernel(),
+             "per_step_ms": {"fused_z_potential": msf[1], "xy_kinetic": msf[2], "finalize": msf[3]},
+             "algorithmic_bytes_per_step": sbf[0] + sbf[1]}
+    c64.fused = False
+    print("fp64 fused z+potential: %.1f steps/s (fused kernel %.3f ms, x-y %.3f ms)"
+          % (fused["steps_per_s"], msf[1], msf[2]), file=sys.stderr, flush=True)
+
     c32, steps_s32, prep32 = None, None, None
     if not args.no_mixed:
         t0 = time.perf_counter()
@@ -528,11 +541,13 @@ def main():
             "mixed_bf16x3": bf,
             "prep_s": {"fp64": prep64, "fp32": prep32},
             "per_step_ms": {"potential": ms[1], "kinetic": ms[2], "finalize": ms[3]},
+            "fused_z_potential_step": fused,
             "hv_per_s": 1e3 / (ms[1] + ms[2]),
             "quadrature": quad,
             "batch_32": batch,
             "adjoint": adj,
-            "algorithmic_bytes_per_step": sb[3],
+            "algorithmic_bytes_per_step": sb[0] + sb[1],
+            "floor_bytes_per_step": sb[3],
             "step_bytes_roofline_frac": sb[3] / (ms[0] / args.steps * 1e-3) / 1e9 / HBM_PEAK_GBS,
         },
         "roofline": {

@@ -42,6 +42,29 @@ __device__ inline double block_sum(double v, double *lds)
     return s;
 }
 
+// three sums over the workgroup at once (one barrier); each is the sum
+// block_sum would give, bitwise; valid in thread 0.  lds: 3 per wave.
+__device__ inline void block_sum3(double a, double b, double c, double *lds, double &sa, double &sb, double &sc)
+{
+    a = wave_sum(a);
+    b = wave_sum(b);
+    c = wave_sum(c);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) {
+        lds[3 * wid] = a;
+        lds[3 * wid + 1] = b;
+        lds[3 * wid + 2] = c;
+    }
+    __syncthreads();
+    sa = sb = sc = 0.0;
+    if (threadIdx.x == 0)
+        for (int w = 0; w < (int)(blockDim.x >> 6); w++) {
+            sa += lds[3 * w];
+            sb += lds[3 * w + 1];
+            sc += lds[3 * w + 2];
+        }
+}
+
 // Fixed-order total of partial[0..n) computed by one workgroup; valid in
 // every thread.  Partials are written with plain stores by the kernel before
 // (kernel boundaries make them visible), so no device-scope fence or atomic
@@ -93,11 +116,9 @@ __device__ inline void finalize_step(const double *__restrict__ pa, const double
         sb[0] += pb[i];
         sv[0] += pc[i];
     }
-    const double t = block_sum((sa[0] + sa[1]) + (sa[2] + sa[3]), red);
-    __syncthreads();
-    const double rr = block_sum((sb[0] + sb[1]) + (sb[2] + sb[3]), red);
-    __syncthreads();
-    const double vv = block_sum((sv[0] + sv[1]) + (sv[2] + sv[3]), red);
+    double t, rr, vv;
+    block_sum3((sa[0] + sa[1]) + (sa[2] + sa[3]), (sb[0] + sb[1]) + (sb[2] + sb[3]),
+               (sv[0] + sv[1]) + (sv[2] + sv[3]), red, t, rr, vv);
     if (threadIdx.x != 0) return;
     const double alpha = t;
     const double b2 = rr - 2.0 * alpha * t + alpha * alpha * vv;
@@ -649,7 +670,7 @@ __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(W, 8))) voi
     double mask0, double mask1, int64_t P, double *__restrict__ partial, double *__restrict__ partial_b,
     double *__restrict__ partial_c)
 {
-    __shared__ double red[kWG / 64];
+    __shared__ double red[3 * kWG / 64];
     double dot = 0.0, rr2 = 0.0, vv2 = 0.0;
     const int p = (int)(blockIdx.x * blockDim.x + threadIdx.x);
     // the larger block's workgroups are dispatched first (blockIdx.y = 0), so
@@ -662,11 +683,8 @@ __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(W, 8))) voi
             vblock_point<(B1 > 0 ? B1 : 1), VT>(X, R, V1, B0, mask1, P, p, dot, rr2, vv2);
     }
     const int w = blockIdx.y * gridDim.x + blockIdx.x;
-    const double s0 = block_sum(dot, red);
-    __syncthreads();
-    const double s1 = block_sum(rr2, red);
-    __syncthreads();
-    const double s2 = block_sum(vv2, red);
+    double s0, s1, s2;
+    block_sum3(dot, rr2, vv2, red, s0, s1, s2);
     if (threadIdx.x == 0) {
         partial[w] = s0;
         partial_b[w] = s1;
@@ -696,7 +714,7 @@ __global__ __launch_bounds__(kWG) void k_vblock_rows(const double *__restrict__ 
                                                      double *__restrict__ partial_c)
 {
     extern __shared__ double xrow[];  // [B][64]
-    __shared__ double red[kWG / 64];
+    __shared__ double red[3 * kWG / 64];
     const int blk = (int)blockIdx.y;
     const int B = blk == 0 ? B0 : B1, plane0 = blk == 0 ? 0 : B0;
     const double mask = blk == 0 ? mask0 : mask1;
@@ -731,11 +749,8 @@ __global__ __launch_bounds__(kWG) void k_vblock_rows(const double *__restrict__ 
         }
     }
     const int w = blockIdx.y * gridDim.x + blockIdx.x;
-    const double s0 = block_sum(dot, red);
-    __syncthreads();
-    const double s1 = block_sum(rr2, red);
-    __syncthreads();
-    const double s2 = block_sum(vv2, red);
+    double s0, s1, s2;
+    block_sum3(dot, rr2, vv2, red, s0, s1, s2);
     if (threadIdx.x == 0) {
         partial[w] = s0;
         partial_b[w] = s1;
@@ -820,7 +835,7 @@ __global__ __launch_bounds__((LB1 >= 0 ? 2 : 1) * kWG) __attribute__((amdgpu_wav
     const double *__restrict__ G1, int nc, int plane1, double mask0, double mask1, int64_t P,
     double *__restrict__ partial, double *__restrict__ partial_b, double *__restrict__ partial_c)
 {
-    __shared__ double red[2 * kWG / 64];
+    __shared__ double red[3 * 2 * kWG / 64];
     double dot = 0.0, rr2 = 0.0, vv2 = 0.0;
     const int half = (int)threadIdx.x / kWG;
     const int p = (int)blockIdx.x * kWG + ((int)threadIdx.x & (kWG - 1));
@@ -834,11 +849,8 @@ __global__ __launch_bounds__((LB1 >= 0 ? 2 : 1) * kWG) __attribute__((amdgpu_wav
             gaunt_point<LB0, PAR0, VT>(X, R, H, G0, nc, 0, mask0, P, p, dot, rr2, vv2);
         }
     }
-    const double s0 = block_sum(dot, red);
-    __syncthreads();
-    const double s1 = block_sum(rr2, red);
-    __syncthreads();
-    const double s2 = block_sum(vv2, red);
+    double s0, s1, s2;
+    block_sum3(dot, rr2, vv2, red, s0, s1, s2);
     if (threadIdx.x == 0) {
         partial[blockIdx.x] = s0;
         partial_b[blockIdx.x] = s1;
@@ -912,13 +924,31 @@ __device__ __forceinline__ void sub_run2(const double *__restrict__ xpa, const d
     (sub_store<PB, RBs>(accb, partb, lane), ...);
 }
 
-// xs: the LDS rows [plane][64] of the workgroup's points
+// x0: this lane's entry of plane 0 (the workgroup's LDS rows, stride 64, or
+// X itself, stride P)
 template <class Pat, typename VT>
-__device__ __forceinline__ void gaunt_sublist(const double (*xs)[64], const VT *__restrict__ hp,
+__device__ __forceinline__ void gaunt_sublist(const double *x0, int64_t xst, const VT *__restrict__ hp,
                                               const double *__restrict__ G, int plane0, double (*part)[64], int lane)
 {
-    sub_run<Pat, VT>(&xs[plane0][lane], hp, G, 64, part, lane, std::make_integer_sequence<int, Pat::B>{},
+    sub_run<Pat, VT>(x0 + (size_t)plane0 * xst, hp, G, xst, part, lane, std::make_integer_sequence<int, Pat::B>{},
                      std::make_integer_sequence<int, Pat::NC>{});
+}
+
+// Where the split kernels' sub-lists read X: the FP64-coefficient kernels
+// stage the group's rows in LDS once (coalesced, each row fetched once); the
+// FP32-coefficient kernels, whose coefficient loads are half the size, read
+// X directly (one memory round trip fewer: 42.9 against 45.1 us at 54^3).
+template <typename VT>
+constexpr bool split_reads_x_directly()
+{
+    return sizeof(VT) == sizeof(float);
+}
+
+// part zeroed by the NT threads
+template <int NR, int NT>
+__device__ __forceinline__ void zero_rows(double (*part)[64])
+{
+    for (int t = (int)threadIdx.x; t < NR * 64; t += NT) (&part[0][0])[t] = 0.0;
 }
 
 // The workgroup's X rows [0, NR) at points group*64 + [0, 64) into xs (the
@@ -964,32 +994,38 @@ __device__ __forceinline__ void vgaunt_split_body(int group, const double *__res
     static_assert(NR % 4 == 0, "rows are shared out evenly over the four waves");
     static_assert(gaunt::split_covers_rows<4, 0>() && gaunt::split_covers_rows<5, 1>(),
                   "every row needs a same-type and a cross partial");
-    __shared__ double xs[NR][64];    // X rows of the 64 points (even rows, then odd rows)
-    __shared__ double part[NR][64];  // same-type + cross partials [row][lane]
-    __shared__ double red[4];
+    constexpr bool kDirect = split_reads_x_directly<VT>();
+    __shared__ double xs[kDirect ? 1 : NR][64];  // X rows of the 64 points (even rows, then odd rows)
+    __shared__ double part[NR][64];              // same-type + cross partials [row][lane]
+    __shared__ double red[3 * 4];
     const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
     const int lane = (int)threadIdx.x & 63;
     const int p = group * 64 + lane;
     const bool live = p < P;
     const int pl = live ? p : (int)(P - 1);
     const VT *hp = H + (size_t)(pl / kVTile) * nc * kVTile + pl % kVTile;
-    stage_rows<NR, 256>(X, P, group, xs, part);
+    if constexpr (kDirect)
+        zero_rows<NR, 256>(part);
+    else
+        stage_rows<NR, 256>(X, P, group, xs, part);
     __syncthreads();
+    const double *x0 = kDirect ? X + pl : &xs[0][lane];
+    const int64_t xst = kDirect ? P : 64;
     if (wave == 0) {
-        gaunt_sublist<gaunt::Pattern<5, 1, gaunt::kCross>, VT>(xs, hp, G1, plane1, &part[B0], lane);
+        gaunt_sublist<gaunt::Pattern<5, 1, gaunt::kCross>, VT>(x0, xst, hp, G1, plane1, &part[B0], lane);
     } else if (wave == 1) {
-        gaunt_sublist<gaunt::Pattern<5, 1, gaunt::kCC>, VT>(xs, hp, G1, plane1, &part[B0], lane);
+        gaunt_sublist<gaunt::Pattern<5, 1, gaunt::kCC>, VT>(x0, xst, hp, G1, plane1, &part[B0], lane);
     } else if (wave == 2) {
         // the even sin-sin columns (M >= 0, L <= 8) are among the odd ones
         using PA = gaunt::Pattern<5, 1, gaunt::kSS>;
         using PB = gaunt::Pattern<4, 0, gaunt::kSS>;
         static_assert(gaunt::ncoef(4) <= gaunt::ncoef(5), "column prefix");
-        sub_run2<PA, PB, VT>(&xs[plane1][lane], &xs[0][lane], hp, G1, G0, 64, &part[B0], &part[0], lane,
+        sub_run2<PA, PB, VT>(x0 + (size_t)plane1 * xst, x0, hp, G1, G0, xst, &part[B0], &part[0], lane,
                              std::make_integer_sequence<int, PA::B>{}, std::make_integer_sequence<int, PB::B>{},
                              std::make_integer_sequence<int, PA::NC>{});
     } else {
-        gaunt_sublist<gaunt::Pattern<4, 0, gaunt::kCC>, VT>(xs, hp, G0, 0, &part[0], lane);
-        gaunt_sublist<gaunt::Pattern<4, 0, gaunt::kCross>, VT>(xs, hp, G0, 0, &part[0], lane);
+        gaunt_sublist<gaunt::Pattern<4, 0, gaunt::kCC>, VT>(x0, xst, hp, G0, 0, &part[0], lane);
+        gaunt_sublist<gaunt::Pattern<4, 0, gaunt::kCross>, VT>(x0, xst, hp, G0, 0, &part[0], lane);
     }
     __syncthreads();
     // rows [9 w, 9 w + 9) of (even rows | odd rows); planes are contiguous
@@ -1001,7 +1037,7 @@ __device__ __forceinline__ void vgaunt_split_body(int group, const double *__res
 #pragma unroll
         for (int k = 0; k < NW; k++) {
             rr[k] = R[(size_t)(r0 + k) * P + p];
-            xx[k] = xs[r0 + k][lane];
+            xx[k] = kDirect ? X[(size_t)(r0 + k) * P + p] : xs[r0 + k][lane];
         }
 #pragma unroll
         for (int k = 0; k < NW; k++) {
@@ -1014,11 +1050,8 @@ __device__ __forceinline__ void vgaunt_split_body(int group, const double *__res
             vv2 += xx[k] * xx[k];
         }
     }
-    const double s0 = block_sum(dot, red);
-    __syncthreads();
-    const double s1 = block_sum(rr2, red);
-    __syncthreads();
-    const double s2 = block_sum(vv2, red);
+    double s0, s1, s2;
+    block_sum3(dot, rr2, vv2, red, s0, s1, s2);
     if (threadIdx.x == 0) {
         partial[group] = s0;
         partial_b[group] = s1;
@@ -1048,23 +1081,29 @@ __device__ __forceinline__ void vgaunt_split1_body(int group, const double *__re
 {
     constexpr int B = gaunt::block_size(LB, PAR), NW = (B + 2) / 3;
     static_assert(gaunt::split_covers_rows<LB, PAR>(), "every row needs a same-type and a cross partial");
-    __shared__ double xs[B][64];    // X rows of the 64 points
-    __shared__ double part[B][64];  // same-type + cross partials [row][lane]
-    __shared__ double red[3];
+    constexpr bool kDirect = split_reads_x_directly<VT>();
+    __shared__ double xs[kDirect ? 1 : B][64];  // X rows of the 64 points
+    __shared__ double part[B][64];              // same-type + cross partials [row][lane]
+    __shared__ double red[3 * 3];
     const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
     const int lane = (int)threadIdx.x & 63;
     const int p = group * 64 + lane;
     const bool live = p < P;
     const int pl = live ? p : (int)(P - 1);
     const VT *hp = H + (size_t)(pl / kVTile) * nc * kVTile + pl % kVTile;
-    stage_rows<B, 192>(X, P, group, xs, part);
-    __syncthreads();
-    if (wave == 0)
-        gaunt_sublist<gaunt::Pattern<LB, PAR, gaunt::kCross>, VT>(xs, hp, G0, 0, &part[0], lane);
-    else if (wave == 1)
-        gaunt_sublist<gaunt::Pattern<LB, PAR, gaunt::kCC>, VT>(xs, hp, G0, 0, &part[0], lane);
+    if constexpr (kDirect)
+        zero_rows<B, 192>(part);
     else
-        gaunt_sublist<gaunt::Pattern<LB, PAR, gaunt::kSS>, VT>(xs, hp, G0, 0, &part[0], lane);
+        stage_rows<B, 192>(X, P, group, xs, part);
+    __syncthreads();
+    const double *x0 = kDirect ? X + pl : &xs[0][lane];
+    const int64_t xst = kDirect ? P : 64;
+    if (wave == 0)
+        gaunt_sublist<gaunt::Pattern<LB, PAR, gaunt::kCross>, VT>(x0, xst, hp, G0, 0, &part[0], lane);
+    else if (wave == 1)
+        gaunt_sublist<gaunt::Pattern<LB, PAR, gaunt::kCC>, VT>(x0, xst, hp, G0, 0, &part[0], lane);
+    else
+        gaunt_sublist<gaunt::Pattern<LB, PAR, gaunt::kSS>, VT>(x0, xst, hp, G0, 0, &part[0], lane);
     __syncthreads();
     double dot = 0.0, rr2 = 0.0, vv2 = 0.0;
     if (live) {
@@ -1074,7 +1113,7 @@ __device__ __forceinline__ void vgaunt_split1_body(int group, const double *__re
         for (int k = 0; k < NW; k++) {
             const int r = r0 + k < B ? r0 + k : B - 1;
             rr[k] = R[(size_t)r * P + p];
-            xx[k] = xs[r][lane];
+            xx[k] = kDirect ? X[(size_t)r * P + p] : xs[r][lane];
         }
 #pragma unroll
         for (int k = 0; k < NW; k++) {
@@ -1088,11 +1127,8 @@ __device__ __forceinline__ void vgaunt_split1_body(int group, const double *__re
             }
         }
     }
-    const double s0 = block_sum(dot, red);
-    __syncthreads();
-    const double s1 = block_sum(rr2, red);
-    __syncthreads();
-    const double s2 = block_sum(vv2, red);
+    double s0, s1, s2;
+    block_sum3(dot, rr2, vv2, red, s0, s1, s2);
     if (threadIdx.x == 0) {
         partial[group] = s0;
         partial_b[group] = s1;
@@ -1274,12 +1310,9 @@ __device__ __forceinline__ void zpot_body(int tile, const double *__restrict__ X
 __device__ inline void zpot_partials(int idx, double dot, double rr2, double vv2, double *__restrict__ pa,
                                      double *__restrict__ pb, double *__restrict__ pc)
 {
-    __shared__ double red[kZpThreads / 64];
-    const double s0 = block_sum(dot, red);
-    __syncthreads();
-    const double s1 = block_sum(rr2, red);
-    __syncthreads();
-    const double s2 = block_sum(vv2, red);
+    __shared__ double red[3 * kZpThreads / 64];
+    double s0, s1, s2;
+    block_sum3(dot, rr2, vv2, red, s0, s1, s2);
     if (threadIdx.x == 0) {
         pa[idx] = s0;
         pb[idx] = s1;
@@ -1336,7 +1369,7 @@ __global__ __launch_bounds__(kWG) void k_finalize(const double *__restrict__ pa,
                                                   Scal *__restrict__ sc, int mode, double *alpha_out,
                                                   double *beta_out, double *beta2_out)
 {
-    __shared__ double red[kWG / 64];
+    __shared__ double red[3 * kWG / 64];
     if (mode == 0) {
         finalize_step(pa, pb, pc, n, sc, alpha_out, beta_out, beta2_out, red);
         return;
@@ -1387,7 +1420,7 @@ __global__ __launch_bounds__(kWG) void k_start_random(double *X, int64_t n, int6
                                                       const int *__restrict__ plane_ref, uint64_t seed,
                                                       double *partial)
 {
-    __shared__ double red[kWG / 64];
+    __shared__ double red[3 * kWG / 64];
     double s = 0.0;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         const int64_t plane = i / P;
@@ -1422,7 +1455,7 @@ __global__ void k_ritz_accum(double *ritz, const double *__restrict__ X, const d
 __global__ __launch_bounds__(kWG) void k_rayleigh_dots(const double *__restrict__ y, const double *__restrict__ hy,
                                                        int64_t n, double e, int mode, double *partial)
 {
-    __shared__ double red[kWG / 64];
+    __shared__ double red[3 * kWG / 64];
     double s0 = 0.0, s1 = 0.0;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         const double a = y[i], b = hy[i];
@@ -1691,7 +1724,7 @@ __global__ __launch_bounds__(kZpThreads) __attribute__((amdgpu_waves_per_eu(3, 8
 
 __global__ __launch_bounds__(kWG) void kb_finalize(const BatchEntry *__restrict__ tab, int nm)
 {
-    __shared__ double red[kWG / 64];
+    __shared__ double red[3 * kWG / 64];
     if ((int)blockIdx.x >= nm) return;
     const BatchEntry &e = tab[blockIdx.x];
     finalize_step(e.pa, e.pb, e.pc, (unsigned)e.nalpha, e.sc, e.alpha, e.beta, e.beta2, red);

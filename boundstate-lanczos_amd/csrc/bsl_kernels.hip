@@ -1696,9 +1696,16 @@ __global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
 // The Gaunt values of a block depend on its shape only, so every member of
 // the launch shares them; they come in as a kernel argument (read-only, so
 // that the unrolled term list takes them through scalar loads).
-// (the ortho block at 3 waves/SIMD: at 4 the table lookup's registers spill)
+// Occupancy: the ortho block with FP64 coefficients at 3 waves/SIMD (at 4
+// the table lookup's registers spill), with FP32 ones (X read directly) at
+// 4; the para block at 4, or 5 with FP32 coefficients.
 template <int LB, int PAR, typename VT>
-__global__ __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(LB == 5 ? 3 : 4, 8))) void kb_split1(
+constexpr int kb_split1_waves()
+{
+    return LB == 5 ? (sizeof(VT) == 8 ? 3 : 4) : (sizeof(VT) == 8 ? 4 : 5);
+}
+template <int LB, int PAR, typename VT>
+__global__ __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(kb_split1_waves<LB, PAR, VT>(), 8))) void kb_split1(
     const BatchEntry *__restrict__ tab, int nm, const double *__restrict__ G)
 {
     const int vb = (int)blockIdx.x, m = batch_find(tab, nm, vb);

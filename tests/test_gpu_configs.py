@@ -7,6 +7,12 @@ against the CPU oracle.
 * cfg2 at its full grid: one Hv of the synthetic 5^12 cage at 54^3 x 36 (the
   benchmark operator, full basis) on a 12 x 14 quadrature, FP64 and FP32,
   on the reference's production box (x_max 0.4 nm).
+* cfg3 at its full size: the benchmark operator (54^3, full basis, 40 x 70)
+  run 10000 steps in FP64 and with FP32 Hv from the same start; the lowest
+  20 good levels of the FP32 run within 0.01 cm^-1 of the FP64 ones, and
+  within 1e-6 cm^-1 after the device's FP64 Rayleigh refinement of their
+  Ritz vectors (the FP64 run's levels are the reference here: the oracle
+  needs minutes per step at this size; its parity is the cfg2 Hv).
 * cfg4: a batch of eight lmax-5 reduced members (para and ortho, NoQuad and
   quadrature calculators, FP32 Hv, 40 x 70, odd and even grids) against the
   oracle's recurrence on the same start vectors, and bitwise against single
@@ -101,6 +107,26 @@ def test_cfg2_hv_54_bf16x3_close_to_oracle(cage54):
         assert c.mult_tol == B.BF16X3_KIN_MULT_TOL
         got = c.hv(v)
     assert _rel(got, ref) < 5e-5
+
+
+def test_cfg3_mixed_levels_54_within_target():
+    from bsl_amd import physics
+    _, kw = cs.baseline_cage_case(54)
+    n = 10000
+    with B.Calculator(**kw) as c64:
+        e64, _ = B.lancbis(*c64.lanczos(n, start="random", seed=0), -10.0, 10.0)
+    with B.Calculator(precision="fp32", **kw) as c32:
+        co = c32.lanczos(n, start="random", seed=0)
+        e32, _ = B.lancbis(*co, -10.0, 10.0, mult_tol=c32.mult_tol)
+        low = np.sort(e64)[:20]
+        err = max(np.min(np.abs(e32 - x)) for x in low) / physics.KJMOL_PER_CM1
+        assert err < 0.01, err
+        sel = np.sort(e32)[:20]
+        evtr, _ = B.trivec(*co, sel, mult_tol=c32.mult_tol)
+        _, eref, res = c32.ritz_refine(n, evtr, len(sel), start="random", seed=0, vectors=False)
+    err_ref = max(np.min(np.abs(eref - x)) for x in low) / physics.KJMOL_PER_CM1
+    assert err_ref < 1e-6, err_ref
+    assert np.max(res) < 1e-3
 
 
 def _cfg4_members():

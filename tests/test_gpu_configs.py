@@ -16,9 +16,10 @@ against the CPU oracle.
 * cfg4: a batch of eight lmax-5 reduced members (para and ortho, NoQuad and
   quadrature calculators, FP32 Hv, 40 x 70, odd and even grids) against the
   oracle's recurrence on the same start vectors, and bitwise against single
-  runs eagerly and through replayed graphs; then 20 lockstep steps of the
-  benchmark's own 32-member set (16 cages x para/ortho on 41^3/47^3/54^3)
-  against two of its members run alone.
+  runs eagerly and through replayed graphs; then the configuration's 5000
+  lockstep steps of the benchmark's own 32-member set (16 cages x
+  para/ortho on 41^3/47^3/54^3) against two of its members run alone
+  (bitwise) and against FP64 runs of them (lowest 10 levels, 0.01 cm^-1).
 * cfg5: Hellmann-Feynman gradients at lmax 5 on the 40 x 70 quadrature
   against the oracle's gradient and against central differences of the
   GPU's converged levels for every kind of parameter.
@@ -164,23 +165,35 @@ def test_cfg4_batch_members_match_oracle_and_single_runs():
         assert np.max(np.abs(b[1:21] - br[1:21]) / br[1:21]) < 2e-5, (i, kw["calculator"], kw["nx"])
 
 
-def test_cfg4_benchmark_batch_short_run():
+def test_cfg4_benchmark_batch_5000_steps():
     """The benchmark's 32 members (seeds 0-15 x para/ortho, 41/47/54 points
-    per axis, lmax 5, 40 x 70, reduced basis, FP32 NoQuad Hv): 20 lockstep
-    steps through replayed graphs equal members 0 (41^3 para) and 5
-    (54^3 ortho) run alone."""
+    per axis, lmax 5, 40 x 70, reduced basis, FP32 NoQuad Hv) for the
+    configuration's 5000 lockstep steps through replayed graphs: members 0
+    (41^3 para) and 5 (54^3 ortho) equal their runs alone, bitwise over all
+    5000 coefficients, and the lowest 10 good levels of each are within
+    0.01 cm^-1 of an FP64 run of the same member."""
+    from bsl_amd import physics
     members = []
     grids = (41, 47, 54)
     for i in range(32):
         _, kw = cs.baseline_cage_case(grids[i % 3], 40, 70, ("Even", "Odd")[i % 2], seed=i // 2)
         members.append(dict(kw, basis="reduced", precision="fp32"))
+    n = 5000
     with B.Batch(members) as bt:
-        got = bt.lanczos(20, start="random", seed=0, graph=True)
+        got = bt.lanczos(n, start="random", seed=0, graph=True)
     for i in (0, 5):
         with B.Calculator(**members[i]) as c:
-            ref = c.lanczos(20, start="random", seed=i)
+            ref = c.lanczos(n, start="random", seed=i)
+            tol = c.mult_tol
         for g, r in zip(got[i], ref):
             assert np.array_equal(g, r), i
+        with B.Calculator(**dict(members[i], precision="fp64")) as c:
+            e64, _ = B.lancbis(*c.lanczos(n, start="random", seed=i), -10.0, 10.0)
+        e32, _ = B.lancbis(*got[i], -10.0, 10.0, mult_tol=tol)
+        low = np.sort(e64)[:10]
+        assert len(low) == 10
+        err = max(np.min(np.abs(e32 - x)) for x in low) / physics.KJMOL_PER_CM1
+        assert err < 0.01, (i, err)
 
 
 CFG5_PARAMS = [("site_q", 0), ("site_q", 1), ("site_pos", 0), ("site_pos", 4), ("site_pos", 8), ("q_h2_h", 0),

@@ -125,7 +125,7 @@ def adjoint_leg(B, args, x_max):
                 return e
             groups = B.multiplets(e, tol=1e-6)
             t0 = time.perf_counter()
-            gr = [c.hf_gradient(psi[gi], np.ones(len(gi))) for gi in groups]
+            gr = c.hf_gradient_sets([(psi[gi], np.ones(len(gi))) for gi in groups])
             t_grad = time.perf_counter() - t0
             return e, res, groups, gr, t_lev, t_grad, c.unclamped_fraction()
 

@@ -159,6 +159,27 @@ class Calculator:
         check(lib().bsl_hf_gradient(self.h, psi.shape[0], psi.ravel(), g, out))
         return split_gradient(out, ns)
 
+    def hf_gradient_sets(self, sets):
+        """The gradients of several sets at once (bsl_hf_gradient_sets): sets
+        is a list of (psi (k, N), g (k)) pairs, e.g. one per degenerate
+        multiplet; returns one dict per set, each equal to hf_gradient of
+        that set alone (up to four sets share a pass over the grid)."""
+        psis, gs, counts = [], [], []
+        for psi, g in sets:
+            psi = np.ascontiguousarray(np.atleast_2d(psi), dtype=np.float64)
+            g = np.ascontiguousarray(np.atleast_1d(g), dtype=np.float64)
+            if psi.shape[1] != self.N or len(g) != psi.shape[0]:
+                raise ValueError("each psi must be (len(g), basis size)")
+            psis.append(psi)
+            gs.append(g)
+            counts.append(psi.shape[0])
+        ns = self.params.n_sites
+        out = np.zeros(len(sets) * (4 * ns + 4))
+        cnt = np.asarray(counts, dtype=np.int32)
+        check(lib().bsl_hf_gradient_sets(self.h, len(sets), cnt.ctypes.data, np.concatenate(psis).ravel(),
+                                         np.concatenate(gs), out))
+        return [split_gradient(out[i * (4 * ns + 4):(i + 1) * (4 * ns + 4)], ns) for i in range(len(sets))]
+
     def vblocks(self):
         nx, ny, nz = self.shape
         out = np.zeros(nx * ny * nz * self.nn * self.nn)

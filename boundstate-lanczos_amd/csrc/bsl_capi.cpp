@@ -1325,9 +1325,15 @@ extern "C" int bsl_clamp_stats(bsl_ctx *c, int64_t *unclamped, int64_t *total)
     return BSL_OK;
 }
 
-extern "C" int bsl_hf_gradient(bsl_ctx *c, int32_t nvec, const double *psi, const double *g, double *grad)
+extern "C" int bsl_hf_gradient_sets(bsl_ctx *c, int32_t nset, const int32_t *counts, const double *psi,
+                                    const double *g, double *grad)
 {
-    if (!c || nvec <= 0 || !psi || !g || !grad) return fail(BSL_E_ARG, "bsl_hf_gradient: bad arguments");
+    if (!c || nset <= 0 || !counts || !psi || !g || !grad) return fail(BSL_E_ARG, "bsl_hf_gradient_sets: bad arguments");
+    int64_t nvec = 0;
+    for (int m = 0; m < nset; m++) {
+        if (counts[m] <= 0) return fail(BSL_E_ARG, "bsl_hf_gradient_sets: every set needs an eigenvector");
+        nvec += counts[m];
+    }
     if (c->p.potential != BSL_POT_CAGE) return fail(BSL_E_STATE, "bsl_hf_gradient: cage potential only");
     BSL_HIP(hipSetDevice(c->device));
     const bsl_params &p = c->p;
@@ -1335,6 +1341,8 @@ extern "C" int bsl_hf_gradient(bsl_ctx *c, int32_t nvec, const double *psi, cons
     const int na = t.na, nb = t.nb, nk = (int)c->carried.size(), ns = p.n_sites, Q = c->Q;
     const int npb = 4, nent = 4 * ns + 4;
     const int nwg = hf_grad_groups(c->P, npb);
+    const int chunk = std::min<int>(nset, 4);  // sets per gradient pass (launch_hf_grad's limit)
+    const size_t pq = (size_t)c->P * Q;
     // quadrature-node values of the carried harmonics and the node weights
     std::vector<double> Y((size_t)nk * Q), wq(Q);
     for (int a = 0; a < na; a++)
@@ -1350,34 +1358,46 @@ extern "C" int bsl_hf_gradient(bsl_ctx *c, int32_t nvec, const double *psi, cons
     std::vector<double> off, site4, gv(g, g + nvec);
     std::vector<int> lj;
     cage_arrays(c, off, site4, lj);
-    double *d_psi = nullptr, *d_vg = nullptr, *d_Y = nullptr, *d_wq = nullptr, *d_g = nullptr, *d_off = nullptr,
+    double *d_psi = nullptr, *d_om = nullptr, *d_Y = nullptr, *d_wq = nullptr, *d_g = nullptr, *d_off = nullptr,
            *d_site = nullptr, *d_part = nullptr, *d_out = nullptr;
     int *d_lj = nullptr;
     int rc = dev_alloc(&d_psi, (size_t)nvec * c->n_kry);
-    for (int k = 0; k < nvec && !rc; k++)
+    for (int64_t k = 0; k < nvec && !rc; k++)
         rc = upload_carried(c, psi + (size_t)k * c->n_full, d_psi + (size_t)k * c->n_kry);
-    if (!rc) rc = dev_alloc(&d_vg, (size_t)c->P * Q);
-    if (!rc) rc = ensure_unclamped(c, d_vg);
+    // one omega grid per set of a pass; the first also serves the clamp mask
+    if (!rc) rc = dev_alloc(&d_om, (size_t)chunk * pq);
+    if (!rc) rc = ensure_unclamped(c, d_om);
     if (!rc) rc = dev_upload(&d_Y, Y);
     if (!rc) rc = dev_upload(&d_wq, wq);
     if (!rc) rc = dev_upload(&d_g, gv);
-    if (!rc) rc = launch_hf_density(d_psi, nvec, d_g, d_Y, d_wq, c->d_unclamped, d_vg, c->P, nk, Q, c->stream);
     if (!rc) rc = dev_upload(&d_off, off);
     if (!rc) rc = dev_upload(&d_site, site4);
     if (!rc) rc = dev_upload(&d_lj, lj);
-    if (!rc) rc = dev_alloc(&d_part, (size_t)nwg * nent);
-    if (!rc) rc = dev_alloc(&d_out, (size_t)nent);
-    if (!rc)
-        rc = launch_hf_grad(d_vg, c->P, Q, c->d_gx, c->d_gy, c->d_gz, p.nx, p.ny, d_off, d_site, d_lj, ns, p.q_h2_h,
-                            p.q_h2_cm, p.lj_a, p.lj_b, npb, d_part, d_out, c->stream);
+    if (!rc) rc = dev_alloc(&d_part, (size_t)nwg * chunk * nent);
+    if (!rc) rc = dev_alloc(&d_out, (size_t)nset * nent);
+    int64_t v0 = 0;  // first eigenvector of the pass
+    for (int m0 = 0; m0 < nset && !rc; m0 += chunk) {
+        const int nm = std::min(chunk, nset - m0);
+        int64_t v = v0;
+        for (int m = 0; m < nm && !rc; m++) {
+            rc = launch_hf_density(d_psi + (size_t)v * c->n_kry, counts[m0 + m], d_g + v, d_Y, d_wq, c->d_unclamped,
+                                   d_om + (size_t)m * pq, c->P, nk, Q, c->stream);
+            v += counts[m0 + m];
+        }
+        if (!rc)
+            rc = launch_hf_grad(d_om, nm, c->P, Q, c->d_gx, c->d_gy, c->d_gz, p.nx, p.ny, d_off, d_site, d_lj, ns,
+                                p.q_h2_h, p.q_h2_cm, p.lj_a, p.lj_b, npb, c->d_unclamped, d_part, d_out + (size_t)m0 * nent,
+                                c->stream);
+        v0 = v;
+    }
     if (!rc) {
-        hipError_t e = hipMemcpyAsync(grad, d_out, sizeof(double) * nent, hipMemcpyDeviceToHost, c->stream);
+        hipError_t e = hipMemcpyAsync(grad, d_out, sizeof(double) * nset * nent, hipMemcpyDeviceToHost, c->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
         if (e != hipSuccess) rc = hip_fail(e, "download gradient", __FILE__, __LINE__);
     }
     (void)hipStreamSynchronize(c->stream);
     dev_free(d_psi);
-    dev_free(d_vg);
+    dev_free(d_om);
     dev_free(d_Y);
     dev_free(d_wq);
     dev_free(d_g);
@@ -1387,6 +1407,12 @@ extern "C" int bsl_hf_gradient(bsl_ctx *c, int32_t nvec, const double *psi, cons
     dev_free(d_part);
     dev_free(d_out);
     return rc;
+}
+
+extern "C" int bsl_hf_gradient(bsl_ctx *c, int32_t nvec, const double *psi, const double *g, double *grad)
+{
+    if (!c || nvec <= 0 || !psi || !g || !grad) return fail(BSL_E_ARG, "bsl_hf_gradient: bad arguments");
+    return bsl_hf_gradient_sets(c, 1, &nvec, psi, g, grad);
 }
 
 extern "C" int bsl_step_bytes(const bsl_ctx *c, double *out)

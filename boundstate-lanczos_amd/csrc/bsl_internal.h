@@ -333,9 +333,13 @@ int launch_hf_density(const double *psi, int nvec, const double *g, const double
                       const uint64_t *unclamped, double *om, int64_t P, int nk, int Q, hipStream_t s);
 int hf_grad_groups(int64_t P, int npb);
 int launch_unclamped_mask(const double *vg, int64_t n, double ceiling, uint64_t *mask, hipStream_t s);
-int launch_hf_grad(const double *om, int64_t P, int Q, const double *gx, const double *gy, const double *gz, int nx,
-                   int ny, const double *off, const double *site4, const int *lj, int ns, double q_h, double q_cm,
-                   double lj_a, double lj_b, int npb, double *partial, double *out, hipStream_t s);
+// nset (1 to 4) sets of eigenvectors at once: omega of set m at om + m P Q;
+// partial: hf_grad_groups(P, npb) * nset * (4 ns + 4) doubles; out:
+// nset * (4 ns + 4), set after set
+int launch_hf_grad(const double *om, int nset, int64_t P, int Q, const double *gx, const double *gy, const double *gz,
+                   int nx, int ny, const double *off, const double *site4, const int *lj, int ns, double q_h,
+                   double q_cm, double lj_a, double lj_b, int npb, const uint64_t *unclamped, double *partial,
+                   double *out, hipStream_t s);
 int quad_points_per_group(int na, int nk);
 // workgroups of the persistent quadrature kernel: its resident workgroups
 // per CU times the CUs of `device`, at most the number of point groups

@@ -312,6 +312,16 @@ int bsl_step_bytes(const bsl_ctx *ctx, double *out);
 // bsl_destroy; later calls skip that pass.  No reference counterpart (the
 // reference has no derivatives).
 int bsl_hf_gradient(bsl_ctx *ctx, int32_t nvec, const double *psi, const double *g, double *grad);
+// The gradients of nset weighted sets at once (e.g. every degenerate
+// multiplet of the lowest levels): set m is the counts[m] eigenvectors that
+// follow the previous sets' in psi, with their weights in g; grad receives
+// nset blocks of 4*n_sites + 4, each equal (bitwise) to bsl_hf_gradient of
+// that set alone.  Up to four sets share one pass over the (point, node)
+// pairs, whose geometry (the distances to every site) is computed once per
+// pass; one potential-sized weight grid per set of a pass (P * Q doubles)
+// is held on the device during the call.  No reference counterpart.
+int bsl_hf_gradient_sets(bsl_ctx *ctx, int32_t nset, const int32_t *counts, const double *psi, const double *g,
+                         double *grad);
 
 // Counts of the (point, quadrature node) pairs of a cage context's potential
 // grid that lie below the ceiling clamp (the pairs bsl_hf_gradient sums

@@ -116,6 +116,43 @@ def test_gpu_gradient_matches_oracle():
         assert abs(got2[name] - ref2[e]) <= 1e-10 * abs(ref2[e])
 
 
+@pytest.mark.gpu
+def test_gpu_gradient_sets_equal_single_sets():
+    """bsl_hf_gradient_sets over six sets of one to three vectors (two passes
+    of up to four sets, the second ragged) gives bitwise the gradients of
+    bsl_hf_gradient on each set alone, and the oracle's within 1e-10."""
+    import bsl_amd as B
+    spec, kw = cs.cage_case(5, 3, 8, 9, "All")
+    o = O.Oracle(spec)
+    ns = len(spec["site_q"])
+    rng = np.random.default_rng(77)
+    sets = []
+    for k in (1, 2, 3, 1, 2, 1):
+        psi = rng.uniform(-1, 1, (k, o.N))
+        psi /= np.linalg.norm(psi, axis=1, keepdims=True)
+        sets.append((psi, rng.uniform(-1, 1, k)))
+    with B.Calculator(**kw) as c:
+        single = [c.hf_gradient(p, g) for p, g in sets]
+    names = ("site_q", "site_pos", "q_h2_h", "q_h2_cm", "lj_a", "lj_b")
+    # a fresh context too: the clamp mask is found by the first call of either entry point
+    for fresh in (False, True):
+        with B.Calculator(**kw) as c:
+            if not fresh:
+                c.hf_gradient(*sets[0])
+            got = c.hf_gradient_sets(sets)
+        assert len(got) == len(sets)
+        for gs, g1 in zip(got, single):
+            for name in names:
+                assert np.array_equal(np.asarray(gs[name]), np.asarray(g1[name])), (fresh, name)
+    for (psi, g), gs in zip(sets, got):
+        ref = o.hf_gradient(psi, g)
+        scale = np.max(np.abs(ref[:4 * ns]))
+        assert np.max(np.abs(gs["site_q"] - ref[0:4 * ns:4])) / scale < 1e-10
+        assert np.max(np.abs(gs["site_pos"] - ref[:4 * ns].reshape(ns, 4)[:, 1:4])) / scale < 1e-10
+        for e, name in zip(range(4 * ns, 4 * ns + 4), names[2:]):
+            assert abs(gs[name] - ref[e]) <= 1e-10 * abs(ref[e])
+
+
 NITER = 800
 
 

@@ -110,7 +110,7 @@ class Calculator:
             return MIXED_MULT_TOL
         return BF16X3_MULT_TOL if self.params.calculator == CALC["quad"] else BF16X3_KIN_MULT_TOL
 
-    def levels(self, niter, count, emin=-1.0e4, emax=1.0e4, start="random", seed=0, tol=1e-8):
+    def levels(self, niter, count, emin=-1.0e4, emax=1.0e4, start="random", seed=0, tol=1e-8, vectors=True):
         """The `count` lowest converged good levels: Loop 1, Cullum-Willoughby,
         Tm eigenvectors, then Loop 2 rebuilds their Ritz vectors by
         recomputing the Krylov vectors (lanczosEigenValuesVectors.cpp:
@@ -118,7 +118,9 @@ class Calculator:
         each level.  The Sturm bisections and error estimates run on the GPU
         (the good eigenvalues of the whole window: 0.04 s against 2.1 s on
         the host for 4000 steps at 41^3).  Returns (energies, unit Ritz
-        vectors (count, N), residual norms)."""
+        vectors (count, N), residual norms); vectors=False leaves the Ritz
+        vectors on the device only (None in their place), where
+        hf_gradient_ritz finds level i as index i."""
         a, b, b2 = self.lanczos(niter, start=start, seed=seed)
         ev, er = lancbis(a, b, b2, emin, emax, device=True, mult_tol=self.mult_tol)
         order = np.argsort(ev)[:count]
@@ -126,8 +128,9 @@ class Calculator:
             raise BslError(0, "the %d lowest levels are not all converged after %d steps" % (count, niter))
         sel = ev[order]
         evtr, _ = trivec(a, b, b2, sel, mult_tol=self.mult_tol)
-        psi, e, r = self.ritz_refine(niter, evtr, count, start=start, seed=seed)
-        psi /= np.linalg.norm(psi, axis=1, keepdims=True)
+        psi, e, r = self.ritz_refine(niter, evtr, count, start=start, seed=seed, vectors=vectors)
+        if psi is not None:
+            psi /= np.linalg.norm(psi, axis=1, keepdims=True)
         return e, psi, r
 
     def ritz_refine(self, niter, evtr, ngood, start="uniform", seed=0, v0=None, vectors=True):
@@ -178,6 +181,28 @@ class Calculator:
         cnt = np.asarray(counts, dtype=np.int32)
         check(lib().bsl_hf_gradient_sets(self.h, len(sets), cnt.ctypes.data, np.concatenate(psis).ravel(),
                                          np.concatenate(gs), out))
+        return [split_gradient(out[i * (4 * ns + 4):(i + 1) * (4 * ns + 4)], ns) for i in range(len(sets))]
+
+    def hf_gradient_ritz(self, sets):
+        """The gradients of sets of the unit Ritz vectors that the last
+        levels() / ritz_refine() left on the device (bsl_hf_gradient_ritz):
+        sets is a list of (indices, g) pairs (index i: the i-th level or
+        evtr entry of that call), e.g. one per multiplet; one dict per set."""
+        idx, gs, counts = [], [], []
+        for ind, g in sets:
+            ind = np.atleast_1d(np.asarray(ind, dtype=np.int32))
+            g = np.atleast_1d(np.asarray(g, dtype=np.float64))
+            if len(ind) != len(g) or len(ind) == 0:
+                raise ValueError("each set needs as many weights as indices, at least one")
+            idx.append(ind)
+            gs.append(g)
+            counts.append(len(ind))
+        ns = self.params.n_sites
+        out = np.zeros(len(sets) * (4 * ns + 4))
+        cnt = np.asarray(counts, dtype=np.int32)
+        ind = np.ascontiguousarray(np.concatenate(idx), dtype=np.int32)
+        check(lib().bsl_hf_gradient_ritz(self.h, len(sets), cnt.ctypes.data, ind.ctypes.data,
+                                         np.ascontiguousarray(np.concatenate(gs)), out))
         return [split_gradient(out[i * (4 * ns + 4):(i + 1) * (4 * ns + 4)], ns) for i in range(len(sets))]
 
     def vblocks(self):

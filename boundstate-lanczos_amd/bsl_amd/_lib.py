@@ -27,7 +27,7 @@ EXPORTS = (
     "bsl_batch_member", "bsl_batch_lanczos", "bsl_batch_time_steps", "bsl_lancbis_device", "bsl_lancbis_tol",
     "bsl_lancbis_device_tol", "bsl_trivec_tol", "bsl_get_potential_form", "bsl_set_potential_form",
     "bsl_gaunt_table", "bsl_lanczos_noalloc", "bsl_clamp_stats",
-    "bsl_ritz_refine", "bsl_get_fusion", "bsl_set_fusion", "bsl_hf_gradient_sets",
+    "bsl_ritz_refine", "bsl_get_fusion", "bsl_set_fusion", "bsl_hf_gradient_sets", "bsl_hf_gradient_ritz",
 )
 
 
@@ -98,6 +98,7 @@ def load():
     lib.bsl_gaunt_table.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p]
     lib.bsl_hf_gradient.argtypes = [C.c_void_p, C.c_int32, _dp, _dp, _dp]
     lib.bsl_hf_gradient_sets.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, _dp, _dp, _dp]
+    lib.bsl_hf_gradient_ritz.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, _dp, _dp]
     lib.bsl_clamp_stats.argtypes = [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
     lib.bsl_batch_create.argtypes = [C.c_int32, C.c_void_p, C.c_void_p]
     lib.bsl_batch_destroy.argtypes = [C.c_void_p]

@@ -487,6 +487,10 @@ using namespace bsl;
 
 extern "C" const char *bsl_last_error(void) { return g_err.c_str(); }
 
+namespace {
+void free_hf_setup(bsl_ctx *c);
+}  // namespace
+
 extern "C" int bsl_destroy(bsl_ctx *c)
 {
     if (!c) return BSL_OK;
@@ -511,7 +515,10 @@ extern "C" int bsl_destroy(bsl_ctx *c)
     dev_free(c->d_gy);
     dev_free(c->d_gz);
     dev_free(c->d_vgrid);
-    dev_free(c->d_unclamped);
+    free_hf_setup(c);
+    for (void *p : {(void *)c->d_hfpsi, (void *)c->d_hfg, (void *)c->d_hfom, (void *)c->d_hfwcm, (void *)c->d_hfpart,
+                    (void *)c->d_hfout})
+        dev_free(p);
     dev_free(c->d_X);
     dev_free(c->d_R);
     dev_free(c->d_T);
@@ -979,7 +986,7 @@ int sum_pairs(bsl_ctx *c, int g, double *s0, double *s1)
 // FP64 Rayleigh quotient <y|H|y>/<y|y> and residual |Hy - e y|/|y| of the
 // device vector y (n_kry, device order) with the context's FP64 operator,
 // whatever its precision; Hy goes to the second half of d_T
-int rayleigh_dev(bsl_ctx *c, const double *y, double *energy, double *resid)
+int rayleigh_dev(bsl_ctx *c, const double *y, double *energy, double *resid, double *norm2 = nullptr)
 {
     int rc = ensure_tmp(c);
     if (rc) return rc;
@@ -1000,6 +1007,7 @@ int rayleigh_dev(bsl_ctx *c, const double *y, double *energy, double *resid)
     if ((rc = sum_pairs(c, g, &r2, &unused))) return rc;
     *energy = e;
     *resid = std::sqrt(r2 / vv);
+    if (norm2) *norm2 = vv;
     return BSL_OK;
 }
 
@@ -1028,6 +1036,7 @@ extern "C" int bsl_ritz(bsl_ctx *c, int32_t niter, const double *v0, int32_t sta
     if (!c || niter <= 0 || ngood < 0 || (ngood > 0 && (!evtr || !ritz))) return fail(BSL_E_ARG, "bad arguments");
     if (ngood == 0) return BSL_OK;
     BSL_HIP(hipSetDevice(c->device));
+    c->ritz_vv.clear();
     int rc = ritz_run(c, niter, v0, start, seed, evtr, ngood);
     if (rc) return rc;
     std::memset(ritz, 0, sizeof(double) * (size_t)c->n_full * ngood);
@@ -1044,8 +1053,12 @@ extern "C" int bsl_ritz_refine(bsl_ctx *c, int32_t niter, const double *v0, int3
         return fail(BSL_E_ARG, "bad arguments");
     if (ngood == 0) return BSL_OK;
     BSL_HIP(hipSetDevice(c->device));
+    c->ritz_vv.clear();
     int rc = ritz_run(c, niter, v0, start, seed, evtr, ngood);
-    for (int g = 0; g < ngood && !rc; g++) rc = rayleigh_dev(c, c->d_ritz + (size_t)g * c->n_kry, energy + g, resid + g);
+    std::vector<double> vv(ngood);
+    for (int g = 0; g < ngood && !rc; g++)
+        rc = rayleigh_dev(c, c->d_ritz + (size_t)g * c->n_kry, energy + g, resid + g, &vv[g]);
+    if (!rc) c->ritz_vv = vv;
     if (!rc && ritz) {
         std::memset(ritz, 0, sizeof(double) * (size_t)c->n_full * ngood);
         for (int g = 0; g < ngood && !rc; g++)
@@ -1278,23 +1291,101 @@ extern "C" int bsl_get_vgrid(bsl_ctx *c, double *v1, double *v2)
 }
 
 namespace {
-// Which nodes are below the ceiling clamp: fixed by the context's potential,
-// found from its grid (regenerated into d_vg, P x Q doubles) once and kept,
-// one bit per (point, node).
-int ensure_unclamped(bsl_ctx *c, double *d_vg)
+void free_hf_setup(bsl_ctx *c)
 {
-    if (c->d_unclamped) return BSL_OK;
-    int rc = make_vgrid(c, d_vg);
-    if (!rc) rc = dev_alloc(&c->d_unclamped, ((size_t)c->P * c->Q + 63) / 64);
-    if (!rc) rc = launch_unclamped_mask(d_vg, c->P * c->Q, c->p.ceiling, c->d_unclamped, c->stream);
-    const hipError_t e = hipStreamSynchronize(c->stream);
-    if (!rc && e != hipSuccess) rc = hip_fail(e, "clamp mask", __FILE__, __LINE__);
-    if (rc) {
-        dev_free(c->d_unclamped);
-        c->d_unclamped = nullptr;
-    }
+    for (void *p : {(void *)c->d_unclamped, (void *)c->d_pairs, (void *)c->d_pair_first, (void *)c->d_hfY,
+                    (void *)c->d_hfwq, (void *)c->d_hfoff, (void *)c->d_hfsite, (void *)c->d_hflj})
+        dev_free(p);
+    c->d_unclamped = nullptr;
+    c->d_pairs = nullptr;
+    c->d_pair_first = nullptr;
+    c->d_hfY = c->d_hfwq = c->d_hfoff = c->d_hfsite = nullptr;
+    c->d_hflj = nullptr;
+    c->hf_ne = 0;
+}
+
+// a workspace buffer of at least `need` doubles (contents not kept when it grows)
+int grow(double *&buf, size_t &cap, size_t need)
+{
+    if (need <= cap) return BSL_OK;
+    dev_free(buf);
+    buf = nullptr;
+    cap = 0;
+    const int rc = dev_alloc(&buf, need);
+    if (!rc) cap = need;
     return rc;
 }
+
+// What the gradients need of a cage context, fixed by its potential and
+// built once: which (point, node) pairs are below the ceiling clamp (one bit
+// each, from the grid regenerated into a temporary P x Q buffer), the list
+// of kept node pairs per group of kHfPoints points that the gradient kernels
+// walk, and the node tables (carried harmonics and weights at the nodes, H
+// offsets, sites).
+int ensure_hf_setup(bsl_ctx *c)
+{
+    if (c->d_unclamped) return BSL_OK;
+    const bsl_params &p = c->p;
+    const Tables &t = c->tab;
+    const int na = t.na, nb = t.nb, nk = (int)c->carried.size(), Q = c->Q;
+    const int ng = hf_grad_groups(c->P, kHfPoints);
+    double *d_vg = nullptr;
+    int *d_count = nullptr;
+    std::vector<int> count(ng);
+    std::vector<int64_t> first(ng + 1, 0);
+    int rc = dev_alloc(&d_vg, (size_t)c->P * Q);
+    if (!rc) rc = make_vgrid(c, d_vg);
+    if (!rc) rc = dev_alloc(&c->d_unclamped, ((size_t)c->P * Q + 63) / 64);
+    if (!rc) rc = launch_unclamped_mask(d_vg, c->P * Q, p.ceiling, c->d_unclamped, c->stream);
+    if (!rc) rc = dev_alloc(&d_count, (size_t)ng);
+    if (!rc) rc = launch_pair_count(c->d_unclamped, c->P, Q, kHfPoints, d_count, c->stream);
+    if (!rc) {
+        hipError_t e = hipMemcpyAsync(count.data(), d_count, sizeof(int) * ng, hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess) rc = hip_fail(e, "pair counts", __FILE__, __LINE__);
+    }
+    for (int g = 0; g < ng && !rc; g++) first[g + 1] = first[g] + count[g];
+    if (!rc) rc = dev_upload(&c->d_pair_first, first);
+    if (!rc) rc = dev_alloc(&c->d_pairs, (size_t)std::max<int64_t>(first[ng], 1));
+    if (!rc) rc = launch_pair_fill(c->d_unclamped, c->P, Q, kHfPoints, c->d_pair_first, c->d_pairs, c->stream);
+    if (!rc) c->hf_ne = first[ng];
+    if (!rc) {
+        // quadrature-node values of the carried harmonics and the node weights
+        std::vector<double> Y((size_t)nk * Q), wq(Q);
+        for (int a = 0; a < na; a++)
+            for (int b = 0; b < nb; b++) {
+                const size_t q0 = (size_t)a * nb + b, q1 = (size_t)na * nb + q0;
+                wq[q0] = wq[q1] = t.wt[a] * t.wp[b];
+                for (int r = 0; r < nk; r++) {
+                    const int n = c->carried[r], mi = t.m[n] + p.lmax;
+                    Y[(size_t)r * Q + q0] = t.L[(size_t)n * na + a] * t.S1[(size_t)mi * nb + b];
+                    Y[(size_t)r * Q + q1] = t.L[(size_t)n * na + a] * t.S2[(size_t)mi * nb + b];
+                }
+            }
+        std::vector<double> off, site4;
+        std::vector<int> lj;
+        cage_arrays(c, off, site4, lj);
+        rc = dev_upload(&c->d_hfY, Y);
+        if (!rc) rc = dev_upload(&c->d_hfwq, wq);
+        if (!rc) rc = dev_upload(&c->d_hfoff, off);
+        if (!rc) rc = dev_upload(&c->d_hfsite, site4);
+        if (!rc) rc = dev_upload(&c->d_hflj, lj);
+    }
+    // the weights of a pass of four sets, first touched here so that the
+    // first gradient call does not pay for mapping them
+    const int nent = 4 * p.n_sites + 4;
+    if (!rc) rc = grow(c->d_hfom, c->hf_cap[2], (size_t)4 * std::max<int64_t>(c->hf_ne, 1));
+    if (!rc) rc = grow(c->d_hfwcm, c->hf_cap[3], (size_t)4 * c->P);
+    if (!rc) rc = grow(c->d_hfpart, c->hf_cap[4], (size_t)ng * 4 * nent);
+    if (!rc) BSL_HIP(hipMemsetAsync(c->d_hfom, 0, sizeof(double) * c->hf_cap[2], c->stream));
+    const hipError_t e = hipStreamSynchronize(c->stream);
+    if (!rc && e != hipSuccess) rc = hip_fail(e, "clamp mask", __FILE__, __LINE__);
+    dev_free(d_count);
+    dev_free(d_vg);
+    if (rc) free_hf_setup(c);
+    return rc;
+}
+
 }  // namespace
 
 extern "C" int bsl_clamp_stats(bsl_ctx *c, int64_t *unclamped, int64_t *total)
@@ -1302,14 +1393,8 @@ extern "C" int bsl_clamp_stats(bsl_ctx *c, int64_t *unclamped, int64_t *total)
     if (!c || !unclamped || !total) return fail(BSL_E_ARG, "null argument");
     if (c->p.potential != BSL_POT_CAGE) return fail(BSL_E_STATE, "bsl_clamp_stats: cage potential only");
     BSL_HIP(hipSetDevice(c->device));
-    int rc = BSL_OK;
-    if (!c->d_unclamped) {
-        double *d_vg = nullptr;
-        rc = dev_alloc(&d_vg, (size_t)c->P * c->Q);
-        if (!rc) rc = ensure_unclamped(c, d_vg);
-        dev_free(d_vg);
-        if (rc) return rc;
-    }
+    const int rc = ensure_hf_setup(c);
+    if (rc) return rc;
     const size_t n = (size_t)c->P * c->Q, words = (n + 63) / 64;
     std::vector<uint64_t> m(words);
     BSL_HIP(hipMemcpyAsync(m.data(), c->d_unclamped, sizeof(uint64_t) * words, hipMemcpyDeviceToHost, c->stream));
@@ -1325,88 +1410,100 @@ extern "C" int bsl_clamp_stats(bsl_ctx *c, int64_t *unclamped, int64_t *total)
     return BSL_OK;
 }
 
-extern "C" int bsl_hf_gradient_sets(bsl_ctx *c, int32_t nset, const int32_t *counts, const double *psi,
-                                    const double *g, double *grad)
+namespace {
+// The passes of bsl_hf_gradient_sets over vectors already in d_hfpsi with
+// their weights in d_hfg (both filled by the caller after hf_prepare).
+int hf_prepare(bsl_ctx *c, int nset, const int32_t *counts, int64_t *nvec)
 {
-    if (!c || nset <= 0 || !counts || !psi || !g || !grad) return fail(BSL_E_ARG, "bsl_hf_gradient_sets: bad arguments");
-    int64_t nvec = 0;
+    if (!c || nset <= 0 || !counts) return fail(BSL_E_ARG, "bsl_hf_gradient: bad arguments");
+    *nvec = 0;
     for (int m = 0; m < nset; m++) {
-        if (counts[m] <= 0) return fail(BSL_E_ARG, "bsl_hf_gradient_sets: every set needs an eigenvector");
-        nvec += counts[m];
+        if (counts[m] <= 0) return fail(BSL_E_ARG, "bsl_hf_gradient: every set needs an eigenvector");
+        *nvec += counts[m];
     }
     if (c->p.potential != BSL_POT_CAGE) return fail(BSL_E_STATE, "bsl_hf_gradient: cage potential only");
     BSL_HIP(hipSetDevice(c->device));
+    const int nent = 4 * c->p.n_sites + 4, nwg = hf_grad_groups(c->P, kHfPoints), chunk = std::min(nset, 4);
+    size_t *cap = c->hf_cap;
+    int rc = ensure_hf_setup(c);
+    if (!rc) rc = grow(c->d_hfpsi, cap[0], (size_t)*nvec * c->n_kry);
+    if (!rc) rc = grow(c->d_hfg, cap[1], (size_t)*nvec);
+    if (!rc) rc = grow(c->d_hfom, cap[2], (size_t)chunk * std::max<int64_t>(c->hf_ne, 1));
+    if (!rc) rc = grow(c->d_hfwcm, cap[3], (size_t)chunk * c->P);
+    if (!rc) rc = grow(c->d_hfpart, cap[4], (size_t)nwg * chunk * nent);
+    if (!rc) rc = grow(c->d_hfout, cap[5], (size_t)nset * nent);
+    return rc;
+}
+
+int hf_passes(bsl_ctx *c, int nset, const int32_t *counts, double *grad)
+{
     const bsl_params &p = c->p;
-    const Tables &t = c->tab;
-    const int na = t.na, nb = t.nb, nk = (int)c->carried.size(), ns = p.n_sites, Q = c->Q;
-    const int npb = 4, nent = 4 * ns + 4;
-    const int nwg = hf_grad_groups(c->P, npb);
+    const int nk = (int)c->carried.size(), ns = p.n_sites, Q = c->Q;
+    const int npb = kHfPoints, nent = 4 * ns + 4;
     const int chunk = std::min<int>(nset, 4);  // sets per gradient pass (launch_hf_grad's limit)
-    const size_t pq = (size_t)c->P * Q;
-    // quadrature-node values of the carried harmonics and the node weights
-    std::vector<double> Y((size_t)nk * Q), wq(Q);
-    for (int a = 0; a < na; a++)
-        for (int b = 0; b < nb; b++) {
-            const size_t q0 = (size_t)a * nb + b, q1 = (size_t)na * nb + q0;
-            wq[q0] = wq[q1] = t.wt[a] * t.wp[b];
-            for (int r = 0; r < nk; r++) {
-                const int n = c->carried[r], mi = t.m[n] + p.lmax;
-                Y[(size_t)r * Q + q0] = t.L[(size_t)n * na + a] * t.S1[(size_t)mi * nb + b];
-                Y[(size_t)r * Q + q1] = t.L[(size_t)n * na + a] * t.S2[(size_t)mi * nb + b];
-            }
-        }
-    std::vector<double> off, site4, gv(g, g + nvec);
-    std::vector<int> lj;
-    cage_arrays(c, off, site4, lj);
-    double *d_psi = nullptr, *d_om = nullptr, *d_Y = nullptr, *d_wq = nullptr, *d_g = nullptr, *d_off = nullptr,
-           *d_site = nullptr, *d_part = nullptr, *d_out = nullptr;
-    int *d_lj = nullptr;
-    int rc = dev_alloc(&d_psi, (size_t)nvec * c->n_kry);
-    for (int64_t k = 0; k < nvec && !rc; k++)
-        rc = upload_carried(c, psi + (size_t)k * c->n_full, d_psi + (size_t)k * c->n_kry);
-    // one omega grid per set of a pass; the first also serves the clamp mask
-    if (!rc) rc = dev_alloc(&d_om, (size_t)chunk * pq);
-    if (!rc) rc = ensure_unclamped(c, d_om);
-    if (!rc) rc = dev_upload(&d_Y, Y);
-    if (!rc) rc = dev_upload(&d_wq, wq);
-    if (!rc) rc = dev_upload(&d_g, gv);
-    if (!rc) rc = dev_upload(&d_off, off);
-    if (!rc) rc = dev_upload(&d_site, site4);
-    if (!rc) rc = dev_upload(&d_lj, lj);
-    if (!rc) rc = dev_alloc(&d_part, (size_t)nwg * chunk * nent);
-    if (!rc) rc = dev_alloc(&d_out, (size_t)nset * nent);
+    const int64_t ne = c->hf_ne;
+    // passes of up to four sets whose eigenvectors' coefficients fit the
+    // density kernel's LDS (a set alone that does not fit fails there)
+    const int vmax = std::max(4, (int)(65536 / (sizeof(double) * kHfPoints * nk)) & ~3);
+    int rc = BSL_OK;
     int64_t v0 = 0;  // first eigenvector of the pass
-    for (int m0 = 0; m0 < nset && !rc; m0 += chunk) {
-        const int nm = std::min(chunk, nset - m0);
-        int64_t v = v0;
-        for (int m = 0; m < nm && !rc; m++) {
-            rc = launch_hf_density(d_psi + (size_t)v * c->n_kry, counts[m0 + m], d_g + v, d_Y, d_wq, c->d_unclamped,
-                                   d_om + (size_t)m * pq, c->P, nk, Q, c->stream);
-            v += counts[m0 + m];
-        }
+    for (int m0 = 0, nm = 0; m0 < nset && !rc; m0 += nm) {
+        int nv = counts[m0];
+        for (nm = 1; nm < chunk && m0 + nm < nset && nv + counts[m0 + nm] <= vmax; nm++) nv += counts[m0 + nm];
+        rc = launch_hf_density(c->d_hfpsi + (size_t)v0 * c->n_kry, nm, counts + m0, c->d_hfg + v0, c->d_hfY,
+                               c->d_hfwq, c->d_unclamped, c->d_pairs, c->d_pair_first, ne, c->d_hfom, c->d_hfwcm,
+                               c->P, nk, Q, c->stream);
         if (!rc)
-            rc = launch_hf_grad(d_om, nm, c->P, Q, c->d_gx, c->d_gy, c->d_gz, p.nx, p.ny, d_off, d_site, d_lj, ns,
-                                p.q_h2_h, p.q_h2_cm, p.lj_a, p.lj_b, npb, c->d_unclamped, d_part, d_out + (size_t)m0 * nent,
-                                c->stream);
-        v0 = v;
+            rc = launch_hf_grad(c->d_hfom, ne, c->d_hfwcm, nm, c->P, Q, c->d_gx, c->d_gy, c->d_gz, p.nx, p.ny,
+                                c->d_hfoff, c->d_hfsite, c->d_hflj, ns, p.q_h2_h, p.q_h2_cm, p.lj_a, p.lj_b, npb,
+                                c->d_pairs, c->d_pair_first, c->d_hfpart, c->d_hfout + (size_t)m0 * nent, c->stream);
+        for (int m = 0; m < nm; m++) v0 += counts[m0 + m];
     }
     if (!rc) {
-        hipError_t e = hipMemcpyAsync(grad, d_out, sizeof(double) * nset * nent, hipMemcpyDeviceToHost, c->stream);
+        hipError_t e = hipMemcpyAsync(grad, c->d_hfout, sizeof(double) * nset * nent, hipMemcpyDeviceToHost, c->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
         if (e != hipSuccess) rc = hip_fail(e, "download gradient", __FILE__, __LINE__);
     }
     (void)hipStreamSynchronize(c->stream);
-    dev_free(d_psi);
-    dev_free(d_om);
-    dev_free(d_Y);
-    dev_free(d_wq);
-    dev_free(d_g);
-    dev_free(d_off);
-    dev_free(d_site);
-    dev_free(d_lj);
-    dev_free(d_part);
-    dev_free(d_out);
     return rc;
+}
+}  // namespace
+
+extern "C" int bsl_hf_gradient_sets(bsl_ctx *c, int32_t nset, const int32_t *counts, const double *psi,
+                                    const double *g, double *grad)
+{
+    if (!psi || !g || !grad) return fail(BSL_E_ARG, "bsl_hf_gradient_sets: bad arguments");
+    int64_t nvec = 0;
+    int rc = hf_prepare(c, nset, counts, &nvec);
+    for (int64_t k = 0; k < nvec && !rc; k++)
+        rc = upload_carried(c, psi + (size_t)k * c->n_full, c->d_hfpsi + (size_t)k * c->n_kry);
+    if (!rc) {
+        const hipError_t e = hipMemcpyAsync(c->d_hfg, g, sizeof(double) * nvec, hipMemcpyHostToDevice, c->stream);
+        if (e != hipSuccess) rc = hip_fail(e, "upload weights", __FILE__, __LINE__);
+    }
+    if (!rc) rc = hf_passes(c, nset, counts, grad);
+    return rc;
+}
+
+extern "C" int bsl_hf_gradient_ritz(bsl_ctx *c, int32_t nset, const int32_t *counts, const int32_t *index,
+                                    const double *g, double *grad)
+{
+    if (!index || !g || !grad) return fail(BSL_E_ARG, "bsl_hf_gradient_ritz: bad arguments");
+    int64_t nvec = 0;
+    int rc = hf_prepare(c, nset, counts, &nvec);
+    if (rc) return rc;
+    const int nr = (int)c->ritz_vv.size();
+    if (nr == 0) return fail(BSL_E_STATE, "bsl_hf_gradient_ritz: no Ritz vectors of bsl_ritz_refine on the context");
+    // unit vectors psi = y / |y|: weight g / |y|^2 on the kept vector y
+    std::vector<double> gs(nvec);
+    for (int64_t k = 0; k < nvec && !rc; k++) {
+        if (index[k] < 0 || index[k] >= nr) return fail(BSL_E_ARG, "bsl_hf_gradient_ritz: Ritz vector index out of range");
+        gs[k] = g[k] / c->ritz_vv[index[k]];
+        BSL_HIP(hipMemcpyAsync(c->d_hfpsi + (size_t)k * c->n_kry, c->d_ritz + (size_t)index[k] * c->n_kry,
+                               sizeof(double) * c->n_kry, hipMemcpyDeviceToDevice, c->stream));
+    }
+    BSL_HIP(hipMemcpyAsync(c->d_hfg, gs.data(), sizeof(double) * nvec, hipMemcpyHostToDevice, c->stream));
+    return hf_passes(c, nset, counts, grad);
 }
 
 extern "C" int bsl_hf_gradient(bsl_ctx *c, int32_t nvec, const double *psi, const double *g, double *grad)

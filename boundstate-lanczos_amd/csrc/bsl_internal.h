@@ -329,17 +329,31 @@ int ctx_start(bsl_ctx *c, const double *v0, int start, uint64_t seed);
 int ctx_step(bsl_ctx *c);
 
 // Hellmann-Feynman adjoint (bsl_adjoint.hip)
-int launch_hf_density(const double *psi, int nvec, const double *g, const double *Y, const double *wq,
-                      const uint64_t *unclamped, double *om, int64_t P, int nk, int Q, hipStream_t s);
+constexpr int kHfPoints = 4;  // points per workgroup of the gradient kernels
 int hf_grad_groups(int64_t P, int npb);
 int launch_unclamped_mask(const double *vg, int64_t n, double ceiling, uint64_t *mask, hipStream_t s);
-// nset (1 to 4) sets of eigenvectors at once: omega of set m at om + m P Q;
-// partial: hf_grad_groups(P, npb) * nset * (4 ns + 4) doubles; out:
-// nset * (4 ns + 4), set after set
-int launch_hf_grad(const double *om, int nset, int64_t P, int Q, const double *gx, const double *gy, const double *gz,
-                   int nx, int ny, const double *off, const double *site4, const int *lj, int ns, double q_h,
-                   double q_cm, double lj_a, double lj_b, int npb, const uint64_t *unclamped, double *partial,
-                   double *out, hipStream_t s);
+// the kept (point, node pair) entries of each group of npb points (npb <=
+// 256, Q even and <= 2^24; node q < Q/2 stands for q and its inversion
+// partner Q - 1 - q, kept if either is unclamped): count[g] per group, then,
+// from first[g] (exclusive prefix sums of the counts, first[groups] =
+// total), entries (pl << 24) | q in pair order
+int launch_pair_count(const uint64_t *unclamped, int64_t P, int Q, int npb, int *count, hipStream_t s);
+int launch_pair_fill(const uint64_t *unclamped, int64_t P, int Q, int npb, const int64_t *first, uint32_t *pairs,
+                     hipStream_t s);
+// nset (1 to 4) sets of counts[m] consecutive eigenvectors (psi, g): the
+// folded weights of set m at om + m ne for the ne entries of the pair list
+// (pairs / first, groups of kHfPoints points) and the H2-centre weights of
+// its points at wcm + m P
+int launch_hf_density(const double *psi, int nset, const int *counts, const double *g, const double *Y,
+                      const double *wq, const uint64_t *unclamped, const uint32_t *pairs, const int64_t *first,
+                      int64_t ne, double *om, double *wcm, int64_t P, int nk, int Q, hipStream_t s);
+// the gradients of those sets (npb = kHfPoints); partial:
+// hf_grad_groups(P, npb) * nset * (4 ns + 4) doubles; out: nset * (4 ns + 4),
+// set after set
+int launch_hf_grad(const double *om, int64_t ne, const double *wcm, int nset, int64_t P, int Q, const double *gx,
+                   const double *gy, const double *gz, int nx, int ny, const double *off, const double *site4,
+                   const int *lj, int ns, double q_h, double q_cm, double lj_a, double lj_b, int npb,
+                   const uint32_t *pairs, const int64_t *first, double *partial, double *out, hipStream_t s);
 int quad_points_per_group(int na, int nk);
 // workgroups of the persistent quadrature kernel: its resident workgroups
 // per CU times the CUs of `device`, at most the number of point groups
@@ -368,12 +382,26 @@ struct bsl_ctx {
     double *d_gx = nullptr, *d_gy = nullptr, *d_gz = nullptr;
     double *d_vgrid = nullptr;         // [P][Q] potential grid (kept for the quadrature calculator)
     uint64_t *d_unclamped = nullptr;   // bit p*Q + q: V(p, q) < ceiling; built by the first gradient call
+    uint32_t *d_pairs = nullptr;       // with it: the kept node pairs of each group of kHfPoints points
+    int64_t *d_pair_first = nullptr;   // (launch_pair_fill), and each group's first entry
+    // gradient workspace, kept between calls: the node tables (harmonics at
+    // the nodes, node weights, H offsets, sites) built with the pair list,
+    // and buffers that grow to the largest request
+    double *d_hfY = nullptr, *d_hfwq = nullptr, *d_hfoff = nullptr, *d_hfsite = nullptr;
+    int *d_hflj = nullptr;
+    double *d_hfpsi = nullptr, *d_hfg = nullptr, *d_hfom = nullptr, *d_hfwcm = nullptr, *d_hfpart = nullptr,
+           *d_hfout = nullptr;
+    size_t hf_cap[6] = {0, 0, 0, 0, 0, 0};  // capacities (doubles) of d_hfpsi .. d_hfout
+    int64_t hf_ne = 0;                        // entries of the pair list
     int Q = 0;
     double *d_X = nullptr, *d_R = nullptr, *d_T = nullptr;
     // Loop-2 workspace, kept between calls (grows to the largest request):
     // Ritz accumulators [ngood][n_kry] and the per-step coefficients
     double *d_ritz = nullptr, *d_rcoef = nullptr;
     size_t ritz_cap = 0, rcoef_cap = 0;
+    // squared norms of the Ritz vectors the last bsl_ritz_refine left in
+    // d_ritz (empty when they are gone: bsl_ritz and Loop-2 calls reuse it)
+    std::vector<double> ritz_vv;
     bsl::Scal *d_scal = nullptr;
     double *d_alpha = nullptr, *d_beta = nullptr, *d_beta2 = nullptr;
     int niter_cap = 0;

@@ -307,10 +307,15 @@ int bsl_step_bytes(const bsl_ctx *ctx, double *out);
 // each eigenvalue).  grad (4*n_sites + 4 doubles): d/d site_q[s] at [4*s],
 // d/d site_pos[s][d] at [4*s + 1 + d]; then d/d q_h2_h, q_h2_cm, lj_a, lj_b
 // at [4*n_sites + 0..3].  The first call on a context evaluates the
-// potential grid once to find the clamped nodes and keeps one bit per
-// (point, node) (P * Q / 8 bytes, Q = 2 * n_theta * n_phi) until
-// bsl_destroy; later calls skip that pass.  No reference counterpart (the
-// reference has no derivatives).
+// potential grid once (in a temporary P * Q doubles) to find the clamped
+// nodes and keeps until bsl_destroy one bit per (point, node) (P * Q / 8
+// bytes, Q = 2 * n_theta * n_phi), the list of kept node pairs (4 bytes
+// each; a node and its inversion partner, the same H2 molecule with the
+// atoms exchanged, count once) and the node tables; later calls skip that
+// pass.  The call's own buffers (the eigenvectors, one weight per kept node
+// pair and set) stay with the context and are reused by later calls of the
+// same or smaller size.  No reference counterpart (the reference has no
+// derivatives).
 int bsl_hf_gradient(bsl_ctx *ctx, int32_t nvec, const double *psi, const double *g, double *grad);
 // The gradients of nset weighted sets at once (e.g. every degenerate
 // multiplet of the lowest levels): set m is the counts[m] eigenvectors that
@@ -318,9 +323,20 @@ int bsl_hf_gradient(bsl_ctx *ctx, int32_t nvec, const double *psi, const double 
 // nset blocks of 4*n_sites + 4, each equal (bitwise) to bsl_hf_gradient of
 // that set alone.  Up to four sets share one pass over the (point, node)
 // pairs, whose geometry (the distances to every site) is computed once per
-// pass; one potential-sized weight grid per set of a pass (P * Q doubles)
-// is held on the device during the call.  No reference counterpart.
+// pass.  No reference counterpart.
 int bsl_hf_gradient_sets(bsl_ctx *ctx, int32_t nset, const int32_t *counts, const double *psi, const double *g,
+                         double *grad);
+// The same for the Ritz vectors that the context's last bsl_ritz_refine
+// left on the device (vector i < its ngood is the one of evtr[i]), so that
+// they never cross PCIe: set m is the counts[m] vectors index[...] that
+// follow the previous sets' in index, with their weights in g.  Each vector
+// y is taken as the unit vector y / |y| (|y|^2 from the refinement; the
+// weight g / |y|^2 carries it), so the result equals bsl_hf_gradient_sets
+// of the downloaded, normalised vectors up to the rounding of that
+// normalisation.  BSL_E_STATE when the context holds no such vectors (a
+// later bsl_ritz or bsl_ritz_refine replaces them).  No reference
+// counterpart.
+int bsl_hf_gradient_ritz(bsl_ctx *ctx, int32_t nset, const int32_t *counts, const int32_t *index, const double *g,
                          double *grad);
 
 // Counts of the (point, quadrature node) pairs of a cage context's potential

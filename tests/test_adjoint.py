@@ -188,6 +188,45 @@ def test_gpu_gradient_matches_finite_differences():
 
 
 @pytest.mark.gpu
+def test_gpu_gradient_of_device_ritz_vectors():
+    """bsl_hf_gradient_ritz on the Ritz vectors levels() leaves on the device
+    equals bsl_hf_gradient_sets of the downloaded unit vectors (within 1e-13
+    of the largest entry: only the normalisation rounds differently), for
+    sets of one and two levels with arbitrary weights; it refuses a context
+    without refined Ritz vectors, one whose vectors a later bsl_ritz
+    replaced, and indices beyond them."""
+    import bsl_amd as B
+    spec, kw = cs.cage_case(6, 3, 8, 8, "All")
+    with B.Calculator(**kw) as c:
+        with pytest.raises(B.BslError):
+            c.hf_gradient_ritz([([0], [1.0])])
+        e, psi, res = c.levels(NITER, 2)
+        sets_idx = [([0], [1.0]), ([1, 0], [0.7, -0.4]), ([1], [2.5]), ([0, 1], [1.0, 1.0]), ([0], [0.3])]
+        got = c.hf_gradient_ritz(sets_idx)
+        ref = c.hf_gradient_sets([(psi[list(i)], g) for i, g in sets_idx])
+        for a, b in zip(got, ref):
+            scale = max(np.max(np.abs(b["site_q"])), np.max(np.abs(b["site_pos"])))
+            assert np.max(np.abs(a["site_q"] - b["site_q"])) <= 1e-13 * scale
+            assert np.max(np.abs(a["site_pos"] - b["site_pos"])) <= 1e-13 * scale
+            for name in ("q_h2_h", "q_h2_cm", "lj_a", "lj_b"):
+                assert abs(a[name] - b[name]) <= 1e-13 * abs(b[name]), name
+        with pytest.raises(B.BslError):
+            c.hf_gradient_ritz([([2], [1.0])])
+        # levels without downloading the vectors: the same gradients
+        e2, none, _ = c.levels(NITER, 2, vectors=False)
+        assert none is None and np.array_equal(e2, e)
+        got2 = c.hf_gradient_ritz(sets_idx)
+        for a, b in zip(got2, got):
+            for name in a:
+                assert np.array_equal(np.asarray(a[name]), np.asarray(b[name])), name
+        a, b, b2 = c.lanczos(NITER, start="random", seed=0)
+        evtr, _ = B.trivec(a, b, b2, e[:1])
+        c.ritz(NITER, evtr, 1, start="random", seed=0)
+        with pytest.raises(B.BslError):
+            c.hf_gradient_ritz([([0], [1.0])])
+
+
+@pytest.mark.gpu
 def test_autograd_wrapper_chain_rule():
     """torch.autograd through the water-model parameters: dL/d(q_O, q_H,
     sigma_O, eps_O) of L = E0 + 2 E1 against central differences."""

@@ -153,6 +153,41 @@ def test_gpu_gradient_sets_equal_single_sets():
             assert abs(gs[name] - ref[e]) <= 1e-10 * abs(ref[e])
 
 
+@pytest.mark.gpu
+def test_gpu_gradient_more_sites_than_one_block():
+    """120 sites (the cage plus a second one 0.9 nm away): the gradient kernel
+    takes its sites 64 at a time, so the pair list is walked twice; every
+    entry against the oracle's gradient within 1e-10, single and in sets."""
+    import bsl_amd as B
+    p1, q1, t1, l1 = cs.cage11_sites()
+    p2, q2, t2, l2 = cs.synthetic_sites(3)
+    sites = (np.concatenate([p1, p2 + np.array([0.9, 0.0, 0.0])]), np.concatenate([q1, q2]),
+             bytes(t1) + bytes(t2), np.concatenate([l1, np.asarray(l2, dtype=np.int32)]))
+    spec, kw = cs.cage_case(5, 3, 8, 9, "All", sites=sites)
+    ns = len(spec["site_q"])
+    assert ns == 120
+    o = O.Oracle(spec)
+    rng = np.random.default_rng(5)
+    sets = []
+    for k in (2, 1, 1):
+        psi = rng.uniform(-1, 1, (k, o.N))
+        psi /= np.linalg.norm(psi, axis=1, keepdims=True)
+        sets.append((psi, rng.uniform(-1, 1, k)))
+    with B.Calculator(**kw) as c:
+        got = [c.hf_gradient(p, g) for p, g in sets]
+        got_sets = c.hf_gradient_sets(sets)
+    for (psi, g), g1, gs in zip(sets, got, got_sets):
+        ref = o.hf_gradient(psi, g)
+        scale = np.max(np.abs(ref[:4 * ns]))
+        for gg in (g1, gs):
+            assert np.max(np.abs(gg["site_q"] - ref[0:4 * ns:4])) / scale < 1e-10
+            assert np.max(np.abs(gg["site_pos"] - ref[:4 * ns].reshape(ns, 4)[:, 1:4])) / scale < 1e-10
+            for e, name in zip(range(4 * ns, 4 * ns + 4), ("q_h2_h", "q_h2_cm", "lj_a", "lj_b")):
+                assert abs(gg[name] - ref[e]) <= 1e-10 * abs(ref[e])
+        for name in g1:
+            assert np.array_equal(np.asarray(g1[name]), np.asarray(gs[name])), name
+
+
 NITER = 800
 
 

@@ -210,12 +210,16 @@ def batch_leg(B, args):
         bt = B.Batch(batch_members(args, calculator, "fp32"))
         prep = time.perf_counter() - t0
         ms = bt.time_steps(4, args.batch_steps, graph=True)
+        # algorithmic bytes of one lockstep step: every member's potential
+        # data and vector passes (bsl_step_bytes)
+        nbytes = sum(float(np.sum(bt.member(i).step_bytes()[:2])) for i in range(32))
         bt.close()
-        out[calculator] = {"members": 32, "member_steps_per_s": 32 * args.batch_steps / (ms / 1e3),
-                           "ms_per_lockstep_step": ms / args.batch_steps, "prep_s": prep,
-                           "steps_timed": args.batch_steps}
-        print("batch %s fp32: prep %.1f s, %.3f ms per lockstep step, %.0f member-steps/s"
-              % (calculator, prep, ms / args.batch_steps, out[calculator]["member_steps_per_s"]),
+        sec = ms / args.batch_steps / 1e3
+        out[calculator] = {"members": 32, "member_steps_per_s": 32 / sec, "ms_per_lockstep_step": 1e3 * sec,
+                           "prep_s": prep, "steps_timed": args.batch_steps,
+                           "algorithmic_bytes_per_lockstep_step": nbytes, "achieved_GBs": nbytes / sec / 1e9}
+        print("batch %s fp32: prep %.1f s, %.3f ms per lockstep step, %.0f member-steps/s, %.0f GB/s"
+              % (calculator, prep, 1e3 * sec, out[calculator]["member_steps_per_s"], nbytes / sec / 1e9),
               file=sys.stderr, flush=True)
     return out
 

@@ -22,24 +22,6 @@ namespace bsl {
 constexpr int kAWG = 256;
 constexpr double kCoulombK = 138.9354846111096;
 
-// mask bit n = [vg[n] < ceiling], 64 nodes per word: one wave per word, the
-// wave's lanes read 64 consecutive entries (coalesced) and lane 0 stores the
-// ballot.  The loop bound is uniform over a wave, so every ballot is whole.
-__global__ __launch_bounds__(kAWG) void k_unclamped_mask(const double *__restrict__ vg, int64_t n, double ceiling,
-                                                         uint64_t *__restrict__ mask)
-{
-    const int lane = threadIdx.x & 63;
-    const int64_t nw = (n + 63) / 64;
-    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const int64_t stride = ((int64_t)gridDim.x * blockDim.x) >> 6;
-    for (int64_t w = wave; w < nw; w += stride) {
-        const int64_t idx = w * 64 + lane;
-        const bool keep = idx < n && vg[idx] < ceiling;
-        const uint64_t bits = __ballot(keep);
-        if (lane == 0) mask[w] = bits;
-    }
-}
-
 // Inversion pairs of nodes.  The Q = 2 na nb nodes are (half, a, b): theta
 // node a (Gauss-Legendre, symmetric about pi/2) and phi = +-phi_b of the
 // Chebyshev angles phi_b = pi (2b + 1) / (2 nb) (symmetric about pi / 2), at
@@ -471,16 +453,6 @@ __global__ __launch_bounds__(kAWG) void k_hf_reduce(const double *__restrict__ p
         __syncthreads();
     }
     if (threadIdx.x == 0) out[e] = red[0];
-}
-
-int launch_unclamped_mask(const double *vg, int64_t n, double ceiling, uint64_t *mask, hipStream_t s)
-{
-    const int64_t nw = (n + 63) / 64;
-    const int64_t groups = (nw + kAWG / 64 - 1) / (kAWG / 64);
-    hipLaunchKernelGGL(k_unclamped_mask, dim3((unsigned)(groups < 65536 ? groups : 65536)), dim3(kAWG), 0, s, vg, n,
-                       ceiling, mask);
-    BSL_HIP(hipGetLastError());
-    return BSL_OK;
 }
 
 int launch_hf_density(const double *psi, int nset, const int *counts, const double *g, const double *Y,

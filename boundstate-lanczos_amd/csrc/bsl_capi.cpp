@@ -305,28 +305,37 @@ void cage_arrays(const bsl_ctx *c, std::vector<double> &off, std::vector<double>
     lj.assign(c->site_lj.begin(), c->site_lj.end());
 }
 
+// the cage potential's clamped grid [P][Q] (device point order) into d_vg,
+// or (mask non-null) only its clamp mask
+int cage_vgrid(bsl_ctx *c, double *d_vg, uint64_t *mask)
+{
+    const bsl_params &p = c->p;
+    std::vector<double> off, site4;
+    std::vector<int> lj;
+    cage_arrays(c, off, site4, lj);
+    CageDev cage;
+    cage.nsites = p.n_sites;
+    double *d_off = nullptr;
+    int rc = dev_upload(&cage.site, site4);
+    if (!rc) rc = dev_upload(&cage.lj, lj);
+    if (!rc) rc = dev_upload(&d_off, off);
+    if (!rc)
+        rc = launch_vgrid_cage(d_vg, mask, c->P, c->Q, c->d_gx, c->d_gy, c->d_gz, p.nx, p.ny, d_off, cage, p.q_h2_h,
+                               p.q_h2_cm, p.lj_a, p.lj_b, p.ceiling, c->stream);
+    (void)hipStreamSynchronize(c->stream);
+    dev_free(cage.site);
+    dev_free(cage.lj);
+    dev_free(d_off);
+    return rc;
+}
+
 // the clamped potential grid [P][Q] (device point order) into d_vg
 int make_vgrid(bsl_ctx *c, double *d_vg)
 {
     const bsl_params &p = c->p;
     int rc = BSL_OK;
     if (p.potential == BSL_POT_CAGE) {
-        std::vector<double> off, site4;
-        std::vector<int> lj;
-        cage_arrays(c, off, site4, lj);
-        CageDev cage;
-        cage.nsites = p.n_sites;
-        double *d_off = nullptr;
-        if (!rc) rc = dev_upload(&cage.site, site4);
-        if (!rc) rc = dev_upload(&cage.lj, lj);
-        if (!rc) rc = dev_upload(&d_off, off);
-        if (!rc)
-            rc = launch_vgrid_cage(d_vg, c->P, c->Q, c->d_gx, c->d_gy, c->d_gz, p.nx, p.ny, d_off, cage, p.q_h2_h,
-                                   p.q_h2_cm, p.lj_a, p.lj_b, p.ceiling, c->stream);
-        (void)hipStreamSynchronize(c->stream);
-        dev_free(cage.site);
-        dev_free(cage.lj);
-        dev_free(d_off);
+        rc = cage_vgrid(c, d_vg, nullptr);
     } else if (p.potential == BSL_POT_CAGE_GRID) {
         std::vector<double> off, site4;
         std::vector<int> lj;
@@ -1318,7 +1327,7 @@ int grow(double *&buf, size_t &cap, size_t need)
 
 // What the gradients need of a cage context, fixed by its potential and
 // built once: which (point, node) pairs are below the ceiling clamp (one bit
-// each, from the grid regenerated into a temporary P x Q buffer), the list
+// each, from a pass of the grid kernel that keeps only the bits), the list
 // of kept node pairs per group of kHfPoints points that the gradient kernels
 // walk, and the node tables (carried harmonics and weights at the nodes, H
 // offsets, sites).
@@ -1329,14 +1338,11 @@ int ensure_hf_setup(bsl_ctx *c)
     const Tables &t = c->tab;
     const int na = t.na, nb = t.nb, nk = (int)c->carried.size(), Q = c->Q;
     const int ng = hf_grad_groups(c->P, kHfPoints);
-    double *d_vg = nullptr;
     int *d_count = nullptr;
     std::vector<int> count(ng);
     std::vector<int64_t> first(ng + 1, 0);
-    int rc = dev_alloc(&d_vg, (size_t)c->P * Q);
-    if (!rc) rc = make_vgrid(c, d_vg);
-    if (!rc) rc = dev_alloc(&c->d_unclamped, ((size_t)c->P * Q + 63) / 64);
-    if (!rc) rc = launch_unclamped_mask(d_vg, c->P * Q, p.ceiling, c->d_unclamped, c->stream);
+    int rc = dev_alloc(&c->d_unclamped, ((size_t)c->P * Q + 63) / 64);
+    if (!rc) rc = cage_vgrid(c, nullptr, c->d_unclamped);
     if (!rc) rc = dev_alloc(&d_count, (size_t)ng);
     if (!rc) rc = launch_pair_count(c->d_unclamped, c->P, Q, kHfPoints, d_count, c->stream);
     if (!rc) {
@@ -1381,7 +1387,6 @@ int ensure_hf_setup(bsl_ctx *c)
     const hipError_t e = hipStreamSynchronize(c->stream);
     if (!rc && e != hipSuccess) rc = hip_fail(e, "clamp mask", __FILE__, __LINE__);
     dev_free(d_count);
-    dev_free(d_vg);
     if (rc) free_hf_setup(c);
     return rc;
 }

@@ -296,9 +296,11 @@ struct CageDev {
     double *site = nullptr;  // [nsites][4]: x, y, z, q
     int *lj = nullptr;
 };
-int launch_vgrid_cage(double *vg, int64_t P, int Q, const double *gx, const double *gy, const double *gz, int nx,
-                      int ny, const double *off /*[Q][3]*/, const CageDev &cage, double q_h, double q_cm,
-                      double lj_a, double lj_b, double ceiling, hipStream_t s);
+// the clamped grid into vg, or (mask non-null, vg unused) only its clamp
+// mask: bit idx = [vg[idx] < ceiling], 64 per word
+int launch_vgrid_cage(double *vg, uint64_t *mask, int64_t P, int Q, const double *gx, const double *gy,
+                      const double *gz, int nx, int ny, const double *off /*[Q][3]*/, const CageDev &cage,
+                      double q_h, double q_cm, double lj_a, double lj_b, double ceiling, hipStream_t s);
 int launch_vgrid_iho(double *vg, int64_t P, int Q, const double *gx, const double *gy, const double *gz, int nx,
                      int ny, double factor, double iho_ceil, double ceiling, hipStream_t s);
 // potential universe of the grid-mapped cage potential
@@ -331,7 +333,6 @@ int ctx_step(bsl_ctx *c);
 // Hellmann-Feynman adjoint (bsl_adjoint.hip)
 constexpr int kHfPoints = 4;  // points per workgroup of the gradient kernels
 int hf_grad_groups(int64_t P, int npb);
-int launch_unclamped_mask(const double *vg, int64_t n, double ceiling, uint64_t *mask, hipStream_t s);
 // the kept (point, node pair) entries of each group of npb points (npb <=
 // 256, Q even and <= 2^24; node q < Q/2 stands for q and its inversion
 // partner Q - 1 - q, kept if either is unclamped): count[g] per group, then,

@@ -1991,22 +1991,30 @@ __device__ inline double rsqrt_nr(double x)
 // vg[p][q], q = (half, a, b): H2 centre at grid point p, H atoms at +-off[q].
 // The H-site terms K q_H q_s / |h - s| are summed in site order with the
 // weights K q_H q_s staged in LDS and 1/|h - s| from rsqrt_nr (a few ulp from
-// the quotient of a correctly rounded square root, per term).
-__global__ __launch_bounds__(kWG) void k_vgrid_cage(double *vg, int64_t P, int Q, const double *__restrict__ gx,
-                                                    const double *__restrict__ gy, const double *__restrict__ gz,
-                                                    int nx, int ny, const double *__restrict__ off,
-                                                    const double *__restrict__ site_g, int ns,
-                                                    const double *__restrict__ cm, double q_h, double ceiling)
+// the quotient of a correctly rounded square root, per term).  MASK: instead
+// of the grid, one bit per (point, node), [vg < ceiling] (the clamp mask of
+// the gradients, bit idx & 63 of word idx >> 6): each wave takes 64
+// consecutive idx (the loop runs in whole waves) and stores the ballot.
+template <bool MASK>
+__global__ __launch_bounds__(kWG) void k_vgrid_cage(double *vg, uint64_t *mask, int64_t P, int Q,
+                                                    const double *__restrict__ gx, const double *__restrict__ gy,
+                                                    const double *__restrict__ gz, int nx, int ny,
+                                                    const double *__restrict__ off, const double *__restrict__ site_g,
+                                                    int ns, const double *__restrict__ cm, double q_h, double ceiling)
 {
     extern __shared__ double site[];  // [ns][4]: x, y, z, K q_H q_s
     for (int t = threadIdx.x; t < 4 * ns; t += blockDim.x)
         site[t] = (t & 3) == 3 ? 138.9354846111096 * q_h * site_g[t] : site_g[t];
     __syncthreads();
     const int64_t total = P * (int64_t)Q;
-    for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
-         idx += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t p = idx / Q;
-        const int q = (int)(idx - p * Q);
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    // whole waves: blockDim and the stride are multiples of 64
+    const int64_t end = MASK ? (total + 63) / 64 * 64 : total;
+    for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < end; idx += stride) {
+        const bool in = idx < total;
+        const int64_t ic = in ? idx : total - 1;
+        const int64_t p = ic / Q;
+        const int q = (int)(ic - p * Q);
         const int i = (int)(p % nx), j = (int)((p / nx) % ny), k = (int)(p / ((int64_t)nx * ny));
         const double x = gx[i], y = gy[j], z = gz[k];
         const double ox = off[3 * q], oy = off[3 * q + 1], oz = off[3 * q + 2];
@@ -2022,7 +2030,12 @@ __global__ __launch_bounds__(kWG) void k_vgrid_cage(double *vg, int64_t P, int Q
         }
         double v = va + vb + cm[p];
         if (v >= ceiling) v = ceiling;
-        vg[idx] = v;
+        if constexpr (MASK) {
+            const uint64_t bits = __ballot(in && v < ceiling);
+            if ((threadIdx.x & 63) == 0) mask[idx >> 6] = bits;
+        } else {
+            vg[idx] = v;
+        }
     }
 }
 
@@ -2048,9 +2061,9 @@ __global__ __launch_bounds__(kWG) void k_vgrid_iho(double *vg, int64_t P, int Q,
     }
 }
 
-int launch_vgrid_cage(double *vg, int64_t P, int Q, const double *gx, const double *gy, const double *gz, int nx,
-                      int ny, const double *off, const CageDev &cage, double q_h, double q_cm, double lj_a,
-                      double lj_b, double ceiling, hipStream_t s)
+int launch_vgrid_cage(double *vg, uint64_t *mask, int64_t P, int Q, const double *gx, const double *gy,
+                      const double *gz, int nx, int ny, const double *off, const CageDev &cage, double q_h,
+                      double q_cm, double lj_a, double lj_b, double ceiling, hipStream_t s)
 {
     double *cm = nullptr;
     BSL_HIP(hipMallocAsync((void **)&cm, sizeof(double) * (size_t)P, s));
@@ -2059,8 +2072,12 @@ int launch_vgrid_cage(double *vg, int64_t P, int Q, const double *gx, const doub
                        gz, nx, ny, cage.site, cage.lj, cage.nsites, q_cm, lj_a, lj_b);
     BSL_HIP(hipGetLastError());
     const size_t lds = sizeof(double) * 4 * cage.nsites;
-    hipLaunchKernelGGL(k_vgrid_cage, dim3(grid_for(P * Q, 65536)), dim3(kWG), lds, s, vg, P, Q, gx, gy, gz, nx,
-                       ny, off, cage.site, cage.nsites, cm, q_h, ceiling);
+    if (mask)
+        hipLaunchKernelGGL(k_vgrid_cage<true>, dim3(grid_for(P * Q, 65536)), dim3(kWG), lds, s, nullptr, mask, P, Q,
+                           gx, gy, gz, nx, ny, off, cage.site, cage.nsites, cm, q_h, ceiling);
+    else
+        hipLaunchKernelGGL(k_vgrid_cage<false>, dim3(grid_for(P * Q, 65536)), dim3(kWG), lds, s, vg, nullptr, P, Q,
+                           gx, gy, gz, nx, ny, off, cage.site, cage.nsites, cm, q_h, ceiling);
     BSL_HIP(hipGetLastError());
     BSL_HIP(hipFreeAsync(cm, s));
     return BSL_OK;

@@ -6,8 +6,9 @@ run the Lanczos recurrence, pick the lowest converged levels, rebuild their
 Ritz vectors with a second Lanczos pass (recomputing the Krylov vectors
 rather than storing them) and refine each level by an FP64 Rayleigh
 quotient.  Backward: Hellmann-Feynman, dE_k/dtheta = <psi_k|dV/dtheta|psi_k>,
-evaluated by the library's adjoint kernels (bsl_hf_gradient) for the
-incoming weights dL/dE_k in one pass.
+evaluated by the library's adjoint kernels for the incoming weights dL/dE_k
+in one pass, on the Ritz vectors the forward pass left on the device
+(bsl_hf_gradient_ritz).
 
 Near-degenerate levels (m-multiplets) are individually differentiable only
 while they stay resolved; their sum is smooth, so losses should use sums over
@@ -45,16 +46,17 @@ class CageLevels(torch.autograd.Function):
         lj = np.asarray(config["lj_sites"], dtype=np.int32)
         calc = Calculator(potential="cage", sites=(pos, q, lj), q_h2_h=float(qh[0]), q_h2_cm=float(qh[1]),
                           lj_a=float(ab[0]), lj_b=float(ab[1]), **config["calc"])
-        e, psi, _ = calc.levels(config["niter"], config["count"], seed=config.get("seed", 0))
-        ctx.calc, ctx.psi = calc, psi
+        # the Ritz vectors stay on the device for the backward pass
+        e, _, _ = calc.levels(config["niter"], config["count"], seed=config.get("seed", 0), vectors=False)
+        ctx.calc = calc
         return torch.as_tensor(e, dtype=torch.float64)
 
     @staticmethod
     def backward(ctx, grad_e):
         g = grad_e.detach().cpu().to(torch.float64).numpy()
-        gr = ctx.calc.hf_gradient(ctx.psi, g)
+        gr = ctx.calc.hf_gradient_ritz([(np.arange(len(g)), g)])[0]
         ctx.calc.close()
-        ctx.calc = ctx.psi = None
+        ctx.calc = None
         return (torch.as_tensor(gr["site_pos"]), torch.as_tensor(gr["site_q"]),
                 torch.tensor([gr["q_h2_h"], gr["q_h2_cm"]], dtype=torch.float64),
                 torch.tensor([gr["lj_a"], gr["lj_b"]], dtype=torch.float64), None)

@@ -154,6 +154,26 @@ def test_gpu_gradient_sets_equal_single_sets():
 
 
 @pytest.mark.gpu
+def test_gpu_clamp_mask_matches_grid_and_oracle():
+    """The gradients' clamp mask (a pass of the grid kernel keeping one bit
+    per (point, node)) counts exactly the nodes of the context's own clamped
+    grid below the ceiling, and those of the oracle's grid, on two grids
+    whose P Q nodes end in a part-filled word."""
+    import bsl_amd as B
+    for n, na, nb in ((5, 8, 9), (6, 7, 5)):
+        spec, kw = cs.cage_case(n, 3, na, nb, "All")
+        with B.Calculator(**kw) as c:
+            frac = c.unclamped_fraction()
+            g1, g2 = c.vgrid()
+        total = g1.size + g2.size  # 18000 and 15120 nodes: a part-filled last word of 64 bits
+        below = np.count_nonzero(g1 < kw["ceiling"]) + np.count_nonzero(g2 < kw["ceiling"])
+        assert 0 < below < total
+        assert round(frac * total) == below
+        o1, o2 = O.Oracle(spec).vgrid()
+        assert np.count_nonzero(np.asarray(o1) < kw["ceiling"]) + np.count_nonzero(np.asarray(o2) < kw["ceiling"]) == below
+
+
+@pytest.mark.gpu
 def test_gpu_gradient_more_sites_than_one_block():
     """120 sites (the cage plus a second one 0.9 nm away): the gradient kernel
     takes its sites 64 at a time, so the pair list is walked twice; every

@@ -154,6 +154,32 @@ def test_gpu_gradient_sets_equal_single_sets():
 
 
 @pytest.mark.gpu
+def test_gpu_gradient_sets_split_by_vector_count():
+    """At lmax 5 (36 carried planes) a density pass holds at most 56
+    eigenvectors: three sets of 20 run as two passes (40, then 20), and a
+    set of 60 alone is refused with an error, not launched; every set's
+    gradient is bitwise the single-set one."""
+    import bsl_amd as B
+    spec, kw = cs.cage_case(5, 5, 8, 9, "All")
+    rng = np.random.default_rng(9)
+    with B.Calculator(**kw) as c:
+        n = c.N
+        sets = []
+        for _ in range(3):
+            psi = rng.uniform(-1, 1, (20, n))
+            psi /= np.linalg.norm(psi, axis=1, keepdims=True)
+            sets.append((psi, rng.uniform(-1, 1, 20)))
+        got = c.hf_gradient_sets(sets)
+        single = [c.hf_gradient(p, g) for p, g in sets]
+        for a, b in zip(got, single):
+            for name in a:
+                assert np.array_equal(np.asarray(a[name]), np.asarray(b[name])), name
+        big = rng.uniform(-1, 1, (60, n))
+        with pytest.raises(B.BslError):
+            c.hf_gradient(big, np.ones(60))
+
+
+@pytest.mark.gpu
 def test_gpu_clamp_mask_matches_grid_and_oracle():
     """The gradients' clamp mask (a pass of the grid kernel keeping one bit
     per (point, node)) counts exactly the nodes of the context's own clamped

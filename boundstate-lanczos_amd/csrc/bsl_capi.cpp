@@ -949,12 +949,14 @@ int ritz_run(bsl_ctx *c, int niter, const double *v0, int start, uint64_t seed, 
     // coefficient of v_j for each vector, zero once the cumulative norm
     // reaches 1 - 1e-16 (lanczosEigenValuesVectors.cpp:404-419)
     std::vector<double> coef((size_t)niter * ngood);
+    int jlast = 0;  // the last step with a non-zero coefficient: later steps add nothing
     for (int g = 0; g < ngood; g++) {
         double cum = 0.0;
         for (int j = 1; j <= niter; j++) {
             const double e = evtr[(j - 1) + (size_t)niter * g];
             cum += std::pow(e, 2.);
             coef[(size_t)(j - 1) * ngood + g] = (cum < (1. - 1.e-16)) ? e : 0.0;
+            if (coef[(size_t)(j - 1) * ngood + g] != 0.0) jlast = std::max(jlast, j);
         }
     }
     int rc = ensure_workspace(&c->d_rcoef, &c->rcoef_cap, coef.size());
@@ -965,8 +967,9 @@ int ritz_run(bsl_ctx *c, int niter, const double *v0, int start, uint64_t seed, 
     if (!rc) rc = ensure_coeffs(c, niter);
     if (!rc) rc = start_lanczos(c, v0, start, seed);
     for (int j = 1; j <= niter && !rc; j++) {
-        rc = launch_ritz_accum(c->d_ritz, c->d_X, c->d_R, c->d_scal, c->d_rcoef + (size_t)(j - 1) * ngood, c->n_kry,
-                               ngood, c->stream);
+        if (j <= jlast)
+            rc = launch_ritz_accum(c->d_ritz, c->d_X, c->d_R, c->d_scal, c->d_rcoef + (size_t)(j - 1) * ngood,
+                                   c->n_kry, ngood, c->stream);
         if (!rc) rc = lanczos_step(c, c->op, j, nullptr);
     }
     // the host coefficient list must outlive the copy

@@ -1380,13 +1380,6 @@ int ensure_hf_setup(bsl_ctx *c)
         if (!rc) rc = dev_upload(&c->d_hfsite, site4);
         if (!rc) rc = dev_upload(&c->d_hflj, lj);
     }
-    // the weights of a pass of four sets, first touched here so that the
-    // first gradient call does not pay for mapping them
-    const int nent = 4 * p.n_sites + 4;
-    if (!rc) rc = grow(c->d_hfom, c->hf_cap[2], (size_t)4 * std::max<int64_t>(c->hf_ne, 1));
-    if (!rc) rc = grow(c->d_hfwcm, c->hf_cap[3], (size_t)4 * c->P);
-    if (!rc) rc = grow(c->d_hfpart, c->hf_cap[4], (size_t)ng * 4 * nent);
-    if (!rc) BSL_HIP(hipMemsetAsync(c->d_hfom, 0, sizeof(double) * c->hf_cap[2], c->stream));
     const hipError_t e = hipStreamSynchronize(c->stream);
     if (!rc && e != hipSuccess) rc = hip_fail(e, "clamp mask", __FILE__, __LINE__);
     dev_free(d_count);

@@ -259,7 +259,7 @@ constexpr int kHfSites = 64;  // sites per pass
 template <int NS>
 __global__ __launch_bounds__(kAWG) void k_hf_grad(const double *__restrict__ om, int64_t ne,
                                                   const double *__restrict__ wcm_g, int64_t P,
-                                                  int Q, const double *__restrict__ gx, const double *__restrict__ gy,
+                                                  const double *__restrict__ gx, const double *__restrict__ gy,
                                                   const double *__restrict__ gz, int nx, int ny,
                                                   const double *__restrict__ off, const double *__restrict__ site_g,
                                                   const int *__restrict__ lj_g, int ns, double q_h, double q_cm,
@@ -491,7 +491,7 @@ int launch_pair_fill(const uint64_t *unclamped, int64_t P, int Q, int npb, const
     return BSL_OK;
 }
 
-int launch_hf_grad(const double *om, int64_t ne, const double *wcm, int nset, int64_t P, int Q, const double *gx,
+int launch_hf_grad(const double *om, int64_t ne, const double *wcm, int nset, int64_t P, const double *gx,
                    const double *gy, const double *gz, int nx, int ny, const double *off, const double *site4,
                    const int *lj, int ns, double q_h, double q_cm, double lj_a, double lj_b, int npb,
                    const uint32_t *pairs, const int64_t *first, double *partial, double *out, hipStream_t s)
@@ -505,8 +505,8 @@ int launch_hf_grad(const double *om, int64_t ne, const double *wcm, int nset, in
     // one instance per set count: a pass of fewer sets carries fewer sums
     // (the one-set instance keeps the occupancy of a single-set kernel)
     auto kern = nset == 1 ? k_hf_grad<1> : nset == 2 ? k_hf_grad<2> : nset == 3 ? k_hf_grad<3> : k_hf_grad<4>;
-    hipLaunchKernelGGL(kern, dim3(nwg), dim3(kAWG), lds, s, om, ne, wcm, P, Q, gx, gy, gz, nx, ny, off, site4, lj, ns,
-                       q_h, q_cm, lj_a, lj_b, npb, pairs, first, partial);
+    hipLaunchKernelGGL(kern, dim3(nwg), dim3(kAWG), lds, s, om, ne, wcm, P, gx, gy, gz, nx, ny, off, site4, lj, ns, q_h,
+                       q_cm, lj_a, lj_b, npb, pairs, first, partial);
     BSL_HIP(hipGetLastError());
     const int nent = (4 * ns + 4) * nset;
     hipLaunchKernelGGL(k_hf_reduce, dim3(nent), dim3(kAWG), 0, s, partial, nwg, nent, out);

@@ -1455,7 +1455,7 @@ int hf_passes(bsl_ctx *c, int nset, const int32_t *counts, double *grad)
                                c->d_hfwq, c->d_unclamped, c->d_pairs, c->d_pair_first, ne, c->d_hfom, c->d_hfwcm,
                                c->P, nk, Q, c->stream);
         if (!rc)
-            rc = launch_hf_grad(c->d_hfom, ne, c->d_hfwcm, nm, c->P, Q, c->d_gx, c->d_gy, c->d_gz, p.nx, p.ny,
+            rc = launch_hf_grad(c->d_hfom, ne, c->d_hfwcm, nm, c->P, c->d_gx, c->d_gy, c->d_gz, p.nx, p.ny,
                                 c->d_hfoff, c->d_hfsite, c->d_hflj, ns, p.q_h2_h, p.q_h2_cm, p.lj_a, p.lj_b, npb,
                                 c->d_pairs, c->d_pair_first, c->d_hfpart, c->d_hfout + (size_t)m0 * nent, c->stream);
         for (int m = 0; m < nm; m++) v0 += counts[m0 + m];

@@ -351,7 +351,7 @@ int launch_hf_density(const double *psi, int nset, const int *counts, const doub
 // the gradients of those sets (npb = kHfPoints); partial:
 // hf_grad_groups(P, npb) * nset * (4 ns + 4) doubles; out: nset * (4 ns + 4),
 // set after set
-int launch_hf_grad(const double *om, int64_t ne, const double *wcm, int nset, int64_t P, int Q, const double *gx,
+int launch_hf_grad(const double *om, int64_t ne, const double *wcm, int nset, int64_t P, const double *gx,
                    const double *gy, const double *gz, int nx, int ny, const double *off, const double *site4,
                    const int *lj, int ns, double q_h, double q_cm, double lj_a, double lj_b, int npb,
                    const uint32_t *pairs, const int64_t *first, double *partial, double *out, hipStream_t s);

@@ -250,9 +250,10 @@ __device__ inline double a_rsqrt(double x)
 // Per site and set only two sums are carried over the H atoms h of the pairs,
 //   S = sum omega / |h - s|,   G = sum omega (h - s) / |h - s|^3,
 // since d/dq_s = k q_h S, its share of d/dq_h = k q_s S and d/ds = k q_h q_s G;
-// the geometry (|h - s|^-1 of both atoms) is shared by the sets, and each
-// set's sums take the same operations in the same order as with that set
-// alone.  The quarters combine in a fixed order.  The H2 centre term only
+// the geometry (|h - s|^-1 of both atoms, and their sum and cubed-weighted
+// offset) is shared by the sets, so a set adds four multiply-adds per pair,
+// and each set's sums take the same operations in the same order as with
+// that set alone.  The quarters combine in a fixed order.  The H2 centre term only
 // depends on the point: its weight is omega_cm(p) = sum_q omega(p, q).
 constexpr int kHfSites = 64;  // sites per pass
 
@@ -311,16 +312,17 @@ __global__ __launch_bounds__(kAWG) void k_hf_grad(const double *__restrict__ om,
                 const double e2x = b.y - sx, e2y = c.x - sy, e2z = c.y - sz;
                 const double i1 = a_rsqrt(e1x * e1x + e1y * e1y + e1z * e1z);
                 const double i2 = a_rsqrt(e2x * e2x + e2y * e2y + e2z * e2z);
-                const double i1s = i1 * i1, i2s = i2 * i2;
+                // the pair's geometry, shared by the sets: |h1 - s|^-1 + |h2 - s|^-1
+                // and sum_a (h_a - s) |h_a - s|^-3; each set adds its weight times them
+                const double si = i1 + i2, b1 = i1 * i1 * i1, b2 = i2 * i2 * i2;
+                const double tx = b1 * e1x + b2 * e2x, ty = b1 * e1y + b2 * e2y, tz = b1 * e1z + b2 * e2z;
 #pragma unroll
                 for (int m = 0; m < NS; m++) {
                     const double w = stw[m][u];
-                    const double w1 = w * i1, w2 = w * i2;
-                    S[m] += w1 + w2;
-                    const double c1 = w1 * i1s, c2 = w2 * i2s;
-                    Gx[m] += c1 * e1x + c2 * e2x;
-                    Gy[m] += c1 * e1y + c2 * e2y;
-                    Gz[m] += c1 * e1z + c2 * e2z;
+                    S[m] = fma(w, si, S[m]);
+                    Gx[m] = fma(w, tx, Gx[m]);
+                    Gy[m] = fma(w, ty, Gy[m]);
+                    Gz[m] = fma(w, tz, Gz[m]);
                 }
             }
         };

@@ -180,6 +180,34 @@ def test_gpu_gradient_sets_split_by_vector_count():
 
 
 @pytest.mark.gpu
+def test_gpu_gradient_reduced_basis_matches_oracle():
+    """Reduced-basis contexts (para: even l, ortho: odd l carried): vectors
+    that vanish on the other planes, gradients against the oracle's of the
+    same full-length vectors within 1e-10."""
+    import bsl_amd as B
+    for sym, par in (("Even", 0), ("Odd", 1)):
+        spec, kw = cs.cage_case(5, 3, 8, 9, sym)
+        o = O.Oracle(spec)
+        ns = len(spec["site_q"])
+        ls = o.qnum()[:, 0]
+        rng = np.random.default_rng(13 + par)
+        psi = rng.uniform(-1, 1, (2, o.N))
+        for n, l in enumerate(ls):
+            if l % 2 != par:
+                psi[:, n * o.P:(n + 1) * o.P] = 0.0
+        psi /= np.linalg.norm(psi, axis=1, keepdims=True)
+        g = np.array([0.6, -1.3])
+        ref = o.hf_gradient(psi, g)
+        with B.Calculator(basis="reduced", **kw) as c:
+            got = c.hf_gradient(psi, g)
+        scale = np.max(np.abs(ref[:4 * ns]))
+        assert np.max(np.abs(got["site_q"] - ref[0:4 * ns:4])) / scale < 1e-10, sym
+        assert np.max(np.abs(got["site_pos"] - ref[:4 * ns].reshape(ns, 4)[:, 1:4])) / scale < 1e-10, sym
+        for e, name in zip(range(4 * ns, 4 * ns + 4), ("q_h2_h", "q_h2_cm", "lj_a", "lj_b")):
+            assert abs(got[name] - ref[e]) <= 1e-10 * abs(ref[e]), (sym, name)
+
+
+@pytest.mark.gpu
 def test_gpu_clamp_mask_matches_grid_and_oracle():
     """The gradients' clamp mask (a pass of the grid kernel keeping one bit
     per (point, node)) counts exactly the nodes of the context's own clamped

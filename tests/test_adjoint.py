@@ -306,6 +306,19 @@ def test_gpu_gradient_of_device_ritz_vectors():
     replaced, and indices beyond them."""
     import bsl_amd as B
     spec, kw = cs.cage_case(6, 3, 8, 8, "All")
+    _ritz_gradient_case(B, kw)
+
+
+@pytest.mark.gpu
+def test_gpu_gradient_of_device_ritz_vectors_mixed_precision():
+    """The same on an FP32-operator context (FP32 Loop 1 and 2, FP64 Ritz
+    vectors and refinement), the adjoint leg's configuration."""
+    import bsl_amd as B
+    spec, kw = cs.cage_case(6, 3, 8, 8, "All")
+    _ritz_gradient_case(B, dict(kw, precision="fp32"))
+
+
+def _ritz_gradient_case(B, kw):
     with B.Calculator(**kw) as c:
         with pytest.raises(B.BslError):
             c.hf_gradient_ritz([([0], [1.0])])
@@ -329,7 +342,7 @@ def test_gpu_gradient_of_device_ritz_vectors():
             for name in a:
                 assert np.array_equal(np.asarray(a[name]), np.asarray(b[name])), name
         a, b, b2 = c.lanczos(NITER, start="random", seed=0)
-        evtr, _ = B.trivec(a, b, b2, e[:1])
+        evtr, _ = B.trivec(a, b, b2, e[:1], mult_tol=c.mult_tol)
         c.ritz(NITER, evtr, 1, start="random", seed=0)
         with pytest.raises(B.BslError):
             c.hf_gradient_ritz([([0], [1.0])])

@@ -166,7 +166,7 @@ class Calculator:
         """The gradients of several sets at once (bsl_hf_gradient_sets): sets
         is a list of (psi (k, N), g (k)) pairs, e.g. one per degenerate
         multiplet; returns one dict per set, each equal to hf_gradient of
-        that set alone (up to four sets share a pass over the grid)."""
+        that set alone (up to eight sets share a pass over the grid)."""
         psis, gs, counts = [], [], []
         for psi, g in sets:
             psi = np.ascontiguousarray(np.atleast_2d(psi), dtype=np.float64)

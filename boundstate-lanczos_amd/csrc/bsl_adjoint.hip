@@ -110,7 +110,7 @@ __global__ __launch_bounds__(kAWG) void k_pair_fill(const uint64_t *__restrict__
 // take the operations of a launch with that set alone (sums over r in
 // order, vectors in order; the point sums over a thread's entries in order,
 // then over the threads in a fixed order).
-constexpr int kHfSets = 4;  // most sets of eigenvectors per launch
+constexpr int kHfSets = kHfDensitySets;  // most sets of eigenvectors per density launch
 struct HfSets {
     int first[kHfSets + 1];
 };
@@ -498,7 +498,7 @@ int launch_hf_grad(const double *om, int64_t ne, const double *wcm, int nset, in
                    const int *lj, int ns, double q_h, double q_cm, double lj_a, double lj_b, int npb,
                    const uint32_t *pairs, const int64_t *first, double *partial, double *out, hipStream_t s)
 {
-    if (nset < 1 || nset > kHfSets) return fail(BSL_E_ARG, "adjoint: 1 to 4 sets of eigenvectors per pass");
+    if (nset < 1 || nset > kHfGradSets) return fail(BSL_E_ARG, "adjoint: 1 to 8 sets of eigenvectors per pass");
     const int nwg = hf_grad_groups(P, npb);
     const size_t lds = sizeof(double) * (4 * (size_t)ns + (size_t)(nset + 3) * npb) + sizeof(int) * ns;
     // dynamic LDS beside the static staging arrays of k_hf_grad
@@ -506,7 +506,13 @@ int launch_hf_grad(const double *om, int64_t ne, const double *wcm, int nset, in
     if (npb > kAWG || lds + lds_static > 96 * 1024) return fail(BSL_E_ARG, "adjoint: too many sites");
     // one instance per set count: a pass of fewer sets carries fewer sums
     // (the one-set instance keeps the occupancy of a single-set kernel)
-    auto kern = nset == 1 ? k_hf_grad<1> : nset == 2 ? k_hf_grad<2> : nset == 3 ? k_hf_grad<3> : k_hf_grad<4>;
+    static_assert(kHfGradSets == 8, "one kernel instance per set count");
+    void (*const kerns[8])(const double *, int64_t, const double *, int64_t, const double *, const double *,
+                           const double *, int, int, const double *, const double *, const int *, int, double,
+                           double, double, double, int, const uint32_t *, const int64_t *, double *) = {
+        k_hf_grad<1>, k_hf_grad<2>, k_hf_grad<3>, k_hf_grad<4>,
+        k_hf_grad<5>, k_hf_grad<6>, k_hf_grad<7>, k_hf_grad<8>};
+    auto kern = kerns[nset - 1];
     hipLaunchKernelGGL(kern, dim3(nwg), dim3(kAWG), lds, s, om, ne, wcm, P, gx, gy, gz, nx, ny, off, site4, lj, ns, q_h,
                        q_cm, lj_a, lj_b, npb, pairs, first, partial);
     BSL_HIP(hipGetLastError());

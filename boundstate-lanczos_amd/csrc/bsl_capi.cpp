@@ -1424,7 +1424,8 @@ int hf_prepare(bsl_ctx *c, int nset, const int32_t *counts, int64_t *nvec)
     }
     if (c->p.potential != BSL_POT_CAGE) return fail(BSL_E_STATE, "bsl_hf_gradient: cage potential only");
     BSL_HIP(hipSetDevice(c->device));
-    const int nent = 4 * c->p.n_sites + 4, nwg = hf_grad_groups(c->P, kHfPoints), chunk = std::min(nset, 4);
+    const int nent = 4 * c->p.n_sites + 4, nwg = hf_grad_groups(c->P, kHfPoints);
+    const int chunk = std::min(nset, kHfGradSets);  // sets of a gradient pass
     size_t *cap = c->hf_cap;
     int rc = ensure_hf_setup(c);
     if (!rc) rc = grow(c->d_hfpsi, cap[0], (size_t)*nvec * c->n_kry);
@@ -1441,24 +1442,32 @@ int hf_passes(bsl_ctx *c, int nset, const int32_t *counts, double *grad)
     const bsl_params &p = c->p;
     const int nk = (int)c->carried.size(), ns = p.n_sites, Q = c->Q;
     const int npb = kHfPoints, nent = 4 * ns + 4;
-    const int chunk = std::min<int>(nset, 4);  // sets per gradient pass (launch_hf_grad's limit)
     const int64_t ne = c->hf_ne;
-    // passes of up to four sets whose eigenvectors' coefficients fit the
-    // density kernel's LDS (a set alone that does not fit fails there)
+    // gradient passes of up to kHfGradSets sets; their weights come from
+    // density launches of up to kHfDensitySets sets whose eigenvectors'
+    // coefficients fit the density kernel's LDS (a set alone that does not
+    // fit fails there), each writing its sets' slots of the pass
     const int vmax = std::max(4, (int)(65536 / (sizeof(double) * kHfPoints * nk)) & ~3);
     int rc = BSL_OK;
-    int64_t v0 = 0;  // first eigenvector of the pass
-    for (int m0 = 0, nm = 0; m0 < nset && !rc; m0 += nm) {
-        int nv = counts[m0];
-        for (nm = 1; nm < chunk && m0 + nm < nset && nv + counts[m0 + nm] <= vmax; nm++) nv += counts[m0 + nm];
-        rc = launch_hf_density(c->d_hfpsi + (size_t)v0 * c->n_kry, nm, counts + m0, c->d_hfg + v0, c->d_hfY,
-                               c->d_hfwq, c->d_unclamped, c->d_pairs, c->d_pair_first, ne, c->d_hfom, c->d_hfwcm,
-                               c->P, nk, Q, c->stream);
+    int64_t v0 = 0;  // first eigenvector of the density launch
+    for (int m0 = 0, np = 0; m0 < nset && !rc; m0 += np) {
+        for (np = 0; np < kHfGradSets && m0 + np < nset && !rc;) {
+            const int m1 = m0 + np;
+            int nv = counts[m1], nm = 1;
+            for (; nm < kHfDensitySets && np + nm < kHfGradSets && m1 + nm < nset && nv + counts[m1 + nm] <= vmax;
+                 nm++)
+                nv += counts[m1 + nm];
+            rc = launch_hf_density(c->d_hfpsi + (size_t)v0 * c->n_kry, nm, counts + m1, c->d_hfg + v0, c->d_hfY,
+                                   c->d_hfwq, c->d_unclamped, c->d_pairs, c->d_pair_first, ne,
+                                   c->d_hfom + (size_t)np * ne, c->d_hfwcm + (size_t)np * c->P, c->P, nk, Q,
+                                   c->stream);
+            v0 += nv;
+            np += nm;
+        }
         if (!rc)
-            rc = launch_hf_grad(c->d_hfom, ne, c->d_hfwcm, nm, c->P, c->d_gx, c->d_gy, c->d_gz, p.nx, p.ny,
+            rc = launch_hf_grad(c->d_hfom, ne, c->d_hfwcm, np, c->P, c->d_gx, c->d_gy, c->d_gz, p.nx, p.ny,
                                 c->d_hfoff, c->d_hfsite, c->d_hflj, ns, p.q_h2_h, p.q_h2_cm, p.lj_a, p.lj_b, npb,
                                 c->d_pairs, c->d_pair_first, c->d_hfpart, c->d_hfout + (size_t)m0 * nent, c->stream);
-        for (int m = 0; m < nm; m++) v0 += counts[m0 + m];
     }
     if (!rc) {
         hipError_t e = hipMemcpyAsync(grad, c->d_hfout, sizeof(double) * nset * nent, hipMemcpyDeviceToHost, c->stream);

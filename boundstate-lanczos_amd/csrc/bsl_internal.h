@@ -331,7 +331,9 @@ int ctx_start(bsl_ctx *c, const double *v0, int start, uint64_t seed);
 int ctx_step(bsl_ctx *c);
 
 // Hellmann-Feynman adjoint (bsl_adjoint.hip)
-constexpr int kHfPoints = 4;  // points per workgroup of the gradient kernels
+constexpr int kHfPoints = 4;       // points per workgroup of the gradient kernels
+constexpr int kHfDensitySets = 4;  // most weight sets of a density launch
+constexpr int kHfGradSets = 8;     // most weight sets of a gradient pass
 int hf_grad_groups(int64_t P, int npb);
 // the kept (point, node pair) entries of each group of npb points (npb <=
 // 256, Q even and <= 2^24; node q < Q/2 stands for q and its inversion
@@ -341,14 +343,15 @@ int hf_grad_groups(int64_t P, int npb);
 int launch_pair_count(const uint64_t *unclamped, int64_t P, int Q, int npb, int *count, hipStream_t s);
 int launch_pair_fill(const uint64_t *unclamped, int64_t P, int Q, int npb, const int64_t *first, uint32_t *pairs,
                      hipStream_t s);
-// nset (1 to 4) sets of counts[m] consecutive eigenvectors (psi, g): the
+// nset (1 to kHfDensitySets) sets of counts[m] consecutive eigenvectors (psi, g): the
 // folded weights of set m at om + m ne for the ne entries of the pair list
 // (pairs / first, groups of kHfPoints points) and the H2-centre weights of
 // its points at wcm + m P
 int launch_hf_density(const double *psi, int nset, const int *counts, const double *g, const double *Y,
                       const double *wq, const uint64_t *unclamped, const uint32_t *pairs, const int64_t *first,
                       int64_t ne, double *om, double *wcm, int64_t P, int nk, int Q, hipStream_t s);
-// the gradients of those sets (npb = kHfPoints); partial:
+// the gradients of nset (1 to kHfGradSets) sets whose weights are at om +
+// m ne and wcm + m P (npb = kHfPoints); partial:
 // hf_grad_groups(P, npb) * nset * (4 ns + 4) doubles; out: nset * (4 ns + 4),
 // set after set
 int launch_hf_grad(const double *om, int64_t ne, const double *wcm, int nset, int64_t P, const double *gx,

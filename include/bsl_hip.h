@@ -321,7 +321,7 @@ int bsl_hf_gradient(bsl_ctx *ctx, int32_t nvec, const double *psi, const double 
 // multiplet of the lowest levels): set m is the counts[m] eigenvectors that
 // follow the previous sets' in psi, with their weights in g; grad receives
 // nset blocks of 4*n_sites + 4, each equal (bitwise) to bsl_hf_gradient of
-// that set alone.  Up to four sets share one pass over the (point, node)
+// that set alone.  Up to eight sets share one pass over the (point, node)
 // pairs, whose geometry (the distances to every site) is computed once per
 // pass.  No reference counterpart.
 int bsl_hf_gradient_sets(bsl_ctx *ctx, int32_t nset, const int32_t *counts, const double *psi, const double *g,

@@ -467,18 +467,20 @@ int autotune(bsl_ctx *c)
             break;
         }
         if ((rc = set_fused(*ops[o], 0))) break;
-        float best = 1e30f;
-        int pick = 1;
+        // each variant's best of three rounds, the rounds interleaved so that
+        // a slow moment of the device does not decide the choice
         const int nvar = ops[o]->vform == BSL_VFORM_HARMONICS ? 2 : 3;
-        for (int v = 0; v < nvar && !rc; v++) {
-            ops[o]->vb_variant = v;
-            float ms = 0;
-            rc = time_potential(c, *ops[o], &ms);
-            if (!rc && ms < best) {
-                best = ms;
-                pick = v;
+        float best[3] = {1e30f, 1e30f, 1e30f};
+        for (int round = 0; round < 3 && !rc; round++)
+            for (int v = 0; v < nvar && !rc; v++) {
+                ops[o]->vb_variant = v;
+                float ms = 0;
+                rc = time_potential(c, *ops[o], &ms);
+                if (!rc && ms < best[v]) best[v] = ms;
             }
-        }
+        int pick = 1;
+        for (int v = 0; v < nvar; v++)
+            if (best[v] < best[pick]) pick = v;
         ops[o]->vb_variant = pick;
     }
     if (!rc) rc = launch_init_scal(c->d_scal, 0.0, 0.0, c->stream);

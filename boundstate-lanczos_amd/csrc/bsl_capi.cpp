@@ -534,6 +534,7 @@ extern "C" int bsl_destroy(bsl_ctx *c)
     dev_free(c->d_R);
     dev_free(c->d_T);
     dev_free(c->d_ritz);
+    dev_free(c->d_rwin);
     dev_free(c->d_rcoef);
     dev_free(c->d_scal);
     dev_free(c->d_alpha);
@@ -968,10 +969,28 @@ int ritz_run(bsl_ctx *c, int niter, const double *v0, int start, uint64_t seed, 
     rc = launch_zero(c->d_ritz, c->n_kry * ngood, c->stream);
     if (!rc) rc = ensure_coeffs(c, niter);
     if (!rc) rc = start_lanczos(c, v0, start, seed);
+    // Lanczos vectors are staged in a window of kRitzWindow and added to the
+    // Ritz vectors a window at a time (each Ritz vector read and written once
+    // per window instead of once per step); a single Ritz vector takes the
+    // per-step sum directly
+    const int win = ngood > 1 ? std::min(kRitzWindow, std::max(jlast, 1)) : 1;
+    if (!rc && win > 1) rc = ensure_workspace(&c->d_rwin, &c->rwin_cap, (size_t)c->n_kry * win);
+    int j0 = 1;  // first step of the window
     for (int j = 1; j <= niter && !rc; j++) {
-        if (j <= jlast)
-            rc = launch_ritz_accum(c->d_ritz, c->d_X, c->d_R, c->d_scal, c->d_rcoef + (size_t)(j - 1) * ngood,
-                                   c->n_kry, ngood, c->stream);
+        if (j <= jlast) {
+            if (win == 1) {
+                rc = launch_ritz_accum(c->d_ritz, c->d_X, c->d_R, c->d_scal, c->d_rcoef + (size_t)(j - 1) * ngood,
+                                       c->n_kry, ngood, c->stream);
+            } else {
+                rc = launch_ritz_stage(c->d_rwin + (size_t)(j - j0) * c->n_kry, c->d_X, c->d_R, c->d_scal, c->n_kry,
+                                       c->stream);
+                if (!rc && (j - j0 + 1 == win || j == jlast)) {
+                    rc = launch_ritz_flush(c->d_ritz, c->d_rwin, c->d_rcoef + (size_t)(j0 - 1) * ngood, j - j0 + 1,
+                                           c->n_kry, ngood, c->stream);
+                    j0 = j + 1;
+                }
+            }
+        }
         if (!rc) rc = lanczos_step(c, c->op, j, nullptr);
     }
     // the host coefficient list must outlive the copy

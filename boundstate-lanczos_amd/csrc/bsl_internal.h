@@ -245,6 +245,12 @@ int launch_zero(double *p, int64_t n, hipStream_t s);
 // ritz[g] += coef[g] (ca X + cb R): the Lanczos vector of the current step
 int launch_ritz_accum(double *ritz, const double *X, const double *R, const Scal *sc, const double *coef, int64_t n,
                       int ngood, hipStream_t s);
+// the same in windows: w = ca X + cb R staged per step, then for the nw <=
+// kRitzWindow staged steps ritz[g] += coef[k][g] w[k] in step order (bitwise
+// the sums of nw k_ritz_accum steps)
+constexpr int kRitzWindow = 16;
+int launch_ritz_stage(double *w, const double *X, const double *R, const Scal *sc, int64_t n, hipStream_t s);
+int launch_ritz_flush(double *ritz, const double *w, const double *coef, int nw, int64_t n, int ngood, hipStream_t s);
 
 // per-workgroup pieces of a Rayleigh quotient (mode 0: y.y and y.hy; mode 1:
 // |hy - e y|^2) into partial[2 b], partial[2 b + 1], b < *groups
@@ -401,8 +407,8 @@ struct bsl_ctx {
     double *d_X = nullptr, *d_R = nullptr, *d_T = nullptr;
     // Loop-2 workspace, kept between calls (grows to the largest request):
     // Ritz accumulators [ngood][n_kry] and the per-step coefficients
-    double *d_ritz = nullptr, *d_rcoef = nullptr;
-    size_t ritz_cap = 0, rcoef_cap = 0;
+    double *d_ritz = nullptr, *d_rcoef = nullptr, *d_rwin = nullptr;  // d_rwin: staged Lanczos vectors
+    size_t ritz_cap = 0, rcoef_cap = 0, rwin_cap = 0;
     // squared norms of the Ritz vectors the last bsl_ritz_refine left in
     // d_ritz (empty when they are gone: bsl_ritz and Loop-2 calls reuse it)
     std::vector<double> ritz_vv;

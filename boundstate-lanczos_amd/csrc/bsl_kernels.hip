@@ -1923,6 +1923,56 @@ int launch_ritz_accum(double *ritz, const double *X, const double *R, const Scal
     return BSL_OK;
 }
 
+// w[i] = ca X[i] + cb R[i]: the Lanczos vector the next step forms, kept
+// for a later flush of the Ritz sums
+__global__ void k_ritz_stage(double *__restrict__ w, const double *__restrict__ X, const double *__restrict__ R,
+                             const Scal *__restrict__ sc, int64_t n)
+{
+    const double ca = sc->ca, cb = sc->cb;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        w[i] = ca * X[i] + cb * R[i];
+}
+
+// ritz[g][i] += coef[k][g] w[k][i] for k = 0 .. nw-1 in order, skipping zero
+// coefficients: the multiply-adds, and their order, of nw steps of
+// k_ritz_accum, with each Ritz vector read and written once per window
+__global__ void k_ritz_flush(double *ritz, const double *__restrict__ w, const double *__restrict__ coef, int nw,
+                             int64_t n, int ngood)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        double wv[kRitzWindow];  // the staged entries of this element, loaded once
+#pragma unroll
+        for (int k = 0; k < kRitzWindow; k++) wv[k] = k < nw ? w[(size_t)k * n + i] : 0.0;
+        for (int g = 0; g < ngood; g++) {
+            bool any = false;
+            for (int k = 0; k < nw; k++) any = any || coef[(size_t)k * ngood + g] != 0.0;
+            if (!any) continue;
+            double r = ritz[(size_t)g * n + i];
+#pragma unroll
+            for (int k = 0; k < kRitzWindow; k++)
+                if (k < nw) {
+                    const double c = coef[(size_t)k * ngood + g];
+                    if (c != 0.0) r += c * wv[k];
+                }
+            ritz[(size_t)g * n + i] = r;
+        }
+    }
+}
+
+int launch_ritz_stage(double *w, const double *X, const double *R, const Scal *sc, int64_t n, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_ritz_stage, dim3(grid_for(n)), dim3(kWG), 0, s, w, X, R, sc, n);
+    BSL_HIP(hipGetLastError());
+    return BSL_OK;
+}
+
+int launch_ritz_flush(double *ritz, const double *w, const double *coef, int nw, int64_t n, int ngood, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_ritz_flush, dim3(grid_for(n)), dim3(kWG), 0, s, ritz, w, coef, nw, n, ngood);
+    BSL_HIP(hipGetLastError());
+    return BSL_OK;
+}
+
 // ---------------------------------------------------------------------------
 // preparation: potential grid
 // ---------------------------------------------------------------------------

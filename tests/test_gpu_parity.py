@@ -226,6 +226,25 @@ def test_ritz_vectors_match_oracle():
         assert _rel(got[g], ref[g]) < 1e-7
 
 
+def test_ritz_windows_equal_single_vector_runs():
+    """Several Ritz vectors are summed a window of 16 staged Lanczos vectors
+    at a time, one alone step by step: the same multiply-adds in the same
+    order, so each vector of a 5-vector Loop 2 (80 steps: five windows) is
+    bitwise the one a Loop 2 for that vector alone gives."""
+    spec, kw = cs.cage_case(5, 2, 6, 6, "All")
+    o, c = O.Oracle(spec), B.Calculator(**kw)
+    niter = 80
+    a, b, b2 = o.lanczos(niter)
+    ev, err = O.lancbis(a, b, b2, -10.0, 10.0)
+    ev = ev[:5]
+    evtr, _ = O.trivec(a, b, b2, ev)
+    got = c.ritz(niter, evtr, len(ev))
+    cols = np.asarray(evtr).reshape(len(ev), niter)
+    for g in range(len(ev)):
+        one = c.ritz(niter, np.ascontiguousarray(cols[g]), 1)
+        assert np.array_equal(got[g], one[0]), g
+
+
 def _oracle_rayleigh(o, y):
     hy = o.hv(y)
     e = float(y @ hy) / float(y @ y)

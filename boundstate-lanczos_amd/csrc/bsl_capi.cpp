@@ -969,29 +969,30 @@ int ritz_run(bsl_ctx *c, int niter, const double *v0, int start, uint64_t seed, 
     rc = launch_zero(c->d_ritz, c->n_kry * ngood, c->stream);
     if (!rc) rc = ensure_coeffs(c, niter);
     if (!rc) rc = start_lanczos(c, v0, start, seed);
-    // Lanczos vectors are staged in a window of kRitzWindow and added to the
-    // Ritz vectors a window at a time (each Ritz vector read and written once
-    // per window instead of once per step); a single Ritz vector takes the
-    // per-step sum directly
+    // Step j writes its Lanczos vector v_j over X, which the swap after the
+    // step makes R.  Several Ritz vectors take the v_j of a window of
+    // kRitzWindow steps, copied aside, in one read and write each; a single
+    // one takes each v_j from R directly.  Either way ritz += coef_j v_j in
+    // step order, with exactly the v_j the recurrence used.
     const int win = ngood > 1 ? std::min(kRitzWindow, std::max(jlast, 1)) : 1;
     if (!rc && win > 1) rc = ensure_workspace(&c->d_rwin, &c->rwin_cap, (size_t)c->n_kry * win);
     int j0 = 1;  // first step of the window
     for (int j = 1; j <= niter && !rc; j++) {
-        if (j <= jlast) {
-            if (win == 1) {
-                rc = launch_ritz_accum(c->d_ritz, c->d_X, c->d_R, c->d_scal, c->d_rcoef + (size_t)(j - 1) * ngood,
-                                       c->n_kry, ngood, c->stream);
-            } else {
-                rc = launch_ritz_stage(c->d_rwin + (size_t)(j - j0) * c->n_kry, c->d_X, c->d_R, c->d_scal, c->n_kry,
-                                       c->stream);
-                if (!rc && (j - j0 + 1 == win || j == jlast)) {
-                    rc = launch_ritz_flush(c->d_ritz, c->d_rwin, c->d_rcoef + (size_t)(j0 - 1) * ngood, j - j0 + 1,
-                                           c->n_kry, ngood, c->stream);
-                    j0 = j + 1;
-                }
+        rc = lanczos_step(c, c->op, j, nullptr);
+        if (rc || j > jlast) continue;
+        const double *coef = c->d_rcoef + (size_t)(j0 - 1) * ngood;
+        if (win == 1) {
+            rc = launch_ritz_flush(c->d_ritz, c->d_R, coef, 1, c->n_kry, ngood, c->stream);
+            j0 = j + 1;
+        } else {
+            const hipError_t e = hipMemcpyAsync(c->d_rwin + (size_t)(j - j0) * c->n_kry, c->d_R,
+                                                sizeof(double) * c->n_kry, hipMemcpyDeviceToDevice, c->stream);
+            if (e != hipSuccess) rc = hip_fail(e, "stage Lanczos vector", __FILE__, __LINE__);
+            if (!rc && (j - j0 + 1 == win || j == jlast)) {
+                rc = launch_ritz_flush(c->d_ritz, c->d_rwin, coef, j - j0 + 1, c->n_kry, ngood, c->stream);
+                j0 = j + 1;
             }
         }
-        if (!rc) rc = lanczos_step(c, c->op, j, nullptr);
     }
     // the host coefficient list must outlive the copy
     const hipError_t e = hipStreamSynchronize(c->stream);

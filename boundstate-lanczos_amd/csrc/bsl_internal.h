@@ -242,14 +242,9 @@ int launch_init_scal(Scal *sc, double ca, double cr, hipStream_t s);
 int launch_start_vector(double *X, int64_t n, int64_t P, const int *plane_ref, int kind, uint64_t seed, Scal *sc,
                         double *partial, int npartial, hipStream_t s);
 int launch_zero(double *p, int64_t n, hipStream_t s);
-// ritz[g] += coef[g] (ca X + cb R): the Lanczos vector of the current step
-int launch_ritz_accum(double *ritz, const double *X, const double *R, const Scal *sc, const double *coef, int64_t n,
-                      int ngood, hipStream_t s);
-// the same in windows: w = ca X + cb R staged per step, then for the nw <=
-// kRitzWindow staged steps ritz[g] += coef[k][g] w[k] in step order (bitwise
-// the sums of nw k_ritz_accum steps)
+// ritz[g] += coef[k][g] w[k] for the nw <= kRitzWindow Lanczos vectors w[k]
+// of consecutive steps, in step order
 constexpr int kRitzWindow = 16;
-int launch_ritz_stage(double *w, const double *X, const double *R, const Scal *sc, int64_t n, hipStream_t s);
 int launch_ritz_flush(double *ritz, const double *w, const double *coef, int nw, int64_t n, int ngood, hipStream_t s);
 
 // per-workgroup pieces of a Rayleigh quotient (mode 0: y.y and y.hy; mode 1:

@@ -1432,21 +1432,6 @@ __global__ __launch_bounds__(kWG) void k_start_random(double *X, int64_t n, int6
     if (threadIdx.x == 0) partial[blockIdx.x] = b;
 }
 
-// ritz[g][i] += coef[g] * (ca X[i] + cb R[i]) (the Lanczos vector the next
-// step forms) for the vectors still accumulating
-__global__ void k_ritz_accum(double *ritz, const double *__restrict__ X, const double *__restrict__ R,
-                             const Scal *__restrict__ sc, const double *__restrict__ coef, int64_t n, int ngood)
-{
-    const double ca = sc->ca, cb = sc->cb;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        const double v = ca * X[i] + cb * R[i];
-        for (int g = 0; g < ngood; g++) {
-            const double c = coef[g];
-            if (c != 0.0) ritz[(size_t)g * n + i] += c * v;
-        }
-    }
-}
-
 // Rayleigh quotient pieces of y and hy = H y, per workgroup (grid-stride
 // shares, summed in a fixed order): mode 0 writes sum y^2 and sum y.hy to
 // partial[2 b], partial[2 b + 1]; mode 1 writes sum (hy - e y)^2 to
@@ -1915,27 +1900,9 @@ int launch_rayleigh_dots(const double *y, const double *hy, int64_t n, double e,
     return BSL_OK;
 }
 
-int launch_ritz_accum(double *ritz, const double *X, const double *R, const Scal *sc, const double *coef, int64_t n,
-                      int ngood, hipStream_t s)
-{
-    hipLaunchKernelGGL(k_ritz_accum, dim3(grid_for(n)), dim3(kWG), 0, s, ritz, X, R, sc, coef, n, ngood);
-    BSL_HIP(hipGetLastError());
-    return BSL_OK;
-}
-
-// w[i] = ca X[i] + cb R[i]: the Lanczos vector the next step forms, kept
-// for a later flush of the Ritz sums
-__global__ void k_ritz_stage(double *__restrict__ w, const double *__restrict__ X, const double *__restrict__ R,
-                             const Scal *__restrict__ sc, int64_t n)
-{
-    const double ca = sc->ca, cb = sc->cb;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-        w[i] = ca * X[i] + cb * R[i];
-}
-
-// ritz[g][i] += coef[k][g] w[k][i] for k = 0 .. nw-1 in order, skipping zero
-// coefficients: the multiply-adds, and their order, of nw steps of
-// k_ritz_accum, with each Ritz vector read and written once per window
+// ritz[g][i] += coef[k][g] w[k][i] for k = 0 .. nw-1 in order (w: the
+// Lanczos vectors of nw consecutive steps), skipping zero coefficients, with
+// each Ritz vector read and written once per window
 __global__ void k_ritz_flush(double *ritz, const double *__restrict__ w, const double *__restrict__ coef, int nw,
                              int64_t n, int ngood)
 {
@@ -1957,13 +1924,6 @@ __global__ void k_ritz_flush(double *ritz, const double *__restrict__ w, const d
             ritz[(size_t)g * n + i] = r;
         }
     }
-}
-
-int launch_ritz_stage(double *w, const double *X, const double *R, const Scal *sc, int64_t n, hipStream_t s)
-{
-    hipLaunchKernelGGL(k_ritz_stage, dim3(grid_for(n)), dim3(kWG), 0, s, w, X, R, sc, n);
-    BSL_HIP(hipGetLastError());
-    return BSL_OK;
 }
 
 int launch_ritz_flush(double *ritz, const double *w, const double *coef, int nw, int64_t n, int ngood, hipStream_t s)

@@ -227,10 +227,11 @@ def test_ritz_vectors_match_oracle():
 
 
 def test_ritz_windows_equal_single_vector_runs():
-    """Several Ritz vectors are summed a window of 16 staged Lanczos vectors
-    at a time, one alone step by step: the same multiply-adds in the same
-    order, so each vector of a 5-vector Loop 2 (80 steps: five windows) is
-    bitwise the one a Loop 2 for that vector alone gives."""
+    """Several Ritz vectors are summed a window of 16 copied Lanczos vectors
+    at a time, one alone step by step from the recurrence's own buffer: the
+    same multiply-adds in the same order, so each vector of a 5-vector Loop 2
+    (80 steps: five windows) is bitwise the one a Loop 2 for that vector
+    alone gives."""
     spec, kw = cs.cage_case(5, 2, 6, 6, "All")
     o, c = O.Oracle(spec), B.Calculator(**kw)
     niter = 80

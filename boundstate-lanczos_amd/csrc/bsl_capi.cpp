@@ -980,16 +980,16 @@ int ritz_run(bsl_ctx *c, int niter, const double *v0, int start, uint64_t seed, 
     for (int j = 1; j <= niter && !rc; j++) {
         rc = lanczos_step(c, c->op, j, nullptr);
         if (rc || j > jlast) continue;
-        const double *coef = c->d_rcoef + (size_t)(j0 - 1) * ngood;
+        const double *wcoef = c->d_rcoef + (size_t)(j0 - 1) * ngood;
         if (win == 1) {
-            rc = launch_ritz_flush(c->d_ritz, c->d_R, coef, 1, c->n_kry, ngood, c->stream);
+            rc = launch_ritz_flush(c->d_ritz, c->d_R, wcoef, 1, c->n_kry, ngood, c->stream);
             j0 = j + 1;
         } else {
             const hipError_t e = hipMemcpyAsync(c->d_rwin + (size_t)(j - j0) * c->n_kry, c->d_R,
                                                 sizeof(double) * c->n_kry, hipMemcpyDeviceToDevice, c->stream);
             if (e != hipSuccess) rc = hip_fail(e, "stage Lanczos vector", __FILE__, __LINE__);
             if (!rc && (j - j0 + 1 == win || j == jlast)) {
-                rc = launch_ritz_flush(c->d_ritz, c->d_rwin, coef, j - j0 + 1, c->n_kry, ngood, c->stream);
+                rc = launch_ritz_flush(c->d_ritz, c->d_rwin, wcoef, j - j0 + 1, c->n_kry, ngood, c->stream);
                 j0 = j + 1;
             }
         }

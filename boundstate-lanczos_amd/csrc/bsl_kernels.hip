@@ -1998,10 +1998,31 @@ __device__ inline double rsqrt_nr(double x)
     return y;
 }
 
+// The charged sites of the cage with their H-site weights K q_H q_s, in site
+// order: hs[n][4] = x, y, z, K q_H q_s, n < *nh (uncharged sites, such as
+// TIP4P's O, give no term, as in the reference's sum).  One thread: a few
+// hundred sites at most.
+__global__ void k_cage_hsites(const double *__restrict__ site_g, int ns, double q_h, double *__restrict__ hs,
+                              int *__restrict__ nh)
+{
+    int n = 0;
+    for (int t = 0; t < ns; t++) {
+        const double w = 138.9354846111096 * q_h * site_g[4 * t + 3];
+        if (w == 0.0) continue;
+        hs[4 * n + 0] = site_g[4 * t + 0];
+        hs[4 * n + 1] = site_g[4 * t + 1];
+        hs[4 * n + 2] = site_g[4 * t + 2];
+        hs[4 * n + 3] = w;
+        n++;
+    }
+    *nh = n;
+}
+
 // vg[p][q], q = (half, a, b): H2 centre at grid point p, H atoms at +-off[q].
-// The H-site terms K q_H q_s / |h - s| are summed in site order with the
-// weights K q_H q_s staged in LDS and 1/|h - s| from rsqrt_nr (a few ulp from
-// the quotient of a correctly rounded square root, per term).  MASK: instead
+// The H-site terms K q_H q_s / |h - s| are summed in site order over the
+// charged sites hs (wave-uniform reads, no branch) with 1/|h - s| from
+// rsqrt_nr (a few ulp from the quotient of a correctly rounded square root,
+// per term).  MASK: instead
 // of the grid, one bit per (point, node), [vg < ceiling] (the clamp mask of
 // the gradients, bit idx & 63 of word idx >> 6): each wave takes 64
 // consecutive idx (the loop runs in whole waves) and stores the ballot.
@@ -2009,13 +2030,11 @@ template <bool MASK>
 __global__ __launch_bounds__(kWG) void k_vgrid_cage(double *vg, uint64_t *mask, int64_t P, int Q,
                                                     const double *__restrict__ gx, const double *__restrict__ gy,
                                                     const double *__restrict__ gz, int nx, int ny,
-                                                    const double *__restrict__ off, const double *__restrict__ site_g,
-                                                    int ns, const double *__restrict__ cm, double q_h, double ceiling)
+                                                    const double *__restrict__ off, const double *__restrict__ hs,
+                                                    const int *__restrict__ nh_g, const double *__restrict__ cm,
+                                                    double ceiling)
 {
-    extern __shared__ double site[];  // [ns][4]: x, y, z, K q_H q_s
-    for (int t = threadIdx.x; t < 4 * ns; t += blockDim.x)
-        site[t] = (t & 3) == 3 ? 138.9354846111096 * q_h * site_g[t] : site_g[t];
-    __syncthreads();
+    const int nh = __builtin_amdgcn_readfirstlane(*nh_g);
     const int64_t total = P * (int64_t)Q;
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     // whole waves: blockDim and the stride are multiples of 64
@@ -2030,9 +2049,8 @@ __global__ __launch_bounds__(kWG) void k_vgrid_cage(double *vg, uint64_t *mask, 
         const double ox = off[3 * q], oy = off[3 * q + 1], oz = off[3 * q + 2];
         const double ax = x + ox, ay = y + oy, az = z + oz, bx = x - ox, by = y - oy, bz = z - oz;
         double va = 0.0, vb = 0.0;
-        for (int t = 0; t < ns; t++) {
-            const double sx = site[4 * t], sy = site[4 * t + 1], sz = site[4 * t + 2], w = site[4 * t + 3];
-            if (w == 0.0) continue;  // uncharged site (TIP4P's O): no term, as in the reference's sum
+        for (int t = 0; t < nh; t++) {
+            const double sx = hs[4 * t], sy = hs[4 * t + 1], sz = hs[4 * t + 2], w = hs[4 * t + 3];
             const double a0 = sx - ax, a1 = sy - ay, a2 = sz - az;
             const double b0 = sx - bx, b1 = sy - by, b2 = sz - bz;
             va = fma(w, rsqrt_nr(fma(a0, a0, fma(a1, a1, a2 * a2))), va);
@@ -2081,14 +2099,20 @@ int launch_vgrid_cage(double *vg, uint64_t *mask, int64_t P, int Q, const double
     hipLaunchKernelGGL(k_cm_potential, dim3((unsigned)((P + kWG - 1) / kWG)), dim3(kWG), lds_cm, s, cm, P, gx, gy,
                        gz, nx, ny, cage.site, cage.lj, cage.nsites, q_cm, lj_a, lj_b);
     BSL_HIP(hipGetLastError());
-    const size_t lds = sizeof(double) * 4 * cage.nsites;
-    if (mask)
-        hipLaunchKernelGGL(k_vgrid_cage<true>, dim3(grid_for(P * Q, 65536)), dim3(kWG), lds, s, nullptr, mask, P, Q,
-                           gx, gy, gz, nx, ny, off, cage.site, cage.nsites, cm, q_h, ceiling);
-    else
-        hipLaunchKernelGGL(k_vgrid_cage<false>, dim3(grid_for(P * Q, 65536)), dim3(kWG), lds, s, vg, nullptr, P, Q,
-                           gx, gy, gz, nx, ny, off, cage.site, cage.nsites, cm, q_h, ceiling);
+    // charged sites [nsites][4] and their count after them
+    double *hs = nullptr;
+    BSL_HIP(hipMallocAsync((void **)&hs, sizeof(double) * (4 * (size_t)cage.nsites + 1), s));
+    int *nh = (int *)(hs + 4 * (size_t)cage.nsites);
+    hipLaunchKernelGGL(k_cage_hsites, dim3(1), dim3(1), 0, s, cage.site, cage.nsites, q_h, hs, nh);
     BSL_HIP(hipGetLastError());
+    if (mask)
+        hipLaunchKernelGGL(k_vgrid_cage<true>, dim3(grid_for(P * Q, 65536)), dim3(kWG), 0, s, nullptr, mask, P, Q, gx,
+                           gy, gz, nx, ny, off, hs, nh, cm, ceiling);
+    else
+        hipLaunchKernelGGL(k_vgrid_cage<false>, dim3(grid_for(P * Q, 65536)), dim3(kWG), 0, s, vg, nullptr, P, Q, gx,
+                           gy, gz, nx, ny, off, hs, nh, cm, ceiling);
+    BSL_HIP(hipGetLastError());
+    BSL_HIP(hipFreeAsync(hs, s));
     BSL_HIP(hipFreeAsync(cm, s));
     return BSL_OK;
 }
